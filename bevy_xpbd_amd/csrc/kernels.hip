@@ -1,0 +1,790 @@
+// HIP kernels for the XPBD cloth hot path on MI355X (gfx950, wave64).
+//
+// Arithmetic restates the reference CPU solver (src/xpbd_cpu.rs) operation
+// by operation: glam's scalar Vec3 order (dot = (x*x + y*y) + z*z,
+// length = sqrtf(dot), component-wise '/' and '*'), correctly rounded sqrtf and
+// division (HIP's default lowering), and no FMA contraction (this file is
+// compiled with -ffp-contract=off).  With that, every kernel reproduces the
+// f32 CPU solver bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace xpbd {
+
+// Reference constants (cloth_compute.rs:82-90).
+constexpr float JACOBI_CORRECTION_CAP = 0.28f;
+constexpr float PREDICT_MAX_SPEED = 12.0f;
+constexpr float GRAB_MAX_PULL = 0.065f;
+constexpr float COLLISION_APPLY_CLAMP = 0.35f;
+constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
+
+__device__ __forceinline__ float len3(float x, float y, float z) { return sqrtf((x * x + y * y) + z * z); }
+
+// predict_particle (xpbd_cpu.rs:38-65); the inverse mass rides in .w.
+__device__ __forceinline__ float4 predict(float4 x, float4 v, uint32_t i, const DevParams& p) {
+    if (x.w <= 0.f) return x;
+    float vx = v.x + p.gx * p.dt;
+    float vy = v.y + p.gy * p.dt;
+    float vz = v.z + p.gz * p.dt;
+    const float speed = len3(vx, vy, vz);
+    if (speed > PREDICT_MAX_SPEED && speed > 0.f) {
+        const float s = PREDICT_MAX_SPEED / speed;
+        vx = vx * s;
+        vy = vy * s;
+        vz = vz * s;
+    }
+    float px = x.x + vx * p.dt;
+    float py = x.y + vy * p.dt;
+    float pz = x.z + vz * p.dt;
+    if (p.grab_active && int32_t(i) == p.grab_idx) {
+        float ux = (p.grab_x - px) * p.grab_k;
+        float uy = (p.grab_y - py) * p.grab_k;
+        float uz = (p.grab_z - pz) * p.grab_k;
+        const float pl = len3(ux, uy, uz);
+        if (pl > GRAB_MAX_PULL && pl > 0.f) {
+            const float s = GRAB_MAX_PULL / pl;
+            ux = ux * s;
+            uy = uy * s;
+            uz = uz * s;
+        }
+        px = px + ux;
+        py = py + uy;
+        pz = pz + uz;
+    }
+    return make_float4(px, py, pz, x.w);
+}
+
+// Ground plane y = 0 then post_velocity (xpbd_cpu.rs:393-409) for one free
+// particle: v = ((x - prev) * inv_dt) * damp.
+__device__ __forceinline__ void finish_particle(float4 nx, float4 prev, uint32_t i, const uint8_t* ground,
+                                                const DevParams& p, float4* X, float4* V) {
+    if (ground && ground[i] && nx.y < 0.f) nx.y = 0.f;
+    X[i] = nx;
+    V[i] = make_float4(((nx.x - prev.x) * p.inv_dt) * p.damp, ((nx.y - prev.y) * p.inv_dt) * p.damp,
+                       ((nx.z - prev.z) * p.inv_dt) * p.damp, 0.f);
+}
+
+// Floats ordered as unsigned integers (for atomicMin/atomicMax bounds).
+__device__ __forceinline__ uint32_t f2o(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t o) {
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+// Block-wide bounds of the rows each thread holds; one atomic per block.
+// -0 is folded to +0 and NaN ignored, as the oracle does.
+__device__ __forceinline__ void block_aabb(bool valid, float x, float y, float z, uint32_t* aabb) {
+    float lo[3], hi[3];
+    const float v[3] = {x + 0.f, y + 0.f, z + 0.f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = valid ? v[a] : INFINITY;
+        hi[a] = valid ? v[a] : -INFINITY;
+        if (lo[a] != lo[a]) lo[a] = INFINITY;
+        if (hi[a] != hi[a]) hi[a] = -INFINITY;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+            hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+        }
+    __shared__ float red[16][6];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) {
+            red[wave][a] = lo[a];
+            red[wave][3 + a] = hi[a];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t nw = (blockDim.x + 63) >> 6;
+        for (uint32_t w = 1; w < nw; ++w)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fminf(lo[a], red[w][a]);
+                hi[a] = fmaxf(hi[a], red[w][3 + a]);
+            }
+        for (int a = 0; a < 3; ++a) {
+            if (lo[a] <= hi[a]) {
+                atomicMin(&aabb[a], f2o(lo[a]));
+                atomicMax(&aabb[3 + a], f2o(hi[a]));
+            }
+        }
+    }
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; this makes consecutive logical
+// tiles land on one XCD so the +-2-row neighbour halo hits that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
+    return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi iteration: atomic-free per-particle gather.  The reference runs
+// jacobi_edges (per constraint dlambda, cloth_sim_jacobi.wgsl:143-180) and then
+// jacobi_gather (cloth_sim_jacobi.wgsl:182-218); since lambda is zero at
+// every read (xpbd_cpu.rs:267-268), dlambda = (-C - alpha~*0) / (w_i + w_j + alpha~)
+// depends only on the pair, and each endpoint recomputes it bit-identically
+// (|a-b| == |b-a|, w_i + w_j == w_j + w_i).  One pass, no lambda buffers.
+// ---------------------------------------------------------------------------
+template <uint32_t MODE>
+__global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
+    const uint32_t nb = gridDim.x;
+    const uint32_t tile = a.remap ? xcd_tile(blockIdx.x, nb) : blockIdx.x;
+    const uint32_t slices_per_block = blockDim.x >> 6;
+    const uint32_t slice = tile * slices_per_block + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t i = slice * 64 + lane;
+    const DevParams& p = a.p;
+    const bool valid = slice < p.nslices && i < p.n;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 out = pi;
+    if (valid) {
+        if (MODE & JM_PREDICT) {
+            pi = predict(a.src[i], a.vel[i], i, p);
+        } else {
+            pi = a.src[i];
+        }
+        out = pi;
+        const float wi = pi.w;
+        if (wi > 0.f) {
+            float ax = 0.f, ay = 0.f, az = 0.f;
+            const uint32_t base = a.sell.soff[slice] + lane;
+            const uint32_t width = a.sell.swid[slice];
+            for (uint32_t k = 0; k < width; ++k) {
+                const uint32_t e = base + k * 64;
+                const uint32_t j = a.sell.j[e];
+                if (j == NO_ENTRY) break;
+                const float rest = a.sell.rest[e];
+                const float comp = a.sell.comp[e];
+                float4 pj;
+                if (MODE & JM_PREDICT) pj = predict(a.src[j], a.vel[j], j, p);
+                else pj = a.src[j];
+                const float wj = pj.w;
+                if (wj <= 0.f && wi <= 0.f) continue;
+                float gx = pi.x - pj.x, gy = pi.y - pj.y, gz = pi.z - pj.z;
+                const float len = len3(gx, gy, gz);
+                if (len < 1e-8f) continue;
+                gx = gx / len;
+                gy = gy / len;
+                gz = gz / len;
+                const float c = len - rest;
+                const float alpha_t = comp * p.inv_dt_sq;
+                const float wsum = wi + wj + alpha_t;
+                if (wsum < 1e-8f) continue;
+                const float dlam = (-c - alpha_t * 0.0f) / wsum;
+                ax = ax + (gx * wi) * dlam;
+                ay = ay + (gy * wi) * dlam;
+                az = az + (gz * wi) * dlam;
+            }
+            float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
+            const float ml = len3(dx, dy, dz);
+            if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
+                const float s = JACOBI_CORRECTION_CAP / ml;
+                dx = dx * s;
+                dy = dy * s;
+                dz = dz * s;
+            }
+            out = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, wi);
+        }
+        if (MODE & JM_FINAL) {
+            if (pi.w > 0.f) finish_particle(out, a.dst[i], i, a.ground, p, a.dst, a.vout);
+        } else {
+            a.dst[i] = out;
+        }
+    }
+    if (MODE & JM_AABB) block_aabb(valid, out.x, out.y, out.z, a.aabb);
+}
+
+// Substep end without a collision pass when the last iterate is in F.
+__global__ void k_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const float4 f = F[i];
+    if (f.w <= 0.f) return;
+    finish_particle(f, X[i], i, ground, p, X, V);
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+__global__ void k_aabb_reset(uint32_t* aabb) {
+    const uint32_t t = threadIdx.x;
+    if (t < 3) aabb[t] = f2o(INFINITY);
+    else if (t < 6) aabb[t] = f2o(-INFINITY);
+}
+
+__global__ void k_aabb_of(const float4* P, uint32_t n, uint32_t* aabb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    float4 v = valid ? P[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    block_aabb(valid, v.x, v.y, v.z, aabb);
+}
+
+// ---------------------------------------------------------------------------
+// Collision broad phase.  The cell side is cell_size * 65/64 (a guard so f32
+// rounding in the key can never put two particles within cell_size two cells
+// apart), doubled until each axis has < 16383 cells and the count fits in 32
+// bits.  Key = ix + gx * (iy + gy * iz), dense over the current bounds, so no
+// two cells share a key.  Mirrors oracle_cell_grid / oracle_cell_keys.
+// ---------------------------------------------------------------------------
+__global__ void k_coll_grid(uint32_t* aabb, CollGrid* g, float cell_size) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = o2f(aabb[a]);
+        hi[a] = o2f(aabb[3 + a]);
+        if (!(lo[a] <= hi[a])) lo[a] = hi[a] = 0.f;
+    }
+    float cell = cell_size * 1.015625f;
+    float inv = 1.0f / cell;
+    uint32_t dims[3] = {1, 1, 1};
+    bool ok = false;
+    for (int it = 0; it < 64; ++it) {
+        inv = 1.0f / cell;
+        float e[3];
+        bool fits = true;
+        for (int a = 0; a < 3; ++a) {
+            e[a] = (hi[a] - lo[a]) * inv;
+            if (!(e[a] < 16383.0f)) fits = false;
+        }
+        if (fits) {
+            for (int a = 0; a < 3; ++a) dims[a] = uint32_t(floorf(e[a])) + 1u;
+            const uint64_t total = uint64_t(dims[0]) * dims[1] * dims[2];
+            if (total <= 0xFFFFFFFFull) {
+                ok = true;
+                break;
+            }
+        }
+        cell = cell * 2.0f;
+    }
+    if (!ok) dims[0] = dims[1] = dims[2] = 1;
+    for (int a = 0; a < 3; ++a) {
+        g->lo[a] = lo[a];
+        g->hi[a] = hi[a];
+        g->dims[a] = dims[a];
+    }
+    g->cell = cell;
+    g->inv_cell = inv;
+    g->pad = 0;
+    for (int a = 0; a < 3; ++a) {   // ready for the next substep's bounds
+        aabb[a] = f2o(INFINITY);
+        aabb[3 + a] = f2o(-INFINITY);
+    }
+}
+
+__device__ __forceinline__ void cell_of(float4 p, const CollGrid& g, uint32_t c[3]) {
+    const float v[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float q = (v[a] - g.lo[a]) * g.inv_cell;
+        q = floorf(q);
+        q = fmaxf(q, 0.0f);
+        q = fminf(q, float(g.dims[a] - 1));
+        c[a] = uint32_t(q);
+    }
+}
+
+__global__ void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grid, uint32_t* keys, uint32_t* ids,
+                            uint2* table, uint32_t slots) {
+    const CollGrid g = *grid;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t c[3];
+        cell_of(F[i], g, c);
+        keys[i] = c[0] + g.dims[0] * (c[1] + g.dims[1] * c[2]);
+        ids[i] = i;
+    }
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += stride)
+        table[s] = make_uint2(EMPTY_KEY, 0u);
+}
+
+// ---- stable LSD radix sort of (key, id), 8-bit digits, 4 passes ----------
+constexpr uint32_t RADIX_BLOCK = 256;
+constexpr uint32_t RADIX_ITEMS = 16;
+constexpr uint32_t RADIX_TILE = RADIX_BLOCK * RADIX_ITEMS;
+
+__global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const uint32_t* keys, uint32_t n, uint32_t shift,
+                                                          uint32_t* hist, uint32_t ntiles) {
+    __shared__ uint32_t h[256];
+    const uint32_t t = threadIdx.x;
+    h[t] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * RADIX_TILE;
+#pragma unroll 4
+    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+        const uint32_t idx = base + r * RADIX_BLOCK + t;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[t * ntiles + blockIdx.x] = h[t];
+}
+
+// Exclusive scan of `total` words in place, one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t* data, uint32_t total) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t chunk = (total + 1023) / 1024;
+    const uint32_t b = t * chunk, e = min(b + chunk, total);
+    uint32_t s = 0;
+    for (uint32_t k = b; k < e; ++k) s += data[k];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;
+    for (uint32_t k = b; k < e; ++k) {
+        const uint32_t v = data[k];
+        data[k] = run;
+        run += v;
+    }
+}
+
+// Stable scatter: within a tile, elements are ranked in input order (round
+// by round; inside a round, wave by wave; inside a wave, by lane) with a
+// 64-lane ballot match on the 8 digit bits.
+__global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* kin, const uint32_t* iin,
+                                                             uint32_t* kout, uint32_t* iout, uint32_t n,
+                                                             uint32_t shift, const uint32_t* hist,
+                                                             uint32_t ntiles) {
+    constexpr uint32_t NW = RADIX_BLOCK / 64;
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t wcnt[NW][256];
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    base[t] = hist[t * ntiles + blockIdx.x];
+    for (uint32_t w = 0; w < NW; ++w) wcnt[w][t] = 0;
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t tile0 = blockIdx.x * RADIX_TILE;
+    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+        const uint32_t idx = tile0 + r * RADIX_BLOCK + t;
+        const bool valid = idx < n;
+        const uint32_t key = valid ? kin[idx] : 0u;
+        const uint32_t id = valid ? iin[idx] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = __popcll(peers & lt);
+        if (valid && rank == 0) wcnt[wave][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base[d] + rank;
+            for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][d];
+            kout[pos] = key;
+            iout[pos] = id;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (uint32_t w = 0; w < NW; ++w) {
+            add += wcnt[w][t];
+            wcnt[w][t] = 0;
+        }
+        base[t] += add;
+        __syncthreads();
+    }
+}
+
+// Hash table cell -> first sorted index (keys are unique at run starts).
+__device__ __forceinline__ uint32_t slot_of(uint32_t key, uint32_t mask) { return (key * 2654435761u) & mask; }
+
+__global__ void k_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t key = skeys[s];
+    if (s > 0 && skeys[s - 1] == key) return;
+    uint32_t h = slot_of(key, mask);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const uint32_t prev = atomicCAS(&table[h].x, EMPTY_KEY, key);
+        if (prev == EMPTY_KEY) {
+            table[h].y = s;
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+__device__ __forceinline__ uint32_t table_find(const uint2* table, uint32_t mask, uint32_t key) {
+    uint32_t h = slot_of(key, mask);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const uint2 e = table[h];
+        if (e.x == key) return e.y;
+        if (e.x == EMPTY_KEY) return EMPTY_KEY;
+        h = (h + 1) & mask;
+    }
+    return EMPTY_KEY;
+}
+
+// Pair kernel of self_collision_resolve (xpbd_cpu.rs:84-121) for the pair
+// (min(i,j), max(i,j)); returns the contribution to particle i.
+__device__ __forceinline__ bool collision_contrib(uint32_t i, uint32_t j, float4 fi, float4 fj, float4 ri,
+                                                  float4 rj, const DevParams& p, float& cx, float& cy,
+                                                  float& cz) {
+    const bool i_lo = i < j;
+    const float4 pa = i_lo ? fi : fj, pb = i_lo ? fj : fi;
+    const float4 ra = i_lo ? ri : rj, rb = i_lo ? rj : ri;
+    const float wa = pa.w, wb = pb.w;
+    if (wa <= 0.f && wb <= 0.f) return false;
+    const float th2 = p.thickness * p.thickness;
+    float dx = pb.x - pa.x, dy = pb.y - pa.y, dz = pb.z - pa.z;
+    const float dist2 = (dx * dx + dy * dy) + dz * dz;
+    if (dist2 > th2 || dist2 < 1e-18f) return false;
+    const float ex = rb.x - ra.x, ey = rb.y - ra.y, ez = rb.z - ra.z;
+    const float rest2 = (ex * ex + ey * ey) + ez * ez;
+    if (dist2 > rest2) return false;
+    float min_d = p.thickness;
+    if (rest2 < th2) min_d = sqrtf(rest2);
+    const float dist = sqrtf(dist2);
+    const float corr = (min_d - dist) * 0.5f * p.coll_scale;
+    if (corr <= 0.f) return false;
+    dx = (dx / dist) * corr;
+    dy = (dy / dist) * corr;
+    dz = (dz / dist) * corr;
+    const float inv_w = 1.0f / fmaxf(wa + wb, 1e-8f);
+    if (i_lo) {
+        cx = (-dx * wa) * inv_w;
+        cy = (-dy * wa) * inv_w;
+        cz = (-dz * wa) * inv_w;
+    } else {
+        cx = (dx * wb) * inv_w;
+        cy = (dy * wb) * inv_w;
+        cz = (dz * wb) * inv_w;
+    }
+    return true;
+}
+
+// Narrow phase + apply + post_velocity, one particle per thread in sorted
+// order.  Contributions are added in ascending partner id — the order the
+// reference's O(N^2) loop adds them (xpbd_cpu.rs:82-123) — by repeatedly
+// selecting the smallest qualifying partner above the last one.  The 3x3
+// rows of neighbouring cells are contiguous runs of the sorted keys, found
+// once through the hash table.
+__global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
+    const uint32_t n = a.p.n;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t i = a.sids[s];
+    const float4 fi = a.F[i];
+    if (fi.w <= 0.f) {
+        if (a.xout != a.prev) a.xout[i] = fi;    // timing runs write scratch
+        return;
+    }
+    const CollGrid g = *a.grid;
+    uint32_t c[3];
+    cell_of(fi, g, c);
+    uint32_t rs[9], rk[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        rs[r] = EMPTY_KEY;
+        rk[r] = 0;
+        const int ny = int(c[1]) + (r % 3) - 1, nz = int(c[2]) + (r / 3) - 1;
+        if (ny < 0 || nz < 0 || ny >= int(g.dims[1]) || nz >= int(g.dims[2])) continue;
+        const uint32_t kb = g.dims[0] * (uint32_t(ny) + g.dims[1] * uint32_t(nz));
+        const uint32_t kmin = kb + (c[0] > 0 ? c[0] - 1 : 0);
+        const uint32_t kmax = kb + (c[0] + 1 < g.dims[0] ? c[0] + 1 : c[0]);
+        for (uint32_t k = kmin; k <= kmax; ++k) {
+            const uint32_t st = table_find(a.table, a.table_mask, k);
+            if (st != EMPTY_KEY) {
+                rs[r] = st;
+                break;
+            }
+        }
+        rk[r] = kmax;
+    }
+    const float4 ri = a.rest4[i];
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    uint32_t last = 0;
+    bool has_last = false;
+    for (;;) {
+        uint32_t best = EMPTY_KEY;
+        float bx = 0.f, by = 0.f, bz = 0.f;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            if (rs[r] == EMPTY_KEY) continue;
+            for (uint32_t idx = rs[r]; idx < n && a.skeys[idx] <= rk[r]; ++idx) {
+                const uint32_t j = a.sids[idx];
+                if (j == i || j >= best || (has_last && j <= last)) continue;
+                float cx, cy, cz;
+                if (collision_contrib(i, j, fi, a.F[j], ri, a.rest4[j], a.p, cx, cy, cz)) {
+                    best = j;
+                    bx = cx;
+                    by = cy;
+                    bz = cz;
+                }
+            }
+        }
+        if (best == EMPTY_KEY) break;
+        ax = ax + bx;
+        ay = ay + by;
+        az = az + bz;
+        last = best;
+        has_last = true;
+    }
+    auto cl = [](float v) {
+        if (v < -COLLISION_APPLY_CLAMP) v = -COLLISION_APPLY_CLAMP;
+        if (v > COLLISION_APPLY_CLAMP) v = COLLISION_APPLY_CLAMP;
+        return v;
+    };
+    const float4 nx = make_float4(fi.x + cl(ax), fi.y + cl(ay), fi.z + cl(az), fi.w);
+    finish_particle(nx, a.prev[i], i, a.ground, a.p, a.xout, a.vout);
+}
+
+// ---------------------------------------------------------------------------
+// Coloured Gauss-Seidel: one persistent launch per substep.  Colours run in
+// order with a grid barrier between them; a colour's constraints share no
+// particle, so the parallel sweep equals the sequential one in colour-sorted
+// order (xpbd_cpu.rs:344-386; gs_edges, cloth_sim.wgsl:154-210).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void grid_barrier(uint32_t* counter, uint32_t* err, uint32_t& epoch) {
+    __syncthreads();
+    epoch += 1;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(counter, 1u);
+        const uint32_t target = epoch * gridDim.x;
+        uint32_t spins = 0;
+        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > (1u << 22)) {   // a few seconds at most: report instead of hanging
+                atomicExch(err, 1u);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    __threadfence();
+}
+
+__global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
+    const DevParams& p = a.p;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t epoch = 0;
+    for (uint32_t i = gt; i < p.n; i += nthreads) a.W[i] = predict(a.X[i], a.V[i], i, p);
+    grid_barrier(a.barrier, a.error, epoch);
+    for (uint32_t it = 0; it < a.iterations; ++it) {
+        for (uint32_t b = 0; b < a.num_batches; ++b) {
+            const uint32_t e1 = a.batch_offsets[b + 1];
+            for (uint32_t e = a.batch_offsets[b] + gt; e < e1; e += nthreads) {
+                const uint2 ij = a.ij[e];
+                const float4 pi = a.W[ij.x], pj = a.W[ij.y];
+                const float wi = pi.w, wj = pj.w;
+                if (wj <= 0.f && wi <= 0.f) continue;
+                const float2 rc = a.rc[e];
+                float gx = pi.x - pj.x, gy = pi.y - pj.y, gz = pi.z - pj.z;
+                const float len = len3(gx, gy, gz);
+                if (len < 1e-8f) continue;
+                gx = gx / len;
+                gy = gy / len;
+                gz = gz / len;
+                const float c = len - rc.x;
+                const float alpha_t = rc.y * p.inv_dt_sq;
+                const float wsum = wi + wj + alpha_t;
+                if (wsum < 1e-8f) continue;
+                const float dlam = (-c - alpha_t * 0.0f) / wsum;
+                const float ox = p.omega * gx, oy = p.omega * gy, oz = p.omega * gz;
+                float ix = (ox * wi) * dlam, iy = (oy * wi) * dlam, iz = (oz * wi) * dlam;
+                float jx = -((ox * wj) * dlam), jy = -((oy * wj) * dlam), jz = -((oz * wj) * dlam);
+                float ml = len3(ix, iy, iz);
+                if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
+                    const float s = JACOBI_CORRECTION_CAP / ml;
+                    ix = ix * s;
+                    iy = iy * s;
+                    iz = iz * s;
+                }
+                ml = len3(jx, jy, jz);
+                if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
+                    const float s = JACOBI_CORRECTION_CAP / ml;
+                    jx = jx * s;
+                    jy = jy * s;
+                    jz = jz * s;
+                }
+                if (wi > 0.f) a.W[ij.x] = make_float4(pi.x + ix, pi.y + iy, pi.z + iz, wi);
+                if (wj > 0.f) a.W[ij.y] = make_float4(pj.x + jx, pj.y + jy, pj.z + jz, wj);
+            }
+            grid_barrier(a.barrier, a.error, epoch);
+        }
+    }
+    if (a.final_write) {
+        for (uint32_t i = gt; i < p.n; i += nthreads) {
+            const float4 w = a.W[i];
+            if (w.w > 0.f) finish_particle(w, a.X[i], i, a.ground, p, a.xout, a.vout);
+        }
+    } else {
+        // bounds of the final iterate for the collision pass that follows
+        const uint32_t rounds = (p.n + nthreads - 1) / nthreads;
+        for (uint32_t r = 0; r < rounds; ++r) {
+            const uint32_t i = r * nthreads + gt;
+            const bool valid = i < p.n;
+            const float4 w = valid ? a.W[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            block_aabb(valid, w.x, w.y, w.z, a.aabb);
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Render publish: finalize_normals (cloth_sim_jacobi.wgsl:527-547) with the
+// face normals gathered per vertex over its incident triangles in ascending
+// order (no fixed-point atomics), fallback +Y; positions copied with w = 0.
+// ---------------------------------------------------------------------------
+__global__ void k_publish(const float4* X, uint32_t n, const uint32_t* tris, const uint32_t* vt_off,
+                          const uint32_t* vt_tri, float4* dpos, float4* dnrm) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 x = X[i];
+    if (dpos) dpos[i] = make_float4(x.x, x.y, x.z, 0.f);
+    if (!dnrm) return;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for (uint32_t k = vt_off[i]; k < vt_off[i + 1]; ++k) {
+        const uint32_t t = vt_tri[k];
+        const float4 p0 = X[tris[3 * t]], p1 = X[tris[3 * t + 1]], p2 = X[tris[3 * t + 2]];
+        const float ax = p1.x - p0.x, ay = p1.y - p0.y, az = p1.z - p0.z;
+        const float bx = p2.x - p0.x, by = p2.y - p0.y, bz = p2.z - p0.z;
+        sx = sx + (ay * bz - by * az);
+        sy = sy + (az * bx - bz * ax);
+        sz = sz + (ax * by - bx * ay);
+    }
+    const float ln = len3(sx, sy, sz);
+    if (ln > 1e-8f) dnrm[i] = make_float4(sx / ln, sy / ln, sz / ln, 0.f);
+    else dnrm[i] = make_float4(0.f, 1.f, 0.f, 0.f);
+}
+
+// Sum over constraints of C^2 (C = |x_i - x_j| - rest), in double.
+__global__ __launch_bounds__(256) void k_residual(const float4* X, const uint2* ij, const float2* rc, uint32_t e,
+                                                 double* out) {
+    double s = 0.0;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < e; k += gridDim.x * blockDim.x) {
+        const uint2 q = ij[k];
+        const float4 a = X[q.x], b = X[q.y];
+        const double dx = double(a.x) - double(b.x), dy = double(a.y) - double(b.y), dz = double(a.z) - double(b.z);
+        const double c = sqrt((dx * dx + dy * dy) + dz * dz) - double(rc[k].x);
+        s += c * c;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+void launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s) {
+    const uint32_t spb = block / 64;
+    const uint32_t blocks = cdiv(a.p.nslices, spb);
+    if (blocks == 0) return;
+    switch (mode) {
+        case 0: hipLaunchKernelGGL(k_jacobi<0>, dim3(blocks), dim3(block), 0, s, a); break;
+        case JM_PREDICT: hipLaunchKernelGGL(k_jacobi<JM_PREDICT>, dim3(blocks), dim3(block), 0, s, a); break;
+        case JM_FINAL: hipLaunchKernelGGL(k_jacobi<JM_FINAL>, dim3(blocks), dim3(block), 0, s, a); break;
+        case JM_AABB: hipLaunchKernelGGL(k_jacobi<JM_AABB>, dim3(blocks), dim3(block), 0, s, a); break;
+        case JM_PREDICT | JM_AABB:
+            hipLaunchKernelGGL(k_jacobi<JM_PREDICT | JM_AABB>, dim3(blocks), dim3(block), 0, s, a);
+            break;
+        default: break;
+    }
+}
+
+void launch_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p, hipStream_t s) {
+    if (p.n == 0) return;
+    hipLaunchKernelGGL(k_finalize, dim3(cdiv(p.n, 256)), dim3(256), 0, s, F, X, V, ground, p);
+}
+
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(std::max(1u, std::min(cdiv(n, 256), 1024u))), dim3(256), 0, s, p, n);
+}
+
+void launch_aabb_reset(uint32_t* aabb, hipStream_t s) { hipLaunchKernelGGL(k_aabb_reset, dim3(1), dim3(64), 0, s, aabb); }
+
+void launch_aabb_of(const float4* P, uint32_t n, uint32_t* aabb, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_aabb_of, dim3(cdiv(n, 256)), dim3(256), 0, s, P, n, aabb);
+}
+
+void launch_coll_grid(uint32_t* aabb, CollGrid* grid, float cell_size, hipStream_t s) {
+    hipLaunchKernelGGL(k_coll_grid, dim3(1), dim3(64), 0, s, aabb, grid, cell_size);
+}
+
+void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grid, uint32_t* keys, uint32_t* ids,
+                      uint2* table, uint32_t slots, hipStream_t s) {
+    const uint32_t work = std::max(n, slots);
+    hipLaunchKernelGGL(k_coll_keys, dim3(std::max(1u, std::min(cdiv(work, 256), 8192u))), dim3(256), 0, s, F, n,
+                       grid, keys, ids, table, slots);
+}
+
+uint32_t radix_hist_words(uint32_t n) { return 256u * std::max(1u, cdiv(n, RADIX_TILE)); }
+
+void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
+                       uint32_t shift, uint32_t* hist, hipStream_t s) {
+    const uint32_t ntiles = std::max(1u, cdiv(n, RADIX_TILE));
+    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, n, shift, hist, ntiles);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, hist, 256u * ntiles);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, iin, kout, iout, n, shift, hist,
+                       ntiles);
+}
+
+void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,
+                       uint32_t* hist, hipStream_t s) {
+    if (n == 0) return;
+    launch_radix_pass(keys, ids, keys_alt, ids_alt, n, 0, hist, s);
+    launch_radix_pass(keys_alt, ids_alt, keys, ids, n, 8, hist, s);
+    launch_radix_pass(keys, ids, keys_alt, ids_alt, n, 16, hist, s);
+    launch_radix_pass(keys_alt, ids_alt, keys, ids, n, 24, hist, s);
+}
+
+void launch_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_coll_table, dim3(cdiv(n, 256)), dim3(256), 0, s, skeys, n, table, mask);
+}
+
+void launch_coll_sweep(const CollArgs& a, hipStream_t s) {
+    if (a.p.n == 0) return;
+    hipLaunchKernelGGL(k_coll_sweep, dim3(cdiv(a.p.n, 256)), dim3(256), 0, s, a);
+}
+
+uint32_t gs_max_blocks(uint32_t block) {
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_substep, int(block), 0) != hipSuccess) return 0;
+    return uint32_t(std::max(0, per_cu) * std::max(0, cus));
+}
+
+void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s) {
+    hipLaunchKernelGGL(k_gs_substep, dim3(blocks), dim3(block), 0, s, a);
+}
+
+void launch_publish(const float4* X, uint32_t n, const uint32_t* tris, const uint32_t* vt_off, const uint32_t* vt_tri,
+                    float4* dpos, float4* dnrm, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_publish, dim3(cdiv(n, 256)), dim3(256), 0, s, X, n, tris, vt_off, vt_tri, dpos, dnrm);
+}
+
+void launch_residual(const float4* X, const uint2* ij, const float2* rc, uint32_t e, double* out, hipStream_t s) {
+    if (e == 0) return;
+    hipLaunchKernelGGL(k_residual, dim3(std::min(cdiv(e, 256), 2048u)), dim3(256), 0, s, X, ij, rc, e, out);
+}
+
+}  // namespace xpbd

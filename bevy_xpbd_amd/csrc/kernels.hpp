@@ -1,0 +1,121 @@
+// Launch interface between the host runtime (cloth_compute.cpp) and the HIP
+// kernels (kernels.hip).  Plain structs passed by value as kernel arguments.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace xpbd {
+
+// Per-substep scalars (ClothSimParamsGpu, cloth_compute.rs:246-273, minus the
+// padding; damp = exp(-k dt) is evaluated once on the host with libm expf,
+// the function Rust's f32::exp calls, xpbd_cpu.rs:402).
+struct DevParams {
+    float dt, inv_dt, inv_dt_sq, omega;
+    float gx, gy, gz, damp;
+    float grab_x, grab_y, grab_z, grab_k;
+    int32_t grab_idx;
+    uint32_t grab_active;
+    float thickness, coll_scale;
+    uint32_t n;
+    uint32_t nslices;
+};
+
+// Sliced-ELL adjacency: 64 consecutive particles per slice (one wavefront),
+// entry k of lane l at soff[s] + k*64 + l.  Each particle's entries keep the
+// reference CSR order (the gather's summation order); a particle's unused
+// tail entries hold j = NO_ENTRY.
+constexpr uint32_t NO_ENTRY = 0xFFFFFFFFu;
+struct Sell {
+    const uint32_t* soff;
+    const uint32_t* swid;
+    const uint32_t* j;
+    const float* rest;
+    const float* comp;
+};
+
+enum JacobiMode : uint32_t {
+    JM_PREDICT = 1,   // inputs are the substep-start state; predict on the fly
+    JM_FINAL = 2,     // write the new state in place + velocity (no collision)
+    JM_AABB = 4,      // accumulate the bounds of the output (collision follows)
+};
+
+struct JacobiArgs {
+    const float4* src;   // JM_PREDICT: X; otherwise the previous iterate
+    const float4* vel;   // JM_PREDICT: V
+    float4* dst;         // iterate out (JM_FINAL: X, in place)
+    float4* vout;        // JM_FINAL: V
+    const uint8_t* ground;
+    uint32_t* aabb;      // JM_AABB: 6 ordered-uint words
+    Sell sell;
+    DevParams p;
+    uint32_t remap;      // 1: XCD-aware block order
+};
+
+// Cell grid of the collision broad phase (device memory, 12 words).
+struct CollGrid {
+    float lo[3], hi[3];
+    float cell, inv_cell;
+    uint32_t dims[3];
+    uint32_t pad;
+};
+
+struct CollArgs {
+    const float4* F;     // positions after the constraint iterations
+    const float4* rest4;
+    const float4* prev;  // X (substep start) for the velocity update
+    float4* xout;
+    float4* vout;
+    const uint8_t* ground;
+    const CollGrid* grid;
+    const uint32_t* skeys;
+    const uint32_t* sids;
+    const uint2* table;
+    uint32_t table_mask;
+    DevParams p;
+};
+
+struct GsArgs {
+    const float4* X;
+    const float4* V;
+    float4* W;           // work buffer (final iterate)
+    float4* xout;        // FINAL: X
+    float4* vout;        // FINAL: V
+    const uint8_t* ground;
+    uint32_t* aabb;
+    const uint2* ij;     // colour-sorted (i, j)
+    const float2* rc;    // colour-sorted (rest, compliance)
+    const uint32_t* batch_offsets;
+    uint32_t num_batches;
+    uint32_t iterations;
+    uint32_t* barrier;
+    uint32_t* error;
+    uint32_t final_write;
+    DevParams p;
+};
+
+// ---- launchers (kernels.hip) ----
+void launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s);
+void launch_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p,
+                     hipStream_t s);
+void launch_aabb_reset(uint32_t* aabb, hipStream_t s);
+void launch_coll_grid(uint32_t* aabb, CollGrid* grid, float cell_size, hipStream_t s);
+void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grid, uint32_t* keys, uint32_t* ids,
+                      uint2* table, uint32_t table_slots, hipStream_t s);
+void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,
+                       uint32_t* hist, hipStream_t s);
+uint32_t radix_hist_words(uint32_t n);
+void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
+                       uint32_t shift, uint32_t* hist, hipStream_t s);
+void launch_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask, hipStream_t s);
+void launch_coll_sweep(const CollArgs& a, hipStream_t s);
+void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
+uint32_t gs_max_blocks(uint32_t block);
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
+void launch_publish(const float4* X, uint32_t n, const uint32_t* tris, const uint32_t* vt_off,
+                    const uint32_t* vt_tri, float4* dpos, float4* dnrm, hipStream_t s);
+void launch_residual(const float4* X, const uint2* ij, const float2* rc, uint32_t e, double* out,
+                     hipStream_t s);
+void launch_aabb_of(const float4* P, uint32_t n, uint32_t* aabb, hipStream_t s);
+
+}  // namespace xpbd

@@ -1,0 +1,913 @@
+// C ABI and host runtime of the XPBD cloth compute path (include/xpbd.h).
+//
+// This is the MI355X counterpart of the reference's render-world systems:
+// init_cloth_sim_jacobi (buffer creation/upload, cloth_jacobi.rs:213-569),
+// run_cloth_sim_jacobi / run_cloth_sim_gauss_seidel (dispatch sequence,
+// cloth_jacobi.rs:675-916, cloth_compute.rs:1272-1522) and the reset path
+// (cloth_jacobi.rs:918-944).  Device state lives in HBM for the handle's
+// lifetime; a frame is enqueued as kernels on one stream and, when enabled,
+// replayed from a captured hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/xpbd.h"
+#include "kernels.hpp"
+#include "mesh.hpp"
+
+using namespace xpbd;
+
+struct xpbd_mesh {
+    Mesh m;
+};
+
+namespace {
+
+constexpr float REFERENCE_FRAME_DELTA_SECS = 1.0f / 60.0f;   // cloth_compute.rs:65
+constexpr uint32_t SUBSTEPS = 36;                             // cloth_compute.rs:59
+constexpr uint32_t INNER_ITERS = 22;                          // cloth_compute.rs:61
+constexpr float THICKNESS = 0.04f;                            // cloth_compute.rs:72
+constexpr float DEFAULT_COLL_SCALE = 0.38f;                   // cloth_compute.rs:74
+constexpr float DEFAULT_LINEAR_AIR_DRAG_PER_SEC = 1.25f;      // cloth_compute.rs:80
+
+uint32_t next_pow2(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v && p < 0x80000000u) p <<= 1;
+    return p;
+}
+
+}  // namespace
+
+struct xpbd_cloth {
+    xpbd_sim_config cfg{};
+    xpbd_sim_uniforms u{};
+    uint32_t n = 0, e = 0, t = 0, nslices = 0, num_batches = 0;
+    std::vector<float> init_pos4;     // reset pose, w = inverse mass
+    std::vector<uint32_t> batch_offsets;
+    double rest_sq_sum = 0.0;
+    bool has_ground = false;
+
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+
+    std::vector<void*> allocs;
+    float4 *X = nullptr, *V = nullptr, *A = nullptr, *B = nullptr, *V2 = nullptr, *rest4 = nullptr;
+    uint32_t *soff = nullptr, *swid = nullptr, *nbj = nullptr;
+    float *nbr = nullptr, *nbc = nullptr;
+    uint8_t* ground = nullptr;
+    uint2* gs_ij = nullptr;
+    float2* gs_rc = nullptr;
+    uint32_t* gs_off = nullptr;
+    uint32_t *keys = nullptr, *ids = nullptr, *keys2 = nullptr, *ids2 = nullptr, *hist = nullptr;
+    uint2* table = nullptr;
+    uint32_t table_slots = 0;
+    uint32_t* aabb = nullptr;
+    CollGrid* grid = nullptr;
+    uint32_t *tris = nullptr, *vt_off = nullptr, *vt_tri = nullptr;
+    uint32_t *barrier = nullptr, *err = nullptr;
+    double* resid = nullptr;
+
+    float4 *render_pos = nullptr, *render_nrm = nullptr;
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool graph_valid = false;
+
+    uint32_t jacobi_block = 256, remap = 1;
+    uint32_t gs_block = 256, gs_blocks = 0;
+
+    std::string msg;
+};
+
+namespace {
+
+int fail(xpbd_cloth* h, int code, const std::string& m) {
+    if (h) h->msg = m;
+    return code;
+}
+
+#define HIPC(h, expr)                                                                                        \
+    do {                                                                                                     \
+        hipError_t _e = (expr);                                                                              \
+        if (_e != hipSuccess)                                                                                \
+            return fail((h), _e == hipErrorOutOfMemory ? XPBD_E_NOMEM : XPBD_E_HIP,                        \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                                 \
+    } while (0)
+
+template <class T>
+int dalloc(xpbd_cloth* h, T** p, size_t count) {
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    HIPC(h, hipMalloc(&q, bytes));
+    h->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return XPBD_OK;
+}
+
+template <class T>
+int upload(xpbd_cloth* h, T* dst, const T* src, size_t count) {
+    if (count == 0) return XPBD_OK;
+    HIPC(h, hipMemcpy(dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return XPBD_OK;
+}
+
+#define TRY(expr)                      \
+    do {                               \
+        int _r = (expr);               \
+        if (_r != XPBD_OK) return _r;  \
+    } while (0)
+
+DevParams dev_params(const xpbd_cloth* h) {
+    const xpbd_sim_uniforms& u = h->u;
+    DevParams p{};
+    p.dt = u.dt;
+    p.inv_dt = u.inv_dt;
+    p.inv_dt_sq = u.inv_dt * u.inv_dt;                 // ClothSimParamsGpu::pack, cloth_compute.rs:325
+    p.omega = u.jacobi_omega;
+    p.gx = u.gravity[0];
+    p.gy = u.gravity[1];
+    p.gz = u.gravity[2];
+    p.damp = std::exp(-u.linear_drag_per_sec * u.dt);  // xpbd_cpu.rs:402 (f32 expf)
+    p.grab_x = u.grab_target[0];
+    p.grab_y = u.grab_target[1];
+    p.grab_z = u.grab_target[2];
+    p.grab_k = u.grab_stiffness;
+    p.grab_idx = u.grab_idx;
+    p.grab_active = u.grab_active;
+    p.thickness = u.thickness;
+    p.coll_scale = u.coll_scale;
+    p.n = h->n;
+    p.nslices = h->nslices;
+    return p;
+}
+
+JacobiArgs jacobi_args(const xpbd_cloth* h, const DevParams& p) {
+    JacobiArgs a{};
+    a.sell = Sell{h->soff, h->swid, h->nbj, h->nbr, h->nbc};
+    a.ground = h->has_ground ? h->ground : nullptr;
+    a.aabb = h->aabb;
+    a.p = p;
+    a.remap = h->remap;
+    return a;
+}
+
+float coll_cell(const xpbd_cloth* h) {
+    return h->cfg.collision_cell > 0.f ? std::max(h->cfg.collision_cell, h->u.thickness) : h->u.thickness;
+}
+
+// Collision chain on iterate F: grid, keys, sort, table, sweep (which also
+// applies the ground plane and the velocity update).  Replaces the
+// reference's clear_atomics .. collide_apply sequence (cloth_jacobi.rs:774-868).
+void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, float4* xout, float4* vout) {
+    hipStream_t s = h->stream;
+    launch_coll_grid(h->aabb, h->grid, coll_cell(h), s);
+    launch_coll_keys(F, h->n, h->grid, h->keys, h->ids, h->table, h->table_slots, s);
+    launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
+    launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, s);
+    CollArgs c{};
+    c.F = F;
+    c.rest4 = h->rest4;
+    c.prev = h->X;
+    c.xout = xout;
+    c.vout = vout;
+    c.ground = h->has_ground ? h->ground : nullptr;
+    c.grid = h->grid;
+    c.skeys = h->keys;
+    c.sids = h->ids;
+    c.table = h->table;
+    c.table_mask = h->table_slots - 1;
+    c.p = p;
+    launch_coll_sweep(c, s);
+}
+
+// One substep (cloth_jacobi.rs:702-883 / cloth_compute.rs:1313-1488).  The
+// state is always (X, V) at substep boundaries; A/B hold the iterates.
+void enqueue_substep(xpbd_cloth* h, bool collision) {
+    hipStream_t s = h->stream;
+    const DevParams p = dev_params(h);
+    const uint32_t iters = std::max(h->cfg.solve_inner_iterations, 1u);
+    const float4* F = nullptr;
+    if (h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL) {
+        launch_zero_u32(h->barrier, 1, s);
+        GsArgs g{};
+        g.X = h->X;
+        g.V = h->V;
+        g.W = h->A;
+        g.xout = h->X;
+        g.vout = h->V;
+        g.ground = h->has_ground ? h->ground : nullptr;
+        g.aabb = h->aabb;
+        g.ij = h->gs_ij;
+        g.rc = h->gs_rc;
+        g.batch_offsets = h->gs_off;
+        g.num_batches = h->e ? h->num_batches : 0;
+        g.iterations = iters;
+        g.barrier = h->barrier;
+        g.error = h->err;
+        g.final_write = collision ? 0u : 1u;
+        g.p = p;
+        launch_gs(g, h->gs_blocks, h->gs_block, s);
+        F = h->A;
+    } else {
+        JacobiArgs a = jacobi_args(h, p);
+        if (iters == 1) {
+            a.src = h->X;
+            a.vel = h->V;
+            a.dst = h->A;
+            launch_jacobi(collision ? (JM_PREDICT | JM_AABB) : JM_PREDICT, a, h->jacobi_block, s);
+            if (!collision) launch_finalize(h->A, h->X, h->V, a.ground, p, s);
+            F = h->A;
+        } else {
+            a.src = h->X;
+            a.vel = h->V;
+            a.dst = h->A;
+            launch_jacobi(JM_PREDICT, a, h->jacobi_block, s);
+            float4* cur = h->A;
+            float4* nxt = h->B;
+            for (uint32_t k = 1; k + 1 < iters; ++k) {
+                a.src = cur;
+                a.dst = nxt;
+                launch_jacobi(0, a, h->jacobi_block, s);
+                std::swap(cur, nxt);
+            }
+            a.src = cur;
+            if (collision) {
+                a.dst = nxt;
+                launch_jacobi(JM_AABB, a, h->jacobi_block, s);
+                F = nxt;
+            } else {
+                a.dst = h->X;
+                a.vout = h->V;
+                launch_jacobi(JM_FINAL, a, h->jacobi_block, s);
+            }
+        }
+    }
+    if (collision) enqueue_collision(h, p, F, h->X, h->V);
+}
+
+bool collision_this_substep(const xpbd_cloth* h, uint32_t si) {
+    if (!(h->u.coll_scale > 1e-12f)) return false;   // cloth_jacobi.rs:699
+    const uint32_t stride = std::max(h->cfg.collision_every_n_substeps, 1u);
+    return si % stride == stride - 1;                  // cloth_jacobi.rs:703-704
+}
+
+void enqueue_frame(xpbd_cloth* h) {
+    const uint32_t S = std::max(h->cfg.solve_substeps, 1u);
+    for (uint32_t si = 0; si < S; ++si) enqueue_substep(h, collision_this_substep(h, si));
+    if (h->render_pos || h->render_nrm)
+        launch_publish(h->X, h->n, h->tris, h->vt_off, h->vt_tri, h->render_pos, h->render_nrm, h->stream);
+}
+
+void drop_graph(xpbd_cloth* h) {
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+    h->graph_valid = false;
+}
+
+int check_device_flag(xpbd_cloth* h) {
+    uint32_t flag = 0;
+    HIPC(h, hipMemcpyAsync(&flag, h->err, sizeof(flag), hipMemcpyDeviceToHost, h->stream));
+    HIPC(h, hipStreamSynchronize(h->stream));
+    if (flag) {
+        uint32_t zero = 0;
+        HIPC(h, hipMemcpy(h->err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        return fail(h, XPBD_E_STATE, "grid barrier wait exceeded its bound (blocks not co-resident)");
+    }
+    return XPBD_OK;
+}
+
+// Sliced-ELL packing of the reference CSR (neighbor_offsets/other/rest/
+// compliance, mesh_prep.rs:588-634; pack_neighbor_gpu, cloth_jacobi.rs:95-111).
+// Indices stay u32 (the reference packs them as f32, exact only below 2^24).
+void build_sell(const Mesh& m, std::vector<uint32_t>& soff, std::vector<uint32_t>& swid, std::vector<uint32_t>& sj,
+                std::vector<float>& sr, std::vector<float>& sc) {
+    const size_t n = m.num_particles();
+    const size_t ns = (n + 63) / 64;
+    soff.assign(ns + 1, 0);
+    swid.assign(ns, 0);
+    for (size_t s = 0; s < ns; ++s) {
+        uint32_t w = 0;
+        for (size_t l = 0; l < 64 && s * 64 + l < n; ++l) {
+            const size_t i = s * 64 + l;
+            w = std::max(w, m.neighbor_offsets[i + 1] - m.neighbor_offsets[i]);
+        }
+        swid[s] = w;
+        soff[s + 1] = soff[s] + w * 64;
+    }
+    const size_t total = soff[ns];
+    sj.assign(total, NO_ENTRY);
+    sr.assign(total, 0.f);
+    sc.assign(total, 0.f);
+    for (size_t i = 0; i < n; ++i) {
+        const size_t s = i / 64, l = i % 64;
+        const uint32_t b = m.neighbor_offsets[i], d = m.neighbor_offsets[i + 1] - b;
+        for (uint32_t k = 0; k < d; ++k) {
+            const size_t e = soff[s] + size_t(k) * 64 + l;
+            sj[e] = m.neighbor_other[b + k];
+            sr[e] = m.neighbor_rest_len[b + k];
+            sc[e] = m.neighbor_compliance[b + k];
+        }
+    }
+}
+
+int validate_mesh(const Mesh& m, std::string& why) {
+    const size_t n = m.num_particles();
+    if (m.positions.size() != 3 * n || m.rest_positions.size() != 3 * n) return why = "position arrays", XPBD_E_INVALID;
+    if (m.neighbor_offsets.size() != n + 1) return why = "neighbor_offsets size", XPBD_E_INVALID;
+    if (n >= 0xFFFFFFF0u) return why = "too many particles", XPBD_E_INVALID;
+    for (size_t k = 0; k < m.num_constraints(); ++k)
+        if (m.constraint_i[k] >= n || m.constraint_j[k] >= n || m.constraint_i[k] == m.constraint_j[k])
+            return why = "constraint endpoint out of range", XPBD_E_INVALID;
+    for (uint32_t v : m.neighbor_other)
+        if (v >= n) return why = "neighbor index out of range", XPBD_E_INVALID;
+    for (size_t i = 0; i < n; ++i)
+        if (m.neighbor_offsets[i] > m.neighbor_offsets[i + 1]) return why = "neighbor_offsets not monotone", XPBD_E_INVALID;
+    if (m.neighbor_offsets[n] != m.neighbor_other.size()) return why = "neighbor_offsets end", XPBD_E_INVALID;
+    for (uint32_t v : m.indices)
+        if (v >= n) return why = "triangle index out of range", XPBD_E_INVALID;
+    if (m.batch_offsets.empty() || m.batch_offsets.back() != m.num_constraints())
+        return why = "batch offsets", XPBD_E_INVALID;
+    return XPBD_OK;
+}
+
+int create_device(xpbd_cloth* h, const Mesh& m) {
+    const size_t n = h->n, e = h->e, t = h->t;
+    TRY(dalloc(h, &h->X, n));
+    TRY(dalloc(h, &h->V, n));
+    TRY(dalloc(h, &h->A, n));
+    TRY(dalloc(h, &h->B, n));
+    TRY(dalloc(h, &h->V2, n));
+    TRY(dalloc(h, &h->rest4, n));
+    std::vector<float> rest4(4 * n);
+    h->init_pos4.assign(4 * n, 0.f);
+    for (size_t i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            rest4[4 * i + a] = m.rest_positions[3 * i + a];
+            h->init_pos4[4 * i + a] = m.positions[3 * i + a];
+        }
+        rest4[4 * i + 3] = m.inv_mass[i];
+        h->init_pos4[4 * i + 3] = m.inv_mass[i];
+    }
+    TRY(upload(h, reinterpret_cast<float*>(h->rest4), rest4.data(), rest4.size()));
+
+    std::vector<uint32_t> soff, swid, sj;
+    std::vector<float> sr, sc;
+    build_sell(m, soff, swid, sj, sr, sc);
+    TRY(dalloc(h, &h->soff, soff.size()));
+    TRY(dalloc(h, &h->swid, swid.size()));
+    TRY(dalloc(h, &h->nbj, sj.size()));
+    TRY(dalloc(h, &h->nbr, sr.size()));
+    TRY(dalloc(h, &h->nbc, sc.size()));
+    TRY(upload(h, h->soff, soff.data(), soff.size()));
+    TRY(upload(h, h->swid, swid.data(), swid.size()));
+    TRY(upload(h, h->nbj, sj.data(), sj.size()));
+    TRY(upload(h, h->nbr, sr.data(), sr.size()));
+    TRY(upload(h, h->nbc, sc.data(), sc.size()));
+
+    h->has_ground = std::any_of(m.ground.begin(), m.ground.end(), [](uint32_t g) { return g != 0; });
+    TRY(dalloc(h, &h->ground, n));
+    {
+        std::vector<uint8_t> g(n, 0);
+        for (size_t i = 0; i < n && i < m.ground.size(); ++i) g[i] = m.ground[i] ? 1 : 0;
+        TRY(upload(h, h->ground, g.data(), n));
+    }
+
+    std::vector<uint2> ij(e);
+    std::vector<float2> rc(e);
+    h->rest_sq_sum = 0.0;
+    for (size_t k = 0; k < e; ++k) {
+        ij[k] = make_uint2(m.constraint_i[k], m.constraint_j[k]);
+        rc[k] = make_float2(m.constraint_rest[k], m.constraint_comp[k]);
+        h->rest_sq_sum += double(m.constraint_rest[k]) * double(m.constraint_rest[k]);
+    }
+    TRY(dalloc(h, &h->gs_ij, e));
+    TRY(dalloc(h, &h->gs_rc, e));
+    TRY(upload(h, h->gs_ij, ij.data(), e));
+    TRY(upload(h, h->gs_rc, rc.data(), e));
+    h->batch_offsets = m.batch_offsets;
+    h->num_batches = uint32_t(m.batch_offsets.size() - 1);
+    TRY(dalloc(h, &h->gs_off, h->batch_offsets.size()));
+    TRY(upload(h, h->gs_off, h->batch_offsets.data(), h->batch_offsets.size()));
+
+    TRY(dalloc(h, &h->keys, n));
+    TRY(dalloc(h, &h->ids, n));
+    TRY(dalloc(h, &h->keys2, n));
+    TRY(dalloc(h, &h->ids2, n));
+    TRY(dalloc(h, &h->hist, radix_hist_words(uint32_t(n))));
+    h->table_slots = next_pow2(uint32_t(std::max<size_t>(2 * n, 64)));
+    TRY(dalloc(h, &h->table, h->table_slots));
+    TRY(dalloc(h, &h->aabb, 8));
+    TRY(dalloc(h, &h->grid, 1));
+
+    // vertex -> incident triangles (ascending) for the normals gather
+    std::vector<uint32_t> vt_off(n + 1, 0), vt_tri(3 * t);
+    for (size_t k = 0; k < 3 * t; ++k) vt_off[m.indices[k] + 1]++;
+    for (size_t i = 0; i < n; ++i) vt_off[i + 1] += vt_off[i];
+    {
+        std::vector<uint32_t> cur(vt_off.begin(), vt_off.end() - 1);
+        for (size_t k = 0; k < t; ++k)
+            for (int c = 0; c < 3; ++c) vt_tri[cur[m.indices[3 * k + c]]++] = uint32_t(k);
+    }
+    TRY(dalloc(h, &h->tris, 3 * t));
+    TRY(dalloc(h, &h->vt_off, n + 1));
+    TRY(dalloc(h, &h->vt_tri, 3 * t));
+    TRY(upload(h, h->tris, m.indices.data(), 3 * t));
+    TRY(upload(h, h->vt_off, vt_off.data(), n + 1));
+    TRY(upload(h, h->vt_tri, vt_tri.data(), 3 * t));
+
+    TRY(dalloc(h, &h->barrier, 4));
+    TRY(dalloc(h, &h->err, 4));
+    TRY(dalloc(h, &h->resid, 2));
+    HIPC(h, hipMemset(h->err, 0, 16));
+    HIPC(h, hipMemset(h->barrier, 0, 16));
+    launch_aabb_reset(h->aabb, h->stream);
+    HIPC(h, hipGetLastError());
+    return XPBD_OK;
+}
+
+int upload_state(xpbd_cloth* h, const float* pos4, const float* vel4) {
+    std::vector<float> x(4 * size_t(h->n)), v(4 * size_t(h->n), 0.f);
+    for (size_t i = 0; i < h->n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            x[4 * i + a] = pos4[4 * i + a];
+            if (vel4) v[4 * i + a] = vel4[4 * i + a];
+        }
+        x[4 * i + 3] = h->init_pos4[4 * i + 3];
+    }
+    HIPC(h, hipStreamSynchronize(h->stream));
+    TRY(upload(h, reinterpret_cast<float*>(h->X), x.data(), x.size()));
+    TRY(upload(h, reinterpret_cast<float*>(h->V), v.data(), v.size()));
+    return XPBD_OK;
+}
+
+int autotune(xpbd_cloth* h) {
+    if (h->n == 0 || h->e == 0) return XPBD_OK;
+    const DevParams p = dev_params(h);
+    HIPC(h, hipMemcpyAsync(h->A, h->X, size_t(h->n) * sizeof(float4), hipMemcpyDeviceToDevice, h->stream));
+    hipEvent_t e0, e1;
+    HIPC(h, hipEventCreate(&e0));
+    HIPC(h, hipEventCreate(&e1));
+    float best = 1e30f;
+    uint32_t best_block = h->jacobi_block, best_remap = h->remap;
+    const uint32_t reps = h->n > (1u << 18) ? 10 : 30;
+    for (uint32_t block : {64u, 128u, 256u, 512u})
+        for (uint32_t remap : {0u, 1u}) {
+            JacobiArgs a = jacobi_args(h, p);
+            a.remap = remap;
+            a.src = h->A;
+            a.dst = h->B;
+            launch_jacobi(0, a, block, h->stream);   // warm-up
+            (void)hipEventRecord(e0, h->stream);
+            for (uint32_t r = 0; r < reps; ++r) {
+                a.src = a.dst;
+                a.dst = (a.dst == h->A) ? h->B : h->A;
+                launch_jacobi(0, a, block, h->stream);
+            }
+            (void)hipEventRecord(e1, h->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) {
+                best = ms;
+                best_block = block;
+                best_remap = remap;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPC(h, hipGetLastError());
+    h->jacobi_block = best_block;
+    h->remap = best_remap;
+    return XPBD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+XPBD_API int xpbd_abi_version(void) { return XPBD_ABI_VERSION; }
+
+// ---------------------------------------------------------------- meshes
+static int wrap_mesh(Mesh* m, xpbd_mesh** out) {
+    if (!m) return XPBD_E_INVALID;
+    auto* w = new (std::nothrow) xpbd_mesh();
+    if (!w) {
+        delete m;
+        return XPBD_E_NOMEM;
+    }
+    w->m = std::move(*m);
+    delete m;
+    *out = w;
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_mesh_grid_cloth_hanging(uint32_t qc, uint32_t qr, float cell, xpbd_mesh** out) {
+    if (!out || qc < 1 || qr < 1 || !(cell > 0.f) || !std::isfinite(cell)) return XPBD_E_INVALID;
+    try {
+        return wrap_mesh(grid_cloth_hanging(qc, qr, cell), out);
+    } catch (const std::bad_alloc&) {
+        return XPBD_E_NOMEM;
+    }
+}
+
+XPBD_API int xpbd_mesh_parse_welded_obj(const char* text, size_t len, xpbd_mesh** out) {
+    if (!out || (!text && len)) return XPBD_E_INVALID;
+    try {
+        return wrap_mesh(parse_welded_obj(std::string(text ? text : "", len)), out);
+    } catch (const std::bad_alloc&) {
+        return XPBD_E_NOMEM;
+    }
+}
+
+XPBD_API int xpbd_mesh_sheet(const xpbd_sheet_desc* d, xpbd_mesh** out) { return xpbd_mesh_batch(d, 1, out); }
+
+XPBD_API int xpbd_mesh_batch(const xpbd_sheet_desc* d, uint32_t count, xpbd_mesh** out) {
+    if (!out || !d || count == 0) return XPBD_E_INVALID;
+    uint64_t total = 0;
+    for (uint32_t c = 0; c < count; ++c) {
+        if (d[c].cols < 2 || d[c].rows < 2 || !(d[c].spacing > 0.f) || d[c].layout > 1 || d[c].pins > 3)
+            return XPBD_E_INVALID;
+        total += uint64_t(d[c].cols) * d[c].rows;
+    }
+    if (total >= 0x40000000ull) return XPBD_E_INVALID;
+    try {
+        return wrap_mesh(sheet_batch(d, count), out);
+    } catch (const std::bad_alloc&) {
+        return XPBD_E_NOMEM;
+    }
+}
+
+static const void* mesh_field(const Mesh& m, int f, uint64_t& count) {
+    switch (f) {
+        case XPBD_MESH_POSITIONS: count = m.positions.size(); return m.positions.data();
+        case XPBD_MESH_REST_POSITIONS: count = m.rest_positions.size(); return m.rest_positions.data();
+        case XPBD_MESH_NORMALS: count = m.normals.size(); return m.normals.data();
+        case XPBD_MESH_UVS: count = m.uvs.size(); return m.uvs.data();
+        case XPBD_MESH_INDICES: count = m.indices.size(); return m.indices.data();
+        case XPBD_MESH_TRI_EDGE_NEIGHBORS: count = m.tri_edge_neighbors.size(); return m.tri_edge_neighbors.data();
+        case XPBD_MESH_INV_MASS: count = m.inv_mass.size(); return m.inv_mass.data();
+        case XPBD_MESH_NEIGHBOR_OTHER: count = m.neighbor_other.size(); return m.neighbor_other.data();
+        case XPBD_MESH_NEIGHBOR_REST_LEN: count = m.neighbor_rest_len.size(); return m.neighbor_rest_len.data();
+        case XPBD_MESH_NEIGHBOR_COMPLIANCE: count = m.neighbor_compliance.size(); return m.neighbor_compliance.data();
+        case XPBD_MESH_NEIGHBOR_CONSTRAINT_ID: count = m.neighbor_constraint_id.size(); return m.neighbor_constraint_id.data();
+        case XPBD_MESH_NEIGHBOR_OFFSETS: count = m.neighbor_offsets.size(); return m.neighbor_offsets.data();
+        case XPBD_MESH_CONSTRAINT_I: count = m.constraint_i.size(); return m.constraint_i.data();
+        case XPBD_MESH_CONSTRAINT_J: count = m.constraint_j.size(); return m.constraint_j.data();
+        case XPBD_MESH_CONSTRAINT_REST: count = m.constraint_rest.size(); return m.constraint_rest.data();
+        case XPBD_MESH_CONSTRAINT_COMPLIANCE: count = m.constraint_comp.size(); return m.constraint_comp.data();
+        case XPBD_MESH_BATCH_OFFSETS: count = m.batch_offsets.size(); return m.batch_offsets.data();
+        case XPBD_MESH_GROUND: count = m.ground.size(); return m.ground.data();
+        case XPBD_MESH_CLOTH_OFFSETS: count = m.cloth_offsets.size(); return m.cloth_offsets.data();
+        default: count = 0; return nullptr;
+    }
+}
+
+XPBD_API uint64_t xpbd_mesh_count(const xpbd_mesh* mesh, int field) {
+    uint64_t c = 0;
+    if (mesh) mesh_field(mesh->m, field, c);
+    return c;
+}
+
+XPBD_API int xpbd_mesh_read(const xpbd_mesh* mesh, int field, void* dst, uint64_t bytes) {
+    if (!mesh || !dst) return XPBD_E_INVALID;
+    uint64_t c = 0;
+    const void* p = mesh_field(mesh->m, field, c);
+    if (!p && c) return XPBD_E_INVALID;
+    if (field < 0 || field >= XPBD_MESH_NUM_FIELDS) return XPBD_E_INVALID;
+    if (bytes < c * 4) return XPBD_E_INVALID;
+    if (c) std::memcpy(dst, p, c * 4);
+    return XPBD_OK;
+}
+
+XPBD_API void xpbd_mesh_destroy(xpbd_mesh* mesh) { delete mesh; }
+
+XPBD_API int xpbd_derive_collision_grid(const float* rest_vec4, uint32_t n, float thickness, xpbd_coll_grid* out) {
+    if (!out || (!rest_vec4 && n)) return XPBD_E_INVALID;
+    derive_collision_grid(rest_vec4, n, thickness, out);
+    return XPBD_OK;
+}
+
+// ---------------------------------------------------------------- config
+XPBD_API void xpbd_sim_uniforms_default(uint32_t solver, uint32_t substeps, xpbd_sim_uniforms* out) {
+    if (!out) return;
+    const uint32_t s = std::max(substeps, 1u);
+    xpbd_sim_uniforms u{};
+    u.dt = REFERENCE_FRAME_DELTA_SECS / float(s);
+    u.inv_dt = 1.0f / u.dt;
+    u.jacobi_omega = solver == XPBD_SOLVER_GAUSS_SEIDEL ? 1.0f : 0.32f;   // cloth_jacobi.rs:946-948
+    u.inner_iterations = INNER_ITERS;
+    u.thickness = THICKNESS;
+    u.coll_scale = DEFAULT_COLL_SCALE;
+    u.gravity[1] = -9.81f;
+    u.grab_idx = -1;
+    u.grab_active = 0;
+    u.grab_stiffness = solver == XPBD_SOLVER_GAUSS_SEIDEL ? 0.45f : 0.28f;
+    u.linear_drag_per_sec = DEFAULT_LINEAR_AIR_DRAG_PER_SEC;
+    *out = u;
+}
+
+XPBD_API void xpbd_sim_config_default(xpbd_sim_config* out) {
+    if (!out) return;
+    xpbd_sim_config c{};
+    c.solve_substeps = SUBSTEPS;
+    c.solve_inner_iterations = INNER_ITERS;
+    c.collision_every_n_substeps = 1;
+    c.solver = XPBD_SOLVER_JACOBI;
+    c.collision_cell = 0.f;
+    c.use_graph = 1;
+    c.autotune = 0;
+    *out = c;
+}
+
+// ---------------------------------------------------------------- cloth
+XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, const xpbd_sim_uniforms* u,
+                         void* stream, xpbd_cloth** out) {
+    if (!mesh || !cfg || !u || !out) return XPBD_E_INVALID;
+    *out = nullptr;
+    if (cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL) return XPBD_E_INVALID;
+    std::string why;
+    if (validate_mesh(mesh->m, why) != XPBD_OK) return XPBD_E_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return XPBD_E_NO_DEVICE;
+    auto* h = new (std::nothrow) xpbd_cloth();
+    if (!h) return XPBD_E_NOMEM;
+    std::unique_ptr<xpbd_cloth, void (*)(xpbd_cloth*)> guard(h, xpbd_destroy);
+    h->cfg = *cfg;
+    h->u = *u;
+    const Mesh& m = mesh->m;
+    h->n = uint32_t(m.num_particles());
+    h->e = uint32_t(m.num_constraints());
+    h->t = uint32_t(m.num_tris());
+    h->nslices = (h->n + 63) / 64;
+    h->u.num_particles = h->n;
+    h->u.num_tris = h->t;
+    if (hipGetDevice(&h->device) != hipSuccess) return XPBD_E_HIP;
+    if (stream) {
+        h->stream = static_cast<hipStream_t>(stream);
+    } else {
+        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return XPBD_E_HIP;
+        h->own_stream = true;
+    }
+    int r = create_device(h, m);
+    if (r != XPBD_OK) return r;
+    r = upload_state(h, h->init_pos4.data(), nullptr);
+    if (r != XPBD_OK) return r;
+    h->gs_blocks = std::min<uint32_t>(gs_max_blocks(h->gs_block), std::max(1u, (h->e + h->gs_block - 1) / h->gs_block));
+    h->gs_blocks = std::max<uint32_t>(h->gs_blocks, 1u);
+    if (cfg->autotune) {
+        r = autotune(h);
+        if (r != XPBD_OK) return r;
+    }
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return XPBD_E_HIP;
+    *out = guard.release();
+    return XPBD_OK;
+}
+
+XPBD_API void xpbd_destroy(xpbd_cloth* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graph(h);
+    for (void* p : h->allocs) (void)hipFree(p);
+    if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+XPBD_API const char* xpbd_last_error(const xpbd_cloth* h) { return h ? h->msg.c_str() : "null handle"; }
+
+XPBD_API int xpbd_set_uniforms(xpbd_cloth* h, const xpbd_sim_uniforms* u) {
+    if (!h || !u) return XPBD_E_INVALID;
+    h->u = *u;
+    h->u.num_particles = h->n;
+    h->u.num_tris = h->t;
+    drop_graph(h);
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_set_config(xpbd_cloth* h, const xpbd_sim_config* cfg) {
+    if (!h || !cfg || cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL) return XPBD_E_INVALID;
+    const bool retune = cfg->autotune && !h->cfg.autotune;
+    h->cfg = *cfg;
+    drop_graph(h);
+    if (retune) return autotune(h);
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_step_frames(xpbd_cloth* h, uint32_t frames) {
+    if (!h) return XPBD_E_INVALID;
+    if (h->n == 0 || frames == 0) return XPBD_OK;
+    if (h->cfg.use_graph) {
+        if (!h->graph_valid) {
+            drop_graph(h);
+            HIPC(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_frame(h);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(h->stream, &g);
+            if (ce != hipSuccess) return fail(h, XPBD_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+            h->graph = g;
+            HIPC(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+            h->graph_valid = true;
+        }
+        for (uint32_t f = 0; f < frames; ++f) HIPC(h, hipGraphLaunch(h->gexec, h->stream));
+    } else {
+        for (uint32_t f = 0; f < frames; ++f) enqueue_frame(h);
+    }
+    HIPC(h, hipGetLastError());
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_step_substeps(xpbd_cloth* h, uint32_t count, uint32_t collision) {
+    if (!h) return XPBD_E_INVALID;
+    if (h->n == 0) return XPBD_OK;
+    const bool coll = collision != 0 && h->u.coll_scale > 1e-12f;
+    for (uint32_t k = 0; k < count; ++k) enqueue_substep(h, coll);
+    HIPC(h, hipGetLastError());
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_sync(xpbd_cloth* h) {
+    if (!h) return XPBD_E_INVALID;
+    HIPC(h, hipStreamSynchronize(h->stream));
+    return check_device_flag(h);
+}
+
+XPBD_API void* xpbd_stream(xpbd_cloth* h) { return h ? static_cast<void*>(h->stream) : nullptr; }
+
+XPBD_API int xpbd_reset(xpbd_cloth* h) {
+    if (!h) return XPBD_E_INVALID;
+    return upload_state(h, h->init_pos4.data(), nullptr);
+}
+
+XPBD_API int xpbd_read_state(xpbd_cloth* h, float* pos4, float* vel4) {
+    if (!h) return XPBD_E_INVALID;
+    HIPC(h, hipStreamSynchronize(h->stream));
+    TRY(check_device_flag(h));
+    const size_t bytes = size_t(h->n) * sizeof(float4);
+    if (pos4) {
+        HIPC(h, hipMemcpy(pos4, h->X, bytes, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < h->n; ++i) pos4[4 * i + 3] = 0.f;
+    }
+    if (vel4) {
+        HIPC(h, hipMemcpy(vel4, h->V, bytes, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < h->n; ++i) vel4[4 * i + 3] = 0.f;
+    }
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_write_state(xpbd_cloth* h, const float* pos4, const float* vel4) {
+    if (!h || !pos4) return XPBD_E_INVALID;
+    return upload_state(h, pos4, vel4);
+}
+
+XPBD_API int xpbd_publish(xpbd_cloth* h, void* dpos, void* dnrm) {
+    if (!h) return XPBD_E_INVALID;
+    launch_publish(h->X, h->n, h->tris, h->vt_off, h->vt_tri, static_cast<float4*>(dpos), static_cast<float4*>(dnrm),
+                   h->stream);
+    HIPC(h, hipGetLastError());
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_bind_render_buffers(xpbd_cloth* h, void* dpos, void* dnrm) {
+    if (!h) return XPBD_E_INVALID;
+    h->render_pos = static_cast<float4*>(dpos);
+    h->render_nrm = static_cast<float4*>(dnrm);
+    drop_graph(h);
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_residual(xpbd_cloth* h, double* out) {
+    if (!h || !out) return XPBD_E_INVALID;
+    double zero[2] = {0.0, 0.0};
+    HIPC(h, hipMemcpyAsync(h->resid, zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+    launch_residual(h->X, h->gs_ij, h->gs_rc, h->e, h->resid, h->stream);
+    double c2 = 0.0;
+    HIPC(h, hipMemcpyAsync(&c2, h->resid, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPC(h, hipStreamSynchronize(h->stream));
+    *out = h->rest_sq_sum > 0.0 ? std::sqrt(c2) / std::sqrt(h->rest_sq_sum) : 0.0;
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* avg_ms) {
+    if (!h || !avg_ms || reps == 0) return XPBD_E_INVALID;
+    if (h->n == 0) return XPBD_E_UNSUPPORTED;
+    hipStream_t s = h->stream;
+    const DevParams p = dev_params(h);
+    HIPC(h, hipMemcpyAsync(h->A, h->X, size_t(h->n) * sizeof(float4), hipMemcpyDeviceToDevice, s));
+    HIPC(h, hipMemcpyAsync(h->B, h->X, size_t(h->n) * sizeof(float4), hipMemcpyDeviceToDevice, s));
+    hipEvent_t e0, e1;
+    HIPC(h, hipEventCreate(&e0));
+    HIPC(h, hipEventCreate(&e1));
+    JacobiArgs a = jacobi_args(h, p);
+    a.src = h->A;
+    a.dst = h->B;
+    auto prep_collision = [&]() {
+        launch_aabb_reset(h->aabb, s);
+        launch_aabb_of(h->A, h->n, h->aabb, s);
+        launch_coll_grid(h->aabb, h->grid, coll_cell(h), s);
+        launch_coll_keys(h->A, h->n, h->grid, h->keys, h->ids, h->table, h->table_slots, s);
+        launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
+        launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, s);
+    };
+    CollArgs c{};
+    if (which == XPBD_KERNEL_COLLISION_SWEEP || which == XPBD_KERNEL_RADIX_PASS) {
+        prep_collision();
+        c.F = h->A;
+        c.rest4 = h->rest4;
+        c.prev = h->X;
+        c.xout = h->B;
+        c.vout = h->V2;
+        c.ground = h->has_ground ? h->ground : nullptr;
+        c.grid = h->grid;
+        c.skeys = h->keys;
+        c.sids = h->ids;
+        c.table = h->table;
+        c.table_mask = h->table_slots - 1;
+        c.p = p;
+    }
+    GsArgs g{};
+    if (which == XPBD_KERNEL_GS_SUBSTEP) {
+        g.X = h->X;
+        g.V = h->V;
+        g.W = h->A;
+        g.xout = h->B;
+        g.vout = h->V2;
+        g.ground = nullptr;
+        g.aabb = h->aabb;
+        g.ij = h->gs_ij;
+        g.rc = h->gs_rc;
+        g.batch_offsets = h->gs_off;
+        g.num_batches = h->e ? h->num_batches : 0;
+        g.iterations = std::max(h->cfg.solve_inner_iterations, 1u);
+        g.barrier = h->barrier;
+        g.error = h->err;
+        g.final_write = 1;
+        g.p = p;
+    }
+    auto one = [&]() {
+        switch (which) {
+            case XPBD_KERNEL_JACOBI_ITER:
+                launch_jacobi(0, a, h->jacobi_block, s);
+                a.src = a.dst;
+                a.dst = (a.dst == h->A) ? h->B : h->A;
+                break;
+            case XPBD_KERNEL_COLLISION_SWEEP: launch_coll_sweep(c, s); break;
+            case XPBD_KERNEL_RADIX_PASS:
+                launch_radix_pass(h->keys, h->ids, h->keys2, h->ids2, h->n, 0, h->hist, s);
+                break;
+            case XPBD_KERNEL_GS_SUBSTEP:
+                launch_zero_u32(h->barrier, 1, s);
+                launch_gs(g, h->gs_blocks, h->gs_block, s);
+                break;
+            default: break;
+        }
+    };
+    if (which < XPBD_KERNEL_JACOBI_ITER || which > XPBD_KERNEL_GS_SUBSTEP) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return XPBD_E_INVALID;
+    }
+    one();   // warm-up
+    HIPC(h, hipEventRecord(e0, s));
+    for (uint32_t r = 0; r < reps; ++r) one();
+    HIPC(h, hipEventRecord(e1, s));
+    HIPC(h, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPC(h, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    launch_aabb_reset(h->aabb, s);
+    HIPC(h, hipGetLastError());
+    TRY(check_device_flag(h));
+    *avg_ms = ms / float(reps);
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t* grid_out, uint32_t* sorted_keys,
+                                       uint32_t* sorted_ids) {
+    if (!h || !grid_out || !sorted_keys || !sorted_ids || !(cell_size > 0.f)) return XPBD_E_INVALID;
+    hipStream_t s = h->stream;
+    launch_aabb_reset(h->aabb, s);
+    launch_aabb_of(h->X, h->n, h->aabb, s);
+    launch_coll_grid(h->aabb, h->grid, cell_size, s);
+    launch_coll_keys(h->X, h->n, h->grid, h->keys, h->ids, h->table, h->table_slots, s);
+    launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
+    HIPC(h, hipGetLastError());
+    HIPC(h, hipStreamSynchronize(s));
+    CollGrid g{};
+    HIPC(h, hipMemcpy(&g, h->grid, sizeof(g), hipMemcpyDeviceToHost));
+    std::memcpy(grid_out, &g, 11 * sizeof(uint32_t));
+    HIPC(h, hipMemcpy(sorted_keys, h->keys, size_t(h->n) * 4, hipMemcpyDeviceToHost));
+    HIPC(h, hipMemcpy(sorted_ids, h->ids, size_t(h->n) * 4, hipMemcpyDeviceToHost));
+    return XPBD_OK;
+}
+
+}  // extern "C"

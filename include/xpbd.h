@@ -1,0 +1,226 @@
+/*
+ * xpbd.h — C ABI of libxpbd_hip.so, the MI355X-native XPBD cloth compute path.
+ *
+ * Drop-in boundary for the reference crate's compute path (mate-h/bevy_xpbd,
+ * crate bevy_softbody).  Each entry point names the reference interface it
+ * replaces as file:line.  Plain pointers and sizes only; no torch or HIP types
+ * appear in the signatures (streams and device pointers are void*).
+ *
+ * Threading: calls on one xpbd_cloth are externally synchronised (the
+ * reference runs its render-world system on one thread).  Different handles
+ * are independent.  All device work of a handle is ordered on its stream.
+ */
+#ifndef XPBD_H
+#define XPBD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XPBD_API __attribute__((visibility("default")))
+#define XPBD_ABI_VERSION 1
+
+/* Status codes.  The reference early-returns silently on missing resources
+ * (cloth_compute.rs:1222-1239) and panics on shader errors
+ * (cloth_jacobi.rs:668); this ABI reports both as codes instead. */
+enum {
+    XPBD_OK = 0,
+    XPBD_E_INVALID = 1,      /* bad argument or inconsistent topology       */
+    XPBD_E_HIP = 2,          /* HIP runtime error (message in last_error)   */
+    XPBD_E_NOMEM = 3,        /* device or host allocation failed            */
+    XPBD_E_UNSUPPORTED = 4,  /* feature not available for this mesh/config */
+    XPBD_E_STATE = 5,        /* wrong call order / device-side guard fired  */
+    XPBD_E_NO_DEVICE = 6     /* no HIP device visible                       */
+};
+
+/* Solver switch: replaces the compile-time cargo features solver-jacobi /
+ * solver-gauss-seidel (Cargo.toml:27-30, lib.rs:3-4) with a runtime enum. */
+enum { XPBD_SOLVER_JACOBI = 0, XPBD_SOLVER_GAUSS_SEIDEL = 1 };
+
+/* ---------------------------------------------------------------------------
+ * L0 — topology preparation (mesh_prep.rs)
+ * ------------------------------------------------------------------------- */
+
+/* ClothMeshData (mesh_prep.rs:103-131).  Opaque; read fields with
+ * xpbd_mesh_count / xpbd_mesh_read.  Constraint rows are in colour-sorted
+ * order and the CSR slices are cursor-scattered in that order, as
+ * finalize_cloth_mesh builds them (mesh_prep.rs:584-635). */
+typedef struct xpbd_mesh xpbd_mesh;
+
+enum {
+    XPBD_MESH_POSITIONS = 0,               /* f32 x 3N  initial state           */
+    XPBD_MESH_REST_POSITIONS = 1,          /* f32 x 3N                          */
+    XPBD_MESH_NORMALS = 2,                 /* f32 x 3N                          */
+    XPBD_MESH_UVS = 3,                     /* f32 x 2N                          */
+    XPBD_MESH_INDICES = 4,                 /* u32 x 3T                          */
+    XPBD_MESH_TRI_EDGE_NEIGHBORS = 5,      /* u32 x 3T (reference builders)     */
+    XPBD_MESH_INV_MASS = 6,                /* f32 x N                           */
+    XPBD_MESH_NEIGHBOR_OTHER = 7,          /* u32 x 2E                          */
+    XPBD_MESH_NEIGHBOR_REST_LEN = 8,       /* f32 x 2E                          */
+    XPBD_MESH_NEIGHBOR_COMPLIANCE = 9,     /* f32 x 2E                          */
+    XPBD_MESH_NEIGHBOR_CONSTRAINT_ID = 10, /* u32 x 2E                          */
+    XPBD_MESH_NEIGHBOR_OFFSETS = 11,       /* u32 x (N+1)                       */
+    XPBD_MESH_CONSTRAINT_I = 12,           /* u32 x E                           */
+    XPBD_MESH_CONSTRAINT_J = 13,           /* u32 x E                           */
+    XPBD_MESH_CONSTRAINT_REST = 14,        /* f32 x E                           */
+    XPBD_MESH_CONSTRAINT_COMPLIANCE = 15,  /* f32 x E                           */
+    XPBD_MESH_BATCH_OFFSETS = 16,          /* u32 x (B+1) colour batch starts   */
+    XPBD_MESH_GROUND = 17,                 /* u32 x N  1 = ground plane y=0     */
+    XPBD_MESH_CLOTH_OFFSETS = 18,          /* u32 x (C+1) first vertex of cloth */
+    XPBD_MESH_NUM_FIELDS = 19
+};
+
+/* grid_cloth_hanging (mesh_prep.rs:363-406). */
+XPBD_API int xpbd_mesh_grid_cloth_hanging(uint32_t quad_cols, uint32_t quad_rows, float cell_size,
+                                          xpbd_mesh** out);
+/* parse_welded_obj (mesh_prep.rs:223-357).  text need not be NUL-terminated. */
+XPBD_API int xpbd_mesh_parse_welded_obj(const char* text, size_t len, xpbd_mesh** out);
+
+/* Benchmark sheets (no reference counterpart: the reference has no seeded
+ * sheet builder).  Axial stretch, both quad diagonals as shear, distance-2
+ * axial bend; uniform inverse mass; explicit pins; seeded jitter. */
+enum { XPBD_LAYOUT_HANGING_XY = 0, XPBD_LAYOUT_HORIZONTAL_XZ = 1 };
+enum { XPBD_PIN_NONE = 0, XPBD_PIN_TWO_CORNERS = 1, XPBD_PIN_TOP_ROW = 2, XPBD_PIN_CENTER = 3 };
+typedef struct xpbd_sheet_desc {
+    uint32_t cols, rows;      /* vertices per row / column (>= 2)          */
+    float spacing;            /* rest grid spacing [m]                     */
+    uint32_t layout;          /* XPBD_LAYOUT_*                             */
+    float height;             /* base y of the sheet [m]                   */
+    float angle;              /* rotation about +Y, horizontal layout      */
+    uint32_t pins;            /* XPBD_PIN_*                                */
+    uint32_t ground;          /* 1: ground plane y = 0 for this cloth      */
+    float inv_mass;           /* inverse mass of free vertices [1/kg]      */
+    float stretch_compliance, shear_compliance, bend_compliance;
+    float jitter;             /* uniform out-of-plane jitter amplitude [m] */
+    uint64_t seed;            /* SplitMix64 seed for the jitter            */
+} xpbd_sheet_desc;
+XPBD_API int xpbd_mesh_sheet(const xpbd_sheet_desc* desc, xpbd_mesh** out);
+/* Many independent cloths in one mesh (block-diagonal constraint graph). */
+XPBD_API int xpbd_mesh_batch(const xpbd_sheet_desc* descs, uint32_t count, xpbd_mesh** out);
+
+XPBD_API uint64_t xpbd_mesh_count(const xpbd_mesh* mesh, int field);
+XPBD_API int xpbd_mesh_read(const xpbd_mesh* mesh, int field, void* dst, uint64_t dst_bytes);
+XPBD_API void xpbd_mesh_destroy(xpbd_mesh* mesh);
+
+/* derive_collision_grid (mesh_prep.rs:16-85); rest is vec4 x n. */
+typedef struct xpbd_coll_grid {
+    float origin[3];
+    float inv_cell;
+    uint32_t dims[3];
+    uint32_t num_cells;
+    uint32_t radix_digits;
+} xpbd_coll_grid;
+XPBD_API int xpbd_derive_collision_grid(const float* rest_vec4, uint32_t n, float thickness,
+                                        xpbd_coll_grid* out);
+
+/* ---------------------------------------------------------------------------
+ * L3/L4 — ClothSimConfig / ClothSimUniforms / ClothComputePlugin
+ * ------------------------------------------------------------------------- */
+
+/* ClothSimConfig scalar fields (cloth_compute.rs:142-179).  The topology
+ * arrays of the reference config come from the xpbd_mesh. */
+typedef struct xpbd_sim_config {
+    uint32_t solve_substeps;              /* S  (default SUBSTEPS = 36)      */
+    uint32_t solve_inner_iterations;      /* I  (default INNER_ITERS = 22)   */
+    uint32_t collision_every_n_substeps;  /* collision when si % N == N-1    */
+    uint32_t solver;                      /* XPBD_SOLVER_*                   */
+    float collision_cell;                 /* hash cell side [m]; 0 = thickness */
+    uint32_t use_graph;                   /* 1: replay frames from a hipGraph */
+    uint32_t autotune;                    /* 1: time launch shapes at create */
+} xpbd_sim_config;
+
+/* ClothSimUniforms (cloth_compute.rs:182-202). */
+typedef struct xpbd_sim_uniforms {
+    float dt, inv_dt;
+    uint32_t num_particles, num_tris;
+    float jacobi_omega;
+    uint32_t inner_iterations;
+    float thickness, coll_scale;
+    float gravity[4];
+    float grab_target[4];
+    int32_t grab_idx;
+    uint32_t grab_active;
+    float grab_stiffness;
+    float linear_drag_per_sec;
+    uint32_t constraint_batch_idx;
+} xpbd_sim_uniforms;
+
+/* ClothSimUniforms::default (cloth_compute.rs:204-243) for `solver`, with
+ * dt = (1/60)/substeps as sync_cloth_solve_budget_to_uniforms sets it
+ * (cloth_compute.rs:419-441). */
+XPBD_API void xpbd_sim_uniforms_default(uint32_t solver, uint32_t substeps, xpbd_sim_uniforms* out);
+/* ClothMeshData::to_sim_config defaults (cloth_compute.rs:1675-1678). */
+XPBD_API void xpbd_sim_config_default(xpbd_sim_config* out);
+
+/* One simulated cloth (or batch): the device state the reference keeps in
+ * ClothSimBuffersJacobi / ClothSimBuffers. */
+typedef struct xpbd_cloth xpbd_cloth;
+
+/* ClothMeshData::to_sim_config + init_cloth_sim (cloth_compute.rs:1599,571;
+ * cloth_jacobi.rs:213): copies the topology to the device and uploads the
+ * initial state.  hip_stream: a hipStream_t to order work on, or NULL for a
+ * library-owned stream. */
+XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, const xpbd_sim_uniforms* u,
+                         void* hip_stream, xpbd_cloth** out);
+XPBD_API void xpbd_destroy(xpbd_cloth* h);
+XPBD_API const char* xpbd_last_error(const xpbd_cloth* h);
+/* Extracted ClothSimUniforms / ClothSimConfig changes (ExtractResourcePlugin,
+ * cloth_compute.rs:458-462). */
+XPBD_API int xpbd_set_uniforms(xpbd_cloth* h, const xpbd_sim_uniforms* u);
+XPBD_API int xpbd_set_config(xpbd_cloth* h, const xpbd_sim_config* cfg);
+
+/* run_cloth_sim (cloth_compute.rs:1197-1269; cloth_jacobi.rs:675-916):
+ * enqueue `frames` frames of S substeps (asynchronous). */
+XPBD_API int xpbd_step_frames(xpbd_cloth* h, uint32_t frames);
+/* `count` substeps with an explicit collision flag (test granularity:
+ * gpu_cpu_parity_jacobi.rs:635-764 runs one substep). */
+XPBD_API int xpbd_step_substeps(xpbd_cloth* h, uint32_t count, uint32_t collision);
+XPBD_API int xpbd_sync(xpbd_cloth* h);
+XPBD_API void* xpbd_stream(xpbd_cloth* h);
+
+/* ClothSimControl::reinit_serial (cloth_compute.rs:509-569,
+ * cloth_jacobi.rs:918-944): state back to the initial pose, zero velocity. */
+XPBD_API int xpbd_reset(xpbd_cloth* h);
+/* State read-back / upload in the reference buffer layout: vec4<f32> per
+ * particle, w = 0 (cloth_jacobi.rs:238-273).  Host pointers; synchronous. */
+XPBD_API int xpbd_read_state(xpbd_cloth* h, float* pos_vec4, float* vel_vec4);
+XPBD_API int xpbd_write_state(xpbd_cloth* h, const float* pos_vec4, const float* vel_vec4);
+
+/* Render publish: finalize_normals (cloth_sim_jacobi.wgsl:527-547) writes
+ * render_positions / render_normals as array<vec4<f32>>, w = 0, indexed by
+ * particle id (cloth_material.rs:16-19, cloth_vertex.wgsl:21-22).  dst_* are
+ * DEVICE pointers (the ClothMatExt storage buffers); either may be NULL. */
+XPBD_API int xpbd_publish(xpbd_cloth* h, void* dst_pos_vec4, void* dst_nrm_vec4);
+/* Bind the render buffers so every frame ends with a publish, as the
+ * reference's normals pass does once per frame (cloth_jacobi.rs:885-913). */
+XPBD_API int xpbd_bind_render_buffers(xpbd_cloth* h, void* dst_pos_vec4, void* dst_nrm_vec4);
+
+/* Relative constraint residual sqrt(sum C^2)/sqrt(sum rest^2) of the current
+ * state (no reference counterpart; the metric the learned warm start is
+ * judged by).  Synchronous. */
+XPBD_API int xpbd_residual(xpbd_cloth* h, double* out);
+
+/* Measurement: average duration [ms] of one launch of kernel `which`
+ * (XPBD_KERNEL_*) on the handle's stream, over `reps` launches bracketed by
+ * HIP events.  The kernel runs on scratch buffers; the state is untouched. */
+enum { XPBD_KERNEL_JACOBI_ITER = 0, XPBD_KERNEL_COLLISION_SWEEP = 1, XPBD_KERNEL_RADIX_PASS = 2,
+       XPBD_KERNEL_GS_SUBSTEP = 3, XPBD_KERNEL_SUBSTEP = 4 };
+XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* avg_ms);
+
+/* Debug/test hook for the collision broad phase: computes the cell grid and
+ * the stable (key, id) radix sort of the CURRENT positions on the device and
+ * copies them back.  grid_out: lo[3], hi[3], cell, inv_cell (floats) then
+ * dims[3] (u32) — 11 words. */
+XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t* grid_out,
+                                       uint32_t* sorted_keys, uint32_t* sorted_ids);
+
+/* Library info. */
+XPBD_API int xpbd_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XPBD_H */

@@ -1,0 +1,40 @@
+"""Shared helpers for the tests: parameter conversion between the product's
+ClothSimUniforms/ClothSimConfig and the oracle's params, and error metrics."""
+import numpy as np
+
+
+def oracle_params(O, X, u, cfg, collision=False, coll_mode=None):
+    p = O.Params()
+    p.dt = u.dt
+    p.inv_dt = u.inv_dt
+    p.jacobi_omega = u.jacobi_omega
+    p.inner_iterations = cfg.solve_inner_iterations
+    p.gravity[:] = list(u.gravity[:3])
+    p.grab_idx = u.grab_idx
+    p.grab_active = u.grab_active
+    p.grab_target[:] = list(u.grab_target[:3])
+    p.grab_stiffness = u.grab_stiffness
+    p.linear_drag_per_sec = u.linear_drag_per_sec
+    p.thickness = u.thickness
+    p.coll_scale = u.coll_scale
+    p.collision = 1 if collision else 0
+    p.solver = 1 if cfg.solver == X.SOLVER_GAUSS_SEIDEL else 0
+    p.coll_mode = O.COLL_GRID if coll_mode is None else coll_mode
+    return p
+
+
+def max_abs(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64)[:, :3] - np.asarray(b, np.float64)[:, :3])))
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(np.asarray(a)[:, :3], dtype=np.float32)
+    b = np.ascontiguousarray(np.asarray(b)[:, :3], dtype=np.float32)
+    return a.tobytes() == b.tobytes()
+
+
+def first_diff(a, b):
+    a = np.asarray(a)[:, :3]
+    b = np.asarray(b)[:, :3]
+    bad = np.nonzero(np.any(a.view(np.uint32) != b.view(np.uint32), axis=1))[0] if a.dtype == np.float32 else []
+    return bad[:5]

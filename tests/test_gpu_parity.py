@@ -1,0 +1,240 @@
+"""HIP path vs the CPU oracle, through the C ABI.  Needs an MI355X.
+
+Bar: bit-exact f32 positions and velocities against the f32 oracle, substep
+by substep (the kernels restate the CPU arithmetic operation by operation);
+the reference's own looser GPU/CPU tolerances (2e-3, 3e-2,
+gpu_cpu_parity_jacobi.rs:912,1015; 2.5e-2, gpu_cpu_parity.rs:977) are
+asserted too, as the reference tests do.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from _helpers import bits_equal, first_diff, max_abs, oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def make(X, mesh, solver=0, S=8, I=10, coll_stride=1, use_graph=0, **uk):
+    cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=I, solver=solver,
+                                   collision_every_n_substeps=coll_stride, use_graph=use_graph)
+    u = X.ClothSimUniforms.default(solver, S, **uk)
+    return X.ClothComputePlugin(mesh, cfg, u), cfg, u
+
+
+def run_pair(O, X, mesh_kw=None, mesh=None, solver=0, S=8, I=10, substeps=8, collision=False, **uk):
+    mesh = mesh or X.ClothMeshData.sheet(**mesh_kw)
+    f = mesh.fields()
+    sim, cfg, u = make(X, mesh, solver, S, I, **uk)
+    s = O.Solver(f)
+    p = oracle_params(O, X, u, cfg, collision=collision)
+    pos = f["positions"].copy()
+    vel = np.zeros_like(pos)
+    for k in range(substeps):
+        note(f"  substep {k}")
+        sim.step_substeps(1, collision)
+        gp, gv = sim.read_state()
+        pos, vel = s.substep(pos, vel, p)
+        err = max_abs(gp, pos)
+        assert bits_equal(gp, pos), f"substep {k}: positions differ (max |d| {err:.3e}) at {first_diff(gp, pos)}"
+        assert bits_equal(gv, vel), f"substep {k}: velocities differ"
+    sim.close()
+    return pos, vel
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_config1_sheet_bit_exact_per_substep(O, X, solver):
+    note(f"case config1 32x32 solver={solver}")
+    run_pair(O, X, dict(cols=32, rows=32, spacing=1 / 31, pins=1, jitter=1e-3, seed=1), solver=solver,
+             S=8, I=10, substeps=16)
+
+
+@pytest.mark.parametrize("iters", [1, 2, 3])
+def test_jacobi_iteration_count_edge_cases(O, X, iters):
+    note(f"case iters={iters}")
+    run_pair(O, X, dict(cols=20, rows=9, spacing=0.05, pins=2, jitter=1e-3, seed=2), S=8, I=iters, substeps=6)
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_reference_triangle_one_substep(O, X, solver):
+    """gpu_cpu_one_substep_single_triangle_positions_close
+    (gpu_cpu_parity_jacobi.rs:817-924; gpu_cpu_parity.rs:821-907): one substep
+    with collision on, gravity (-1,-2,0.5), SUBSTEPS=36, INNER_ITERS=22."""
+    note(f"case triangle solver={solver}")
+    mesh = X.ClothMeshData.parse_welded_obj("\nv 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0 1\nf 1/1 2/2 3/3\n")
+    pos, _ = run_pair(O, X, mesh=mesh, solver=solver, S=36, I=22, substeps=1, collision=True,
+                      gravity=(-1.0, -2.0, 0.5), grab_stiffness=0.0)
+    assert np.isfinite(pos).all()
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_reference_grid_18x18_one_substep(O, X, solver):
+    """gpu_cpu_grid_cloth_substep_positions_close (gpu_cpu_parity_jacobi.rs:926-1027,
+    tolerance 3e-2; gpu_cpu_parity.rs:909-989, 2.5e-2) — here bit-exact."""
+    note(f"case grid18 solver={solver}")
+    mesh = X.ClothMeshData.grid_cloth_hanging(18, 18, 0.042)
+    run_pair(O, X, mesh=mesh, solver=solver, S=36, I=22, substeps=3, collision=True)
+
+
+def test_reference_grab_and_frames(O, X):
+    """A whole frame schedule with the grab term active (predict, xpbd_cpu.rs:56-63)."""
+    note("case grab frames")
+    mesh = X.ClothMeshData.grid_cloth_hanging(16, 16, 0.045)
+    f = mesh.fields()
+    sim, cfg, u = make(X, mesh, 0, S=12, I=8, grab_idx=5, grab_active=1, grab_target=(0.3, 0.2, 0.4))
+    sim.step_frames(3)
+    gp, gv = sim.read_state()
+    p = oracle_params(O, X, u, cfg, collision=True)
+    pos, vel = O.Solver(f).frames(f["positions"], np.zeros_like(f["positions"]), p, 12, 1, 3)
+    assert bits_equal(gp, pos) and bits_equal(gv, vel), max_abs(gp, pos)
+    sim.close()
+
+
+def test_folding_sheet_with_collision_bit_exact(O, X):
+    """Config-2 style: horizontal sheet, centre pin, falls and folds onto
+    itself; contacts must be resolved identically to the brute-force order."""
+    note("case fold 40x40")
+    kw = dict(cols=40, rows=40, spacing=1 / 39, layout=1, height=1.0, angle=0.9, pins=3)
+    mesh = X.ClothMeshData.sheet(**kw)
+    f = mesh.fields()
+    h = 1 / 39
+    th = 2 * (0.4 * h)
+    sim, cfg, u = make(X, mesh, 0, S=4, I=8, thickness=th)
+    s = O.Solver(f)
+    p = oracle_params(O, X, u, cfg, collision=True)
+    pos = f["positions"].copy()
+    vel = np.zeros_like(pos)
+    contacts = 0
+    for frame in range(40):
+        sim.step_frames(1)
+        pos, vel = s.frames(pos, vel, p, 4, 1, 1)
+        if frame % 5 == 4:
+            note(f"  frame {frame}")
+            gp, gv = sim.read_state()
+            assert bits_equal(gp, pos), f"frame {frame}: max |d| {max_abs(gp, pos):.3e}"
+            assert bits_equal(gv, vel)
+    # the brute-force loop agrees with the grid oracle on the final state
+    pb = O.params()
+    q1 = s.self_collision(pos, th, 0.38, O.COLL_BRUTE)
+    q2 = s.self_collision(pos, th, 0.38, O.COLL_GRID)
+    assert q1.tobytes() == q2.tobytes()
+    contacts = int(np.any(q1 != pos, axis=1).sum())
+    note(f"  particles in contact at the end: {contacts}")
+    assert contacts > 0
+    sim.close()
+
+
+def test_collision_keys_and_sort_bit_exact(O, X):
+    note("case keys")
+    mesh = X.ClothMeshData.sheet(cols=64, rows=48, spacing=0.01, layout=1, height=1.0, angle=0.3, pins=3,
+                                 jitter=2e-3, seed=4)
+    sim, _, _ = make(X, mesh)
+    pos, _ = sim.read_state()
+    pos[:, :3] += np.random.default_rng(1).normal(0, 0.05, (len(pos), 3)).astype(np.float32)
+    sim.write_state(pos)
+    for cell in (0.008, 0.02, 0.1):
+        g, keys, ids = sim.debug_collision_keys(cell)
+        og = O.cell_grid(pos, cell)
+        assert np.array(og.lo[:], np.float32).tobytes() == g["lo"].tobytes()
+        assert list(og.dims) == list(g["dims"])
+        assert np.float32(og.inv_cell) == g["inv_cell"]
+        ok = O.cell_keys(pos, og)
+        sk, si = O.sort_pairs(ok)
+        assert (keys == sk).all() and (ids == si).all()
+    sim.close()
+
+
+def test_publish_layout_and_normals(O, X):
+    import torch
+    note("case publish")
+    mesh = X.ClothMeshData.sheet(cols=33, rows=21, spacing=0.03, pins=2, jitter=1e-3, seed=3)
+    f = mesh.fields()
+    sim, _, _ = make(X, mesh, S=4, I=6)
+    sim.step_frames(2)
+    dpos = torch.full((sim.n, 4), 7.0, device="cuda:0")
+    dnrm = torch.full((sim.n, 4), 7.0, device="cuda:0")
+    sim.sync()
+    sim.publish(dpos.data_ptr(), dnrm.data_ptr())
+    sim.sync()
+    gp, _ = sim.read_state()
+    rp = dpos.cpu().numpy()
+    rn = dnrm.cpu().numpy()
+    assert (rp[:, 3] == 0).all() and (rn[:, 3] == 0).all()
+    assert bits_equal(rp, gp)
+    on = O.vertex_normals(gp[:, :3], f["indices"])
+    assert bits_equal(rn, on)
+    sim.close()
+
+
+def test_graph_replay_equals_eager(X):
+    note("case graph")
+    kw = dict(cols=48, rows=48, spacing=1 / 47, layout=1, height=1.0, angle=0.4, pins=3)
+    th = 2 * 0.4 / 47
+    a, _, _ = make(X, X.ClothMeshData.sheet(**kw), S=4, I=8, use_graph=1, thickness=th)
+    b, _, _ = make(X, X.ClothMeshData.sheet(**kw), S=4, I=8, use_graph=0, thickness=th)
+    for k in range(4):
+        a.step_frames(5)
+        b.step_frames(5)
+        pa, va = a.read_state()
+        pb, vb = b.read_state()
+        assert bits_equal(pa, pb) and bits_equal(va, vb), f"replay {k}"
+    a.close()
+    b.close()
+
+
+def test_reset_and_state_roundtrip(X):
+    note("case reset")
+    mesh = X.ClothMeshData.sheet(cols=16, rows=16, spacing=0.05, pins=1, jitter=1e-3, seed=5)
+    f = mesh.fields()
+    sim, _, _ = make(X, mesh, S=4, I=4)
+    p0, v0 = sim.read_state()
+    assert bits_equal(p0, f["positions"]) and (v0 == 0).all() and (p0[:, 3] == 0).all()
+    sim.step_frames(3)
+    p1, _ = sim.read_state()
+    assert not bits_equal(p1, p0)
+    sim.control.reinit_serial += 1
+    sim.control.sim_paused = True
+    assert sim.run_cloth_sim() is False
+    p2, v2 = sim.read_state()
+    assert bits_equal(p2, f["positions"]) and (v2 == 0).all()
+    sim.control.step_serial += 1
+    assert sim.run_cloth_sim() is True
+    sim.close()
+
+
+def test_residual_matches_oracle(O, X):
+    note("case residual")
+    mesh = X.ClothMeshData.sheet(cols=40, rows=30, spacing=0.02, pins=2, jitter=1e-3, seed=6)
+    f = mesh.fields()
+    sim, _, _ = make(X, mesh, S=8, I=10)
+    sim.step_frames(2)
+    gp, _ = sim.read_state()
+    r_gpu = sim.residual()
+    r_cpu = O.Solver(f).residual(gp)
+    assert r_gpu > 0 and abs(r_gpu - r_cpu) <= 1e-9 * max(r_cpu, 1e-30) + 1e-15
+    sim.close()
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_config2_256_collision_substeps_bit_exact(O, X, solver):
+    """256x256 horizontal sheet, centre pin, seeded rotation, self-collision."""
+    note(f"case 256 solver={solver}")
+    h = 1 / 255
+    rng = np.random.default_rng(2)
+    kw = dict(cols=256, rows=256, spacing=h, layout=1, height=1.0, angle=float(rng.uniform(0, 2 * np.pi)), pins=3)
+    run_pair(O, X, kw, solver=solver, S=4, I=8, substeps=3, collision=True, thickness=2 * (0.4 * h),
+             jacobi_omega=0.32 if solver == 0 else 1.0)
+
+
+def test_config3_1024_two_substeps_bit_exact(O, X):
+    note("case 1024")
+    h = 1 / 1023
+    O.set_threads(16)
+    kw = dict(cols=1024, rows=1024, spacing=h, pins=2, jitter=1e-3, seed=3)
+    run_pair(O, X, kw, S=8, I=10, substeps=2, collision=True, thickness=2 * (0.4 * h))
