@@ -38,3 +38,44 @@ def first_diff(a, b):
     b = np.asarray(b)[:, :3]
     bad = np.nonzero(np.any(a.view(np.uint32) != b.view(np.uint32), axis=1))[0] if a.dtype == np.float32 else []
     return bad[:5]
+
+
+class DeviceBuffer:
+    """Raw device memory through the HIP runtime (for the render-buffer
+    boundary, which takes device pointers)."""
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        import ctypes as C
+        if cls._hip is None:
+            h = C.CDLL("libamdhip64.so")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes: int):
+        import ctypes as C
+        self.nbytes = nbytes
+        self.ptr = C.c_void_p()
+        assert self.hip().hipMalloc(C.byref(self.ptr), nbytes) == 0
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes == self.nbytes
+        assert self.hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) == 0
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes == self.nbytes
+        assert self.hip().hipDeviceSynchronize() == 0
+        assert self.hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None

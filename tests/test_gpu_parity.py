@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 
-from _helpers import bits_equal, first_diff, max_abs, oracle_params
+from _helpers import DeviceBuffer, bits_equal, first_diff, max_abs, oracle_params
 
 pytestmark = pytest.mark.gpu
 
@@ -150,26 +150,36 @@ def test_collision_keys_and_sort_bit_exact(O, X):
     sim.close()
 
 
-def test_publish_layout_and_normals(O, X):
-    import torch
-    note("case publish")
+@pytest.mark.parametrize("bound", [False, True])
+def test_publish_layout_and_normals(O, X, bound):
+    """finalize_normals / render_positions layout (cloth_sim_jacobi.wgsl:527-547):
+    vec4<f32>, w = 0, indexed by particle id; normals = normalised sum of
+    incident face normals, checked bit-exact against the oracle."""
+    note(f"case publish bound={bound}")
     mesh = X.ClothMeshData.sheet(cols=33, rows=21, spacing=0.03, pins=2, jitter=1e-3, seed=3)
     f = mesh.fields()
-    sim, _, _ = make(X, mesh, S=4, I=6)
-    sim.step_frames(2)
-    dpos = torch.full((sim.n, 4), 7.0, device="cuda:0")
-    dnrm = torch.full((sim.n, 4), 7.0, device="cuda:0")
-    sim.sync()
-    sim.publish(dpos.data_ptr(), dnrm.data_ptr())
+    sim, _, _ = make(X, mesh, S=4, I=6, use_graph=1)
+    dpos = DeviceBuffer(16 * sim.n)
+    dnrm = DeviceBuffer(16 * sim.n)
+    dpos.upload(np.full((sim.n, 4), 7.0, np.float32))
+    dnrm.upload(np.full((sim.n, 4), 7.0, np.float32))
+    if bound:
+        sim.bind_render_buffers(dpos.ptr.value, dnrm.ptr.value)
+        sim.step_frames(2)
+    else:
+        sim.step_frames(2)
+        sim.publish(dpos.ptr.value, dnrm.ptr.value)
     sim.sync()
     gp, _ = sim.read_state()
-    rp = dpos.cpu().numpy()
-    rn = dnrm.cpu().numpy()
+    rp = dpos.download(np.float32, (sim.n, 4))
+    rn = dnrm.download(np.float32, (sim.n, 4))
     assert (rp[:, 3] == 0).all() and (rn[:, 3] == 0).all()
     assert bits_equal(rp, gp)
     on = O.vertex_normals(gp[:, :3], f["indices"])
     assert bits_equal(rn, on)
     sim.close()
+    dpos.free()
+    dnrm.free()
 
 
 def test_graph_replay_equals_eager(X):
