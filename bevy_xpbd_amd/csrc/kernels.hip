@@ -69,27 +69,20 @@ __device__ __forceinline__ void finish_particle(float4 nx, float4 prev, uint32_t
                        ((nx.z - prev.z) * p.inv_dt) * p.damp, 0.f);
 }
 
-// Floats ordered as unsigned integers (for atomicMin/atomicMax bounds).
-__device__ __forceinline__ uint32_t f2o(float f) {
-    const uint32_t u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float o2f(uint32_t o) {
-    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
-}
-
-// Block-wide bounds of the rows each thread holds; one atomic per block.
-// -0 is folded to +0 and NaN ignored, as the oracle does.
-__device__ __forceinline__ void block_aabb(bool valid, float x, float y, float z, uint32_t* aabb) {
-    float lo[3], hi[3];
+// Block-wide bounds of the rows each thread holds, written as one partial
+// (lo xyz, hi xyz) per block; the grid kernel reduces the partials.  -0 is
+// folded to +0 and NaN ignored, as the oracle does.
+__device__ __forceinline__ void aabb_fold(bool valid, float x, float y, float z, float lo[3], float hi[3]) {
     const float v[3] = {x + 0.f, y + 0.f, z + 0.f};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = valid ? v[a] : INFINITY;
-        hi[a] = valid ? v[a] : -INFINITY;
-        if (lo[a] != lo[a]) lo[a] = INFINITY;
-        if (hi[a] != hi[a]) hi[a] = -INFINITY;
-    }
+    for (int a = 0; a < 3; ++a)
+        if (valid && v[a] == v[a]) {
+            lo[a] = fminf(lo[a], v[a]);
+            hi[a] = fmaxf(hi[a], v[a]);
+        }
+}
+
+__device__ __forceinline__ void block_aabb_store(float lo[3], float hi[3], float* part, uint32_t slot) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
@@ -105,20 +98,14 @@ __device__ __forceinline__ void block_aabb(bool valid, float x, float y, float z
             red[wave][3 + a] = hi[a];
         }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 6) {
         const uint32_t nw = (blockDim.x + 63) >> 6;
-        for (uint32_t w = 1; w < nw; ++w)
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = fminf(lo[a], red[w][a]);
-                hi[a] = fmaxf(hi[a], red[w][3 + a]);
-            }
-        for (int a = 0; a < 3; ++a) {
-            if (lo[a] <= hi[a]) {
-                atomicMin(&aabb[a], f2o(lo[a]));
-                atomicMax(&aabb[3 + a], f2o(hi[a]));
-            }
-        }
+        const int a = threadIdx.x;
+        float r = red[0][a];
+        for (uint32_t w = 1; w < nw; ++w) r = a < 3 ? fminf(r, red[w][a]) : fmaxf(r, red[w][a]);
+        part[size_t(slot) * 8 + a] = r;
     }
+    __syncthreads();
 }
 
 // Blocks are dealt round-robin to the 8 XCDs; this makes consecutive logical
@@ -136,6 +123,33 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
 // depends only on the pair, and each endpoint recomputes it bit-identically
 // (|a-b| == |b-a|, w_i + w_j == w_j + w_i).  One pass, no lambda buffers.
 // ---------------------------------------------------------------------------
+// One constraint entry of the gather: accumulates (n * w_i) * dlambda in the
+// reference's order of operations (xpbd_cpu.rs:159-182, 206-222).
+__device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float rest, float comp, const DevParams& p,
+                                             float& ax, float& ay, float& az) {
+    const float wi = pi.w, wj = pj.w;
+    if (wj <= 0.f && wi <= 0.f) return;
+    float gx = pi.x - pj.x, gy = pi.y - pj.y, gz = pi.z - pj.z;
+    const float len = len3(gx, gy, gz);
+    if (len < 1e-8f) return;
+    gx = gx / len;
+    gy = gy / len;
+    gz = gz / len;
+    const float c = len - rest;
+    const float alpha_t = comp * p.inv_dt_sq;
+    const float wsum = wi + wj + alpha_t;
+    if (wsum < 1e-8f) return;
+    const float dlam = (-c - alpha_t * 0.0f) / wsum;
+    ax = ax + (gx * wi) * dlam;
+    ay = ay + (gy * wi) * dlam;
+    az = az + (gz * wi) * dlam;
+}
+
+// Entries are fetched JU at a time (all index/rest/compliance loads, then all
+// neighbour gathers, then the arithmetic in entry order), so each wavefront
+// keeps several dependent load chains in flight instead of one.
+constexpr uint32_t JU = 4;
+
 template <uint32_t MODE>
 __global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
     const uint32_t nb = gridDim.x;
@@ -149,50 +163,46 @@ __global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 out = pi;
     if (valid) {
-        if (MODE & JM_PREDICT) {
-            pi = predict(a.src[i], a.vel[i], i, p);
-        } else {
-            pi = a.src[i];
-        }
+        if (MODE & JM_PREDICT) pi = predict(a.src[i], a.vel[i], i, p);
+        else pi = a.src[i];
         out = pi;
         const float wi = pi.w;
         if (wi > 0.f) {
             float ax = 0.f, ay = 0.f, az = 0.f;
             const uint32_t base = a.sell.soff[slice] + lane;
             const uint32_t width = a.sell.swid[slice];
-            for (uint32_t k = 0; k < width; ++k) {
-                const uint32_t e = base + k * 64;
-                const uint32_t j = a.sell.j[e];
-                if (j == NO_ENTRY) break;
-                const float rest = a.sell.rest[e];
-                const float comp = a.sell.comp[e];
-                float4 pj;
-                if (MODE & JM_PREDICT) pj = predict(a.src[j], a.vel[j], j, p);
-                else pj = a.src[j];
-                const float wj = pj.w;
-                if (wj <= 0.f && wi <= 0.f) continue;
-                float gx = pi.x - pj.x, gy = pi.y - pj.y, gz = pi.z - pj.z;
-                const float len = len3(gx, gy, gz);
-                if (len < 1e-8f) continue;
-                gx = gx / len;
-                gy = gy / len;
-                gz = gz / len;
-                const float c = len - rest;
-                const float alpha_t = comp * p.inv_dt_sq;
-                const float wsum = wi + wj + alpha_t;
-                if (wsum < 1e-8f) continue;
-                const float dlam = (-c - alpha_t * 0.0f) / wsum;
-                ax = ax + (gx * wi) * dlam;
-                ay = ay + (gy * wi) * dlam;
-                az = az + (gz * wi) * dlam;
+            for (uint32_t k0 = 0; k0 < width; k0 += JU) {
+                uint32_t jj[JU];
+                float rr[JU], cc[JU];
+                float4 pp[JU];
+#pragma unroll
+                for (uint32_t u = 0; u < JU; ++u) {
+                    const uint32_t k = k0 + u;
+                    const uint32_t e = base + k * 64;
+                    jj[u] = k < width ? __builtin_nontemporal_load(a.sell.j + e) : NO_ENTRY;
+                    rr[u] = k < width ? __builtin_nontemporal_load(a.sell.rest + e) : 0.f;
+                    cc[u] = k < width ? __builtin_nontemporal_load(a.sell.comp + e) : 0.f;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < JU; ++u) {
+                    if (jj[u] != NO_ENTRY) {
+                        if (MODE & JM_PREDICT) pp[u] = predict(a.src[jj[u]], a.vel[jj[u]], jj[u], p);
+                        else pp[u] = a.src[jj[u]];
+                    } else {
+                        pp[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    }
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < JU; ++u)
+                    if (jj[u] != NO_ENTRY) jacobi_entry(pi, pp[u], rr[u], cc[u], p, ax, ay, az);
             }
             float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
             const float ml = len3(dx, dy, dz);
             if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
-                const float s = JACOBI_CORRECTION_CAP / ml;
-                dx = dx * s;
-                dy = dy * s;
-                dz = dz * s;
+                const float sc = JACOBI_CORRECTION_CAP / ml;
+                dx = dx * sc;
+                dy = dy * sc;
+                dz = dz * sc;
             }
             out = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, wi);
         }
@@ -202,7 +212,11 @@ __global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
             a.dst[i] = out;
         }
     }
-    if (MODE & JM_AABB) block_aabb(valid, out.x, out.y, out.z, a.aabb);
+    if (MODE & JM_AABB) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        aabb_fold(valid, out.x, out.y, out.z, lo, hi);
+        block_aabb_store(lo, hi, a.aabb, tile);
+    }
 }
 
 // Substep end without a collision pass when the last iterate is in F.
@@ -218,17 +232,13 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
-__global__ void k_aabb_reset(uint32_t* aabb) {
-    const uint32_t t = threadIdx.x;
-    if (t < 3) aabb[t] = f2o(INFINITY);
-    else if (t < 6) aabb[t] = f2o(-INFINITY);
-}
-
-__global__ void k_aabb_of(const float4* P, uint32_t n, uint32_t* aabb) {
+__global__ void k_aabb_of(const float4* P, uint32_t n, float* part) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
-    float4 v = valid ? P[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    block_aabb(valid, v.x, v.y, v.z, aabb);
+    const float4 v = valid ? P[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    aabb_fold(valid, v.x, v.y, v.z, lo, hi);
+    block_aabb_store(lo, hi, part, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -238,14 +248,35 @@ __global__ void k_aabb_of(const float4* P, uint32_t n, uint32_t* aabb) {
 // bits.  Key = ix + gx * (iy + gy * iz), dense over the current bounds, so no
 // two cells share a key.  Mirrors oracle_cell_grid / oracle_cell_keys.
 // ---------------------------------------------------------------------------
-__global__ void k_coll_grid(uint32_t* aabb, CollGrid* g, float cell_size) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    float lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = o2f(aabb[a]);
-        hi[a] = o2f(aabb[3 + a]);
+__global__ __launch_bounds__(1024) void k_coll_grid(const float* part, uint32_t nparts, CollGrid* g,
+                                                     float cell_size) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], part[size_t(k) * 8 + a]);
+            hi[a] = fmaxf(hi[a], part[size_t(k) * 8 + 3 + a]);
+        }
+    __shared__ float red[16][6];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+            hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int a = 0; a < 3; ++a) {
+            red[threadIdx.x >> 6][a] = lo[a];
+            red[threadIdx.x >> 6][3 + a] = hi[a];
+        }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (uint32_t w = 1; w < (blockDim.x >> 6); ++w)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], red[w][a]);
+            hi[a] = fmaxf(hi[a], red[w][3 + a]);
+        }
+    for (int a = 0; a < 3; ++a)
         if (!(lo[a] <= hi[a])) lo[a] = hi[a] = 0.f;
-    }
     float cell = cell_size * 1.015625f;
     float inv = 1.0f / cell;
     uint32_t dims[3] = {1, 1, 1};
@@ -277,10 +308,6 @@ __global__ void k_coll_grid(uint32_t* aabb, CollGrid* g, float cell_size) {
     g->cell = cell;
     g->inv_cell = inv;
     g->pad = 0;
-    for (int a = 0; a < 3; ++a) {   // ready for the next substep's bounds
-        aabb[a] = f2o(INFINITY);
-        aabb[3 + a] = f2o(-INFINITY);
-    }
 }
 
 __device__ __forceinline__ void cell_of(float4 p, const CollGrid& g, uint32_t c[3]) {
@@ -330,28 +357,39 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const uint32_t* keys
     hist[t * ntiles + blockIdx.x] = h[t];
 }
 
-// Exclusive scan of `total` words in place, one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t* data, uint32_t total) {
-    __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x;
-    const uint32_t chunk = (total + 1023) / 1024;
-    const uint32_t b = t * chunk, e = min(b + chunk, total);
-    uint32_t s = 0;
-    for (uint32_t k = b; k < e; ++k) s += data[k];
-    part[t] = s;
+// Exclusive scan of 256 values held one per thread (block of 256).
+__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* lds, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off, 64);
+        if (lane >= uint32_t(off)) incl += t;
+    }
+    if (lane == 63) lds[wave] = incl;
     __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = t >= off ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; ++w) before += lds[w];
+    total = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return before + incl - v;
+}
+
+// Per digit (one block each): exclusive prefix of the per-tile counts over
+// tiles, in place; the digit's total goes to dtot[d].
+__global__ __launch_bounds__(256) void k_radix_scan_tiles(uint32_t* hist, uint32_t ntiles, uint32_t* dtot) {
+    __shared__ uint32_t lds[4];
+    uint32_t* row = hist + size_t(blockIdx.x) * ntiles;
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < ntiles; b += 256) {
+        const uint32_t k = b + threadIdx.x;
+        const uint32_t v = k < ntiles ? row[k] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan_256(v, lds, tot);
+        if (k < ntiles) row[k] = carry + ex;
+        carry += tot;
     }
-    uint32_t run = part[t] - s;
-    for (uint32_t k = b; k < e; ++k) {
-        const uint32_t v = data[k];
-        data[k] = run;
-        run += v;
-    }
+    if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
 }
 
 // Stable scatter: within a tile, elements are ranked in input order (round
@@ -360,12 +398,15 @@ __global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t* data, uint32_
 __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* kin, const uint32_t* iin,
                                                              uint32_t* kout, uint32_t* iout, uint32_t n,
                                                              uint32_t shift, const uint32_t* hist,
-                                                             uint32_t ntiles) {
+                                                             uint32_t ntiles, const uint32_t* dtot) {
     constexpr uint32_t NW = RADIX_BLOCK / 64;
     __shared__ uint32_t base[256];
     __shared__ uint32_t wcnt[NW][256];
+    __shared__ uint32_t lds[4];
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    base[t] = hist[t * ntiles + blockIdx.x];
+    uint32_t tot;
+    const uint32_t digit_base = block_exclusive_scan_256(dtot[t], lds, tot);
+    base[t] = digit_base + hist[t * ntiles + blockIdx.x];
     for (uint32_t w = 0; w < NW; ++w) wcnt[w][t] = 0;
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -403,17 +444,21 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
     }
 }
 
-// Hash table cell -> first sorted index (keys are unique at run starts).
+// Hash table: group of 4 consecutive x-cells (key >> 2) -> first sorted
+// index of the group.  A 3-cell row [kmin, kmax] spans one or two groups, so
+// one probe (rarely two) finds where its particles start; one insert per
+// occupied group keeps the atomic traffic ~4x below one per cell.
+constexpr uint32_t GROUP_SHIFT = 2;
 __device__ __forceinline__ uint32_t slot_of(uint32_t key, uint32_t mask) { return (key * 2654435761u) & mask; }
 
 __global__ void k_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const uint32_t key = skeys[s];
-    if (s > 0 && skeys[s - 1] == key) return;
-    uint32_t h = slot_of(key, mask);
+    const uint32_t g = skeys[s] >> GROUP_SHIFT;
+    if (s > 0 && (skeys[s - 1] >> GROUP_SHIFT) == g) return;
+    uint32_t h = slot_of(g, mask);
     for (uint32_t probe = 0; probe <= mask; ++probe) {
-        const uint32_t prev = atomicCAS(&table[h].x, EMPTY_KEY, key);
+        const uint32_t prev = atomicCAS(&table[h].x, EMPTY_KEY, g);
         if (prev == EMPTY_KEY) {
             table[h].y = s;
             return;
@@ -422,11 +467,11 @@ __global__ void k_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, ui
     }
 }
 
-__device__ __forceinline__ uint32_t table_find(const uint2* table, uint32_t mask, uint32_t key) {
-    uint32_t h = slot_of(key, mask);
+__device__ __forceinline__ uint32_t table_find(const uint2* table, uint32_t mask, uint32_t g) {
+    uint32_t h = slot_of(g, mask);
     for (uint32_t probe = 0; probe <= mask; ++probe) {
         const uint2 e = table[h];
-        if (e.x == key) return e.y;
+        if (e.x == g) return e.y;
         if (e.x == EMPTY_KEY) return EMPTY_KEY;
         h = (h + 1) & mask;
     }
@@ -434,19 +479,19 @@ __device__ __forceinline__ uint32_t table_find(const uint2* table, uint32_t mask
 }
 
 // Pair kernel of self_collision_resolve (xpbd_cpu.rs:84-121) for the pair
-// (min(i,j), max(i,j)); returns the contribution to particle i.
-__device__ __forceinline__ bool collision_contrib(uint32_t i, uint32_t j, float4 fi, float4 fj, float4 ri,
-                                                  float4 rj, const DevParams& p, float& cx, float& cy,
-                                                  float& cz) {
+// (min(i,j), max(i,j)); returns the contribution to particle i.  The rest
+// positions are read only for pairs inside the contact distance.
+__device__ __forceinline__ bool collision_contrib(uint32_t i, uint32_t j, float4 fi, float4 fj, const float4* rest4,
+                                                  const DevParams& p, float& cx, float& cy, float& cz) {
     const bool i_lo = i < j;
     const float4 pa = i_lo ? fi : fj, pb = i_lo ? fj : fi;
-    const float4 ra = i_lo ? ri : rj, rb = i_lo ? rj : ri;
     const float wa = pa.w, wb = pb.w;
     if (wa <= 0.f && wb <= 0.f) return false;
     const float th2 = p.thickness * p.thickness;
     float dx = pb.x - pa.x, dy = pb.y - pa.y, dz = pb.z - pa.z;
     const float dist2 = (dx * dx + dy * dy) + dz * dz;
     if (dist2 > th2 || dist2 < 1e-18f) return false;
+    const float4 ra = rest4[i_lo ? i : j], rb = rest4[i_lo ? j : i];
     const float ex = rb.x - ra.x, ey = rb.y - ra.y, ez = rb.z - ra.z;
     const float rest2 = (ex * ex + ey * ey) + ez * ez;
     if (dist2 > rest2) return false;
@@ -472,11 +517,43 @@ __device__ __forceinline__ bool collision_contrib(uint32_t i, uint32_t j, float4
 }
 
 // Narrow phase + apply + post_velocity, one particle per thread in sorted
-// order.  Contributions are added in ascending partner id — the order the
-// reference's O(N^2) loop adds them (xpbd_cpu.rs:82-123) — by repeatedly
-// selecting the smallest qualifying partner above the last one.  The 3x3
-// rows of neighbouring cells are contiguous runs of the sorted keys, found
-// once through the hash table.
+// order.  Contributions must be added in ascending partner id — the order the
+// reference's O(N^2) loop adds them (xpbd_cpu.rs:82-123).  One pass over the
+// 3x3 neighbour rows (each a contiguous run of the sorted keys) keeps up to
+// CMAX contacts in a list sorted by partner id (registers); a particle with
+// more contacts falls back to repeated selection of the next-smallest partner.
+constexpr int CMAX = 4;
+
+struct RowRange {
+    uint32_t start, kmin, kmax;
+};
+
+__device__ __forceinline__ void sweep_rows(const CollArgs& a, const CollGrid& g, const uint32_t c[3],
+                                           RowRange rows[9]) {
+    uint32_t g0[9], g1[9], st[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        rows[r].start = EMPTY_KEY;
+        rows[r].kmin = 1u;
+        rows[r].kmax = 0u;
+        g0[r] = g1[r] = EMPTY_KEY;
+        const int ny = int(c[1]) + (r % 3) - 1, nz = int(c[2]) + (r / 3) - 1;
+        if (ny < 0 || nz < 0 || ny >= int(g.dims[1]) || nz >= int(g.dims[2])) continue;
+        const uint32_t kb = g.dims[0] * (uint32_t(ny) + g.dims[1] * uint32_t(nz));
+        rows[r].kmin = kb + (c[0] > 0 ? c[0] - 1 : 0);
+        rows[r].kmax = kb + (c[0] + 1 < g.dims[0] ? c[0] + 1 : c[0]);
+        g0[r] = rows[r].kmin >> GROUP_SHIFT;
+        g1[r] = rows[r].kmax >> GROUP_SHIFT;
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) st[r] = g0[r] != EMPTY_KEY ? table_find(a.table, a.table_mask, g0[r]) : EMPTY_KEY;
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+        if (st[r] == EMPTY_KEY && g1[r] != g0[r] && g1[r] != EMPTY_KEY) st[r] = table_find(a.table, a.table_mask, g1[r]);
+#pragma unroll
+    for (int r = 0; r < 9; ++r) rows[r].start = st[r];
+}
+
 __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     const uint32_t n = a.p.n;
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -490,53 +567,86 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     const CollGrid g = *a.grid;
     uint32_t c[3];
     cell_of(fi, g, c);
-    uint32_t rs[9], rk[9];
+    RowRange rows[9];
+    sweep_rows(a, g, c, rows);
+
+    // one pass: contacts kept sorted by partner id
+    uint32_t lj[CMAX];
+    float lx[CMAX], ly[CMAX], lz[CMAX];
+#pragma unroll
+    for (int q = 0; q < CMAX; ++q) {
+        lj[q] = EMPTY_KEY;
+        lx[q] = ly[q] = lz[q] = 0.f;
+    }
+    int count = 0;
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-        rs[r] = EMPTY_KEY;
-        rk[r] = 0;
-        const int ny = int(c[1]) + (r % 3) - 1, nz = int(c[2]) + (r / 3) - 1;
-        if (ny < 0 || nz < 0 || ny >= int(g.dims[1]) || nz >= int(g.dims[2])) continue;
-        const uint32_t kb = g.dims[0] * (uint32_t(ny) + g.dims[1] * uint32_t(nz));
-        const uint32_t kmin = kb + (c[0] > 0 ? c[0] - 1 : 0);
-        const uint32_t kmax = kb + (c[0] + 1 < g.dims[0] ? c[0] + 1 : c[0]);
-        for (uint32_t k = kmin; k <= kmax; ++k) {
-            const uint32_t st = table_find(a.table, a.table_mask, k);
-            if (st != EMPTY_KEY) {
-                rs[r] = st;
-                break;
-            }
-        }
-        rk[r] = kmax;
-    }
-    const float4 ri = a.rest4[i];
-    float ax = 0.f, ay = 0.f, az = 0.f;
-    uint32_t last = 0;
-    bool has_last = false;
-    for (;;) {
-        uint32_t best = EMPTY_KEY;
-        float bx = 0.f, by = 0.f, bz = 0.f;
+        if (rows[r].start == EMPTY_KEY) continue;
+        for (uint32_t idx = rows[r].start; idx < n; ++idx) {
+            const uint32_t key = a.skeys[idx];
+            if (key > rows[r].kmax) break;
+            if (key < rows[r].kmin) continue;
+            const uint32_t j = a.sids[idx];
+            if (j == i) continue;
+            float cx, cy, cz;
+            if (!collision_contrib(i, j, fi, a.F[j], a.rest4, a.p, cx, cy, cz)) continue;
+            ++count;
+            if (count > CMAX) continue;
+            // insert at the end, then bubble down to keep ascending partner ids
+            lj[CMAX - 1] = j;
+            lx[CMAX - 1] = cx;
+            ly[CMAX - 1] = cy;
+            lz[CMAX - 1] = cz;
 #pragma unroll
-        for (int r = 0; r < 9; ++r) {
-            if (rs[r] == EMPTY_KEY) continue;
-            for (uint32_t idx = rs[r]; idx < n && a.skeys[idx] <= rk[r]; ++idx) {
-                const uint32_t j = a.sids[idx];
-                if (j == i || j >= best || (has_last && j <= last)) continue;
-                float cx, cy, cz;
-                if (collision_contrib(i, j, fi, a.F[j], ri, a.rest4[j], a.p, cx, cy, cz)) {
-                    best = j;
-                    bx = cx;
-                    by = cy;
-                    bz = cz;
+            for (int q = CMAX - 1; q > 0; --q) {
+                if (lj[q] < lj[q - 1]) {
+                    uint32_t tj = lj[q]; lj[q] = lj[q - 1]; lj[q - 1] = tj;
+                    float t = lx[q]; lx[q] = lx[q - 1]; lx[q - 1] = t;
+                    t = ly[q]; ly[q] = ly[q - 1]; ly[q - 1] = t;
+                    t = lz[q]; lz[q] = lz[q - 1]; lz[q - 1] = t;
                 }
             }
         }
-        if (best == EMPTY_KEY) break;
-        ax = ax + bx;
-        ay = ay + by;
-        az = az + bz;
-        last = best;
-        has_last = true;
+    }
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    if (count <= CMAX) {
+#pragma unroll
+        for (int q = 0; q < CMAX; ++q)
+            if (q < count) {
+                ax = ax + lx[q];
+                ay = ay + ly[q];
+                az = az + lz[q];
+            }
+    } else {
+        uint32_t last = 0;
+        bool has_last = false;
+        for (;;) {
+            uint32_t best = EMPTY_KEY;
+            float bx = 0.f, by = 0.f, bz = 0.f;
+            for (int r = 0; r < 9; ++r) {
+                if (rows[r].start == EMPTY_KEY) continue;
+                for (uint32_t idx = rows[r].start; idx < n; ++idx) {
+                    const uint32_t key = a.skeys[idx];
+                    if (key > rows[r].kmax) break;
+                    if (key < rows[r].kmin) continue;
+                    const uint32_t j = a.sids[idx];
+                    if (j == i || j >= best || (has_last && j <= last)) continue;
+                    float cx, cy, cz;
+                    if (collision_contrib(i, j, fi, a.F[j], a.rest4, a.p, cx, cy, cz)) {
+                        best = j;
+                        bx = cx;
+                        by = cy;
+                        bz = cz;
+                    }
+                }
+            }
+            if (best == EMPTY_KEY) break;
+            ax = ax + bx;
+            ay = ay + by;
+            az = az + bz;
+            last = best;
+            has_last = true;
+        }
     }
     auto cl = [](float v) {
         if (v < -COLLISION_APPLY_CLAMP) v = -COLLISION_APPLY_CLAMP;
@@ -545,6 +655,13 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     };
     const float4 nx = make_float4(fi.x + cl(ax), fi.y + cl(ay), fi.z + cl(az), fi.w);
     finish_particle(nx, a.prev[i], i, a.ground, a.p, a.xout, a.vout);
+}
+
+// Standalone predict (xpbd_cpu.rs:251-264): P = predict(X, V).
+__global__ void k_predict(const float4* X, const float4* V, float4* P, DevParams p) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    P[i] = predict(X[i], V[i], i, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -630,14 +747,12 @@ __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
         }
     } else {
         // bounds of the final iterate for the collision pass that follows
-        const uint32_t rounds = (p.n + nthreads - 1) / nthreads;
-        for (uint32_t r = 0; r < rounds; ++r) {
-            const uint32_t i = r * nthreads + gt;
-            const bool valid = i < p.n;
-            const float4 w = valid ? a.W[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-            block_aabb(valid, w.x, w.y, w.z, a.aabb);
-            __syncthreads();
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = gt; i < p.n; i += nthreads) {
+            const float4 w = a.W[i];
+            aabb_fold(true, w.x, w.y, w.z, lo, hi);
         }
+        block_aabb_store(lo, hi, a.aabb, blockIdx.x);
     }
 }
 
@@ -691,10 +806,11 @@ __global__ __launch_bounds__(256) void k_residual(const float4* X, const uint2* 
 // ---------------------------------------------------------------------------
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-void launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s) {
-    const uint32_t spb = block / 64;
-    const uint32_t blocks = cdiv(a.p.nslices, spb);
-    if (blocks == 0) return;
+uint32_t jacobi_blocks(uint32_t nslices, uint32_t block) { return cdiv(nslices, block / 64); }
+
+uint32_t launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s) {
+    const uint32_t blocks = jacobi_blocks(a.p.nslices, block);
+    if (blocks == 0) return 0;
     switch (mode) {
         case 0: hipLaunchKernelGGL(k_jacobi<0>, dim3(blocks), dim3(block), 0, s, a); break;
         case JM_PREDICT: hipLaunchKernelGGL(k_jacobi<JM_PREDICT>, dim3(blocks), dim3(block), 0, s, a); break;
@@ -705,6 +821,12 @@ void launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream
             break;
         default: break;
     }
+    return blocks;
+}
+
+void launch_predict(const float4* X, const float4* V, float4* P, DevParams p, hipStream_t s) {
+    if (p.n == 0) return;
+    hipLaunchKernelGGL(k_predict, dim3(cdiv(p.n, 256)), dim3(256), 0, s, X, V, P, p);
 }
 
 void launch_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p, hipStream_t s) {
@@ -716,15 +838,14 @@ void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_u32, dim3(std::max(1u, std::min(cdiv(n, 256), 1024u))), dim3(256), 0, s, p, n);
 }
 
-void launch_aabb_reset(uint32_t* aabb, hipStream_t s) { hipLaunchKernelGGL(k_aabb_reset, dim3(1), dim3(64), 0, s, aabb); }
-
-void launch_aabb_of(const float4* P, uint32_t n, uint32_t* aabb, hipStream_t s) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_aabb_of, dim3(cdiv(n, 256)), dim3(256), 0, s, P, n, aabb);
+uint32_t launch_aabb_of(const float4* P, uint32_t n, float* part, hipStream_t s) {
+    const uint32_t blocks = std::max(1u, cdiv(n, 256));
+    hipLaunchKernelGGL(k_aabb_of, dim3(blocks), dim3(256), 0, s, P, n, part);
+    return blocks;
 }
 
-void launch_coll_grid(uint32_t* aabb, CollGrid* grid, float cell_size, hipStream_t s) {
-    hipLaunchKernelGGL(k_coll_grid, dim3(1), dim3(64), 0, s, aabb, grid, cell_size);
+void launch_coll_grid(const float* part, uint32_t nparts, CollGrid* grid, float cell_size, hipStream_t s) {
+    hipLaunchKernelGGL(k_coll_grid, dim3(1), dim3(1024), 0, s, part, nparts, grid, cell_size);
 }
 
 void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grid, uint32_t* keys, uint32_t* ids,
@@ -734,15 +855,16 @@ void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grid, uint32_
                        grid, keys, ids, table, slots);
 }
 
-uint32_t radix_hist_words(uint32_t n) { return 256u * std::max(1u, cdiv(n, RADIX_TILE)); }
+uint32_t radix_hist_words(uint32_t n) { return 256u * std::max(1u, cdiv(n, RADIX_TILE)) + 256u; }
 
 void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
                        uint32_t shift, uint32_t* hist, hipStream_t s) {
     const uint32_t ntiles = std::max(1u, cdiv(n, RADIX_TILE));
+    uint32_t* dtot = hist + 256u * ntiles;
     hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, n, shift, hist, ntiles);
-    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, hist, 256u * ntiles);
+    hipLaunchKernelGGL(k_radix_scan_tiles, dim3(256), dim3(256), 0, s, hist, ntiles, dtot);
     hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, iin, kout, iout, n, shift, hist,
-                       ntiles);
+                       ntiles, dtot);
 }
 
 void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,

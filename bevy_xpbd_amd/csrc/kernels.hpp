@@ -1,4 +1,4 @@
-// Launch interface between the host runtime (cloth_compute.cpp) and the HIP
+// Launch interface between the host runtime (xpbd_api.cpp) and the HIP
 // kernels (kernels.hip).  Plain structs passed by value as kernel arguments.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -46,7 +46,7 @@ struct JacobiArgs {
     float4* dst;         // iterate out (JM_FINAL: X, in place)
     float4* vout;        // JM_FINAL: V
     const uint8_t* ground;
-    uint32_t* aabb;      // JM_AABB: 6 ordered-uint words
+    float* aabb;         // JM_AABB: one (lo xyz, hi xyz) partial per block, stride 8
     Sell sell;
     DevParams p;
     uint32_t remap;      // 1: XCD-aware block order
@@ -82,7 +82,7 @@ struct GsArgs {
     float4* xout;        // FINAL: X
     float4* vout;        // FINAL: V
     const uint8_t* ground;
-    uint32_t* aabb;
+    float* aabb;         // one bounds partial per block (collision follows)
     const uint2* ij;     // colour-sorted (i, j)
     const float2* rc;    // colour-sorted (rest, compliance)
     const uint32_t* batch_offsets;
@@ -95,11 +95,12 @@ struct GsArgs {
 };
 
 // ---- launchers (kernels.hip) ----
-void launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s);
+uint32_t jacobi_blocks(uint32_t nslices, uint32_t block);
+uint32_t launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s);
+void launch_predict(const float4* X, const float4* V, float4* P, DevParams p, hipStream_t s);
 void launch_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p,
                      hipStream_t s);
-void launch_aabb_reset(uint32_t* aabb, hipStream_t s);
-void launch_coll_grid(uint32_t* aabb, CollGrid* grid, float cell_size, hipStream_t s);
+void launch_coll_grid(const float* part, uint32_t nparts, CollGrid* grid, float cell_size, hipStream_t s);
 void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grid, uint32_t* keys, uint32_t* ids,
                       uint2* table, uint32_t table_slots, hipStream_t s);
 void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,
@@ -116,6 +117,6 @@ void launch_publish(const float4* X, uint32_t n, const uint32_t* tris, const uin
                     const uint32_t* vt_tri, float4* dpos, float4* dnrm, hipStream_t s);
 void launch_residual(const float4* X, const uint2* ij, const float2* rc, uint32_t e, double* out,
                      hipStream_t s);
-void launch_aabb_of(const float4* P, uint32_t n, uint32_t* aabb, hipStream_t s);
+uint32_t launch_aabb_of(const float4* P, uint32_t n, float* part, hipStream_t s);
 
 }  // namespace xpbd

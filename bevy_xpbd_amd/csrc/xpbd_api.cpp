@@ -69,7 +69,8 @@ struct xpbd_cloth {
     uint32_t *keys = nullptr, *ids = nullptr, *keys2 = nullptr, *ids2 = nullptr, *hist = nullptr;
     uint2* table = nullptr;
     uint32_t table_slots = 0;
-    uint32_t* aabb = nullptr;
+    float* aabb = nullptr;         // bounds partials, 8 floats per producing block
+    uint32_t aabb_slots = 0;
     CollGrid* grid = nullptr;
     uint32_t *tris = nullptr, *vt_off = nullptr, *vt_tri = nullptr;
     uint32_t *barrier = nullptr, *err = nullptr;
@@ -81,7 +82,7 @@ struct xpbd_cloth {
     hipGraphExec_t gexec = nullptr;
     bool graph_valid = false;
 
-    uint32_t jacobi_block = 256, remap = 1;
+    uint32_t jacobi_block = 256, remap = 1, fused_predict = 0;
     uint32_t gs_block = 256, gs_blocks = 0;
 
     std::string msg;
@@ -166,9 +167,10 @@ float coll_cell(const xpbd_cloth* h) {
 // Collision chain on iterate F: grid, keys, sort, table, sweep (which also
 // applies the ground plane and the velocity update).  Replaces the
 // reference's clear_atomics .. collide_apply sequence (cloth_jacobi.rs:774-868).
-void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, float4* xout, float4* vout) {
+void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint32_t nparts, float4* xout,
+                       float4* vout) {
     hipStream_t s = h->stream;
-    launch_coll_grid(h->aabb, h->grid, coll_cell(h), s);
+    launch_coll_grid(h->aabb, nparts, h->grid, coll_cell(h), s);
     launch_coll_keys(F, h->n, h->grid, h->keys, h->ids, h->table, h->table_slots, s);
     launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
     launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, s);
@@ -195,6 +197,7 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
     const DevParams p = dev_params(h);
     const uint32_t iters = std::max(h->cfg.solve_inner_iterations, 1u);
     const float4* F = nullptr;
+    uint32_t nparts = 0;
     if (h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL) {
         launch_zero_u32(h->barrier, 1, s);
         GsArgs g{};
@@ -216,13 +219,37 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         g.p = p;
         launch_gs(g, h->gs_blocks, h->gs_block, s);
         F = h->A;
+        nparts = h->gs_blocks;
+    } else if (!h->fused_predict) {
+        // predict -> A, then iterations ping-pong A/B; the last one writes the
+        // state in place (no collision) or the iterate F the collision reads.
+        JacobiArgs a = jacobi_args(h, p);
+        launch_predict(h->X, h->V, h->A, p, s);
+        float4* cur = h->A;
+        float4* nxt = h->B;
+        for (uint32_t k = 0; k < iters; ++k) {
+            a.src = cur;
+            if (k + 1 < iters) {
+                a.dst = nxt;
+                launch_jacobi(0, a, h->jacobi_block, s);
+                std::swap(cur, nxt);
+            } else if (collision) {
+                a.dst = nxt;
+                nparts = launch_jacobi(JM_AABB, a, h->jacobi_block, s);
+                F = nxt;
+            } else {
+                a.dst = h->X;
+                a.vout = h->V;
+                launch_jacobi(JM_FINAL, a, h->jacobi_block, s);
+            }
+        }
     } else {
         JacobiArgs a = jacobi_args(h, p);
         if (iters == 1) {
             a.src = h->X;
             a.vel = h->V;
             a.dst = h->A;
-            launch_jacobi(collision ? (JM_PREDICT | JM_AABB) : JM_PREDICT, a, h->jacobi_block, s);
+            nparts = launch_jacobi(collision ? (JM_PREDICT | JM_AABB) : JM_PREDICT, a, h->jacobi_block, s);
             if (!collision) launch_finalize(h->A, h->X, h->V, a.ground, p, s);
             F = h->A;
         } else {
@@ -241,7 +268,7 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
             a.src = cur;
             if (collision) {
                 a.dst = nxt;
-                launch_jacobi(JM_AABB, a, h->jacobi_block, s);
+                nparts = launch_jacobi(JM_AABB, a, h->jacobi_block, s);
                 F = nxt;
             } else {
                 a.dst = h->X;
@@ -250,7 +277,7 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
             }
         }
     }
-    if (collision) enqueue_collision(h, p, F, h->X, h->V);
+    if (collision) enqueue_collision(h, p, F, nparts, h->X, h->V);
 }
 
 bool collision_this_substep(const xpbd_cloth* h, uint32_t si) {
@@ -404,9 +431,10 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(dalloc(h, &h->keys2, n));
     TRY(dalloc(h, &h->ids2, n));
     TRY(dalloc(h, &h->hist, radix_hist_words(uint32_t(n))));
-    h->table_slots = next_pow2(uint32_t(std::max<size_t>(2 * n, 64)));
+    h->table_slots = next_pow2(uint32_t(std::max<size_t>(n, 64)));   // one entry per 4-cell group
     TRY(dalloc(h, &h->table, h->table_slots));
-    TRY(dalloc(h, &h->aabb, 8));
+    h->aabb_slots = std::max<uint32_t>({h->nslices, (h->n + 255) / 256, 4096u, 1u});
+    TRY(dalloc(h, &h->aabb, size_t(h->aabb_slots) * 8));
     TRY(dalloc(h, &h->grid, 1));
 
     // vertex -> incident triangles (ascending) for the normals gather
@@ -430,8 +458,6 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(dalloc(h, &h->resid, 2));
     HIPC(h, hipMemset(h->err, 0, 16));
     HIPC(h, hipMemset(h->barrier, 0, 16));
-    launch_aabb_reset(h->aabb, h->stream);
-    HIPC(h, hipGetLastError());
     return XPBD_OK;
 }
 
@@ -448,6 +474,11 @@ int upload_state(xpbd_cloth* h, const float* pos4, const float* vel4) {
     TRY(upload(h, reinterpret_cast<float*>(h->X), x.data(), x.size()));
     TRY(upload(h, reinterpret_cast<float*>(h->V), v.data(), v.size()));
     return XPBD_OK;
+}
+
+void apply_tuning(xpbd_cloth* h) {
+    if (h->cfg.jacobi_block) h->jacobi_block = h->cfg.jacobi_block;
+    h->fused_predict = h->cfg.fused_predict ? 1u : 0u;
 }
 
 int autotune(xpbd_cloth* h) {
@@ -626,6 +657,8 @@ XPBD_API void xpbd_sim_config_default(xpbd_sim_config* out) {
     c.collision_cell = 0.f;
     c.use_graph = 1;
     c.autotune = 0;
+    c.jacobi_block = 0;
+    c.fused_predict = 0;
     *out = c;
 }
 
@@ -635,6 +668,7 @@ XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, cons
     if (!mesh || !cfg || !u || !out) return XPBD_E_INVALID;
     *out = nullptr;
     if (cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL) return XPBD_E_INVALID;
+    if (cfg->jacobi_block && (cfg->jacobi_block % 64 || cfg->jacobi_block > 512)) return XPBD_E_INVALID;
     std::string why;
     if (validate_mesh(mesh->m, why) != XPBD_OK) return XPBD_E_INVALID;
     int ndev = 0;
@@ -664,6 +698,7 @@ XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, cons
     if (r != XPBD_OK) return r;
     h->gs_blocks = std::min<uint32_t>(gs_max_blocks(h->gs_block), std::max(1u, (h->e + h->gs_block - 1) / h->gs_block));
     h->gs_blocks = std::max<uint32_t>(h->gs_blocks, 1u);
+    apply_tuning(h);
     if (cfg->autotune) {
         r = autotune(h);
         if (r != XPBD_OK) return r;
@@ -695,8 +730,10 @@ XPBD_API int xpbd_set_uniforms(xpbd_cloth* h, const xpbd_sim_uniforms* u) {
 
 XPBD_API int xpbd_set_config(xpbd_cloth* h, const xpbd_sim_config* cfg) {
     if (!h || !cfg || cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL) return XPBD_E_INVALID;
+    if (cfg->jacobi_block && (cfg->jacobi_block % 64 || cfg->jacobi_block > 512)) return XPBD_E_INVALID;
     const bool retune = cfg->autotune && !h->cfg.autotune;
     h->cfg = *cfg;
+    apply_tuning(h);
     drop_graph(h);
     if (retune) return autotune(h);
     return XPBD_OK;
@@ -810,9 +847,8 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
     a.src = h->A;
     a.dst = h->B;
     auto prep_collision = [&]() {
-        launch_aabb_reset(h->aabb, s);
-        launch_aabb_of(h->A, h->n, h->aabb, s);
-        launch_coll_grid(h->aabb, h->grid, coll_cell(h), s);
+        const uint32_t np = launch_aabb_of(h->A, h->n, h->aabb, s);
+        launch_coll_grid(h->aabb, np, h->grid, coll_cell(h), s);
         launch_coll_keys(h->A, h->n, h->grid, h->keys, h->ids, h->table, h->table_slots, s);
         launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
         launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, s);
@@ -884,7 +920,6 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
     HIPC(h, hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    launch_aabb_reset(h->aabb, s);
     HIPC(h, hipGetLastError());
     TRY(check_device_flag(h));
     *avg_ms = ms / float(reps);
@@ -895,9 +930,8 @@ XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t*
                                        uint32_t* sorted_ids) {
     if (!h || !grid_out || !sorted_keys || !sorted_ids || !(cell_size > 0.f)) return XPBD_E_INVALID;
     hipStream_t s = h->stream;
-    launch_aabb_reset(h->aabb, s);
-    launch_aabb_of(h->X, h->n, h->aabb, s);
-    launch_coll_grid(h->aabb, h->grid, cell_size, s);
+    const uint32_t np = launch_aabb_of(h->X, h->n, h->aabb, s);
+    launch_coll_grid(h->aabb, np, h->grid, cell_size, s);
     launch_coll_keys(h->X, h->n, h->grid, h->keys, h->ids, h->table, h->table_slots, s);
     launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
     HIPC(h, hipGetLastError());

@@ -130,6 +130,8 @@ typedef struct xpbd_sim_config {
     float collision_cell;                 /* hash cell side [m]; 0 = thickness */
     uint32_t use_graph;                   /* 1: replay frames from a hipGraph */
     uint32_t autotune;                    /* 1: time launch shapes at create */
+    uint32_t jacobi_block;                /* Jacobi workgroup size 64..512; 0 = 256 or autotuned */
+    uint32_t fused_predict;               /* 1: predict inside the first iteration */
 } xpbd_sim_config;
 
 /* ClothSimUniforms (cloth_compute.rs:182-202). */

@@ -21,17 +21,19 @@ def note(msg):
     sys.stdout.flush()
 
 
-def make(X, mesh, solver=0, S=8, I=10, coll_stride=1, use_graph=0, **uk):
+def make(X, mesh, solver=0, S=8, I=10, coll_stride=1, use_graph=0, fused_predict=0, jacobi_block=0, **uk):
     cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=I, solver=solver,
-                                   collision_every_n_substeps=coll_stride, use_graph=use_graph)
+                                   collision_every_n_substeps=coll_stride, use_graph=use_graph,
+                                   fused_predict=fused_predict, jacobi_block=jacobi_block)
     u = X.ClothSimUniforms.default(solver, S, **uk)
     return X.ClothComputePlugin(mesh, cfg, u), cfg, u
 
 
-def run_pair(O, X, mesh_kw=None, mesh=None, solver=0, S=8, I=10, substeps=8, collision=False, **uk):
+def run_pair(O, X, mesh_kw=None, mesh=None, solver=0, S=8, I=10, substeps=8, collision=False, fused_predict=0,
+             jacobi_block=0, **uk):
     mesh = mesh or X.ClothMeshData.sheet(**mesh_kw)
     f = mesh.fields()
-    sim, cfg, u = make(X, mesh, solver, S, I, **uk)
+    sim, cfg, u = make(X, mesh, solver, S, I, fused_predict=fused_predict, jacobi_block=jacobi_block, **uk)
     s = O.Solver(f)
     p = oracle_params(O, X, u, cfg, collision=collision)
     pos = f["positions"].copy()
@@ -55,10 +57,43 @@ def test_config1_sheet_bit_exact_per_substep(O, X, solver):
              S=8, I=10, substeps=16)
 
 
+@pytest.mark.parametrize("fused", [0, 1])
 @pytest.mark.parametrize("iters", [1, 2, 3])
-def test_jacobi_iteration_count_edge_cases(O, X, iters):
-    note(f"case iters={iters}")
-    run_pair(O, X, dict(cols=20, rows=9, spacing=0.05, pins=2, jitter=1e-3, seed=2), S=8, I=iters, substeps=6)
+def test_jacobi_iteration_count_edge_cases(O, X, iters, fused):
+    note(f"case iters={iters} fused_predict={fused}")
+    for coll in (False, True):
+        run_pair(O, X, dict(cols=20, rows=9, spacing=0.05, pins=2, jitter=1e-3, seed=2), S=8, I=iters,
+                 substeps=4, collision=coll, fused_predict=fused)
+
+
+@pytest.mark.parametrize("block", [64, 128, 512])
+def test_jacobi_block_sizes(O, X, block):
+    note(f"case block={block}")
+    run_pair(O, X, dict(cols=70, rows=33, spacing=0.03, pins=2, jitter=1e-3, seed=8), S=8, I=5, substeps=3,
+             collision=True, jacobi_block=block)
+
+
+def test_collision_dense_cluster_overflow_path(O, X):
+    """A sheet squeezed to a tenth of its size: every particle touches many
+    partners, exercising the >4-contact selection path of the sweep."""
+    note("case dense cluster")
+    mesh = X.ClothMeshData.sheet(cols=24, rows=24, spacing=0.01, pins=0, jitter=1e-3, seed=11)
+    f = mesh.fields()
+    sim, cfg, u = make(X, mesh, 0, S=8, I=2, thickness=0.004)
+    pos = f["positions"].copy()
+    pos *= np.float32(0.1)
+    sim.write_state(pos)
+    sim.step_substeps(1, True)
+    gp, gv = sim.read_state()
+    p = oracle_params(O, X, u, cfg, collision=True)
+    s = O.Solver(f)
+    op, ov = s.substep(pos, np.zeros_like(pos), p)
+    assert bits_equal(gp, op) and bits_equal(gv, ov), max_abs(gp, op)
+    # contacts per particle in the pre-collision iterate exceed the 4-entry list
+    p0 = O.params(substeps=8, inner_iterations=2, omega=float(u.jacobi_omega), thickness=0.004)
+    q, _ = s.substep(pos, np.zeros_like(pos), p0)
+    d = np.linalg.norm(q[:, None, :] - q[None, :, :], axis=2)
+    assert (np.sum(d <= 0.004, axis=1) - 1).max() > 4
 
 
 @pytest.mark.parametrize("solver", [0, 1])
