@@ -82,7 +82,7 @@ class ClothSimConfig(C.Structure):
         ("solve_substeps", C.c_uint32), ("solve_inner_iterations", C.c_uint32),
         ("collision_every_n_substeps", C.c_uint32), ("solver", C.c_uint32),
         ("collision_cell", C.c_float), ("use_graph", C.c_uint32), ("autotune", C.c_uint32),
-        ("jacobi_block", C.c_uint32), ("fused_predict", C.c_uint32),
+        ("jacobi_block", C.c_uint32), ("fused_predict", C.c_uint32), ("plain_adjacency", C.c_uint32),
     ]
 
     @classmethod

@@ -150,7 +150,7 @@ __device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float rest, f
 // keeps several dependent load chains in flight instead of one.
 constexpr uint32_t JU = 4;
 
-template <uint32_t MODE>
+template <uint32_t MODE, bool PACKED>
 __global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
     const uint32_t nb = gridDim.x;
     const uint32_t tile = a.remap ? xcd_tile(blockIdx.x, nb) : blockIdx.x;
@@ -175,13 +175,34 @@ __global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
                 uint32_t jj[JU];
                 float rr[JU], cc[JU];
                 float4 pp[JU];
+                if (PACKED) {
+                    uint32_t w[JU];
 #pragma unroll
-                for (uint32_t u = 0; u < JU; ++u) {
-                    const uint32_t k = k0 + u;
-                    const uint32_t e = base + k * 64;
-                    jj[u] = k < width ? __builtin_nontemporal_load(a.sell.j + e) : NO_ENTRY;
-                    rr[u] = k < width ? __builtin_nontemporal_load(a.sell.rest + e) : 0.f;
-                    cc[u] = k < width ? __builtin_nontemporal_load(a.sell.comp + e) : 0.f;
+                    for (uint32_t u = 0; u < JU; ++u) {
+                        const uint32_t k = k0 + u;
+                        w[u] = k < width ? __builtin_nontemporal_load(a.sell.packed + base + k * 64) : NO_ENTRY;
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < JU; ++u) {
+                        if (w[u] != NO_ENTRY) {
+                            jj[u] = uint32_t(int32_t(i) + int32_t(int16_t(w[u] & 0xFFFFu)));
+                            const float2 rc = a.sell.pairs[w[u] >> 16];
+                            rr[u] = rc.x;
+                            cc[u] = rc.y;
+                        } else {
+                            jj[u] = NO_ENTRY;
+                            rr[u] = cc[u] = 0.f;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (uint32_t u = 0; u < JU; ++u) {
+                        const uint32_t k = k0 + u;
+                        const uint32_t e = base + k * 64;
+                        jj[u] = k < width ? __builtin_nontemporal_load(a.sell.j + e) : NO_ENTRY;
+                        rr[u] = k < width ? __builtin_nontemporal_load(a.sell.rest + e) : 0.f;
+                        cc[u] = k < width ? __builtin_nontemporal_load(a.sell.comp + e) : 0.f;
+                    }
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < JU; ++u) {
@@ -808,19 +829,25 @@ static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 uint32_t jacobi_blocks(uint32_t nslices, uint32_t block) { return cdiv(nslices, block / 64); }
 
-uint32_t launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s) {
-    const uint32_t blocks = jacobi_blocks(a.p.nslices, block);
-    if (blocks == 0) return 0;
+template <bool PACKED>
+static void launch_jacobi_t(uint32_t mode, const JacobiArgs& a, uint32_t blocks, uint32_t block, hipStream_t s) {
     switch (mode) {
-        case 0: hipLaunchKernelGGL(k_jacobi<0>, dim3(blocks), dim3(block), 0, s, a); break;
-        case JM_PREDICT: hipLaunchKernelGGL(k_jacobi<JM_PREDICT>, dim3(blocks), dim3(block), 0, s, a); break;
-        case JM_FINAL: hipLaunchKernelGGL(k_jacobi<JM_FINAL>, dim3(blocks), dim3(block), 0, s, a); break;
-        case JM_AABB: hipLaunchKernelGGL(k_jacobi<JM_AABB>, dim3(blocks), dim3(block), 0, s, a); break;
+        case 0: hipLaunchKernelGGL((k_jacobi<0, PACKED>), dim3(blocks), dim3(block), 0, s, a); break;
+        case JM_PREDICT: hipLaunchKernelGGL((k_jacobi<JM_PREDICT, PACKED>), dim3(blocks), dim3(block), 0, s, a); break;
+        case JM_FINAL: hipLaunchKernelGGL((k_jacobi<JM_FINAL, PACKED>), dim3(blocks), dim3(block), 0, s, a); break;
+        case JM_AABB: hipLaunchKernelGGL((k_jacobi<JM_AABB, PACKED>), dim3(blocks), dim3(block), 0, s, a); break;
         case JM_PREDICT | JM_AABB:
-            hipLaunchKernelGGL(k_jacobi<JM_PREDICT | JM_AABB>, dim3(blocks), dim3(block), 0, s, a);
+            hipLaunchKernelGGL((k_jacobi<JM_PREDICT | JM_AABB, PACKED>), dim3(blocks), dim3(block), 0, s, a);
             break;
         default: break;
     }
+}
+
+uint32_t launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s) {
+    const uint32_t blocks = jacobi_blocks(a.p.nslices, block);
+    if (blocks == 0) return 0;
+    if (a.sell.packed) launch_jacobi_t<true>(mode, a, blocks, block, s);
+    else launch_jacobi_t<false>(mode, a, blocks, block, s);
     return blocks;
 }
 

@@ -32,6 +32,11 @@ struct Sell {
     const uint32_t* j;
     const float* rest;
     const float* comp;
+    // Packed form (used when every entry fits): one u32 per entry,
+    // (pair index << 16) | (uint16)(j - i), with pair = (rest, compliance)
+    // looked up in a table of the mesh's distinct pairs.  NO_ENTRY pads.
+    const uint32_t* packed;
+    const float2* pairs;
 };
 
 enum JacobiMode : uint32_t {

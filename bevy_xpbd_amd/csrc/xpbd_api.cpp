@@ -62,6 +62,9 @@ struct xpbd_cloth {
     float4 *X = nullptr, *V = nullptr, *A = nullptr, *B = nullptr, *V2 = nullptr, *rest4 = nullptr;
     uint32_t *soff = nullptr, *swid = nullptr, *nbj = nullptr;
     float *nbr = nullptr, *nbc = nullptr;
+    uint32_t* nbpk = nullptr;       // packed adjacency (nullptr: plain layout in use)
+    float2* pairs = nullptr;
+    uint32_t num_pairs = 0;
     uint8_t* ground = nullptr;
     uint2* gs_ij = nullptr;
     float2* gs_rc = nullptr;
@@ -152,7 +155,7 @@ DevParams dev_params(const xpbd_cloth* h) {
 
 JacobiArgs jacobi_args(const xpbd_cloth* h, const DevParams& p) {
     JacobiArgs a{};
-    a.sell = Sell{h->soff, h->swid, h->nbj, h->nbr, h->nbc};
+    a.sell = Sell{h->soff, h->swid, h->nbj, h->nbr, h->nbc, h->nbpk, h->pairs};
     a.ground = h->has_ground ? h->ground : nullptr;
     a.aabb = h->aabb;
     a.p = p;
@@ -347,6 +350,46 @@ void build_sell(const Mesh& m, std::vector<uint32_t>& soff, std::vector<uint32_t
     }
 }
 
+// Packed adjacency: (pair << 16) | (uint16)(j - i) per entry, pair indexing
+// the mesh's distinct (rest, compliance) values.  Returns false when a delta
+// does not fit in 16 bits or there are more than 65535 distinct pairs.
+bool pack_sell(const Mesh& m, const std::vector<uint32_t>& soff, std::vector<uint32_t>& packed,
+               std::vector<float2>& pairs) {
+    const size_t n = m.num_particles(), e = m.num_constraints();
+    std::vector<uint64_t> key(e);
+    for (size_t k = 0; k < e; ++k) {
+        uint32_t rb, cb;
+        std::memcpy(&rb, &m.constraint_rest[k], 4);
+        std::memcpy(&cb, &m.constraint_comp[k], 4);
+        key[k] = (uint64_t(rb) << 32) | cb;
+    }
+    std::vector<uint64_t> uniq(key);
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    if (uniq.size() > 65535) return false;
+    std::vector<uint16_t> pid(e);
+    for (size_t k = 0; k < e; ++k)
+        pid[k] = uint16_t(std::lower_bound(uniq.begin(), uniq.end(), key[k]) - uniq.begin());
+    pairs.resize(uniq.size());
+    for (size_t q = 0; q < uniq.size(); ++q) {
+        const uint32_t rb = uint32_t(uniq[q] >> 32), cb = uint32_t(uniq[q]);
+        std::memcpy(&pairs[q].x, &rb, 4);
+        std::memcpy(&pairs[q].y, &cb, 4);
+    }
+    packed.assign(soff.back(), NO_ENTRY);
+    for (size_t i = 0; i < n; ++i) {
+        const size_t s = i / 64, l = i % 64;
+        const uint32_t b = m.neighbor_offsets[i], d = m.neighbor_offsets[i + 1] - b;
+        for (uint32_t k = 0; k < d; ++k) {
+            const int64_t delta = int64_t(m.neighbor_other[b + k]) - int64_t(i);
+            if (delta < -32768 || delta > 32767) return false;
+            const uint32_t q = pid[m.neighbor_constraint_id[b + k]];
+            packed[soff[s] + size_t(k) * 64 + l] = (q << 16) | (uint32_t(delta) & 0xFFFFu);
+        }
+    }
+    return true;
+}
+
 int validate_mesh(const Mesh& m, std::string& why) {
     const size_t n = m.num_particles();
     if (m.positions.size() != 3 * n || m.rest_positions.size() != 3 * n) return why = "position arrays", XPBD_E_INVALID;
@@ -400,6 +443,17 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(upload(h, h->nbj, sj.data(), sj.size()));
     TRY(upload(h, h->nbr, sr.data(), sr.size()));
     TRY(upload(h, h->nbc, sc.data(), sc.size()));
+    if (!h->cfg.plain_adjacency) {
+        std::vector<uint32_t> pk;
+        std::vector<float2> pairs;
+        if (pack_sell(m, soff, pk, pairs)) {
+            TRY(dalloc(h, &h->nbpk, pk.size()));
+            TRY(dalloc(h, &h->pairs, pairs.size()));
+            TRY(upload(h, h->nbpk, pk.data(), pk.size()));
+            TRY(upload(h, h->pairs, pairs.data(), pairs.size()));
+            h->num_pairs = uint32_t(pairs.size());
+        }
+    }
 
     h->has_ground = std::any_of(m.ground.begin(), m.ground.end(), [](uint32_t g) { return g != 0; });
     TRY(dalloc(h, &h->ground, n));

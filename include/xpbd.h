@@ -132,6 +132,8 @@ typedef struct xpbd_sim_config {
     uint32_t autotune;                    /* 1: time launch shapes at create */
     uint32_t jacobi_block;                /* Jacobi workgroup size 64..512; 0 = 256 or autotuned */
     uint32_t fused_predict;               /* 1: predict inside the first iteration */
+    uint32_t plain_adjacency;             /* 1: 12-byte adjacency entries even when the
+                                             4-byte packed form fits (read at create) */
 } xpbd_sim_config;
 
 /* ClothSimUniforms (cloth_compute.rs:182-202). */

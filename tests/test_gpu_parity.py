@@ -21,19 +21,22 @@ def note(msg):
     sys.stdout.flush()
 
 
-def make(X, mesh, solver=0, S=8, I=10, coll_stride=1, use_graph=0, fused_predict=0, jacobi_block=0, **uk):
+def make(X, mesh, solver=0, S=8, I=10, coll_stride=1, use_graph=0, fused_predict=0, jacobi_block=0,
+         plain_adjacency=0, **uk):
     cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=I, solver=solver,
                                    collision_every_n_substeps=coll_stride, use_graph=use_graph,
-                                   fused_predict=fused_predict, jacobi_block=jacobi_block)
+                                   fused_predict=fused_predict, jacobi_block=jacobi_block,
+                                   plain_adjacency=plain_adjacency)
     u = X.ClothSimUniforms.default(solver, S, **uk)
     return X.ClothComputePlugin(mesh, cfg, u), cfg, u
 
 
 def run_pair(O, X, mesh_kw=None, mesh=None, solver=0, S=8, I=10, substeps=8, collision=False, fused_predict=0,
-             jacobi_block=0, **uk):
+             jacobi_block=0, plain_adjacency=0, **uk):
     mesh = mesh or X.ClothMeshData.sheet(**mesh_kw)
     f = mesh.fields()
-    sim, cfg, u = make(X, mesh, solver, S, I, fused_predict=fused_predict, jacobi_block=jacobi_block, **uk)
+    sim, cfg, u = make(X, mesh, solver, S, I, fused_predict=fused_predict, jacobi_block=jacobi_block,
+                       plain_adjacency=plain_adjacency, **uk)
     s = O.Solver(f)
     p = oracle_params(O, X, u, cfg, collision=collision)
     pos = f["positions"].copy()
@@ -57,13 +60,21 @@ def test_config1_sheet_bit_exact_per_substep(O, X, solver):
              S=8, I=10, substeps=16)
 
 
+@pytest.mark.parametrize("plain", [0, 1])
 @pytest.mark.parametrize("fused", [0, 1])
 @pytest.mark.parametrize("iters", [1, 2, 3])
-def test_jacobi_iteration_count_edge_cases(O, X, iters, fused):
-    note(f"case iters={iters} fused_predict={fused}")
+def test_jacobi_iteration_count_edge_cases(O, X, iters, fused, plain):
+    note(f"case iters={iters} fused_predict={fused} plain_adjacency={plain}")
     for coll in (False, True):
         run_pair(O, X, dict(cols=20, rows=9, spacing=0.05, pins=2, jitter=1e-3, seed=2), S=8, I=iters,
-                 substeps=4, collision=coll, fused_predict=fused)
+                 substeps=4, collision=coll, fused_predict=fused, plain_adjacency=plain)
+
+
+def test_reference_grid_plain_adjacency(O, X):
+    """The 12-byte adjacency path on the reference's own grid builder."""
+    note("case grid18 plain")
+    mesh = X.ClothMeshData.grid_cloth_hanging(18, 18, 0.042)
+    run_pair(O, X, mesh=mesh, S=36, I=22, substeps=2, collision=True, plain_adjacency=1)
 
 
 @pytest.mark.parametrize("block", [64, 128, 512])
