@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the per-kernel timing launches (for rocprofv3 counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -98,6 +100,12 @@ def main():
                                    autotune=0 if args.no_autotune else 1)
     u = X.ClothSimUniforms.default(X.SOLVER_JACOBI, S, thickness=float(thickness))
     sim = X.ClothComputePlugin(mesh, cfg, u)
+    if args.kernels_only:
+        jac_ms = sim.time_kernel(X.KERNEL_JACOBI_ITER, 20)
+        sweep_ms = sim.time_kernel(X.KERNEL_COLLISION_SWEEP, 5)
+        log(f"bench: kernels only: jacobi {jac_ms * 1e3:.1f} us, sweep {sweep_ms * 1e3:.1f} us")
+        sim.close()
+        return
     log(f"bench: device state uploaded; warmup {args.warmup} frames")
     sim.step_frames(args.warmup)
     sim.sync()

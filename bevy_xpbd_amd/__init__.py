@@ -22,6 +22,7 @@ from .api import (  # noqa: F401
     XpbdError,
     derive_collision_grid,
     lib,
+    selftest_division,
     KERNEL_JACOBI_ITER,
     KERNEL_COLLISION_SWEEP,
     KERNEL_RADIX_PASS,

@@ -83,6 +83,7 @@ class ClothSimConfig(C.Structure):
         ("collision_every_n_substeps", C.c_uint32), ("solver", C.c_uint32),
         ("collision_cell", C.c_float), ("use_graph", C.c_uint32), ("autotune", C.c_uint32),
         ("jacobi_block", C.c_uint32), ("fused_predict", C.c_uint32), ("plain_adjacency", C.c_uint32),
+        ("general_division", C.c_uint32),
     ]
 
     @classmethod
@@ -162,6 +163,7 @@ def lib():
             "xpbd_residual": (i32, [vp, C.POINTER(C.c_double)]),
             "xpbd_time_kernel": (i32, [vp, i32, u32, C.POINTER(C.c_float)]),
             "xpbd_debug_collision_keys": (i32, [vp, f32, vp, vp, vp]),
+            "xpbd_selftest_division": (i32, [i32, i32, u64, u64, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -384,6 +386,14 @@ class ClothComputePlugin:
              "cell": grid[6:7].view(np.float32)[0], "inv_cell": grid[7:8].view(np.float32)[0],
              "dims": grid[8:11].copy()}
         return g, keys, ids
+
+
+def selftest_division(exp_lo: int = -60, exp_hi: int = 60, pairs: int = 1 << 28, seed: int = 1) -> dict:
+    """Run the device self-test of the Jacobi gather's division (needs a GPU)."""
+    out = np.zeros(5, np.uint32)
+    _check(lib().xpbd_selftest_division(exp_lo, exp_hi, pairs, seed, _ptr(out)), "xpbd_selftest_division")
+    return {"rcp_mismatches": int(out[0]), "first_bad_b": int(out[1]), "div_mismatches": int(out[2]),
+            "first_bad_pair": (int(out[3]), int(out[4]))}
 
 
 class ClothMaterialPlugin:

@@ -123,8 +123,34 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
 // depends only on the pair, and each endpoint recomputes it bit-identically
 // (|a-b| == |b-a|, w_i + w_j == w_j + w_i).  One pass, no lambda buffers.
 // ---------------------------------------------------------------------------
+// Correctly rounded division without the general div_scale/div_fixup
+// expansion, valid where no operand or result leaves the normal range.
+// y = RN(1/b) from the hardware reciprocal and one FMA Newton step; then
+// q = RN(a*y), r = a - b*q (exact with FMA), RN(q + r*y) = RN(a/b)
+// (Markstein's theorem: y the correctly rounded reciprocal, q within one ulp).
+// That the refined y equals RN(1/b) for every significand is checked
+// exhaustively on the device by xpbd_selftest (tests/test_gpu_numerics.py).
+__device__ __forceinline__ float rcp_rn(float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y0, 1.0f);
+    return __builtin_fmaf(e, y0, y0);
+}
+__device__ __forceinline__ float div_rn(float a, float b, float y) {
+    const float q = a * y;
+    const float r = __builtin_fmaf(-b, q, a);
+    const float v = __builtin_fmaf(r, y, q);
+    return q == 0.f ? q : v;
+}
+// |x| in [2^-60, 2^60] or x == 0: inside the range where div_rn is exact for
+// the quotients this file forms (|a/b| <= 2^60, all intermediates normal).
+__device__ __forceinline__ bool div_safe(float x) {
+    const float ax = fabsf(x);
+    return ax == 0.f || (ax >= 8.67361738e-19f && ax <= 1.15292150e+18f);
+}
+
 // One constraint entry of the gather: accumulates (n * w_i) * dlambda in the
 // reference's order of operations (xpbd_cpu.rs:159-182, 206-222).
+template <bool FASTDIV>
 __device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float rest, float comp, const DevParams& p,
                                              float& ax, float& ay, float& az) {
     const float wi = pi.w, wj = pj.w;
@@ -132,14 +158,25 @@ __device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float rest, f
     float gx = pi.x - pj.x, gy = pi.y - pj.y, gz = pi.z - pj.z;
     const float len = len3(gx, gy, gz);
     if (len < 1e-8f) return;
-    gx = gx / len;
-    gy = gy / len;
-    gz = gz / len;
     const float c = len - rest;
     const float alpha_t = comp * p.inv_dt_sq;
     const float wsum = wi + wj + alpha_t;
     if (wsum < 1e-8f) return;
-    const float dlam = (-c - alpha_t * 0.0f) / wsum;
+    const float num = -c - alpha_t * 0.0f;
+    float dlam;
+    if (FASTDIV && div_safe(len) && div_safe(gx) && div_safe(gy) && div_safe(gz) && div_safe(wsum) &&
+        div_safe(num)) {
+        const float yl = rcp_rn(len);
+        gx = div_rn(gx, len, yl);
+        gy = div_rn(gy, len, yl);
+        gz = div_rn(gz, len, yl);
+        dlam = div_rn(num, wsum, rcp_rn(wsum));
+    } else {
+        gx = gx / len;
+        gy = gy / len;
+        gz = gz / len;
+        dlam = num / wsum;
+    }
     ax = ax + (gx * wi) * dlam;
     ay = ay + (gy * wi) * dlam;
     az = az + (gz * wi) * dlam;
@@ -150,7 +187,7 @@ __device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float rest, f
 // keeps several dependent load chains in flight instead of one.
 constexpr uint32_t JU = 4;
 
-template <uint32_t MODE, bool PACKED>
+template <uint32_t MODE, bool PACKED, bool FASTDIV>
 __global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
     const uint32_t nb = gridDim.x;
     const uint32_t tile = a.remap ? xcd_tile(blockIdx.x, nb) : blockIdx.x;
@@ -215,7 +252,7 @@ __global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < JU; ++u)
-                    if (jj[u] != NO_ENTRY) jacobi_entry(pi, pp[u], rr[u], cc[u], p, ax, ay, az);
+                    if (jj[u] != NO_ENTRY) jacobi_entry<FASTDIV>(pi, pp[u], rr[u], cc[u], p, ax, ay, az);
             }
             float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
             const float ml = len3(dx, dy, dz);
@@ -829,15 +866,21 @@ static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 uint32_t jacobi_blocks(uint32_t nslices, uint32_t block) { return cdiv(nslices, block / 64); }
 
-template <bool PACKED>
+template <bool PACKED, bool FASTDIV>
 static void launch_jacobi_t(uint32_t mode, const JacobiArgs& a, uint32_t blocks, uint32_t block, hipStream_t s) {
     switch (mode) {
-        case 0: hipLaunchKernelGGL((k_jacobi<0, PACKED>), dim3(blocks), dim3(block), 0, s, a); break;
-        case JM_PREDICT: hipLaunchKernelGGL((k_jacobi<JM_PREDICT, PACKED>), dim3(blocks), dim3(block), 0, s, a); break;
-        case JM_FINAL: hipLaunchKernelGGL((k_jacobi<JM_FINAL, PACKED>), dim3(blocks), dim3(block), 0, s, a); break;
-        case JM_AABB: hipLaunchKernelGGL((k_jacobi<JM_AABB, PACKED>), dim3(blocks), dim3(block), 0, s, a); break;
+        case 0: hipLaunchKernelGGL((k_jacobi<0, PACKED, FASTDIV>), dim3(blocks), dim3(block), 0, s, a); break;
+        case JM_PREDICT:
+            hipLaunchKernelGGL((k_jacobi<JM_PREDICT, PACKED, FASTDIV>), dim3(blocks), dim3(block), 0, s, a);
+            break;
+        case JM_FINAL:
+            hipLaunchKernelGGL((k_jacobi<JM_FINAL, PACKED, FASTDIV>), dim3(blocks), dim3(block), 0, s, a);
+            break;
+        case JM_AABB:
+            hipLaunchKernelGGL((k_jacobi<JM_AABB, PACKED, FASTDIV>), dim3(blocks), dim3(block), 0, s, a);
+            break;
         case JM_PREDICT | JM_AABB:
-            hipLaunchKernelGGL((k_jacobi<JM_PREDICT | JM_AABB, PACKED>), dim3(blocks), dim3(block), 0, s, a);
+            hipLaunchKernelGGL((k_jacobi<JM_PREDICT | JM_AABB, PACKED, FASTDIV>), dim3(blocks), dim3(block), 0, s, a);
             break;
         default: break;
     }
@@ -846,9 +889,68 @@ static void launch_jacobi_t(uint32_t mode, const JacobiArgs& a, uint32_t blocks,
 uint32_t launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s) {
     const uint32_t blocks = jacobi_blocks(a.p.nslices, block);
     if (blocks == 0) return 0;
-    if (a.sell.packed) launch_jacobi_t<true>(mode, a, blocks, block, s);
-    else launch_jacobi_t<false>(mode, a, blocks, block, s);
+    if (a.sell.packed) {
+        if (a.fastdiv) launch_jacobi_t<true, true>(mode, a, blocks, block, s);
+        else launch_jacobi_t<true, false>(mode, a, blocks, block, s);
+    } else {
+        if (a.fastdiv) launch_jacobi_t<false, true>(mode, a, blocks, block, s);
+        else launch_jacobi_t<false, false>(mode, a, blocks, block, s);
+    }
     return blocks;
+}
+
+// ---------------------------------------------------------------------------
+// Device self-test of div_rn / rcp_rn: the refined reciprocal against the
+// correctly rounded 1.0f / b for every significand at the given exponents,
+// and div_rn against '/' on pseudo-random pairs.  Counts mismatches.
+// ---------------------------------------------------------------------------
+__global__ void k_selftest_rcp(int exp_lo, int exp_hi, uint32_t* bad, uint32_t* first) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (1u << 23)) return;
+    for (int e = exp_lo; e <= exp_hi; ++e)
+        for (int sgn = 0; sgn < 2; ++sgn) {
+            const float b = __uint_as_float((uint32_t(sgn) << 31) | (uint32_t(e + 127) << 23) | m);
+            const float y = rcp_rn(b);
+            const float ref = 1.0f / b;
+            if (__float_as_uint(y) != __float_as_uint(ref)) {
+                if (atomicAdd(bad, 1u) == 0) *first = __float_as_uint(b);
+            }
+        }
+}
+
+__device__ __forceinline__ uint32_t mix32(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return uint32_t(x);
+}
+
+__global__ void k_selftest_div(uint64_t n, uint64_t seed, uint32_t* bad, uint32_t* first) {
+    for (uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t ua = mix32(seed * 0x9E3779B97F4A7C15ull + 2 * k);
+        const uint32_t ub = mix32(seed * 0x9E3779B97F4A7C15ull + 2 * k + 1);
+        // exponents spread over [-40, 40] (safe range), random significands and signs
+        const float a = __uint_as_float((ua & 0x807FFFFFu) | (uint32_t(87 + (ua >> 24) % 81) << 23));
+        const float b = __uint_as_float((ub & 0x807FFFFFu) | (uint32_t(87 + (ub >> 24) % 81) << 23));
+        const float q = div_rn(a, b, rcp_rn(b));
+        const float ref = a / b;
+        if (__float_as_uint(q) != __float_as_uint(ref)) {
+            if (atomicAdd(bad, 1u) == 0) {
+                first[0] = __float_as_uint(a);
+                first[1] = __float_as_uint(b);
+            }
+        }
+    }
+}
+
+void launch_selftest_rcp(int exp_lo, int exp_hi, uint32_t* bad, uint32_t* first, hipStream_t s) {
+    hipLaunchKernelGGL(k_selftest_rcp, dim3((1u << 23) / 256), dim3(256), 0, s, exp_lo, exp_hi, bad, first);
+}
+
+void launch_selftest_div(uint64_t n, uint64_t seed, uint32_t* bad, uint32_t* first, hipStream_t s) {
+    hipLaunchKernelGGL(k_selftest_div, dim3(8192), dim3(256), 0, s, n, seed, bad, first);
 }
 
 void launch_predict(const float4* X, const float4* V, float4* P, DevParams p, hipStream_t s) {

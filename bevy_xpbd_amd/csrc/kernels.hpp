@@ -55,6 +55,7 @@ struct JacobiArgs {
     Sell sell;
     DevParams p;
     uint32_t remap;      // 1: XCD-aware block order
+    uint32_t fastdiv;    // 1: shared-reciprocal correctly rounded divisions
 };
 
 // Cell grid of the collision broad phase (device memory, 12 words).
@@ -123,5 +124,7 @@ void launch_publish(const float4* X, uint32_t n, const uint32_t* tris, const uin
 void launch_residual(const float4* X, const uint2* ij, const float2* rc, uint32_t e, double* out,
                      hipStream_t s);
 uint32_t launch_aabb_of(const float4* P, uint32_t n, float* part, hipStream_t s);
+void launch_selftest_rcp(int exp_lo, int exp_hi, uint32_t* bad, uint32_t* first, hipStream_t s);
+void launch_selftest_div(uint64_t n, uint64_t seed, uint32_t* bad, uint32_t* first, hipStream_t s);
 
 }  // namespace xpbd

@@ -160,6 +160,7 @@ JacobiArgs jacobi_args(const xpbd_cloth* h, const DevParams& p) {
     a.aabb = h->aabb;
     a.p = p;
     a.remap = h->remap;
+    a.fastdiv = h->cfg.general_division ? 0u : 1u;
     return a;
 }
 
@@ -978,6 +979,30 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
     TRY(check_device_flag(h));
     *avg_ms = ms / float(reps);
     return XPBD_OK;
+}
+
+XPBD_API int xpbd_selftest_division(int exp_lo, int exp_hi, uint64_t pairs, uint64_t seed, uint32_t* out5) {
+    if (!out5 || exp_lo > exp_hi || exp_lo < -126 || exp_hi > 127) return XPBD_E_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return XPBD_E_NO_DEVICE;
+    uint32_t* d = nullptr;
+    if (hipMalloc(&d, 8 * sizeof(uint32_t)) != hipSuccess) return XPBD_E_NOMEM;
+    hipStream_t s = nullptr;
+    int rc = XPBD_OK;
+    if (hipMemset(d, 0, 8 * sizeof(uint32_t)) != hipSuccess || hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        rc = XPBD_E_HIP;
+    } else {
+        launch_selftest_rcp(exp_lo, exp_hi, d, d + 1, s);
+        if (pairs) launch_selftest_div(pairs, seed, d + 2, d + 3, s);
+        uint32_t h[8] = {0};
+        if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess ||
+            hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = XPBD_E_HIP;
+        std::memcpy(out5, h, 5 * sizeof(uint32_t));
+    }
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(d);
+    return rc;
 }
 
 XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t* grid_out, uint32_t* sorted_keys,

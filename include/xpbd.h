@@ -134,6 +134,9 @@ typedef struct xpbd_sim_config {
     uint32_t fused_predict;               /* 1: predict inside the first iteration */
     uint32_t plain_adjacency;             /* 1: 12-byte adjacency entries even when the
                                              4-byte packed form fits (read at create) */
+    uint32_t general_division;            /* 1: the compiler's general f32 division in
+                                             the Jacobi gather instead of the shared-
+                                             reciprocal form (both correctly rounded) */
 } xpbd_sim_config;
 
 /* ClothSimUniforms (cloth_compute.rs:182-202). */
@@ -220,6 +223,14 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
  * dims[3] (u32) — 11 words. */
 XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t* grid_out,
                                        uint32_t* sorted_keys, uint32_t* sorted_ids);
+
+/* Device self-test of the shared-reciprocal division used by the Jacobi
+ * gather (no reference counterpart; a numerics check of this library):
+ * out[0] = significands b (every one, signs +-, exponents exp_lo..exp_hi)
+ * whose refined reciprocal differs from the correctly rounded 1/b,
+ * out[1] = the first such b (bits); out[2] = mismatches of the quotient
+ * against '/' over `pairs` pseudo-random pairs, out[3..4] = first (a, b). */
+XPBD_API int xpbd_selftest_division(int exp_lo, int exp_hi, uint64_t pairs, uint64_t seed, uint32_t* out5);
 
 /* Library info. */
 XPBD_API int xpbd_abi_version(void);
