@@ -182,13 +182,38 @@ __device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float rest, f
     az = az + (gz * wi) * dlam;
 }
 
-// Entries are fetched JU at a time (all index/rest/compliance loads, then all
-// neighbour gathers, then the arithmetic in entry order), so each wavefront
-// keeps several dependent load chains in flight instead of one.
-constexpr uint32_t JU = 4;
+// Entries are processed JU at a time: all adjacency loads of the chunk, then
+// all neighbour gathers, then the arithmetic in entry order — several load
+// chains in flight per wavefront.  Padding entries point at the particle
+// itself (zero length: skipped by the reference's len < 1e-8 rule), so no
+// load is conditional.  The slice width is wavefront-uniform.
+constexpr uint32_t JU = 6;
+constexpr uint32_t LDS_PAIRS_MAX = 1024;   // (rest, compliance) table staged in LDS up to this size
+
+template <bool PACKED>
+__device__ __forceinline__ void load_entry(const JacobiArgs& a, const float2* lpairs, bool lds, uint32_t i, uint32_t e,
+                                           uint32_t& j, float& rest, float& comp) {
+    if (PACKED) {
+        const uint32_t w = __builtin_nontemporal_load(a.sell.packed + e);
+        j = uint32_t(int32_t(i) + int32_t(int16_t(w & 0xFFFFu)));
+        const float2 rc = lds ? lpairs[w >> 16] : a.sell.pairs[w >> 16];
+        rest = rc.x;
+        comp = rc.y;
+    } else {
+        j = __builtin_nontemporal_load(a.sell.j + e);
+        rest = __builtin_nontemporal_load(a.sell.rest + e);
+        comp = __builtin_nontemporal_load(a.sell.comp + e);
+    }
+}
 
 template <uint32_t MODE, bool PACKED, bool FASTDIV>
 __global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
+    __shared__ float2 lpairs[LDS_PAIRS_MAX];
+    const bool lds = PACKED && a.num_pairs <= LDS_PAIRS_MAX;
+    if (lds) {
+        for (uint32_t q = threadIdx.x; q < a.num_pairs; q += blockDim.x) lpairs[q] = a.sell.pairs[q];
+        __syncthreads();
+    }
     const uint32_t nb = gridDim.x;
     const uint32_t tile = a.remap ? xcd_tile(blockIdx.x, nb) : blockIdx.x;
     const uint32_t slices_per_block = blockDim.x >> 6;
@@ -208,51 +233,27 @@ __global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
             float ax = 0.f, ay = 0.f, az = 0.f;
             const uint32_t base = a.sell.soff[slice] + lane;
             const uint32_t width = a.sell.swid[slice];
-            for (uint32_t k0 = 0; k0 < width; k0 += JU) {
+            uint32_t k0 = 0;
+            for (; k0 + JU <= width; k0 += JU) {
                 uint32_t jj[JU];
                 float rr[JU], cc[JU];
                 float4 pp[JU];
-                if (PACKED) {
-                    uint32_t w[JU];
 #pragma unroll
-                    for (uint32_t u = 0; u < JU; ++u) {
-                        const uint32_t k = k0 + u;
-                        w[u] = k < width ? __builtin_nontemporal_load(a.sell.packed + base + k * 64) : NO_ENTRY;
-                    }
-#pragma unroll
-                    for (uint32_t u = 0; u < JU; ++u) {
-                        if (w[u] != NO_ENTRY) {
-                            jj[u] = uint32_t(int32_t(i) + int32_t(int16_t(w[u] & 0xFFFFu)));
-                            const float2 rc = a.sell.pairs[w[u] >> 16];
-                            rr[u] = rc.x;
-                            cc[u] = rc.y;
-                        } else {
-                            jj[u] = NO_ENTRY;
-                            rr[u] = cc[u] = 0.f;
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (uint32_t u = 0; u < JU; ++u) {
-                        const uint32_t k = k0 + u;
-                        const uint32_t e = base + k * 64;
-                        jj[u] = k < width ? __builtin_nontemporal_load(a.sell.j + e) : NO_ENTRY;
-                        rr[u] = k < width ? __builtin_nontemporal_load(a.sell.rest + e) : 0.f;
-                        cc[u] = k < width ? __builtin_nontemporal_load(a.sell.comp + e) : 0.f;
-                    }
-                }
+                for (uint32_t u = 0; u < JU; ++u) load_entry<PACKED>(a, lpairs, lds, i, base + (k0 + u) * 64, jj[u], rr[u], cc[u]);
 #pragma unroll
                 for (uint32_t u = 0; u < JU; ++u) {
-                    if (jj[u] != NO_ENTRY) {
-                        if (MODE & JM_PREDICT) pp[u] = predict(a.src[jj[u]], a.vel[jj[u]], jj[u], p);
-                        else pp[u] = a.src[jj[u]];
-                    } else {
-                        pp[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    }
+                    if (MODE & JM_PREDICT) pp[u] = predict(a.src[jj[u]], a.vel[jj[u]], jj[u], p);
+                    else pp[u] = a.src[jj[u]];
                 }
 #pragma unroll
-                for (uint32_t u = 0; u < JU; ++u)
-                    if (jj[u] != NO_ENTRY) jacobi_entry<FASTDIV>(pi, pp[u], rr[u], cc[u], p, ax, ay, az);
+                for (uint32_t u = 0; u < JU; ++u) jacobi_entry<FASTDIV>(pi, pp[u], rr[u], cc[u], p, ax, ay, az);
+            }
+            for (; k0 < width; ++k0) {
+                uint32_t j;
+                float rest, comp;
+                load_entry<PACKED>(a, lpairs, lds, i, base + k0 * 64, j, rest, comp);
+                const float4 pj = (MODE & JM_PREDICT) ? predict(a.src[j], a.vel[j], j, p) : a.src[j];
+                jacobi_entry<FASTDIV>(pi, pj, rest, comp, p, ax, ay, az);
             }
             float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
             const float ml = len3(dx, dy, dz);

@@ -24,8 +24,8 @@ struct DevParams {
 // Sliced-ELL adjacency: 64 consecutive particles per slice (one wavefront),
 // entry k of lane l at soff[s] + k*64 + l.  Each particle's entries keep the
 // reference CSR order (the gather's summation order); a particle's unused
-// tail entries hold j = NO_ENTRY.
-constexpr uint32_t NO_ENTRY = 0xFFFFFFFFu;
+// tail entries point at the particle itself (j = i, rest = compliance = 0;
+// packed: 0), which the gather skips as a zero-length constraint.
 struct Sell {
     const uint32_t* soff;
     const uint32_t* swid;
@@ -34,7 +34,7 @@ struct Sell {
     const float* comp;
     // Packed form (used when every entry fits): one u32 per entry,
     // (pair index << 16) | (uint16)(j - i), with pair = (rest, compliance)
-    // looked up in a table of the mesh's distinct pairs.  NO_ENTRY pads.
+    // looked up in a table of the mesh's distinct pairs; padding is 0.
     const uint32_t* packed;
     const float2* pairs;
 };
@@ -56,6 +56,7 @@ struct JacobiArgs {
     DevParams p;
     uint32_t remap;      // 1: XCD-aware block order
     uint32_t fastdiv;    // 1: shared-reciprocal correctly rounded divisions
+    uint32_t num_pairs;  // packed: size of sell.pairs
 };
 
 // Cell grid of the collision broad phase (device memory, 12 words).

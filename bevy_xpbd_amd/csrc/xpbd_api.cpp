@@ -161,6 +161,7 @@ JacobiArgs jacobi_args(const xpbd_cloth* h, const DevParams& p) {
     a.p = p;
     a.remap = h->remap;
     a.fastdiv = h->cfg.general_division ? 0u : 1u;
+    a.num_pairs = h->num_pairs;
     return a;
 }
 
@@ -336,9 +337,12 @@ void build_sell(const Mesh& m, std::vector<uint32_t>& soff, std::vector<uint32_t
         soff[s + 1] = soff[s] + w * 64;
     }
     const size_t total = soff[ns];
-    sj.assign(total, NO_ENTRY);
+    sj.assign(total, 0u);
     sr.assign(total, 0.f);
     sc.assign(total, 0.f);
+    for (size_t s = 0; s < ns; ++s)   // padding: the particle itself (zero length, skipped)
+        for (size_t k = 0; k < swid[s]; ++k)
+            for (size_t l = 0; l < 64; ++l) sj[soff[s] + k * 64 + l] = uint32_t(std::min(s * 64 + l, n - 1));
     for (size_t i = 0; i < n; ++i) {
         const size_t s = i / 64, l = i % 64;
         const uint32_t b = m.neighbor_offsets[i], d = m.neighbor_offsets[i + 1] - b;
@@ -377,7 +381,7 @@ bool pack_sell(const Mesh& m, const std::vector<uint32_t>& soff, std::vector<uin
         std::memcpy(&pairs[q].x, &rb, 4);
         std::memcpy(&pairs[q].y, &cb, 4);
     }
-    packed.assign(soff.back(), NO_ENTRY);
+    packed.assign(soff.back(), 0u);   // padding: delta 0, pair 0 -> the particle itself
     for (size_t i = 0; i < n; ++i) {
         const size_t s = i / 64, l = i % 64;
         const uint32_t b = m.neighbor_offsets[i], d = m.neighbor_offsets[i + 1] - b;
