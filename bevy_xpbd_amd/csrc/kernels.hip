@@ -451,30 +451,48 @@ __global__ __launch_bounds__(256) void k_radix_scan_tiles(uint32_t* hist, uint32
     if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
 }
 
-// Stable scatter: within a tile, elements are ranked in input order (round
-// by round; inside a round, wave by wave; inside a wave, by lane) with a
-// 64-lane ballot match on the 8 digit bits.
+// Stable scatter.  Each tile is first sorted by digit in LDS (ranks in input
+// order: round by round; inside a round, wave by wave; inside a wave, by lane,
+// from a 64-lane ballot match on the 8 digit bits), then written out in LDS
+// order, so each digit's run of the tile leaves as contiguous stores.
 __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* kin, const uint32_t* iin,
                                                              uint32_t* kout, uint32_t* iout, uint32_t n,
                                                              uint32_t shift, const uint32_t* hist,
                                                              uint32_t ntiles, const uint32_t* dtot) {
     constexpr uint32_t NW = RADIX_BLOCK / 64;
-    __shared__ uint32_t base[256];
+    __shared__ uint32_t gbase[256];      // global position of this tile's first element of digit d
+    __shared__ uint32_t lstart[256];     // tile-local start of digit d
+    __shared__ uint32_t run[256];        // running local count of digit d
     __shared__ uint32_t wcnt[NW][256];
+    __shared__ uint32_t skey[RADIX_TILE];
+    __shared__ uint32_t sid[RADIX_TILE];
     __shared__ uint32_t lds[4];
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    uint32_t tot;
-    const uint32_t digit_base = block_exclusive_scan_256(dtot[t], lds, tot);
-    base[t] = digit_base + hist[t * ntiles + blockIdx.x];
+    const uint32_t tile0 = blockIdx.x * RADIX_TILE;
+    const uint32_t count = min(RADIX_TILE, n - tile0);
+    // tile-local histogram
+    run[t] = 0;
     for (uint32_t w = 0; w < NW; ++w) wcnt[w][t] = 0;
     __syncthreads();
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t tile0 = blockIdx.x * RADIX_TILE;
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
-        const uint32_t idx = tile0 + r * RADIX_BLOCK + t;
-        const bool valid = idx < n;
-        const uint32_t key = valid ? kin[idx] : 0u;
-        const uint32_t id = valid ? iin[idx] : 0u;
+        const uint32_t k = r * RADIX_BLOCK + t;
+        if (k < count) atomicAdd(&run[(kin[tile0 + k] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    uint32_t tot;
+    const uint32_t mine = run[t];
+    const uint32_t lex = block_exclusive_scan_256(mine, lds, tot);
+    const uint32_t digit_base = block_exclusive_scan_256(dtot[t], lds, tot);
+    lstart[t] = lex;
+    gbase[t] = digit_base + hist[t * ntiles + blockIdx.x];
+    run[t] = 0;
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+        const uint32_t k = r * RADIX_BLOCK + t;
+        const bool valid = k < count;
+        const uint32_t key = valid ? kin[tile0 + k] : 0u;
+        const uint32_t id = valid ? iin[tile0 + k] : 0u;
         const uint32_t d = (key >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -487,10 +505,10 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
         if (valid && rank == 0) wcnt[wave][d] = __popcll(peers);
         __syncthreads();
         if (valid) {
-            uint32_t pos = base[d] + rank;
+            uint32_t pos = lstart[d] + run[d] + rank;
             for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][d];
-            kout[pos] = key;
-            iout[pos] = id;
+            skey[pos] = key;
+            sid[pos] = id;
         }
         __syncthreads();
         uint32_t add = 0;
@@ -498,8 +516,20 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
             add += wcnt[w][t];
             wcnt[w][t] = 0;
         }
-        base[t] += add;
+        run[t] += add;
         __syncthreads();
+    }
+    // write out in local sorted order: element q of the tile goes to
+    // gbase[d] + (q - lstart[d])
+    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+        const uint32_t q = r * RADIX_BLOCK + t;
+        if (q < count) {
+            const uint32_t key = skey[q];
+            const uint32_t d = (key >> shift) & 255u;
+            const uint32_t dst = gbase[d] + (q - lstart[d]);
+            kout[dst] = key;
+            iout[dst] = sid[q];
+        }
     }
 }
 
@@ -508,7 +538,12 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
 // one probe (rarely two) finds where its particles start; one insert per
 // occupied group keeps the atomic traffic ~4x below one per cell.
 constexpr uint32_t GROUP_SHIFT = 2;
-__device__ __forceinline__ uint32_t slot_of(uint32_t key, uint32_t mask) { return (key * 2654435761u) & mask; }
+// Slot of a group key: the low 3 bits stay (8 consecutive groups = 32 cells
+// share one 64-byte line, so threads walking the sorted order probe the same
+// lines); the blocks of 8 are scattered by a multiplicative mix.
+__device__ __forceinline__ uint32_t slot_of(uint32_t g, uint32_t mask) {
+    return ((((g >> 3) * 2654435761u) << 3) | (g & 7u)) & mask;
+}
 
 __global__ void k_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
