@@ -83,7 +83,7 @@ class ClothSimConfig(C.Structure):
         ("collision_every_n_substeps", C.c_uint32), ("solver", C.c_uint32),
         ("collision_cell", C.c_float), ("use_graph", C.c_uint32), ("autotune", C.c_uint32),
         ("jacobi_block", C.c_uint32), ("fused_predict", C.c_uint32), ("plain_adjacency", C.c_uint32),
-        ("general_division", C.c_uint32),
+        ("fast_division", C.c_uint32), ("xcd_remap", C.c_uint32),
     ]
 
     @classmethod

@@ -129,7 +129,8 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
 // q = RN(a*y), r = a - b*q (exact with FMA), RN(q + r*y) = RN(a/b)
 // (Markstein's theorem: y the correctly rounded reciprocal, q within one ulp).
 // That the refined y equals RN(1/b) for every significand is checked
-// exhaustively on the device by xpbd_selftest (tests/test_gpu_numerics.py).
+// exhaustively on the device by xpbd_selftest_division.  Off by default: the
+// range guards cost more than the shorter division saves in the gather.
 __device__ __forceinline__ float rcp_rn(float b) {
     const float y0 = __builtin_amdgcn_rcpf(b);
     const float e = __builtin_fmaf(-b, y0, 1.0f);
@@ -397,7 +398,7 @@ __global__ void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grid, u
 
 // ---- stable LSD radix sort of (key, id), 8-bit digits, 4 passes ----------
 constexpr uint32_t RADIX_BLOCK = 256;
-constexpr uint32_t RADIX_ITEMS = 16;
+constexpr uint32_t RADIX_ITEMS = 4;
 constexpr uint32_t RADIX_TILE = RADIX_BLOCK * RADIX_ITEMS;
 
 __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const uint32_t* keys, uint32_t n, uint32_t shift,
@@ -648,17 +649,26 @@ __device__ __forceinline__ void sweep_rows(const CollArgs& a, const CollGrid& g,
     for (int r = 0; r < 9; ++r) rows[r].start = st[r];
 }
 
+// Squared distance of two particles; symmetric bit for bit (the components
+// of a - b and b - a differ only in sign), so it can screen candidates before
+// the canonical (min id, max id) orientation is known.
+__device__ __forceinline__ float dist2_of(float4 a, float4 b) {
+    const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
 __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     const uint32_t n = a.p.n;
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    const float4 fi = a.sP[s];
     const uint32_t i = a.sids[s];
-    const float4 fi = a.F[i];
     if (fi.w <= 0.f) {
         if (a.xout != a.prev) a.xout[i] = fi;    // timing runs write scratch
         return;
     }
     const CollGrid g = *a.grid;
+    const float th2 = a.p.thickness * a.p.thickness;
     uint32_t c[3];
     cell_of(fi, g, c);
     RowRange rows[9];
@@ -678,12 +688,14 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
         if (rows[r].start == EMPTY_KEY) continue;
         for (uint32_t idx = rows[r].start; idx < n; ++idx) {
             const uint32_t key = a.skeys[idx];
+            const float4 fj = a.sP[idx];
             if (key > rows[r].kmax) break;
-            if (key < rows[r].kmin) continue;
+            if (key < rows[r].kmin || idx == s) continue;
+            const float d2 = dist2_of(fi, fj);
+            if (d2 > th2 || d2 < 1e-18f) continue;
             const uint32_t j = a.sids[idx];
-            if (j == i) continue;
             float cx, cy, cz;
-            if (!collision_contrib(i, j, fi, a.F[j], a.rest4, a.p, cx, cy, cz)) continue;
+            if (!collision_contrib(i, j, fi, fj, a.rest4, a.p, cx, cy, cz)) continue;
             ++count;
             if (count > CMAX) continue;
             // insert at the end, then bubble down to keep ascending partner ids
@@ -722,11 +734,14 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
                 for (uint32_t idx = rows[r].start; idx < n; ++idx) {
                     const uint32_t key = a.skeys[idx];
                     if (key > rows[r].kmax) break;
-                    if (key < rows[r].kmin) continue;
+                    if (key < rows[r].kmin || idx == s) continue;
+                    const float4 fj = a.sP[idx];
+                    const float d2 = dist2_of(fi, fj);
+                    if (d2 > th2 || d2 < 1e-18f) continue;
                     const uint32_t j = a.sids[idx];
-                    if (j == i || j >= best || (has_last && j <= last)) continue;
+                    if (j >= best || (has_last && j <= last)) continue;
                     float cx, cy, cz;
-                    if (collision_contrib(i, j, fi, a.F[j], a.rest4, a.p, cx, cy, cz)) {
+                    if (collision_contrib(i, j, fi, fj, a.rest4, a.p, cx, cy, cz)) {
                         best = j;
                         bx = cx;
                         by = cy;
@@ -749,6 +764,12 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     };
     const float4 nx = make_float4(fi.x + cl(ax), fi.y + cl(ay), fi.z + cl(az), fi.w);
     finish_particle(nx, a.prev[i], i, a.ground, a.p, a.xout, a.vout);
+}
+
+// Positions permuted into sorted (cell) order: sP[s] = F[sids[s]].
+__global__ void k_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float4* sP) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) sP[s] = F[sids[s]];
 }
 
 // Standalone predict (xpbd_cpu.rs:251-264): P = predict(X, V).
@@ -1044,6 +1065,11 @@ void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32
 void launch_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_coll_table, dim3(cdiv(n, 256)), dim3(256), 0, s, skeys, n, table, mask);
+}
+
+void launch_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float4* sP, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_coll_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, F, sids, n, sP);
 }
 
 void launch_coll_sweep(const CollArgs& a, hipStream_t s) {

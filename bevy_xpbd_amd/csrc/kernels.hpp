@@ -69,6 +69,7 @@ struct CollGrid {
 
 struct CollArgs {
     const float4* F;     // positions after the constraint iterations
+    const float4* sP;    // F permuted into sorted order
     const float4* rest4;
     const float4* prev;  // X (substep start) for the velocity update
     float4* xout;
@@ -116,6 +117,7 @@ uint32_t radix_hist_words(uint32_t n);
 void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
                        uint32_t shift, uint32_t* hist, hipStream_t s);
 void launch_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask, hipStream_t s);
+void launch_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float4* sP, hipStream_t s);
 void launch_coll_sweep(const CollArgs& a, hipStream_t s);
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
 uint32_t gs_max_blocks(uint32_t block);

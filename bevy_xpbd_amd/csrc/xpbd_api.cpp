@@ -60,6 +60,7 @@ struct xpbd_cloth {
 
     std::vector<void*> allocs;
     float4 *X = nullptr, *V = nullptr, *A = nullptr, *B = nullptr, *V2 = nullptr, *rest4 = nullptr;
+    float4* sP = nullptr;   // collision: positions in sorted order
     uint32_t *soff = nullptr, *swid = nullptr, *nbj = nullptr;
     float *nbr = nullptr, *nbc = nullptr;
     uint32_t* nbpk = nullptr;       // packed adjacency (nullptr: plain layout in use)
@@ -160,7 +161,7 @@ JacobiArgs jacobi_args(const xpbd_cloth* h, const DevParams& p) {
     a.aabb = h->aabb;
     a.p = p;
     a.remap = h->remap;
-    a.fastdiv = h->cfg.general_division ? 0u : 1u;
+    a.fastdiv = h->cfg.fast_division ? 1u : 0u;
     a.num_pairs = h->num_pairs;
     return a;
 }
@@ -179,8 +180,10 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     launch_coll_keys(F, h->n, h->grid, h->keys, h->ids, h->table, h->table_slots, s);
     launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
     launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, s);
+    launch_coll_gather(F, h->ids, h->n, h->sP, s);
     CollArgs c{};
     c.F = F;
+    c.sP = h->sP;
     c.rest4 = h->rest4;
     c.prev = h->X;
     c.xout = xout;
@@ -423,6 +426,7 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(dalloc(h, &h->B, n));
     TRY(dalloc(h, &h->V2, n));
     TRY(dalloc(h, &h->rest4, n));
+    TRY(dalloc(h, &h->sP, n));
     std::vector<float> rest4(4 * n);
     h->init_pos4.assign(4 * n, 0.f);
     for (size_t i = 0; i < n; ++i) {
@@ -537,6 +541,7 @@ int upload_state(xpbd_cloth* h, const float* pos4, const float* vel4) {
 
 void apply_tuning(xpbd_cloth* h) {
     if (h->cfg.jacobi_block) h->jacobi_block = h->cfg.jacobi_block;
+    if (h->cfg.xcd_remap) h->remap = h->cfg.xcd_remap == 2 ? 1u : 0u;
     h->fused_predict = h->cfg.fused_predict ? 1u : 0u;
 }
 
@@ -911,11 +916,13 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         launch_coll_keys(h->A, h->n, h->grid, h->keys, h->ids, h->table, h->table_slots, s);
         launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
         launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, s);
+        launch_coll_gather(h->A, h->ids, h->n, h->sP, s);
     };
     CollArgs c{};
     if (which == XPBD_KERNEL_COLLISION_SWEEP || which == XPBD_KERNEL_RADIX_PASS) {
         prep_collision();
         c.F = h->A;
+        c.sP = h->sP;
         c.rest4 = h->rest4;
         c.prev = h->X;
         c.xout = h->B;

@@ -134,9 +134,12 @@ typedef struct xpbd_sim_config {
     uint32_t fused_predict;               /* 1: predict inside the first iteration */
     uint32_t plain_adjacency;             /* 1: 12-byte adjacency entries even when the
                                              4-byte packed form fits (read at create) */
-    uint32_t general_division;            /* 1: the compiler's general f32 division in
-                                             the Jacobi gather instead of the shared-
-                                             reciprocal form (both correctly rounded) */
+    uint32_t fast_division;               /* 1: shared-reciprocal division (Markstein
+                                             step, range-guarded) in the Jacobi gather
+                                             instead of the compiler's general f32
+                                             division; both correctly rounded */
+    uint32_t xcd_remap;                   /* Jacobi block order: 0 = default/autotuned,
+                                             1 = hardware order, 2 = XCD-contiguous tiles */
 } xpbd_sim_config;
 
 /* ClothSimUniforms (cloth_compute.rs:182-202). */

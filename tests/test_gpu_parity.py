@@ -22,21 +22,21 @@ def note(msg):
 
 
 def make(X, mesh, solver=0, S=8, I=10, coll_stride=1, use_graph=0, fused_predict=0, jacobi_block=0,
-         plain_adjacency=0, general_division=0, **uk):
+         plain_adjacency=0, fast_division=0, **uk):
     cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=I, solver=solver,
                                    collision_every_n_substeps=coll_stride, use_graph=use_graph,
                                    fused_predict=fused_predict, jacobi_block=jacobi_block,
-                                   plain_adjacency=plain_adjacency, general_division=general_division)
+                                   plain_adjacency=plain_adjacency, fast_division=fast_division)
     u = X.ClothSimUniforms.default(solver, S, **uk)
     return X.ClothComputePlugin(mesh, cfg, u), cfg, u
 
 
 def run_pair(O, X, mesh_kw=None, mesh=None, solver=0, S=8, I=10, substeps=8, collision=False, fused_predict=0,
-             jacobi_block=0, plain_adjacency=0, general_division=0, **uk):
+             jacobi_block=0, plain_adjacency=0, fast_division=0, **uk):
     mesh = mesh or X.ClothMeshData.sheet(**mesh_kw)
     f = mesh.fields()
     sim, cfg, u = make(X, mesh, solver, S, I, fused_predict=fused_predict, jacobi_block=jacobi_block,
-                       plain_adjacency=plain_adjacency, general_division=general_division, **uk)
+                       plain_adjacency=plain_adjacency, fast_division=fast_division, **uk)
     s = O.Solver(f)
     p = oracle_params(O, X, u, cfg, collision=collision)
     pos = f["positions"].copy()
@@ -77,12 +77,12 @@ def test_reference_grid_plain_adjacency(O, X):
     run_pair(O, X, mesh=mesh, S=36, I=22, substeps=2, collision=True, plain_adjacency=1)
 
 
-@pytest.mark.parametrize("general_division", [0, 1])
+@pytest.mark.parametrize("fast_division", [0, 1])
 @pytest.mark.parametrize("block", [64, 128, 512])
-def test_jacobi_block_sizes(O, X, block, general_division):
-    note(f"case block={block} general_division={general_division}")
+def test_jacobi_block_sizes(O, X, block, fast_division):
+    note(f"case block={block} fast_division={fast_division}")
     run_pair(O, X, dict(cols=70, rows=33, spacing=0.03, pins=2, jitter=1e-3, seed=8), S=8, I=5, substeps=3,
-             collision=True, jacobi_block=block, general_division=general_division)
+             collision=True, jacobi_block=block, fast_division=fast_division)
 
 
 def test_collision_dense_cluster_overflow_path(O, X):
