@@ -618,6 +618,7 @@ __device__ __forceinline__ bool collision_contrib(uint32_t i, uint32_t j, float4
 // CMAX contacts in a list sorted by partner id (registers); a particle with
 // more contacts falls back to repeated selection of the next-smallest partner.
 constexpr int CMAX = 4;
+constexpr uint32_t SB = 4;   // candidates fetched per step of a row scan
 
 struct RowRange {
     uint32_t start, kmin, kmax;
@@ -683,35 +684,55 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
         lx[q] = ly[q] = lz[q] = 0.f;
     }
     int count = 0;
-#pragma unroll
     for (int r = 0; r < 9; ++r) {
-        if (rows[r].start == EMPTY_KEY) continue;
-        for (uint32_t idx = rows[r].start; idx < n; ++idx) {
-            const uint32_t key = a.skeys[idx];
-            const float4 fj = a.sP[idx];
-            if (key > rows[r].kmax) break;
-            if (key < rows[r].kmin || idx == s) continue;
-            const float d2 = dist2_of(fi, fj);
-            if (d2 > th2 || d2 < 1e-18f) continue;
-            const uint32_t j = a.sids[idx];
-            float cx, cy, cz;
-            if (!collision_contrib(i, j, fi, fj, a.rest4, a.p, cx, cy, cz)) continue;
-            ++count;
-            if (count > CMAX) continue;
-            // insert at the end, then bubble down to keep ascending partner ids
-            lj[CMAX - 1] = j;
-            lx[CMAX - 1] = cx;
-            ly[CMAX - 1] = cy;
-            lz[CMAX - 1] = cz;
+        const uint32_t kmin = rows[r].kmin, kmax = rows[r].kmax;
+        uint32_t idx = rows[r].start;
+        bool done = idx == EMPTY_KEY;
+        while (!done) {
+            // SB candidates' loads issued together (indices clamped in range;
+            // slots past the end count as "beyond the row")
+            uint32_t kk[SB];
+            float4 ff[SB];
 #pragma unroll
-            for (int q = CMAX - 1; q > 0; --q) {
-                if (lj[q] < lj[q - 1]) {
-                    uint32_t tj = lj[q]; lj[q] = lj[q - 1]; lj[q - 1] = tj;
-                    float t = lx[q]; lx[q] = lx[q - 1]; lx[q - 1] = t;
-                    t = ly[q]; ly[q] = ly[q - 1]; ly[q - 1] = t;
-                    t = lz[q]; lz[q] = lz[q - 1]; lz[q - 1] = t;
+            for (uint32_t u = 0; u < SB; ++u) {
+                const uint32_t q = min(idx + u, n - 1);
+                kk[u] = idx + u < n ? a.skeys[q] : EMPTY_KEY;
+                ff[u] = a.sP[q];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < SB; ++u) {
+                if (done) break;
+                if (kk[u] > kmax) {
+                    done = true;
+                    break;
+                }
+                const uint32_t q = idx + u;
+                if (kk[u] < kmin || q == s) continue;
+                const float4 fj = ff[u];
+                const float d2 = dist2_of(fi, fj);
+                if (d2 > th2 || d2 < 1e-18f) continue;
+                const uint32_t j = a.sids[q];
+                float cx, cy, cz;
+                if (!collision_contrib(i, j, fi, fj, a.rest4, a.p, cx, cy, cz)) continue;
+                ++count;
+                if (count > CMAX) continue;
+                // insert at the end, then bubble down to keep ascending partner ids
+                lj[CMAX - 1] = j;
+                lx[CMAX - 1] = cx;
+                ly[CMAX - 1] = cy;
+                lz[CMAX - 1] = cz;
+#pragma unroll
+                for (int w = CMAX - 1; w > 0; --w) {
+                    if (lj[w] < lj[w - 1]) {
+                        uint32_t tj = lj[w]; lj[w] = lj[w - 1]; lj[w - 1] = tj;
+                        float t = lx[w]; lx[w] = lx[w - 1]; lx[w - 1] = t;
+                        t = ly[w]; ly[w] = ly[w - 1]; ly[w - 1] = t;
+                        t = lz[w]; lz[w] = lz[w - 1]; lz[w - 1] = t;
+                    }
                 }
             }
+            idx += SB;
+            if (idx >= n) done = true;
         }
     }
     float ax = 0.f, ay = 0.f, az = 0.f;

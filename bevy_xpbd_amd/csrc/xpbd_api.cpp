@@ -494,7 +494,9 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(dalloc(h, &h->keys2, n));
     TRY(dalloc(h, &h->ids2, n));
     TRY(dalloc(h, &h->hist, radix_hist_words(uint32_t(n))));
-    h->table_slots = next_pow2(uint32_t(std::max<size_t>(n, 64)));   // one entry per 4-cell group
+    // one entry per occupied 4-cell group: at most n groups (a stretched
+    // sheet puts nearly every particle in its own group), load <= 1/2
+    h->table_slots = next_pow2(uint32_t(std::max<size_t>(2 * n, 64)));
     TRY(dalloc(h, &h->table, h->table_slots));
     h->aabb_slots = std::max<uint32_t>({h->nslices, (h->n + 255) / 256, 4096u, 1u});
     TRY(dalloc(h, &h->aabb, size_t(h->aabb_slots) * 8));
