@@ -83,7 +83,7 @@ class ClothSimConfig(C.Structure):
         ("collision_every_n_substeps", C.c_uint32), ("solver", C.c_uint32),
         ("collision_cell", C.c_float), ("use_graph", C.c_uint32), ("autotune", C.c_uint32),
         ("jacobi_block", C.c_uint32), ("fused_predict", C.c_uint32), ("plain_adjacency", C.c_uint32),
-        ("fast_division", C.c_uint32), ("xcd_remap", C.c_uint32),
+        ("fast_division", C.c_uint32), ("xcd_remap", C.c_uint32), ("persistent_jacobi", C.c_uint32),
     ]
 
     @classmethod
@@ -376,16 +376,16 @@ class ClothComputePlugin:
         _check(lib().xpbd_time_kernel(self._h, which, reps, C.byref(ms)), "xpbd_time_kernel", self._h)
         return ms.value
 
-    def debug_collision_keys(self, cell_size: float):
-        grid = np.zeros(11, np.uint32)
+    def debug_collision_keys(self, cell_size: float, num_cloths: int = 1):
+        grid = np.zeros((num_cloths, 12), np.uint32)
         keys = np.zeros(self.n, np.uint32)
         ids = np.zeros(self.n, np.uint32)
         _check(lib().xpbd_debug_collision_keys(self._h, cell_size, _ptr(grid), _ptr(keys), _ptr(ids)),
                "xpbd_debug_collision_keys", self._h)
-        g = {"lo": grid[0:3].view(np.float32).copy(), "hi": grid[3:6].view(np.float32).copy(),
-             "cell": grid[6:7].view(np.float32)[0], "inv_cell": grid[7:8].view(np.float32)[0],
-             "dims": grid[8:11].copy()}
-        return g, keys, ids
+        gs = [{"lo": g[0:3].view(np.float32).copy(), "hi": g[3:6].view(np.float32).copy(),
+               "cell": g[6:7].view(np.float32)[0], "inv_cell": g[7:8].view(np.float32)[0],
+               "dims": g[8:11].copy(), "key_base": int(g[11])} for g in grid]
+        return (gs[0] if num_cloths == 1 else gs), keys, ids
 
 
 def selftest_division(exp_lo: int = -60, exp_hi: int = 60, pairs: int = 1 << 28, seed: int = 1) -> dict:

@@ -207,71 +207,84 @@ __device__ __forceinline__ void load_entry(const JacobiArgs& a, const float2* lp
     }
 }
 
+// One particle of a Jacobi iteration (the body shared by the per-iteration
+// kernel and the persistent one).  Returns the new position; writes it (or,
+// with JM_FINAL, the new state and velocity).
 template <uint32_t MODE, bool PACKED, bool FASTDIV>
-__global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
-    __shared__ float2 lpairs[LDS_PAIRS_MAX];
+__device__ __forceinline__ float4 jacobi_particle(const JacobiArgs& a, const float2* lpairs, bool lds,
+                                                  uint32_t slice, uint32_t lane, bool& valid) {
+    const uint32_t i = slice * 64 + lane;
+    const DevParams& p = a.p;
+    valid = slice < p.nslices && i < p.n;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 out = pi;
+    if (!valid) return out;
+    if (MODE & JM_PREDICT) pi = predict(a.src[i], a.vel[i], i, p);
+    else pi = a.src[i];
+    out = pi;
+    const float wi = pi.w;
+    if (wi > 0.f) {
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        const uint32_t base = a.sell.soff[slice] + lane;
+        const uint32_t width = a.sell.swid[slice];
+        uint32_t k0 = 0;
+        for (; k0 + JU <= width; k0 += JU) {
+            uint32_t jj[JU];
+            float rr[JU], cc[JU];
+            float4 pp[JU];
+#pragma unroll
+            for (uint32_t u = 0; u < JU; ++u) load_entry<PACKED>(a, lpairs, lds, i, base + (k0 + u) * 64, jj[u], rr[u], cc[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < JU; ++u) {
+                if (MODE & JM_PREDICT) pp[u] = predict(a.src[jj[u]], a.vel[jj[u]], jj[u], p);
+                else pp[u] = a.src[jj[u]];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < JU; ++u) jacobi_entry<FASTDIV>(pi, pp[u], rr[u], cc[u], p, ax, ay, az);
+        }
+        for (; k0 < width; ++k0) {
+            uint32_t j;
+            float rest, comp;
+            load_entry<PACKED>(a, lpairs, lds, i, base + k0 * 64, j, rest, comp);
+            const float4 pj = (MODE & JM_PREDICT) ? predict(a.src[j], a.vel[j], j, p) : a.src[j];
+            jacobi_entry<FASTDIV>(pi, pj, rest, comp, p, ax, ay, az);
+        }
+        float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
+        const float ml = len3(dx, dy, dz);
+        if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
+            const float sc = JACOBI_CORRECTION_CAP / ml;
+            dx = dx * sc;
+            dy = dy * sc;
+            dz = dz * sc;
+        }
+        out = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, wi);
+    }
+    if (MODE & JM_FINAL) {
+        if (pi.w > 0.f) finish_particle(out, a.dst[i], i, a.ground, p, a.dst, a.vout);
+    } else {
+        a.dst[i] = out;
+    }
+    return out;
+}
+
+template <bool PACKED>
+__device__ __forceinline__ bool stage_pairs(const JacobiArgs& a, float2* lpairs) {
     const bool lds = PACKED && a.num_pairs <= LDS_PAIRS_MAX;
     if (lds) {
         for (uint32_t q = threadIdx.x; q < a.num_pairs; q += blockDim.x) lpairs[q] = a.sell.pairs[q];
         __syncthreads();
     }
-    const uint32_t nb = gridDim.x;
-    const uint32_t tile = a.remap ? xcd_tile(blockIdx.x, nb) : blockIdx.x;
-    const uint32_t slices_per_block = blockDim.x >> 6;
-    const uint32_t slice = tile * slices_per_block + (threadIdx.x >> 6);
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t i = slice * 64 + lane;
-    const DevParams& p = a.p;
-    const bool valid = slice < p.nslices && i < p.n;
-    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 out = pi;
-    if (valid) {
-        if (MODE & JM_PREDICT) pi = predict(a.src[i], a.vel[i], i, p);
-        else pi = a.src[i];
-        out = pi;
-        const float wi = pi.w;
-        if (wi > 0.f) {
-            float ax = 0.f, ay = 0.f, az = 0.f;
-            const uint32_t base = a.sell.soff[slice] + lane;
-            const uint32_t width = a.sell.swid[slice];
-            uint32_t k0 = 0;
-            for (; k0 + JU <= width; k0 += JU) {
-                uint32_t jj[JU];
-                float rr[JU], cc[JU];
-                float4 pp[JU];
-#pragma unroll
-                for (uint32_t u = 0; u < JU; ++u) load_entry<PACKED>(a, lpairs, lds, i, base + (k0 + u) * 64, jj[u], rr[u], cc[u]);
-#pragma unroll
-                for (uint32_t u = 0; u < JU; ++u) {
-                    if (MODE & JM_PREDICT) pp[u] = predict(a.src[jj[u]], a.vel[jj[u]], jj[u], p);
-                    else pp[u] = a.src[jj[u]];
-                }
-#pragma unroll
-                for (uint32_t u = 0; u < JU; ++u) jacobi_entry<FASTDIV>(pi, pp[u], rr[u], cc[u], p, ax, ay, az);
-            }
-            for (; k0 < width; ++k0) {
-                uint32_t j;
-                float rest, comp;
-                load_entry<PACKED>(a, lpairs, lds, i, base + k0 * 64, j, rest, comp);
-                const float4 pj = (MODE & JM_PREDICT) ? predict(a.src[j], a.vel[j], j, p) : a.src[j];
-                jacobi_entry<FASTDIV>(pi, pj, rest, comp, p, ax, ay, az);
-            }
-            float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
-            const float ml = len3(dx, dy, dz);
-            if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
-                const float sc = JACOBI_CORRECTION_CAP / ml;
-                dx = dx * sc;
-                dy = dy * sc;
-                dz = dz * sc;
-            }
-            out = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, wi);
-        }
-        if (MODE & JM_FINAL) {
-            if (pi.w > 0.f) finish_particle(out, a.dst[i], i, a.ground, p, a.dst, a.vout);
-        } else {
-            a.dst[i] = out;
-        }
-    }
+    return lds;
+}
+
+template <uint32_t MODE, bool PACKED, bool FASTDIV>
+__global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
+    __shared__ float2 lpairs[LDS_PAIRS_MAX];
+    const bool lds = stage_pairs<PACKED>(a, lpairs);
+    const uint32_t tile = a.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t slice = tile * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    bool valid;
+    const float4 out = jacobi_particle<MODE, PACKED, FASTDIV>(a, lpairs, lds, slice, threadIdx.x & 63, valid);
     if (MODE & JM_AABB) {
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         aabb_fold(valid, out.x, out.y, out.z, lo, hi);
@@ -302,12 +315,53 @@ __global__ void k_aabb_of(const float4* P, uint32_t n, float* part) {
 }
 
 // ---------------------------------------------------------------------------
-// Collision broad phase.  The cell side is cell_size * 65/64 (a guard so f32
-// rounding in the key can never put two particles within cell_size two cells
-// apart), doubled until each axis has < 16383 cells and the count fits in 32
-// bits.  Key = ix + gx * (iy + gy * iz), dense over the current bounds, so no
-// two cells share a key.  Mirrors oracle_cell_grid / oracle_cell_keys.
+// Collision broad phase.  Each cloth gets a grid over its current bounds: the
+// cell side is cell_size * 65/64 (a guard so f32 rounding in the key can never
+// put two particles within cell_size two cells apart), doubled until each axis
+// has < 16383 cells and the cloth's cell count fits its key range
+// (2^(32-b) - 2 cells, b = bits numbering the cloths; b = 0 for one cloth).
+// Key = (cloth << (32-b)) + ix + gx * (iy + gy * iz): dense per cloth, so no
+// two cells share a key and different cloths never meet.  Mirrors
+// oracle_cell_grid / oracle_cell_keys / oracle_cell_keys_batch.
 // ---------------------------------------------------------------------------
+__device__ void grid_from_bounds(float lo[3], float hi[3], float cell_size, uint64_t max_cells, uint32_t key_base,
+                                 CollGrid* g) {
+    for (int a = 0; a < 3; ++a)
+        if (!(lo[a] <= hi[a])) lo[a] = hi[a] = 0.f;
+    float cell = cell_size * 1.015625f;
+    float inv = 1.0f / cell;
+    uint32_t dims[3] = {1, 1, 1};
+    bool ok = false;
+    for (int it = 0; it < 64; ++it) {
+        inv = 1.0f / cell;
+        float e[3];
+        bool fits = true;
+        for (int a = 0; a < 3; ++a) {
+            e[a] = (hi[a] - lo[a]) * inv;
+            if (!(e[a] < 16383.0f)) fits = false;
+        }
+        if (fits) {
+            for (int a = 0; a < 3; ++a) dims[a] = uint32_t(floorf(e[a])) + 1u;
+            const uint64_t total = uint64_t(dims[0]) * dims[1] * dims[2];
+            if (total <= max_cells) {
+                ok = true;
+                break;
+            }
+        }
+        cell = cell * 2.0f;
+    }
+    if (!ok) dims[0] = dims[1] = dims[2] = 1;
+    for (int a = 0; a < 3; ++a) {
+        g->lo[a] = lo[a];
+        g->hi[a] = hi[a];
+        g->dims[a] = dims[a];
+    }
+    g->cell = cell;
+    g->inv_cell = inv;
+    g->key_base = key_base;
+}
+
+// One cloth: reduce the per-block bounds partials of the producing kernel.
 __global__ __launch_bounds__(1024) void k_coll_grid(const float* part, uint32_t nparts, CollGrid* g,
                                                      float cell_size) {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -335,39 +389,32 @@ __global__ __launch_bounds__(1024) void k_coll_grid(const float* part, uint32_t 
             lo[a] = fminf(lo[a], red[w][a]);
             hi[a] = fmaxf(hi[a], red[w][3 + a]);
         }
-    for (int a = 0; a < 3; ++a)
-        if (!(lo[a] <= hi[a])) lo[a] = hi[a] = 0.f;
-    float cell = cell_size * 1.015625f;
-    float inv = 1.0f / cell;
-    uint32_t dims[3] = {1, 1, 1};
-    bool ok = false;
-    for (int it = 0; it < 64; ++it) {
-        inv = 1.0f / cell;
-        float e[3];
-        bool fits = true;
-        for (int a = 0; a < 3; ++a) {
-            e[a] = (hi[a] - lo[a]) * inv;
-            if (!(e[a] < 16383.0f)) fits = false;
-        }
-        if (fits) {
-            for (int a = 0; a < 3; ++a) dims[a] = uint32_t(floorf(e[a])) + 1u;
-            const uint64_t total = uint64_t(dims[0]) * dims[1] * dims[2];
-            if (total <= 0xFFFFFFFFull) {
-                ok = true;
-                break;
-            }
-        }
-        cell = cell * 2.0f;
+    grid_from_bounds(lo, hi, cell_size, 0xFFFFFFFEull, 0u, g);
+}
+
+// Many cloths: one block per cloth reduces its particle range.
+__global__ __launch_bounds__(256) void k_cloth_aabb(const float4* P, const uint32_t* cloth_off, float* part) {
+    const uint32_t c = blockIdx.x;
+    const uint32_t b = cloth_off[c], e = cloth_off[c + 1];
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const float4 v = P[i];
+        aabb_fold(true, v.x, v.y, v.z, lo, hi);
     }
-    if (!ok) dims[0] = dims[1] = dims[2] = 1;
+    block_aabb_store(lo, hi, part, c);
+}
+
+__global__ void k_coll_grids(const float* part, uint32_t nclothes, float cell_size, uint32_t local_bits,
+                             CollGrid* grids) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nclothes) return;
+    float lo[3], hi[3];
     for (int a = 0; a < 3; ++a) {
-        g->lo[a] = lo[a];
-        g->hi[a] = hi[a];
-        g->dims[a] = dims[a];
+        lo[a] = part[size_t(c) * 8 + a];
+        hi[a] = part[size_t(c) * 8 + 3 + a];
     }
-    g->cell = cell;
-    g->inv_cell = inv;
-    g->pad = 0;
+    const uint64_t cap = (local_bits >= 32 ? (1ull << 32) : (1ull << local_bits)) - 2;
+    grid_from_bounds(lo, hi, cell_size, cap, local_bits >= 32 ? 0u : (c << local_bits), &grids[c]);
 }
 
 __device__ __forceinline__ void cell_of(float4 p, const CollGrid& g, uint32_t c[3]) {
@@ -382,14 +429,14 @@ __device__ __forceinline__ void cell_of(float4 p, const CollGrid& g, uint32_t c[
     }
 }
 
-__global__ void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grid, uint32_t* keys, uint32_t* ids,
-                            uint2* table, uint32_t slots) {
-    const CollGrid g = *grid;
+__global__ void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of,
+                            uint32_t* keys, uint32_t* ids, uint2* table, uint32_t slots) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const CollGrid& g = grids[cloth_of ? cloth_of[i] : 0u];
         uint32_t c[3];
         cell_of(F[i], g, c);
-        keys[i] = c[0] + g.dims[0] * (c[1] + g.dims[1] * c[2]);
+        keys[i] = g.key_base + (c[0] + g.dims[0] * (c[1] + g.dims[1] * c[2]));
         ids[i] = i;
     }
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += stride)
@@ -635,7 +682,7 @@ __device__ __forceinline__ void sweep_rows(const CollArgs& a, const CollGrid& g,
         g0[r] = g1[r] = EMPTY_KEY;
         const int ny = int(c[1]) + (r % 3) - 1, nz = int(c[2]) + (r / 3) - 1;
         if (ny < 0 || nz < 0 || ny >= int(g.dims[1]) || nz >= int(g.dims[2])) continue;
-        const uint32_t kb = g.dims[0] * (uint32_t(ny) + g.dims[1] * uint32_t(nz));
+        const uint32_t kb = g.key_base + g.dims[0] * (uint32_t(ny) + g.dims[1] * uint32_t(nz));
         rows[r].kmin = kb + (c[0] > 0 ? c[0] - 1 : 0);
         rows[r].kmax = kb + (c[0] + 1 < g.dims[0] ? c[0] + 1 : c[0]);
         g0[r] = rows[r].kmin >> GROUP_SHIFT;
@@ -668,7 +715,7 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
         if (a.xout != a.prev) a.xout[i] = fi;    // timing runs write scratch
         return;
     }
-    const CollGrid g = *a.grid;
+    const CollGrid g = a.grids[a.cloth_of ? a.cloth_of[i] : 0u];
     const float th2 = a.p.thickness * a.p.thickness;
     uint32_t c[3];
     cell_of(fi, g, c);
@@ -893,6 +940,48 @@ __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Persistent Jacobi substep: predict and all I iterations in one launch, a
+// grid barrier between iterations (the batch layout's "one launch per
+// substep").  Blocks walk the slices in a grid-stride loop; the result is
+// identical to the per-iteration kernels.
+// ---------------------------------------------------------------------------
+template <bool PACKED, bool FASTDIV>
+__global__ __launch_bounds__(256) void k_jacobi_persistent(JacobiArgs a, PersistArgs q) {
+    __shared__ float2 lpairs[LDS_PAIRS_MAX];
+    const bool lds = stage_pairs<PACKED>(a, lpairs);
+    const DevParams& p = a.p;
+    uint32_t epoch = 0;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += nthreads) q.A[i] = predict(q.X[i], q.V[i], i, p);
+    grid_barrier(q.barrier, q.error, epoch);
+    const uint32_t spb = blockDim.x >> 6;
+    const uint32_t groups = (p.nslices + spb - 1) / spb;
+    JacobiArgs b = a;
+    float4* cur = q.A;
+    float4* nxt = q.B;
+    for (uint32_t k = 0; k < q.iterations; ++k) {
+        const bool last = k + 1 == q.iterations;
+        b.src = cur;
+        for (uint32_t gi = blockIdx.x; gi < groups; gi += gridDim.x) {
+            const uint32_t slice = gi * spb + (threadIdx.x >> 6);
+            bool valid;
+            if (last && q.final_write) {
+                b.dst = q.X;
+                b.vout = q.V;
+                jacobi_particle<JM_FINAL, PACKED, FASTDIV>(b, lpairs, lds, slice, threadIdx.x & 63, valid);
+            } else {
+                b.dst = nxt;
+                jacobi_particle<0, PACKED, FASTDIV>(b, lpairs, lds, slice, threadIdx.x & 63, valid);
+            }
+        }
+        if (!last) grid_barrier(q.barrier, q.error, epoch);
+        float4* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Render publish: finalize_normals (cloth_sim_jacobi.wgsl:527-547) with the
 // face normals gathered per vertex over its incident triangles in ascending
 // order (no fixed-point atomics), fallback +Y; positions copied with w = 0.
@@ -1055,11 +1144,18 @@ void launch_coll_grid(const float* part, uint32_t nparts, CollGrid* grid, float 
     hipLaunchKernelGGL(k_coll_grid, dim3(1), dim3(1024), 0, s, part, nparts, grid, cell_size);
 }
 
-void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grid, uint32_t* keys, uint32_t* ids,
-                      uint2* table, uint32_t slots, hipStream_t s) {
+void launch_coll_grids(const float4* P, const uint32_t* cloth_off, uint32_t nclothes, float* part, float cell_size,
+                       uint32_t local_bits, CollGrid* grids, hipStream_t s) {
+    hipLaunchKernelGGL(k_cloth_aabb, dim3(nclothes), dim3(256), 0, s, P, cloth_off, part);
+    hipLaunchKernelGGL(k_coll_grids, dim3(cdiv(nclothes, 64)), dim3(64), 0, s, part, nclothes, cell_size, local_bits,
+                       grids);
+}
+
+void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
+                      uint32_t* ids, uint2* table, uint32_t slots, hipStream_t s) {
     const uint32_t work = std::max(n, slots);
     hipLaunchKernelGGL(k_coll_keys, dim3(std::max(1u, std::min(cdiv(work, 256), 8192u))), dim3(256), 0, s, F, n,
-                       grid, keys, ids, table, slots);
+                       grids, cloth_of, keys, ids, table, slots);
 }
 
 uint32_t radix_hist_words(uint32_t n) { return 256u * std::max(1u, cdiv(n, RADIX_TILE)) + 256u; }
@@ -1096,6 +1192,29 @@ void launch_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float
 void launch_coll_sweep(const CollArgs& a, hipStream_t s) {
     if (a.p.n == 0) return;
     hipLaunchKernelGGL(k_coll_sweep, dim3(cdiv(a.p.n, 256)), dim3(256), 0, s, a);
+}
+
+uint32_t persistent_max_blocks(bool packed, bool fastdiv) {
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    hipError_t e;
+    if (packed) e = fastdiv ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jacobi_persistent<true, true>, 256, 0)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jacobi_persistent<true, false>, 256, 0);
+    else e = fastdiv ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jacobi_persistent<false, true>, 256, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jacobi_persistent<false, false>, 256, 0);
+    if (e != hipSuccess) return 0;
+    return uint32_t(std::max(0, per_cu) * std::max(0, cus));
+}
+
+void launch_jacobi_persistent(const JacobiArgs& a, const PersistArgs& q, uint32_t blocks, hipStream_t s) {
+    if (a.sell.packed) {
+        if (a.fastdiv) hipLaunchKernelGGL((k_jacobi_persistent<true, true>), dim3(blocks), dim3(256), 0, s, a, q);
+        else hipLaunchKernelGGL((k_jacobi_persistent<true, false>), dim3(blocks), dim3(256), 0, s, a, q);
+    } else {
+        if (a.fastdiv) hipLaunchKernelGGL((k_jacobi_persistent<false, true>), dim3(blocks), dim3(256), 0, s, a, q);
+        else hipLaunchKernelGGL((k_jacobi_persistent<false, false>), dim3(blocks), dim3(256), 0, s, a, q);
+    }
 }
 
 uint32_t gs_max_blocks(uint32_t block) {

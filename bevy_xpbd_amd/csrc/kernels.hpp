@@ -59,12 +59,26 @@ struct JacobiArgs {
     uint32_t num_pairs;  // packed: size of sell.pairs
 };
 
-// Cell grid of the collision broad phase (device memory, 12 words).
+// Persistent Jacobi substep: predict X,V -> A, iterate A/B; the last iterate
+// becomes the state X,V (final_write) or is left for a collision pass in A
+// (even iteration count) or B (odd).
+struct PersistArgs {
+    float4* X;
+    float4* V;
+    float4* A;
+    float4* B;
+    uint32_t iterations;
+    uint32_t final_write;
+    uint32_t* barrier;
+    uint32_t* error;
+};
+
+// Cell grid of one cloth in the collision broad phase (device memory, 12 words).
 struct CollGrid {
     float lo[3], hi[3];
     float cell, inv_cell;
     uint32_t dims[3];
-    uint32_t pad;
+    uint32_t key_base;   // cloth << (32 - b): first key of the cloth's range
 };
 
 struct CollArgs {
@@ -75,7 +89,8 @@ struct CollArgs {
     float4* xout;
     float4* vout;
     const uint8_t* ground;
-    const CollGrid* grid;
+    const CollGrid* grids;      // one per cloth
+    const uint32_t* cloth_of;   // particle -> cloth (nullptr: one cloth)
     const uint32_t* skeys;
     const uint32_t* sids;
     const uint2* table;
@@ -109,8 +124,10 @@ void launch_predict(const float4* X, const float4* V, float4* P, DevParams p, hi
 void launch_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p,
                      hipStream_t s);
 void launch_coll_grid(const float* part, uint32_t nparts, CollGrid* grid, float cell_size, hipStream_t s);
-void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grid, uint32_t* keys, uint32_t* ids,
-                      uint2* table, uint32_t table_slots, hipStream_t s);
+void launch_coll_grids(const float4* P, const uint32_t* cloth_off, uint32_t nclothes, float* part, float cell_size,
+                       uint32_t local_bits, CollGrid* grids, hipStream_t s);
+void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
+                      uint32_t* ids, uint2* table, uint32_t table_slots, hipStream_t s);
 void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,
                        uint32_t* hist, hipStream_t s);
 uint32_t radix_hist_words(uint32_t n);
@@ -121,6 +138,8 @@ void launch_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float
 void launch_coll_sweep(const CollArgs& a, hipStream_t s);
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
 uint32_t gs_max_blocks(uint32_t block);
+uint32_t persistent_max_blocks(bool packed, bool fastdiv);
+void launch_jacobi_persistent(const JacobiArgs& a, const PersistArgs& q, uint32_t blocks, hipStream_t s);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
 void launch_publish(const float4* X, uint32_t n, const uint32_t* tris, const uint32_t* vt_off,
                     const uint32_t* vt_tri, float4* dpos, float4* dnrm, hipStream_t s);

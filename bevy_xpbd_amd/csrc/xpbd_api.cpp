@@ -73,9 +73,12 @@ struct xpbd_cloth {
     uint32_t *keys = nullptr, *ids = nullptr, *keys2 = nullptr, *ids2 = nullptr, *hist = nullptr;
     uint2* table = nullptr;
     uint32_t table_slots = 0;
-    float* aabb = nullptr;         // bounds partials, 8 floats per producing block
+    float* aabb = nullptr;         // bounds partials, 8 floats per producing block / cloth
     uint32_t aabb_slots = 0;
-    CollGrid* grid = nullptr;
+    CollGrid* grids = nullptr;     // one per cloth
+    uint32_t nclothes = 1, local_bits = 32;
+    uint32_t* cloth_of = nullptr;  // particle -> cloth (nullptr for one cloth)
+    uint32_t* cloth_off = nullptr; // C + 1 vertex offsets
     uint32_t *tris = nullptr, *vt_off = nullptr, *vt_tri = nullptr;
     uint32_t *barrier = nullptr, *err = nullptr;
     double* resid = nullptr;
@@ -88,6 +91,7 @@ struct xpbd_cloth {
 
     uint32_t jacobi_block = 256, remap = 1, fused_predict = 0;
     uint32_t gs_block = 256, gs_blocks = 0;
+    uint32_t pj_blocks = 0;        // persistent Jacobi grid (co-resident blocks)
 
     std::string msg;
 };
@@ -173,11 +177,18 @@ float coll_cell(const xpbd_cloth* h) {
 // Collision chain on iterate F: grid, keys, sort, table, sweep (which also
 // applies the ground plane and the velocity update).  Replaces the
 // reference's clear_atomics .. collide_apply sequence (cloth_jacobi.rs:774-868).
+// Cell grids of the current iterate: one cloth reduces the bounds partials
+// its last iteration wrote; many cloths get one bounds block each.
+void enqueue_grids(xpbd_cloth* h, const float4* F, uint32_t nparts, float cell, hipStream_t s) {
+    if (h->nclothes == 1) launch_coll_grid(h->aabb, nparts, h->grids, cell, s);
+    else launch_coll_grids(F, h->cloth_off, h->nclothes, h->aabb, cell, h->local_bits, h->grids, s);
+}
+
 void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint32_t nparts, float4* xout,
                        float4* vout) {
     hipStream_t s = h->stream;
-    launch_coll_grid(h->aabb, nparts, h->grid, coll_cell(h), s);
-    launch_coll_keys(F, h->n, h->grid, h->keys, h->ids, h->table, h->table_slots, s);
+    enqueue_grids(h, F, nparts, coll_cell(h), s);
+    launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->table_slots, s);
     launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
     launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, s);
     launch_coll_gather(F, h->ids, h->n, h->sP, s);
@@ -189,7 +200,8 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     c.xout = xout;
     c.vout = vout;
     c.ground = h->has_ground ? h->ground : nullptr;
-    c.grid = h->grid;
+    c.grids = h->grids;
+    c.cloth_of = h->cloth_of;
     c.skeys = h->keys;
     c.sids = h->ids;
     c.table = h->table;
@@ -228,6 +240,26 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         launch_gs(g, h->gs_blocks, h->gs_block, s);
         F = h->A;
         nparts = h->gs_blocks;
+    } else if (h->cfg.persistent_jacobi) {
+        // predict and all iterations in one launch; the result lands in X,V
+        // (no collision) or in A (even I) / B (odd I) for the collision pass.
+        launch_zero_u32(h->barrier, 1, s);
+        JacobiArgs a = jacobi_args(h, p);
+        a.vel = nullptr;
+        PersistArgs q{};
+        q.X = h->X;
+        q.V = h->V;
+        q.A = h->A;
+        q.B = h->B;
+        q.iterations = iters;
+        q.final_write = collision ? 0u : 1u;
+        q.barrier = h->barrier;
+        q.error = h->err;
+        launch_jacobi_persistent(a, q, h->pj_blocks, s);
+        if (collision) {
+            F = (iters % 2 == 0) ? h->A : h->B;
+            if (h->nclothes == 1) nparts = launch_aabb_of(F, h->n, h->aabb, s);
+        }
     } else if (!h->fused_predict) {
         // predict -> A, then iterations ping-pong A/B; the last one writes the
         // state in place (no collision) or the iterate F the collision reads.
@@ -243,7 +275,7 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
                 std::swap(cur, nxt);
             } else if (collision) {
                 a.dst = nxt;
-                nparts = launch_jacobi(JM_AABB, a, h->jacobi_block, s);
+                nparts = launch_jacobi(h->nclothes == 1 ? JM_AABB : 0u, a, h->jacobi_block, s);
                 F = nxt;
             } else {
                 a.dst = h->X;
@@ -498,9 +530,26 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     // sheet puts nearly every particle in its own group), load <= 1/2
     h->table_slots = next_pow2(uint32_t(std::max<size_t>(2 * n, 64)));
     TRY(dalloc(h, &h->table, h->table_slots));
-    h->aabb_slots = std::max<uint32_t>({h->nslices, (h->n + 255) / 256, 4096u, 1u});
+    h->aabb_slots = std::max<uint32_t>({h->nslices, (h->n + 255) / 256, 4096u, uint32_t(m.cloth_offsets.size()), 1u});
     TRY(dalloc(h, &h->aabb, size_t(h->aabb_slots) * 8));
-    TRY(dalloc(h, &h->grid, 1));
+    {
+        std::vector<uint32_t> offs = m.cloth_offsets;
+        if (offs.size() < 2 || offs.front() != 0 || offs.back() != n) offs = {0u, uint32_t(n)};
+        h->nclothes = uint32_t(offs.size() - 1);
+        uint32_t bits = 0;
+        while ((1u << bits) < h->nclothes) ++bits;
+        h->local_bits = h->nclothes <= 1 ? 32u : 32u - bits;
+        TRY(dalloc(h, &h->grids, h->nclothes));
+        if (h->nclothes > 1) {
+            std::vector<uint32_t> cof(n);
+            for (uint32_t c = 0; c < h->nclothes; ++c)
+                for (uint32_t i = offs[c]; i < offs[c + 1]; ++i) cof[i] = c;
+            TRY(dalloc(h, &h->cloth_of, n));
+            TRY(dalloc(h, &h->cloth_off, offs.size()));
+            TRY(upload(h, h->cloth_of, cof.data(), n));
+            TRY(upload(h, h->cloth_off, offs.data(), offs.size()));
+        }
+    }
 
     // vertex -> incident triangles (ascending) for the normals gather
     std::vector<uint32_t> vt_off(n + 1, 0), vt_tri(3 * t);
@@ -545,6 +594,9 @@ void apply_tuning(xpbd_cloth* h) {
     if (h->cfg.jacobi_block) h->jacobi_block = h->cfg.jacobi_block;
     if (h->cfg.xcd_remap) h->remap = h->cfg.xcd_remap == 2 ? 1u : 0u;
     h->fused_predict = h->cfg.fused_predict ? 1u : 0u;
+    // persistent Jacobi: every block must be resident at once (grid barrier)
+    const uint32_t groups = std::max(1u, (h->nslices + 3) / 4);
+    h->pj_blocks = std::max(1u, std::min(persistent_max_blocks(h->nbpk != nullptr, h->cfg.fast_division != 0), groups));
 }
 
 int autotune(xpbd_cloth* h) {
@@ -913,9 +965,9 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
     a.src = h->A;
     a.dst = h->B;
     auto prep_collision = [&]() {
-        const uint32_t np = launch_aabb_of(h->A, h->n, h->aabb, s);
-        launch_coll_grid(h->aabb, np, h->grid, coll_cell(h), s);
-        launch_coll_keys(h->A, h->n, h->grid, h->keys, h->ids, h->table, h->table_slots, s);
+        const uint32_t np = h->nclothes == 1 ? launch_aabb_of(h->A, h->n, h->aabb, s) : 0u;
+        enqueue_grids(h, h->A, np, coll_cell(h), s);
+        launch_coll_keys(h->A, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->table_slots, s);
         launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
         launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, s);
         launch_coll_gather(h->A, h->ids, h->n, h->sP, s);
@@ -930,7 +982,8 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         c.xout = h->B;
         c.vout = h->V2;
         c.ground = h->has_ground ? h->ground : nullptr;
-        c.grid = h->grid;
+        c.grids = h->grids;
+        c.cloth_of = h->cloth_of;
         c.skeys = h->keys;
         c.sids = h->ids;
         c.table = h->table;
@@ -1022,15 +1075,13 @@ XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t*
                                        uint32_t* sorted_ids) {
     if (!h || !grid_out || !sorted_keys || !sorted_ids || !(cell_size > 0.f)) return XPBD_E_INVALID;
     hipStream_t s = h->stream;
-    const uint32_t np = launch_aabb_of(h->X, h->n, h->aabb, s);
-    launch_coll_grid(h->aabb, np, h->grid, cell_size, s);
-    launch_coll_keys(h->X, h->n, h->grid, h->keys, h->ids, h->table, h->table_slots, s);
+    const uint32_t np = h->nclothes == 1 ? launch_aabb_of(h->X, h->n, h->aabb, s) : 0u;
+    enqueue_grids(h, h->X, np, cell_size, s);
+    launch_coll_keys(h->X, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->table_slots, s);
     launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
     HIPC(h, hipGetLastError());
     HIPC(h, hipStreamSynchronize(s));
-    CollGrid g{};
-    HIPC(h, hipMemcpy(&g, h->grid, sizeof(g), hipMemcpyDeviceToHost));
-    std::memcpy(grid_out, &g, 11 * sizeof(uint32_t));
+    HIPC(h, hipMemcpy(grid_out, h->grids, size_t(h->nclothes) * sizeof(CollGrid), hipMemcpyDeviceToHost));
     HIPC(h, hipMemcpy(sorted_keys, h->keys, size_t(h->n) * 4, hipMemcpyDeviceToHost));
     HIPC(h, hipMemcpy(sorted_ids, h->ids, size_t(h->n) * 4, hipMemcpyDeviceToHost));
     return XPBD_OK;

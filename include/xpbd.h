@@ -140,6 +140,9 @@ typedef struct xpbd_sim_config {
                                              division; both correctly rounded */
     uint32_t xcd_remap;                   /* Jacobi block order: 0 = default/autotuned,
                                              1 = hardware order, 2 = XCD-contiguous tiles */
+    uint32_t persistent_jacobi;           /* 1: predict + all I Jacobi iterations of a
+                                             substep in one co-resident launch with grid
+                                             barriers (same results) */
 } xpbd_sim_config;
 
 /* ClothSimUniforms (cloth_compute.rs:182-202). */
@@ -220,10 +223,10 @@ enum { XPBD_KERNEL_JACOBI_ITER = 0, XPBD_KERNEL_COLLISION_SWEEP = 1, XPBD_KERNEL
        XPBD_KERNEL_GS_SUBSTEP = 3, XPBD_KERNEL_SUBSTEP = 4 };
 XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* avg_ms);
 
-/* Debug/test hook for the collision broad phase: computes the cell grid and
+/* Debug/test hook for the collision broad phase: computes the cell grids and
  * the stable (key, id) radix sort of the CURRENT positions on the device and
- * copies them back.  grid_out: lo[3], hi[3], cell, inv_cell (floats) then
- * dims[3] (u32) — 11 words. */
+ * copies them back.  grid_out: 12 words per cloth — lo[3], hi[3], cell,
+ * inv_cell (floats), dims[3], key base (u32). */
 XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t* grid_out,
                                        uint32_t* sorted_keys, uint32_t* sorted_ids);
 

@@ -60,7 +60,8 @@ class SolverIn(C.Structure):
         ("n", C.c_uint32), ("inv_mass", C.c_void_p), ("rest_pos", C.c_void_p),
         ("ground", C.c_void_p), ("e", C.c_uint32), ("ci", C.c_void_p), ("cj", C.c_void_p),
         ("crest", C.c_void_p), ("ccomp", C.c_void_p), ("noffs", C.c_void_p),
-        ("nother", C.c_void_p), ("neid", C.c_void_p),
+        ("nother", C.c_void_p), ("neid", C.c_void_p), ("num_cloths", C.c_uint32),
+        ("cloth_offsets", C.c_void_p),
     ]
 
 
@@ -119,6 +120,7 @@ def lib():
         L.oracle_cell_grid.argtypes = [vp, u32, f32, C.POINTER(Grid)]
         L.oracle_cell_keys.argtypes = [vp, u32, C.POINTER(Grid), vp]
         L.oracle_sort_pairs.argtypes = [vp, u32, vp, vp]
+        L.oracle_cell_keys_batch.argtypes = [vp, vp, u32, f32, vp, vp]
         L.oracle_vertex_normals.argtypes = [vp, u32, vp, u32, vp]
         L.oracle_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -207,11 +209,14 @@ class Solver:
             "noffs": np.ascontiguousarray(mesh["neighbor_offsets"], np.uint32),
             "nother": np.ascontiguousarray(mesh["neighbor_other"], np.uint32),
             "neid": np.ascontiguousarray(mesh["neighbor_constraint_id"], np.uint32),
+            "coffs": np.ascontiguousarray(mesh.get("cloth_offsets", np.array([0, n])), np.uint32),
         }
         a = self.arrays
+        nc = max(len(a["coffs"]) - 1, 1)
         self.sin = SolverIn(n, _ptr(a["inv_mass"]), _ptr(a["rest"]), _ptr(a["ground"]),
                             len(a["ci"]), _ptr(a["ci"]), _ptr(a["cj"]), _ptr(a["crest"]),
-                            _ptr(a["ccomp"]), _ptr(a["noffs"]), _ptr(a["nother"]), _ptr(a["neid"]))
+                            _ptr(a["ccomp"]), _ptr(a["noffs"]), _ptr(a["nother"]), _ptr(a["neid"]),
+                            nc, _ptr(a["coffs"]) if nc > 1 else None)
 
     @staticmethod
     def _state(pos, vel, dtype):
@@ -293,6 +298,16 @@ def cell_keys(pos4: np.ndarray, g: Grid) -> np.ndarray:
     keys = np.empty(len(pos4), np.uint32)
     lib().oracle_cell_keys(_ptr(pos4), len(pos4), C.byref(g), _ptr(keys))
     return keys
+
+
+def cell_keys_batch(pos4: np.ndarray, cloth_offsets: np.ndarray, cell_size: float):
+    pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+    offs = np.ascontiguousarray(cloth_offsets, dtype=np.uint32)
+    nc = len(offs) - 1
+    grids = (Grid * nc)()
+    keys = np.empty(len(pos4), np.uint32)
+    lib().oracle_cell_keys_batch(_ptr(pos4), _ptr(offs), nc, cell_size, grids, _ptr(keys))
+    return list(grids), keys
 
 
 def sort_pairs(keys: np.ndarray):

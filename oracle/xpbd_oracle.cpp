@@ -809,17 +809,36 @@ void apply_collision_accum(T* pos, const om_solver_in* in, const std::vector<V3<
     }
 }
 
+// Cloth c's particle range; self-collision pairs never cross cloths (each
+// cloth is its own self_collision_resolve call in the reference's terms).
+inline void cloth_range(const om_solver_in* in, uint32_t c, size_t& b, size_t& e) {
+    if (in->num_cloths <= 1 || !in->cloth_offsets) {
+        b = 0;
+        e = in->n;
+    } else {
+        b = in->cloth_offsets[c];
+        e = in->cloth_offsets[c + 1];
+    }
+}
+inline uint32_t cloth_count(const om_solver_in* in) {
+    return (in->num_cloths <= 1 || !in->cloth_offsets) ? 1u : in->num_cloths;
+}
+
 template <class T>
 void self_collision_brute(const om_solver_in* in, T* pos, T thickness, T coll_scale) {
     const size_t n = in->n;
     std::vector<V3<T>> accum(n, v3<T>(0, 0, 0));
-    for (size_t i = 0; i < n; ++i)
-        for (size_t j = i + 1; j < n; ++j) {
-            V3<T> di, dj;
-            if (!collision_pair(pos, in, i, j, thickness, coll_scale, di, dj)) continue;
-            if (T(in->inv_mass[i]) > T(0)) accum[i] = accum[i] + di;
-            if (T(in->inv_mass[j]) > T(0)) accum[j] = accum[j] + dj;
-        }
+    for (uint32_t c = 0; c < cloth_count(in); ++c) {
+        size_t b, e;
+        cloth_range(in, c, b, e);
+        for (size_t i = b; i < e; ++i)
+            for (size_t j = i + 1; j < e; ++j) {
+                V3<T> di, dj;
+                if (!collision_pair(pos, in, i, j, thickness, coll_scale, di, dj)) continue;
+                if (T(in->inv_mass[i]) > T(0)) accum[i] = accum[i] + di;
+                if (T(in->inv_mass[j]) > T(0)) accum[j] = accum[j] + dj;
+            }
+    }
     apply_collision_accum(pos, in, accum);
 }
 
@@ -828,11 +847,12 @@ void self_collision_brute(const om_solver_in* in, T* pos, T thickness, T coll_sc
 // order the O(N^2) loop above adds them.  Cells are 2x the contact distance so
 // every pair within `thickness` lies in the 27-cell neighbourhood.
 template <class T>
-void self_collision_grid(const om_solver_in* in, T* pos, T thickness, T coll_scale) {
-    const size_t n = in->n;
-    if (n == 0) return;
+void self_collision_grid_range(const om_solver_in* in, const T* pos, size_t b, size_t e, T thickness,
+                               T coll_scale, std::vector<V3<T>>& accum) {
+    const size_t m = e - b;
+    if (m == 0) return;
     double lo[3] = {INFINITY, INFINITY, INFINITY};
-    for (size_t i = 0; i < n; ++i)
+    for (size_t i = b; i < e; ++i)
         for (int a = 0; a < 3; ++a) lo[a] = std::fmin(lo[a], double(pos[3 * i + a]));
     for (int a = 0; a < 3; ++a)
         if (!std::isfinite(lo[a])) lo[a] = 0.0;
@@ -845,24 +865,23 @@ void self_collision_grid(const om_solver_in* in, T* pos, T thickness, T coll_sca
         }
     };
     auto key_of = [](const int64_t c[3]) { return (uint64_t(c[2]) << 42) | (uint64_t(c[1]) << 21) | uint64_t(c[0]); };
-    std::vector<std::pair<uint64_t, uint32_t>> keyed(n);
-    for (size_t i = 0; i < n; ++i) {
+    std::vector<std::pair<uint64_t, uint32_t>> keyed(m);
+    for (size_t i = b; i < e; ++i) {
         int64_t c[3];
         cell_of(i, c);
-        keyed[i] = {key_of(c), uint32_t(i)};
+        keyed[i - b] = {key_of(c), uint32_t(i)};
     }
     std::sort(keyed.begin(), keyed.end());
     std::unordered_map<uint64_t, std::pair<uint32_t, uint32_t>> ranges;
-    ranges.reserve(n * 2);
-    for (size_t k = 0; k < n;) {
-        size_t e = k;
-        while (e < n && keyed[e].first == keyed[k].first) ++e;
-        ranges[keyed[k].first] = {uint32_t(k), uint32_t(e)};
-        k = e;
+    ranges.reserve(m * 2);
+    for (size_t k = 0; k < m;) {
+        size_t q = k;
+        while (q < m && keyed[q].first == keyed[k].first) ++q;
+        ranges[keyed[k].first] = {uint32_t(k), uint32_t(q)};
+        k = q;
     }
-    std::vector<V3<T>> accum(n, v3<T>(0, 0, 0));
 #pragma omp parallel for schedule(dynamic, 1024)
-    for (int64_t ii = 0; ii < int64_t(n); ++ii) {
+    for (int64_t ii = int64_t(b); ii < int64_t(e); ++ii) {
         const size_t i = size_t(ii);
         if (T(in->inv_mass[i]) <= T(0)) continue;
         int64_t c[3];
@@ -882,11 +901,21 @@ void self_collision_grid(const om_solver_in* in, T* pos, T thickness, T coll_sca
         V3<T> acc = v3<T>(0, 0, 0);
         for (uint32_t j : cand) {
             V3<T> dlo, dhi;
-            size_t a = std::min<size_t>(i, j), b = std::max<size_t>(i, j);
-            if (!collision_pair(pos, in, a, b, thickness, coll_scale, dlo, dhi)) continue;
+            size_t a = std::min<size_t>(i, j), bb = std::max<size_t>(i, j);
+            if (!collision_pair(pos, in, a, bb, thickness, coll_scale, dlo, dhi)) continue;
             acc = acc + (i == a ? dlo : dhi);
         }
         accum[i] = acc;
+    }
+}
+
+template <class T>
+void self_collision_grid(const om_solver_in* in, T* pos, T thickness, T coll_scale) {
+    std::vector<V3<T>> accum(in->n, v3<T>(0, 0, 0));
+    for (uint32_t c = 0; c < cloth_count(in); ++c) {
+        size_t b, e;
+        cloth_range(in, c, b, e);
+        self_collision_grid_range(in, pos, b, e, thickness, coll_scale, accum);
     }
     apply_collision_accum(pos, in, accum);
 }
@@ -1261,12 +1290,12 @@ void oracle_self_collision_f64(const om_solver_in* in, double* pos, double thick
 // rest-pose grid instead).  Per substep: axis-aligned bounds of the current
 // positions; cell = cell_size * 65/64; the cell doubles until every axis has
 // fewer than 16383 cells and the cell count fits in 32 bits.
-void oracle_cell_grid(const float* pos4, uint32_t n, float cell_size, om_grid* g) {
+static void cell_grid_capped(const float* pos4, size_t b, size_t e, float cell_size, uint64_t max_cells, om_grid* g) {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = 0; i < n; ++i)
+    for (size_t i = b; i < e; ++i)
         for (int a = 0; a < 3; ++a) {
-            float v = pos4[4 * size_t(i) + a] + 0.0f;  // -0 folds to +0
-            if (v < lo[a]) lo[a] = v;                  // NaN never compares: ignored
+            float v = pos4[4 * i + a] + 0.0f;  // -0 folds to +0
+            if (v < lo[a]) lo[a] = v;          // NaN never compares: ignored
             if (v > hi[a]) hi[a] = v;
         }
     for (int a = 0; a < 3; ++a)
@@ -1277,25 +1306,23 @@ void oracle_cell_grid(const float* pos4, uint32_t n, float cell_size, om_grid* g
     bool ok = false;
     for (int it = 0; it < 64; ++it) {
         inv = 1.0f / cell;
-        float e[3];
+        float ex[3];
         bool fits = true;
         for (int a = 0; a < 3; ++a) {
-            e[a] = (hi[a] - lo[a]) * inv;
-            if (!(e[a] < 16383.0f)) fits = false;
+            ex[a] = (hi[a] - lo[a]) * inv;
+            if (!(ex[a] < 16383.0f)) fits = false;
         }
         if (fits) {
-            for (int a = 0; a < 3; ++a) dims[a] = uint32_t(std::floor(e[a])) + 1u;
-            uint64_t total = uint64_t(dims[0]) * dims[1] * dims[2];
-            if (total <= 0xFFFFFFFFull) {
+            for (int a = 0; a < 3; ++a) dims[a] = uint32_t(std::floor(ex[a])) + 1u;
+            const uint64_t total = uint64_t(dims[0]) * dims[1] * dims[2];
+            if (total <= max_cells) {
                 ok = true;
                 break;
             }
         }
         cell = cell * 2.0f;
     }
-    if (!ok) {
-        dims[0] = dims[1] = dims[2] = 1;
-    }
+    if (!ok) dims[0] = dims[1] = dims[2] = 1;
     for (int a = 0; a < 3; ++a) {
         g->lo[a] = lo[a];
         g->hi[a] = hi[a];
@@ -1303,6 +1330,16 @@ void oracle_cell_grid(const float* pos4, uint32_t n, float cell_size, om_grid* g
     }
     g->cell = cell;
     g->inv_cell = inv;
+}
+
+// The build's collision cell key (no reference counterpart; the reference's
+// collision_flat_packed, cloth_sim_jacobi.wgsl:243-253, clamps into a capped
+// rest-pose grid instead).  Per substep: axis-aligned bounds of the current
+// positions; cell = cell_size * 65/64; the cell doubles until every axis has
+// fewer than 16383 cells and the cell count fits the key (2^32 - 2 cells for
+// one cloth).
+void oracle_cell_grid(const float* pos4, uint32_t n, float cell_size, om_grid* g) {
+    cell_grid_capped(pos4, 0, n, cell_size, 0xFFFFFFFEull, g);
 }
 
 void oracle_cell_keys(const float* pos4, uint32_t n, const om_grid* g, uint32_t* keys) {
@@ -1317,6 +1354,26 @@ void oracle_cell_keys(const float* pos4, uint32_t n, const om_grid* g, uint32_t*
             c[a] = uint32_t(q);
         }
         keys[i] = c[0] + g->dims[0] * (c[1] + g->dims[1] * c[2]);
+    }
+}
+
+static uint32_t cloth_key_bits(uint32_t c) {
+    uint32_t b = 0;
+    while ((1u << b) < c) ++b;
+    return c <= 1 ? 0u : b;
+}
+
+void oracle_cell_keys_batch(const float* pos4, const uint32_t* offs, uint32_t nc, float cell_size, om_grid* grids,
+                            uint32_t* keys) {
+    const uint32_t cb = cloth_key_bits(nc);
+    const uint32_t lb = 32 - cb;
+    const uint64_t cap = (cb == 0 ? (1ull << 32) : (1ull << lb)) - 2;
+    for (uint32_t c = 0; c < nc; ++c) {
+        const size_t b = offs[c], e = offs[c + 1];
+        cell_grid_capped(pos4, b, e, cell_size, cap, &grids[c]);
+        const uint32_t base = cb == 0 ? 0u : (c << lb);
+        oracle_cell_keys(pos4 + 4 * b, uint32_t(e - b), &grids[c], keys + b);
+        for (size_t i = b; i < e; ++i) keys[i] += base;
     }
 }
 

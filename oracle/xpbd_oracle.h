@@ -100,6 +100,8 @@ typedef struct om_solver_in {
     const uint32_t* noffs;      /* N+1 */
     const uint32_t* nother;     /* 2E  */
     const uint32_t* neid;       /* 2E  */
+    uint32_t num_cloths;        /* 0 or 1: one cloth; else cloths_offsets has C+1 */
+    const uint32_t* cloth_offsets;  /* self-collision stays within each cloth */
 } om_solver_in;
 
 enum { OM_SOLVER_JACOBI = 0, OM_SOLVER_GS = 1 };
@@ -144,6 +146,11 @@ typedef struct om_grid {
 } om_grid;
 void oracle_cell_grid(const float* pos4, uint32_t n, float cell_size, om_grid* g);
 void oracle_cell_keys(const float* pos4, uint32_t n, const om_grid* g, uint32_t* keys);
+/* Many cloths: cloth c gets its own grid over its particles, limited to
+ * 2^(32 - b) - 2 cells (b = bits to number the cloths, 0 for one cloth), and
+ * key = (c << (32 - b)) | local key.  grids: C entries. */
+void oracle_cell_keys_batch(const float* pos4, const uint32_t* cloth_offsets, uint32_t num_cloths,
+                            float cell_size, om_grid* grids, uint32_t* keys);
 void oracle_sort_pairs(const uint32_t* keys, uint32_t n, uint32_t* sorted_keys, uint32_t* sorted_ids);
 
 /* Relative constraint residual: sqrt(sum C^2) / sqrt(sum rest^2) (double). */

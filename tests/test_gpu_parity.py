@@ -22,21 +22,23 @@ def note(msg):
 
 
 def make(X, mesh, solver=0, S=8, I=10, coll_stride=1, use_graph=0, fused_predict=0, jacobi_block=0,
-         plain_adjacency=0, fast_division=0, **uk):
+         plain_adjacency=0, fast_division=0, persistent_jacobi=0, **uk):
     cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=I, solver=solver,
                                    collision_every_n_substeps=coll_stride, use_graph=use_graph,
                                    fused_predict=fused_predict, jacobi_block=jacobi_block,
-                                   plain_adjacency=plain_adjacency, fast_division=fast_division)
+                                   plain_adjacency=plain_adjacency, fast_division=fast_division,
+                                   persistent_jacobi=persistent_jacobi)
     u = X.ClothSimUniforms.default(solver, S, **uk)
     return X.ClothComputePlugin(mesh, cfg, u), cfg, u
 
 
 def run_pair(O, X, mesh_kw=None, mesh=None, solver=0, S=8, I=10, substeps=8, collision=False, fused_predict=0,
-             jacobi_block=0, plain_adjacency=0, fast_division=0, **uk):
+             jacobi_block=0, plain_adjacency=0, fast_division=0, persistent_jacobi=0, **uk):
     mesh = mesh or X.ClothMeshData.sheet(**mesh_kw)
     f = mesh.fields()
     sim, cfg, u = make(X, mesh, solver, S, I, fused_predict=fused_predict, jacobi_block=jacobi_block,
-                       plain_adjacency=plain_adjacency, fast_division=fast_division, **uk)
+                       plain_adjacency=plain_adjacency, fast_division=fast_division,
+                       persistent_jacobi=persistent_jacobi, **uk)
     s = O.Solver(f)
     p = oracle_params(O, X, u, cfg, collision=collision)
     pos = f["positions"].copy()
@@ -68,6 +70,17 @@ def test_jacobi_iteration_count_edge_cases(O, X, iters, fused, plain):
     for coll in (False, True):
         run_pair(O, X, dict(cols=20, rows=9, spacing=0.05, pins=2, jitter=1e-3, seed=2), S=8, I=iters,
                  substeps=4, collision=coll, fused_predict=fused, plain_adjacency=plain)
+
+
+@pytest.mark.parametrize("plain", [0, 1])
+@pytest.mark.parametrize("iters", [1, 2, 3, 10])
+def test_persistent_jacobi_bit_exact(O, X, iters, plain):
+    """Predict and all iterations of a substep in one launch (grid barriers
+    between iterations): the same results as one launch per iteration."""
+    note(f"case persistent iters={iters} plain_adjacency={plain}")
+    for coll in (False, True):
+        run_pair(O, X, dict(cols=70, rows=33, spacing=0.03, pins=2, jitter=1e-3, seed=9), S=8, I=iters,
+                 substeps=3, collision=coll, plain_adjacency=plain, persistent_jacobi=1)
 
 
 def test_reference_grid_plain_adjacency(O, X):
@@ -229,11 +242,13 @@ def test_publish_layout_and_normals(O, X, bound):
     dnrm.free()
 
 
-def test_graph_replay_equals_eager(X):
-    note("case graph")
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_graph_replay_equals_eager(X, persistent):
+    note(f"case graph persistent={persistent}")
     kw = dict(cols=48, rows=48, spacing=1 / 47, layout=1, height=1.0, angle=0.4, pins=3)
     th = 2 * 0.4 / 47
-    a, _, _ = make(X, X.ClothMeshData.sheet(**kw), S=4, I=8, use_graph=1, thickness=th)
+    a, _, _ = make(X, X.ClothMeshData.sheet(**kw), S=4, I=8, use_graph=1, thickness=th,
+                   persistent_jacobi=persistent)
     b, _, _ = make(X, X.ClothMeshData.sheet(**kw), S=4, I=8, use_graph=0, thickness=th)
     for k in range(4):
         a.step_frames(5)
