@@ -853,33 +853,103 @@ __global__ void k_predict(const float4* X, const float4* V, float4* P, DevParams
 // particle, so the parallel sweep equals the sequential one in colour-sorted
 // order (xpbd_cpu.rs:344-386; gs_edges, cloth_sim.wgsl:154-210).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void grid_barrier(uint32_t* counter, uint32_t* err, uint32_t& epoch) {
-    __syncthreads();
-    epoch += 1;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        atomicAdd(counter, 1u);
-        const uint32_t target = epoch * gridDim.x;
-        uint32_t spins = 0;
-        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(2);
-            if (++spins > (1u << 22)) {   // a few seconds at most: report instead of hanging
-                atomicExch(err, 1u);
-                break;
-            }
+// Grid barrier for co-resident persistent launches.  Device memory from
+// hipMalloc is cached in each XCD's L2 without coherence between XCDs, so data
+// stored before a barrier has to be written back from the producing XCD's L2
+// and invalidated in the consuming one.  Doing that in every block (an
+// agent-scope release + acquire per block) costs ~95 us per barrier at 2048
+// blocks; here the last block to arrive on each XCD does it once for the whole
+// XCD (one L2 writeback per XCD) and the others only acquire (invalidate).  Blocks find their XCD
+// from the XCC_ID hardware register.  Every wait is bounded: on expiry the
+// error word is set and the kernel runs to its end instead of hanging.
+//
+// Words (each hot counter on its own 256-byte line, zeroed before the launch):
+// [0] XCD leaders arrived, [L] generation, [2L] blocks counted at set-up,
+// [(4+x)L] blocks of XCD x arrived, [(12+x)L] XCD x generation,
+// [(20+x)L] blocks resident on XCD x.
+constexpr uint32_t GS_LINE = 64;
+static_assert(GRID_SYNC_WORDS >= 28 * GS_LINE, "grid barrier words");
+
+__device__ __forceinline__ uint32_t gs_load(uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t gs_add(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gs_wait(uint32_t* p, uint32_t v, uint32_t* err) {
+    uint32_t spins = 0;
+    while (gs_load(p) < v) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1u << 22)) {   // seconds at most: report instead of hanging
+            atomicExch(err, 1u);
+            return;
         }
     }
-    __syncthreads();
-    __threadfence();
 }
+
+struct GridSync {
+    uint32_t* w;
+    uint32_t* err;
+    uint32_t xcd, nx, nxcd, epoch;
+
+    __device__ void init(uint32_t* words, uint32_t* error) {
+        __shared__ uint32_t sh[2];
+        w = words;
+        err = error;
+        epoch = 0;
+        if (threadIdx.x == 0) {
+            const uint32_t x = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u;   // HW_REG_XCC_ID[3:0]
+            gs_add(w + GS_LINE * (20 + x), 1u);
+            __hip_atomic_fetch_add(w + GS_LINE * 2, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            gs_wait(w + GS_LINE * 2, gridDim.x, err);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            uint32_t n = 0;
+            for (uint32_t k = 0; k < 8; ++k) n += gs_load(w + GS_LINE * (20 + k)) ? 1u : 0u;
+            sh[0] = x;
+            sh[1] = gs_load(w + GS_LINE * (20 + x)) | (n << 16);
+        }
+        __syncthreads();
+        xcd = sh[0];
+        nx = sh[1] & 0xFFFFu;
+        nxcd = sh[1] >> 16;
+    }
+
+    __device__ __forceinline__ void sync() {
+        // __syncthreads does not wait for global stores (within a CU they are
+        // ordered through the shared L1): each wave drains its own first, so
+        // they are in L2 before the XCD leader writes L2 back.
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        epoch += 1;
+        if (threadIdx.x == 0) {
+            const uint32_t old = gs_add(w + GS_LINE * (4 + xcd), 1u);
+            if (old + 1 == epoch * nx) {   // last block of this XCD
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // write back this XCD's L2
+                if (gs_add(w, 1u) + 1 == epoch * nxcd)
+                    __hip_atomic_store(w + GS_LINE, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                gs_wait(w + GS_LINE, epoch, err);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // invalidate this XCD's L2
+                __hip_atomic_store(w + GS_LINE * (12 + xcd), epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                gs_wait(w + GS_LINE * (12 + xcd), epoch, err);
+                // drop stale lines of this CU's L1 (gfx950 has no L1-only
+                // invalidate that works here: the workgroup-scope one is a
+                // no-op, so this is the agent-scope one, once per block)
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+        }
+        __syncthreads();
+    }
+};
 
 __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
     const DevParams& p = a.p;
     const uint32_t nthreads = gridDim.x * blockDim.x;
     const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t epoch = 0;
+    GridSync gsync;
+    gsync.init(a.barrier, a.error);
     for (uint32_t i = gt; i < p.n; i += nthreads) a.W[i] = predict(a.X[i], a.V[i], i, p);
-    grid_barrier(a.barrier, a.error, epoch);
+    gsync.sync();
     for (uint32_t it = 0; it < a.iterations; ++it) {
         for (uint32_t b = 0; b < a.num_batches; ++b) {
             const uint32_t e1 = a.batch_offsets[b + 1];
@@ -920,7 +990,7 @@ __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
                 if (wi > 0.f) a.W[ij.x] = make_float4(pi.x + ix, pi.y + iy, pi.z + iz, wi);
                 if (wj > 0.f) a.W[ij.y] = make_float4(pj.x + jx, pj.y + jy, pj.z + jz, wj);
             }
-            grid_barrier(a.barrier, a.error, epoch);
+            gsync.sync();
         }
     }
     if (a.final_write) {
@@ -950,10 +1020,11 @@ __global__ __launch_bounds__(256) void k_jacobi_persistent(JacobiArgs a, Persist
     __shared__ float2 lpairs[LDS_PAIRS_MAX];
     const bool lds = stage_pairs<PACKED>(a, lpairs);
     const DevParams& p = a.p;
-    uint32_t epoch = 0;
+    GridSync gsync;
+    gsync.init(q.barrier, q.error);
     const uint32_t nthreads = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += nthreads) q.A[i] = predict(q.X[i], q.V[i], i, p);
-    grid_barrier(q.barrier, q.error, epoch);
+    gsync.sync();
     const uint32_t spb = blockDim.x >> 6;
     const uint32_t groups = (p.nslices + spb - 1) / spb;
     JacobiArgs b = a;
@@ -974,7 +1045,7 @@ __global__ __launch_bounds__(256) void k_jacobi_persistent(JacobiArgs a, Persist
                 jacobi_particle<0, PACKED, FASTDIV>(b, lpairs, lds, slice, threadIdx.x & 63, valid);
             }
         }
-        if (!last) grid_barrier(q.barrier, q.error, epoch);
+        if (!last) gsync.sync();
         float4* t = cur;
         cur = nxt;
         nxt = t;

@@ -98,6 +98,10 @@ struct CollArgs {
     DevParams p;
 };
 
+// Words of the grid-barrier state of a persistent launch (zeroed before each
+// launch with launch_zero_u32).
+constexpr uint32_t GRID_SYNC_WORDS = 2048;
+
 struct GsArgs {
     const float4* X;
     const float4* V;

@@ -219,7 +219,7 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
     const float4* F = nullptr;
     uint32_t nparts = 0;
     if (h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL) {
-        launch_zero_u32(h->barrier, 1, s);
+        launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
         GsArgs g{};
         g.X = h->X;
         g.V = h->V;
@@ -243,7 +243,7 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
     } else if (h->cfg.persistent_jacobi) {
         // predict and all iterations in one launch; the result lands in X,V
         // (no collision) or in A (even I) / B (odd I) for the collision pass.
-        launch_zero_u32(h->barrier, 1, s);
+        launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
         JacobiArgs a = jacobi_args(h, p);
         a.vel = nullptr;
         PersistArgs q{};
@@ -567,11 +567,11 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(upload(h, h->vt_off, vt_off.data(), n + 1));
     TRY(upload(h, h->vt_tri, vt_tri.data(), 3 * t));
 
-    TRY(dalloc(h, &h->barrier, 4));
+    TRY(dalloc(h, &h->barrier, GRID_SYNC_WORDS));
     TRY(dalloc(h, &h->err, 4));
     TRY(dalloc(h, &h->resid, 2));
     HIPC(h, hipMemset(h->err, 0, 16));
-    HIPC(h, hipMemset(h->barrier, 0, 16));
+    HIPC(h, hipMemset(h->barrier, 0, GRID_SYNC_WORDS * sizeof(uint32_t)));
     return XPBD_OK;
 }
 
@@ -814,7 +814,14 @@ XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, cons
     if (r != XPBD_OK) return r;
     r = upload_state(h, h->init_pos4.data(), nullptr);
     if (r != XPBD_OK) return r;
-    h->gs_blocks = std::min<uint32_t>(gs_max_blocks(h->gs_block), std::max(1u, (h->e + h->gs_block - 1) / h->gs_block));
+    // GS grid: enough threads for the largest colour (one constraint each) and
+    // a quarter of the particles, within what can be resident at once (every
+    // block takes part in each grid barrier, whose cost grows with the grid).
+    uint32_t widest = 0;
+    for (size_t b = 0; b + 1 < h->batch_offsets.size(); ++b)
+        widest = std::max(widest, h->batch_offsets[b + 1] - h->batch_offsets[b]);
+    const uint32_t gs_work = std::max(widest, (h->n + 3) / 4);
+    h->gs_blocks = std::min<uint32_t>(gs_max_blocks(h->gs_block), std::max(1u, (gs_work + h->gs_block - 1) / h->gs_block));
     h->gs_blocks = std::max<uint32_t>(h->gs_blocks, 1u);
     apply_tuning(h);
     if (cfg->autotune) {
@@ -1021,7 +1028,7 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
                 launch_radix_pass(h->keys, h->ids, h->keys2, h->ids2, h->n, 0, h->hist, s);
                 break;
             case XPBD_KERNEL_GS_SUBSTEP:
-                launch_zero_u32(h->barrier, 1, s);
+                launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
                 launch_gs(g, h->gs_blocks, h->gs_block, s);
                 break;
             default: break;
