@@ -21,6 +21,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILE_FRAMES = 10
 
 
 def log(msg: str):
@@ -103,7 +104,9 @@ def main():
     if args.kernels_only:
         jac_ms = sim.time_kernel(X.KERNEL_JACOBI_ITER, 20)
         sweep_ms = sim.time_kernel(X.KERNEL_COLLISION_SWEEP, 5)
-        log(f"bench: kernels only: jacobi {jac_ms * 1e3:.1f} us, sweep {sweep_ms * 1e3:.1f} us")
+        copy_ms = sim.time_kernel(X.KERNEL_COPY, 10)
+        log(f"bench: kernels only: jacobi {jac_ms * 1e3:.1f} us, sweep {sweep_ms * 1e3:.1f} us, "
+            f"copy of {32 * n} B {copy_ms * 1e3:.1f} us")
         sim.close()
         return
     log(f"bench: device state uploaded; warmup {args.warmup} frames")
@@ -119,10 +122,15 @@ def main():
     ms_per_step = 1000.0 * el / args.steps
     log(f"bench: {value:.1f} substeps/s ({ms_per_step:.3f} ms/frame)")
 
-    # dominant kernel: one Jacobi iteration, timed with HIP events on the handle's stream
-    jac_ms = sim.time_kernel(X.KERNEL_JACOBI_ITER, 50)
-    sweep_ms = sim.time_kernel(X.KERNEL_COLLISION_SWEEP, 20)
+    # dominant kernel: one Jacobi iteration.  Its mean launch duration comes
+    # from HIP events on the handle's stream around every iteration launch of
+    # PROFILE_FRAMES further frames of the same run (eager, live state).
+    jac_ms, jac_launches = sim.profile_frames(PROFILE_FRAMES, X.KERNEL_JACOBI_ITER)
+    sweep_ms, _ = sim.profile_frames(PROFILE_FRAMES, X.KERNEL_COLLISION_SWEEP)
+    jac_rep_ms = sim.time_kernel(X.KERNEL_JACOBI_ITER, 50)
     radix_ms = sim.time_kernel(X.KERNEL_RADIX_PASS, 20)
+    copy_ms = sim.time_kernel(X.KERNEL_COPY, 20)
+    pers_ms = sim.time_kernel(X.KERNEL_JACOBI_PERSISTENT, 5)
     b_iter = algorithmic_bytes_per_iteration(n, e)
     achieved = b_iter / (jac_ms * 1e-3) / 1e9
     table_slots = 1 << max(6, (2 * n - 1).bit_length())
@@ -169,11 +177,14 @@ def main():
                    "residual_after_timed_run": residual,
                    "substep_algorithmic_GBps": round(substep_gbs, 1),
                    "substep_hbm_frac": round(substep_gbs / HBM_PEAK_GBS, 4),
-                   "collision_sweep_us": round(sweep_ms * 1e3, 2), "radix_pass_us": round(radix_ms * 1e3, 2)},
+                   "collision_sweep_us": round(sweep_ms * 1e3, 2), "radix_pass_us": round(radix_ms * 1e3, 2),
+                   "copy_kernel_GBps": round(32 * n / (copy_ms * 1e-3) / 1e9, 1),
+                   "persistent_predict_plus_iterations_us": round(pers_ms * 1e3, 2)},
         "roofline": {"bound": "hbm", "kernel": "k_jacobi (one iteration)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "avg_launch_us": round(jac_ms * 1e3, 2),
-                     "algorithmic_bytes_per_launch": b_iter},
+                     "launches_timed": jac_launches, "algorithmic_bytes_per_launch": b_iter,
+                     "back_to_back_launch_us": round(jac_rep_ms * 1e3, 2)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

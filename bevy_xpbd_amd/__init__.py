@@ -27,6 +27,8 @@ from .api import (  # noqa: F401
     KERNEL_COLLISION_SWEEP,
     KERNEL_RADIX_PASS,
     KERNEL_GS_SUBSTEP,
+    KERNEL_COPY,
+    KERNEL_JACOBI_PERSISTENT,
     SOLVER_JACOBI,
     SOLVER_GAUSS_SEIDEL,
     LAYOUT_HANGING_XY,

@@ -21,6 +21,7 @@ SOLVER_JACOBI, SOLVER_GAUSS_SEIDEL = 0, 1
 LAYOUT_HANGING_XY, LAYOUT_HORIZONTAL_XZ = 0, 1
 PIN_NONE, PIN_TWO_CORNERS, PIN_TOP_ROW, PIN_CENTER = 0, 1, 2, 3
 KERNEL_JACOBI_ITER, KERNEL_COLLISION_SWEEP, KERNEL_RADIX_PASS, KERNEL_GS_SUBSTEP = 0, 1, 2, 3
+KERNEL_COPY, KERNEL_JACOBI_PERSISTENT = 4, 5
 
 _STATUS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "unsupported",
            5: "bad state", 6: "no HIP device"}
@@ -162,6 +163,7 @@ def lib():
             "xpbd_bind_render_buffers": (i32, [vp, vp, vp]),
             "xpbd_residual": (i32, [vp, C.POINTER(C.c_double)]),
             "xpbd_time_kernel": (i32, [vp, i32, u32, C.POINTER(C.c_float)]),
+            "xpbd_profile_frames": (i32, [vp, u32, i32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
             "xpbd_debug_collision_keys": (i32, [vp, f32, vp, vp, vp]),
             "xpbd_selftest_division": (i32, [i32, i32, u64, u64, vp]),
         }
@@ -375,6 +377,15 @@ class ClothComputePlugin:
         ms = C.c_float()
         _check(lib().xpbd_time_kernel(self._h, which, reps, C.byref(ms)), "xpbd_time_kernel", self._h)
         return ms.value
+
+    def profile_frames(self, frames: int, which: int):
+        """Advance `frames` frames eagerly with HIP events around every launch
+        of kernel kind `which`; returns (mean duration in ms, launches)."""
+        ms = C.c_float()
+        cnt = C.c_uint32()
+        _check(lib().xpbd_profile_frames(self._h, frames, which, C.byref(ms), C.byref(cnt)), "xpbd_profile_frames",
+               self._h)
+        return ms.value, cnt.value
 
     def debug_collision_keys(self, cell_size: float, num_cloths: int = 1):
         grid = np.zeros((num_cloths, 12), np.uint32)

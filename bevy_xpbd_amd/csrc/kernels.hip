@@ -301,6 +301,11 @@ __global__ void k_finalize(const float4* F, float4* X, float4* V, const uint8_t*
     finish_particle(f, X[i], i, ground, p, X, V);
 }
 
+__global__ void k_copy4(const float4* __restrict__ a, float4* __restrict__ b, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
@@ -731,6 +736,9 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
         lx[q] = ly[q] = lz[q] = 0.f;
     }
     int count = 0;
+    // rows[] is indexed with compile-time indices only (unrolled loops), so it
+    // stays in registers instead of the per-thread scratch segment
+#pragma unroll
     for (int r = 0; r < 9; ++r) {
         const uint32_t kmin = rows[r].kmin, kmax = rows[r].kmax;
         uint32_t idx = rows[r].start;
@@ -797,6 +805,7 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
         for (;;) {
             uint32_t best = EMPTY_KEY;
             float bx = 0.f, by = 0.f, bz = 0.f;
+#pragma unroll
             for (int r = 0; r < 9; ++r) {
                 if (rows[r].start == EMPTY_KEY) continue;
                 for (uint32_t idx = rows[r].start; idx < n; ++idx) {
@@ -1199,6 +1208,10 @@ void launch_predict(const float4* X, const float4* V, float4* P, DevParams p, hi
 void launch_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p, hipStream_t s) {
     if (p.n == 0) return;
     hipLaunchKernelGGL(k_finalize, dim3(cdiv(p.n, 256)), dim3(256), 0, s, F, X, V, ground, p);
+}
+
+void launch_copy4(const float4* a, float4* b, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy4, dim3(std::max(1u, cdiv(n, 256))), dim3(256), 0, s, a, b, n);
 }
 
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {

@@ -142,6 +142,7 @@ void launch_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float
 void launch_coll_sweep(const CollArgs& a, hipStream_t s);
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
 uint32_t gs_max_blocks(uint32_t block);
+void launch_copy4(const float4* a, float4* b, uint32_t n, hipStream_t s);
 uint32_t persistent_max_blocks(bool packed, bool fastdiv);
 void launch_jacobi_persistent(const JacobiArgs& a, const PersistArgs& q, uint32_t blocks, hipStream_t s);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);

@@ -93,6 +93,14 @@ struct xpbd_cloth {
     uint32_t gs_block = 256, gs_blocks = 0;
     uint32_t pj_blocks = 0;        // persistent Jacobi grid (co-resident blocks)
 
+    // instrumented frames (xpbd_profile_frames): events around every launch of
+    // one kernel kind, enqueued eagerly
+    struct Probe {
+        int which = -1;
+        std::vector<hipEvent_t> ev;   // pairs
+        size_t used = 0;
+    } probe;
+
     std::string msg;
 };
 
@@ -184,6 +192,21 @@ void enqueue_grids(xpbd_cloth* h, const float4* F, uint32_t nparts, float cell, 
     else launch_coll_grids(F, h->cloth_off, h->nclothes, h->aabb, cell, h->local_bits, h->grids, s);
 }
 
+// Instrumented frames: bracket a launch of kind `which` with a pair of events.
+struct ProbeScope {
+    xpbd_cloth* h;
+    bool on;
+    ProbeScope(xpbd_cloth* hh, int which) : h(hh), on(hh->probe.which == which && hh->probe.used + 2 <= hh->probe.ev.size()) {
+        if (on) (void)hipEventRecord(h->probe.ev[h->probe.used], h->stream);
+    }
+    ~ProbeScope() {
+        if (on) {
+            (void)hipEventRecord(h->probe.ev[h->probe.used + 1], h->stream);
+            h->probe.used += 2;
+        }
+    }
+};
+
 void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint32_t nparts, float4* xout,
                        float4* vout) {
     hipStream_t s = h->stream;
@@ -207,6 +230,7 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     c.table = h->table;
     c.table_mask = h->table_slots - 1;
     c.p = p;
+    ProbeScope ps(h, XPBD_KERNEL_COLLISION_SWEEP);
     launch_coll_sweep(c, s);
 }
 
@@ -237,7 +261,10 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         g.error = h->err;
         g.final_write = collision ? 0u : 1u;
         g.p = p;
-        launch_gs(g, h->gs_blocks, h->gs_block, s);
+        {
+            ProbeScope ps(h, XPBD_KERNEL_GS_SUBSTEP);
+            launch_gs(g, h->gs_blocks, h->gs_block, s);
+        }
         F = h->A;
         nparts = h->gs_blocks;
     } else if (h->cfg.persistent_jacobi) {
@@ -255,7 +282,10 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         q.final_write = collision ? 0u : 1u;
         q.barrier = h->barrier;
         q.error = h->err;
-        launch_jacobi_persistent(a, q, h->pj_blocks, s);
+        {
+            ProbeScope ps(h, XPBD_KERNEL_JACOBI_PERSISTENT);
+            launch_jacobi_persistent(a, q, h->pj_blocks, s);
+        }
         if (collision) {
             F = (iters % 2 == 0) ? h->A : h->B;
             if (h->nclothes == 1) nparts = launch_aabb_of(F, h->n, h->aabb, s);
@@ -271,6 +301,7 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
             a.src = cur;
             if (k + 1 < iters) {
                 a.dst = nxt;
+                ProbeScope ps(h, XPBD_KERNEL_JACOBI_ITER);
                 launch_jacobi(0, a, h->jacobi_block, s);
                 std::swap(cur, nxt);
             } else if (collision) {
@@ -302,6 +333,7 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
             for (uint32_t k = 1; k + 1 < iters; ++k) {
                 a.src = cur;
                 a.dst = nxt;
+                ProbeScope ps(h, XPBD_KERNEL_JACOBI_ITER);
                 launch_jacobi(0, a, h->jacobi_block, s);
                 std::swap(cur, nxt);
             }
@@ -1031,10 +1063,26 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
                 launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
                 launch_gs(g, h->gs_blocks, h->gs_block, s);
                 break;
+            case XPBD_KERNEL_COPY: launch_copy4(h->X, h->B, h->n, s); break;
+            case XPBD_KERNEL_JACOBI_PERSISTENT: {
+                launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
+                JacobiArgs pa = jacobi_args(h, p);
+                PersistArgs q{};
+                q.X = h->X;   // read only: the result stays in A/B
+                q.V = h->V;
+                q.A = h->A;
+                q.B = h->B;
+                q.iterations = std::max(h->cfg.solve_inner_iterations, 1u);
+                q.final_write = 0;
+                q.barrier = h->barrier;
+                q.error = h->err;
+                launch_jacobi_persistent(pa, q, h->pj_blocks, s);
+                break;
+            }
             default: break;
         }
     };
-    if (which < XPBD_KERNEL_JACOBI_ITER || which > XPBD_KERNEL_GS_SUBSTEP) {
+    if (which < XPBD_KERNEL_JACOBI_ITER || which > XPBD_KERNEL_JACOBI_PERSISTENT) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         return XPBD_E_INVALID;
@@ -1051,6 +1099,53 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
     HIPC(h, hipGetLastError());
     TRY(check_device_flag(h));
     *avg_ms = ms / float(reps);
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_profile_frames(xpbd_cloth* h, uint32_t frames, int which, float* avg_ms, uint32_t* launches) {
+    if (!h || !avg_ms || frames == 0) return XPBD_E_INVALID;
+    if (which != XPBD_KERNEL_JACOBI_ITER && which != XPBD_KERNEL_COLLISION_SWEEP && which != XPBD_KERNEL_GS_SUBSTEP &&
+        which != XPBD_KERNEL_JACOBI_PERSISTENT)
+        return XPBD_E_INVALID;
+    if (h->n == 0) return XPBD_E_UNSUPPORTED;
+    const uint32_t S = std::max(h->cfg.solve_substeps, 1u);
+    const uint32_t I = std::max(h->cfg.solve_inner_iterations, 1u);
+    const size_t pairs = size_t(S) * I + 1;   // upper bound of launches of one kind per frame
+    auto& pr = h->probe;
+    pr.ev.assign(2 * pairs, nullptr);
+    int rc = XPBD_OK;
+    for (auto& e : pr.ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = XPBD_E_HIP;
+    double total = 0.0;
+    uint32_t count = 0;
+    for (uint32_t f = 0; f < frames && rc == XPBD_OK; ++f) {
+        pr.which = which;
+        pr.used = 0;
+        enqueue_frame(h);
+        pr.which = -1;
+        if (hipStreamSynchronize(h->stream) != hipSuccess) {
+            rc = fail(h, XPBD_E_HIP, "profile_frames: stream synchronize failed");
+            break;
+        }
+        for (size_t k = 0; k + 1 < pr.used; k += 2) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, pr.ev[k], pr.ev[k + 1]) != hipSuccess) {
+                rc = fail(h, XPBD_E_HIP, "profile_frames: event timing failed");
+                break;
+            }
+            total += ms;
+            ++count;
+        }
+    }
+    pr.which = -1;
+    for (auto& e : pr.ev)
+        if (e) (void)hipEventDestroy(e);
+    pr.ev.clear();
+    if (rc != XPBD_OK) return rc;
+    HIPC(h, hipGetLastError());
+    TRY(check_device_flag(h));
+    *avg_ms = count ? float(total / count) : 0.f;
+    if (launches) *launches = count;
     return XPBD_OK;
 }
 

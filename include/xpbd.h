@@ -218,10 +218,20 @@ XPBD_API int xpbd_residual(xpbd_cloth* h, double* out);
 
 /* Measurement: average duration [ms] of one launch of kernel `which`
  * (XPBD_KERNEL_*) on the handle's stream, over `reps` launches bracketed by
- * HIP events.  The kernel runs on scratch buffers; the state is untouched. */
+ * HIP events.  The kernel runs on scratch buffers; the state is untouched.
+ * COPY is a plain 16N-byte read + 16N-byte write of the position buffer (the
+ * calibration for hardware byte counters); JACOBI_PERSISTENT is one substep's
+ * predict + I iterations in the persistent kernel. */
 enum { XPBD_KERNEL_JACOBI_ITER = 0, XPBD_KERNEL_COLLISION_SWEEP = 1, XPBD_KERNEL_RADIX_PASS = 2,
-       XPBD_KERNEL_GS_SUBSTEP = 3, XPBD_KERNEL_SUBSTEP = 4 };
+       XPBD_KERNEL_GS_SUBSTEP = 3, XPBD_KERNEL_COPY = 4, XPBD_KERNEL_JACOBI_PERSISTENT = 5 };
 XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* avg_ms);
+
+/* Measurement on the live state: advances the simulation by `frames` frames,
+ * enqueued eagerly (no graph) with a pair of HIP events around every launch
+ * of kind `which` (JACOBI_ITER, COLLISION_SWEEP, GS_SUBSTEP or
+ * JACOBI_PERSISTENT, whichever the configuration runs), and returns their
+ * mean duration [ms] and their number.  Synchronous. */
+XPBD_API int xpbd_profile_frames(xpbd_cloth* h, uint32_t frames, int which, float* avg_ms, uint32_t* launches);
 
 /* Debug/test hook for the collision broad phase: computes the cell grids and
  * the stable (key, id) radix sort of the CURRENT positions on the device and

@@ -260,6 +260,25 @@ def test_graph_replay_equals_eager(X, persistent):
     b.close()
 
 
+def test_profile_frames_counts_launches_and_steps_like_frames(X):
+    """The instrumented frames advance the state exactly as step_frames does
+    and time one event pair per launch of the chosen kind."""
+    note("case profile_frames")
+    kw = dict(cols=40, rows=30, spacing=0.02, pins=2, jitter=1e-3, seed=12)
+    a, _, _ = make(X, X.ClothMeshData.sheet(**kw), S=4, I=6, use_graph=1, thickness=0.016)
+    b, _, _ = make(X, X.ClothMeshData.sheet(**kw), S=4, I=6, use_graph=1, thickness=0.016)
+    ms, cnt = a.profile_frames(3, X.KERNEL_JACOBI_ITER)
+    assert cnt == 3 * 4 * 5 and ms > 0
+    ms, cnt = a.profile_frames(2, X.KERNEL_COLLISION_SWEEP)
+    assert cnt == 2 * 4 and ms > 0
+    b.step_frames(5)
+    pa, va = a.read_state()
+    pb, vb = b.read_state()
+    assert bits_equal(pa, pb) and bits_equal(va, vb)
+    a.close()
+    b.close()
+
+
 def test_reset_and_state_roundtrip(X):
     note("case reset")
     mesh = X.ClothMeshData.sheet(cols=16, rows=16, spacing=0.05, pins=1, jitter=1e-3, seed=5)
