@@ -6,6 +6,11 @@ Workload (default): the 1024x1024 self-colliding sheet — 1,048,576 particles,
 2r), Jacobi, 8 substeps x 10 iterations per frame.  A "step" is one frame
 (8 substeps) replayed from the captured hipGraph; inputs are resident in HBM.
 
+`--workload batch512`: 512 cloths of uneven sizes (sides U{16..192} per axis,
+log-uniform compliances, pins from {two corners, top row, none + ground}),
+self-collision within each cloth, same solver settings.  `--persistent` runs
+predict + all iterations of a substep as one persistent launch.
+
 Prints one JSON line (rank 0).  `--gpus N` with N > 1 exits: this build runs
 on one GPU.
 """
@@ -69,11 +74,33 @@ def cpu_baseline(mesh_fields, u, cfg, budget_s: float, threads: int):
     return done / el, done, el
 
 
+def build_workload(X, name: str):
+    """Returns (mesh, thickness, description, config extras)."""
+    if name == "sheet1024":
+        kw = sheet_1024()
+        h = kw["spacing"]
+        mesh = X.ClothMeshData.sheet(**kw)
+        return mesh, 2.0 * (0.4 * h), ("1024x1024 self-colliding sheet, plain Jacobi, 8 substeps x 10 iterations "
+                                       "per frame (step = 1 frame)"), {}
+    if name == "batch512":
+        from bevy_xpbd_amd.workloads import batch_descs
+        spacing = 0.01
+        descs, modes = batch_descs(count=512, seed=4, spacing=spacing)
+        mesh = X.ClothMeshData.batch(descs)
+        extra = {"cloths": len(descs), "sides": "U{16..192} per axis", "compliance": "log-uniform [1e-8, 1e-3] per class",
+                 "pins": {m: modes.count(m) for m in sorted(set(modes))}, "seed": 4, "spacing_m": spacing}
+        return mesh, 2.0 * (0.4 * spacing), ("512-cloth uneven batch, self-collision within each cloth, plain "
+                                             "Jacobi, 8 substeps x 10 iterations per frame (step = 1 frame)"), extra
+    raise SystemExit(f"bench.py: unknown workload {name!r}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="sheet1024", choices=["sheet1024", "batch512"])
+    ap.add_argument("--persistent", action="store_true", help="one persistent Jacobi launch per substep")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -89,16 +116,14 @@ def main():
     import bevy_xpbd_amd as X
 
     S, I = 8, 10
-    kw = sheet_1024()
-    h = kw["spacing"]
-    thickness = 2.0 * (0.4 * h)
-    log("bench: building 1024x1024 sheet topology")
+    log(f"bench: building {args.workload} topology")
     t0 = time.perf_counter()
-    mesh = X.ClothMeshData.sheet(**kw)
+    mesh, thickness, workload, extra = build_workload(X, args.workload)
     n, e = mesh.num_particles, mesh.num_constraints
     log(f"bench: topology {n} particles, {e} constraints in {time.perf_counter() - t0:.1f}s")
     cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=I, use_graph=1,
-                                   autotune=0 if args.no_autotune else 1)
+                                   autotune=0 if args.no_autotune else 1,
+                                   persistent_jacobi=1 if args.persistent else 0)
     u = X.ClothSimUniforms.default(X.SOLVER_JACOBI, S, thickness=float(thickness))
     sim = X.ClothComputePlugin(mesh, cfg, u)
     if args.kernels_only:
@@ -127,25 +152,34 @@ def main():
     # PROFILE_FRAMES further frames of the same run (eager, live state).
     jac_ms, jac_launches = sim.profile_frames(PROFILE_FRAMES, X.KERNEL_JACOBI_ITER)
     sweep_ms, _ = sim.profile_frames(PROFILE_FRAMES, X.KERNEL_COLLISION_SWEEP)
+    pers_live_ms = None
+    if args.persistent:
+        pers_live_ms, _ = sim.profile_frames(PROFILE_FRAMES, X.KERNEL_JACOBI_PERSISTENT)
     jac_rep_ms = sim.time_kernel(X.KERNEL_JACOBI_ITER, 50)
     radix_ms = sim.time_kernel(X.KERNEL_RADIX_PASS, 20)
     copy_ms = sim.time_kernel(X.KERNEL_COPY, 20)
     pers_ms = sim.time_kernel(X.KERNEL_JACOBI_PERSISTENT, 5)
     b_iter = algorithmic_bytes_per_iteration(n, e)
-    achieved = b_iter / (jac_ms * 1e-3) / 1e9
     table_slots = 1 << max(6, (2 * n - 1).bit_length())
     b_sub = algorithmic_bytes_per_substep(n, e, I, True, table_slots)
     substep_gbs = b_sub * value / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "jacobi_iter_1024_traffic.json")
-    if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    if args.workload == "sheet1024":
+        tpath = os.path.join(ROOT, "profiles", "jacobi_iter_1024_traffic.json")
+        if os.path.exists(tpath):
+            try:
+                traffic = json.load(open(tpath)).get("bytes_per_launch")
+            except Exception:
+                traffic = None
     residual = sim.residual()
     sim.close()
-    log(f"bench: jacobi iteration {jac_ms * 1e3:.1f} us = {achieved:.0f} GB/s algorithmic; "
+    if jac_launches:
+        achieved = b_iter / (jac_ms * 1e-3) / 1e9
+        rkernel, r_us, r_bytes = "k_jacobi (one iteration)", jac_ms * 1e3, b_iter
+    else:   # persistent: the dominant launch is the whole predict + I iterations
+        achieved = (I * b_iter) / (pers_live_ms * 1e-3) / 1e9
+        rkernel, r_us, r_bytes = "k_jacobi_persistent (predict + I iterations)", pers_live_ms * 1e3, I * b_iter
+    log(f"bench: dominant kernel {rkernel} {r_us:.1f} us = {achieved:.0f} GB/s algorithmic; "
         f"collision sweep {sweep_ms * 1e3:.1f} us; radix pass {radix_ms * 1e3:.1f} us")
 
     cpu = None
@@ -154,9 +188,20 @@ def main():
         log(f"bench: CPU baseline (oracle, {threads} threads, ~{args.cpu_budget:.0f}s)")
         rate, done, cel = cpu_baseline(mesh.fields(), u, cfg, args.cpu_budget, threads)
         cpu = {"value": round(rate, 4), "unit": "substeps/s", "cores": threads, "kind": "port",
-               "sample": f"{done} substeps of the same 1024x1024 workload (8x10 Jacobi + hash-grid "
+               "sample": f"{done} substeps of the same workload ({args.workload}: 8x10 Jacobi + hash-grid "
                          f"self-collision) from the initial state, {cel:.1f}s"}
 
+    config = {"workload": workload, "particles": n, "constraints": e, "substeps_per_step": S, "iterations": I,
+              "collision": "every substep, contact distance 0.8 x spacing",
+              "solve_launch": "persistent (one launch per substep)" if args.persistent else
+                              "one launch per iteration (hipGraph)",
+              "residual_after_timed_run": residual,
+              "substep_algorithmic_GBps": round(substep_gbs, 1),
+              "substep_hbm_frac": round(substep_gbs / HBM_PEAK_GBS, 4),
+              "collision_sweep_us": round(sweep_ms * 1e3, 2), "radix_pass_us": round(radix_ms * 1e3, 2),
+              "copy_kernel_GBps": round(32 * n / (copy_ms * 1e-3) / 1e9, 1),
+              "persistent_predict_plus_iterations_us": round(pers_ms * 1e3, 2)}
+    config.update(extra)
     out = {
         "metric": "cloth substeps/s at fixed constraint residual; learned vs plain Jacobi; HBM %",
         "value": round(value, 2),
@@ -170,21 +215,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "1024x1024 self-colliding sheet, plain Jacobi, 8 substeps x 10 iterations "
-                               "per frame (step = 1 frame)",
-                   "particles": n, "constraints": e, "substeps_per_step": S, "iterations": I,
-                   "collision": "every substep, contact distance 0.8 x spacing",
-                   "residual_after_timed_run": residual,
-                   "substep_algorithmic_GBps": round(substep_gbs, 1),
-                   "substep_hbm_frac": round(substep_gbs / HBM_PEAK_GBS, 4),
-                   "collision_sweep_us": round(sweep_ms * 1e3, 2), "radix_pass_us": round(radix_ms * 1e3, 2),
-                   "copy_kernel_GBps": round(32 * n / (copy_ms * 1e-3) / 1e9, 1),
-                   "persistent_predict_plus_iterations_us": round(pers_ms * 1e3, 2)},
-        "roofline": {"bound": "hbm", "kernel": "k_jacobi (one iteration)", "achieved": round(achieved, 1),
+        "config": config,
+        "roofline": {"bound": "hbm", "kernel": rkernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "avg_launch_us": round(jac_ms * 1e3, 2),
-                     "launches_timed": jac_launches, "algorithmic_bytes_per_launch": b_iter,
-                     "back_to_back_launch_us": round(jac_rep_ms * 1e3, 2)},
+                     "traffic": traffic, "avg_launch_us": round(r_us, 2),
+                     "launches_timed": jac_launches, "algorithmic_bytes_per_launch": r_bytes,
+                     "back_to_back_iteration_us": round(jac_rep_ms * 1e3, 2)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -593,9 +593,15 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
 constexpr uint32_t GROUP_SHIFT = 2;
 // Slot of a group key: the low 3 bits stay (8 consecutive groups = 32 cells
 // share one 64-byte line, so threads walking the sorted order probe the same
-// lines); the blocks of 8 are scattered by a multiplicative mix.
+// lines); the block index g >> 3 is placed by Fibonacci hashing, taking the
+// HIGH bits of (g >> 3) * 2^32/phi.  Those depend on every key bit (a batch
+// keeps its cloth index in the top bits; the low bits of a product would
+// ignore it and pile all cloths onto the same slots), and runs of consecutive
+// blocks, which a dense cell grid produces, land nearly evenly spread.
 __device__ __forceinline__ uint32_t slot_of(uint32_t g, uint32_t mask) {
-    return ((((g >> 3) * 2654435761u) << 3) | (g & 7u)) & mask;
+    const uint32_t bits = __popc(mask >> 3);   // log2(slots / 8); the table has >= 64 slots
+    const uint32_t blk = ((g >> 3) * 2654435761u) >> (32u - bits);
+    return ((blk << 3) | (g & 7u)) & mask;
 }
 
 __global__ void k_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask) {
