@@ -18,7 +18,10 @@ from .api import (  # noqa: F401
     ClothSimConfig,
     ClothSimControl,
     ClothSimUniforms,
+    DeviceArray,
     SheetDesc,
+    WarmStartMLP,
+    MLP_PARAMS,
     XpbdError,
     derive_collision_grid,
     lib,

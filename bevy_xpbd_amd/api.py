@@ -47,6 +47,7 @@ MESH_FIELDS = {
     "batch_offsets": (16, np.uint32, 1),
     "ground": (17, np.uint32, 1),
     "cloth_offsets": (18, np.uint32, 1),
+    "cloth_cols": (19, np.uint32, 1),
 }
 
 
@@ -166,6 +167,18 @@ def lib():
             "xpbd_profile_frames": (i32, [vp, u32, i32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
             "xpbd_debug_collision_keys": (i32, [vp, f32, vp, vp, vp]),
             "xpbd_selftest_division": (i32, [i32, i32, u64, u64, vp]),
+            "xpbd_mlp_create": (i32, [u64, pp]),
+            "xpbd_mlp_destroy": (None, [vp]),
+            "xpbd_mlp_last_error": (C.c_char_p, [vp]),
+            "xpbd_mlp_get_params": (i32, [vp, vp, C.POINTER(C.c_float)]),
+            "xpbd_mlp_set_params": (i32, [vp, vp, f32]),
+            "xpbd_mlp_forward": (i32, [vp, vp, u32, vp]),
+            "xpbd_mlp_gradient": (i32, [vp, vp, vp, u32, vp, C.POINTER(C.c_float)]),
+            "xpbd_mlp_train": (i32, [vp, vp, vp, u32, u32, u32, f32, vp]),
+            "xpbd_warm_sample": (i32, [vp, u32, vp, vp]),
+            "xpbd_warm_features": (i32, [vp, vp]),
+            "xpbd_set_warm_start": (i32, [vp, vp, u32, f32]),
+            "xpbd_warm_stats": (i32, [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), u32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -387,6 +400,26 @@ class ClothComputePlugin:
                self._h)
         return ms.value, cnt.value
 
+    def warm_sample(self, iterations: int, features: DeviceArray, targets: DeviceArray):
+        """Training pairs of the current state: features (N x 32 bf16) and
+        (iterated - predicted) / spacing (N x 3 f32), both on the device."""
+        _check(lib().xpbd_warm_sample(self._h, iterations, features.ptr, targets.ptr), "xpbd_warm_sample", self._h)
+
+    def warm_features(self, features: DeviceArray):
+        _check(lib().xpbd_warm_features(self._h, features.ptr), "xpbd_warm_features", self._h)
+
+    def set_warm_start(self, mlp, warm_iterations: int = 2, residual_threshold: float = 1e-4):
+        """Attach (mlp) or detach (None) the learned warm start."""
+        self._warm = mlp   # keep the network alive while attached
+        _check(lib().xpbd_set_warm_start(self._h, mlp._h if mlp is not None else None, warm_iterations,
+                                         residual_threshold), "xpbd_set_warm_start", self._h)
+
+    def warm_stats(self, reset: bool = False):
+        sub, fb = C.c_uint32(), C.c_uint32()
+        _check(lib().xpbd_warm_stats(self._h, C.byref(sub), C.byref(fb), 1 if reset else 0), "xpbd_warm_stats",
+               self._h)
+        return sub.value, fb.value
+
     def debug_collision_keys(self, cell_size: float, num_cloths: int = 1):
         grid = np.zeros((num_cloths, 12), np.uint32)
         keys = np.zeros(self.n, np.uint32)
@@ -397,6 +430,135 @@ class ClothComputePlugin:
                "cell": g[6:7].view(np.float32)[0], "inv_cell": g[7:8].view(np.float32)[0],
                "dims": g[8:11].copy(), "key_base": int(g[11])} for g in grid]
         return (gs[0] if num_cloths == 1 else gs), keys, ids
+
+
+class DeviceArray:
+    """Device memory from the HIP runtime (hipMalloc via ctypes) with numpy
+    upload / download; the boundary takes device buffers as plain pointers."""
+    _hip = None
+
+    @classmethod
+    def _rt(cls):
+        if cls._hip is None:
+            h = C.CDLL("libamdhip64.so")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipMalloc.restype = C.c_int
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipFree.restype = C.c_int
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipMemcpy.restype = C.c_int
+            h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+            h.hipMemset.restype = C.c_int
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        rc = self._rt().hipMalloc(C.byref(p), max(self.nbytes, 256))
+        if rc != 0:
+            raise XpbdError(XPBD_E_NOMEM, "hipMalloc", f"hip error {rc}")
+        self.ptr = p.value
+        self._rt().hipMemset(C.c_void_p(self.ptr), 0, max(self.nbytes, 256))
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a, offset_bytes: int = 0):
+        a = np.ascontiguousarray(a)
+        if offset_bytes + a.nbytes > self.nbytes:
+            raise ValueError("upload beyond the device array")
+        rc = self._rt().hipMemcpy(C.c_void_p(self.ptr + offset_bytes), a.ctypes.data_as(C.c_void_p), a.nbytes, 1)
+        if rc != 0:
+            raise XpbdError(XPBD_E_HIP, "hipMemcpy", f"hip error {rc}")
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        rc = self._rt().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes, 2)
+        if rc != 0:
+            raise XpbdError(XPBD_E_HIP, "hipMemcpy", f"hip error {rc}")
+        return out
+
+    def at(self, offset_bytes: int) -> int:
+        return self.ptr + offset_bytes
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self._rt().hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+MLP_PARAMS = 7312
+MLP_IN = 32
+
+
+class WarmStartMLP:
+    """The learned warm start's network (30 -> 64 -> 64 -> 3, GELU; BF16
+    MFMA forward / backward with FP32 accumulate, Adam) on the GPU."""
+
+    def __init__(self, seed: int = 1):
+        h = C.c_void_p()
+        _check(lib().xpbd_mlp_create(seed, C.byref(h)), "xpbd_mlp_create")
+        self._h = h
+
+    def _chk(self, code, what):
+        if code != XPBD_OK:
+            msg = lib().xpbd_mlp_last_error(self._h)
+            raise XpbdError(code, what, msg.decode() if msg else "")
+
+    def params(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        sc = C.c_float()
+        self._chk(lib().xpbd_mlp_get_params(self._h, _ptr(p), C.byref(sc)), "xpbd_mlp_get_params")
+        return p, sc.value
+
+    def set_params(self, params, out_scale: float):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != MLP_PARAMS:
+            raise ValueError(f"expected {MLP_PARAMS} parameters")
+        self._chk(lib().xpbd_mlp_set_params(self._h, _ptr(p), float(out_scale)), "xpbd_mlp_set_params")
+
+    def forward(self, features: DeviceArray, rows: int) -> np.ndarray:
+        out = DeviceArray((rows, 3), np.float32)
+        self._chk(lib().xpbd_mlp_forward(self._h, features.ptr, rows, out.ptr), "xpbd_mlp_forward")
+        return out.numpy()
+
+    def gradient(self, features: DeviceArray, targets: DeviceArray, rows: int):
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        self._chk(lib().xpbd_mlp_gradient(self._h, features.ptr, targets.ptr, rows, _ptr(g), C.byref(loss)),
+                  "xpbd_mlp_gradient")
+        return g, loss.value
+
+    def train(self, features: DeviceArray, targets: DeviceArray, rows: int, steps: int, batch: int,
+              lr: float) -> np.ndarray:
+        curve = np.zeros(steps, np.float32)
+        self._chk(lib().xpbd_mlp_train(self._h, features.ptr, targets.ptr, rows, steps, batch, lr, _ptr(curve)),
+                  "xpbd_mlp_train")
+        return curve
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            lib().xpbd_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def selftest_division(exp_lo: int = -60, exp_hi: int = 60, pairs: int = 1 << 28, seed: int = 1) -> dict:

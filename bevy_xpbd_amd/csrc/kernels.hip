@@ -279,6 +279,7 @@ __device__ __forceinline__ bool stage_pairs(const JacobiArgs& a, float2* lpairs)
 
 template <uint32_t MODE, bool PACKED, bool FASTDIV>
 __global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
+    if (a.gate && *a.gate == 0u) return;   // uniform over the grid
     __shared__ float2 lpairs[LDS_PAIRS_MAX];
     const bool lds = stage_pairs<PACKED>(a, lpairs);
     const uint32_t tile = a.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -862,6 +863,30 @@ __global__ void k_predict(const float4* X, const float4* V, float4* P, DevParams
     P[i] = predict(X[i], V[i], i, p);
 }
 
+__global__ void k_predict_gated(const float4* X, const float4* V, float4* P, DevParams p, const uint32_t* gate) {
+    if (*gate == 0u) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    P[i] = predict(X[i], V[i], i, p);
+}
+
+__global__ void k_copy4_unless(const float4* a, float4* b, uint32_t n, const uint32_t* gate) {
+    if (*gate != 0u) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// Warm-start acceptance: relative residual against the threshold, plus the
+// substep / fallback counters.  One thread.
+__global__ void k_warm_flag(const double* resid, double rest_sq, float threshold, uint32_t* flag) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double rel = rest_sq > 0.0 ? sqrt(resid[0]) / sqrt(rest_sq) : 0.0;
+    const uint32_t f = rel > double(threshold) ? 1u : 0u;
+    flag[0] = f;
+    flag[1] = flag[1] + 1u;
+    flag[2] = flag[2] + f;
+}
+
 // ---------------------------------------------------------------------------
 // Coloured Gauss-Seidel: one persistent launch per substep.  Colours run in
 // order with a grid barrier between them; a colour's constraints share no
@@ -1209,6 +1234,21 @@ void launch_selftest_div(uint64_t n, uint64_t seed, uint32_t* bad, uint32_t* fir
 void launch_predict(const float4* X, const float4* V, float4* P, DevParams p, hipStream_t s) {
     if (p.n == 0) return;
     hipLaunchKernelGGL(k_predict, dim3(cdiv(p.n, 256)), dim3(256), 0, s, X, V, P, p);
+}
+
+void launch_predict_gated(const float4* X, const float4* V, float4* P, DevParams p, const uint32_t* gate,
+                          hipStream_t s) {
+    if (p.n == 0) return;
+    hipLaunchKernelGGL(k_predict_gated, dim3(cdiv(p.n, 256)), dim3(256), 0, s, X, V, P, p, gate);
+}
+
+void launch_copy4_unless(const float4* a, float4* b, uint32_t n, const uint32_t* gate, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_copy4_unless, dim3(cdiv(n, 256)), dim3(256), 0, s, a, b, n, gate);
+}
+
+void launch_warm_flag(const double* resid, double rest_sq, float threshold, uint32_t* flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_warm_flag, dim3(1), dim3(64), 0, s, resid, rest_sq, threshold, flag);
 }
 
 void launch_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p, hipStream_t s) {

@@ -57,6 +57,7 @@ struct JacobiArgs {
     uint32_t remap;      // 1: XCD-aware block order
     uint32_t fastdiv;    // 1: shared-reciprocal correctly rounded divisions
     uint32_t num_pairs;  // packed: size of sell.pairs
+    const uint32_t* gate;   // non-null: the launch does nothing unless *gate != 0
 };
 
 // Persistent Jacobi substep: predict X,V -> A, iterate A/B; the last iterate
@@ -143,6 +144,13 @@ void launch_coll_sweep(const CollArgs& a, hipStream_t s);
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
 uint32_t gs_max_blocks(uint32_t block);
 void launch_copy4(const float4* a, float4* b, uint32_t n, hipStream_t s);
+// gated variants for the warm-start fallback: run only if *gate != 0
+void launch_predict_gated(const float4* X, const float4* V, float4* P, DevParams p, const uint32_t* gate,
+                          hipStream_t s);
+// copy a -> b only if *gate == 0
+void launch_copy4_unless(const float4* a, float4* b, uint32_t n, const uint32_t* gate, hipStream_t s);
+// flag[0] = sqrt(resid) / sqrt(rest_sq) > threshold; flag[1] += 1; flag[2] += flag[0]
+void launch_warm_flag(const double* resid, double rest_sq, float threshold, uint32_t* flag, hipStream_t s);
 uint32_t persistent_max_blocks(bool packed, bool fastdiv);
 void launch_jacobi_persistent(const JacobiArgs& a, const PersistArgs& q, uint32_t blocks, hipStream_t s);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);

@@ -27,6 +27,7 @@ struct Mesh {
     std::vector<uint32_t> batch_offsets;
     std::vector<uint32_t> ground;
     std::vector<uint32_t> cloth_offsets;
+    std::vector<uint32_t> cloth_cols;   // grid columns per cloth (empty: not a grid mesh)
 
     size_t num_particles() const { return inv_mass.size(); }
     size_t num_constraints() const { return constraint_i.size(); }

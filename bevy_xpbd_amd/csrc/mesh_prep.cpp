@@ -352,7 +352,9 @@ Mesh* grid_cloth_hanging(uint32_t qc, uint32_t qr, float cell) {
             uint32_t a = iy * vc + ix, b = (iy + 1) * vc + ix + 1;
             extras.push_back({a, b, dist(pos, a, b), CROSS_DIAG_COMPLIANCE});
         }
-    return finalize(std::move(pos), std::move(nrm), std::move(uvs), std::move(idx), std::move(extras));
+    Mesh* m = finalize(std::move(pos), std::move(nrm), std::move(uvs), std::move(idx), std::move(extras));
+    if (m) m->cloth_cols = {vc};
+    return m;
 }
 
 Mesh* parse_welded_obj(const std::string& text) {
@@ -538,6 +540,7 @@ Mesh* sheet_batch(const xpbd_sheet_desc* descs, uint32_t count) {
         for (uint32_t iy = 0; iy + 2 < R; ++iy)
             for (uint32_t ix = 0; ix < C; ++ix) add(iy * C + ix, (iy + 2) * C + ix, d.bend_compliance);
         m->cloth_offsets.push_back(uint32_t(m->inv_mass.size()));
+        m->cloth_cols.push_back(C);
     }
     assemble_constraints(*m, m->inv_mass.size(), std::move(cons));
     return m;

@@ -1,0 +1,480 @@
+// Learned warm start kernels for gfx950: features, BF16 MFMA forward /
+// backward (FP32 accumulate), Adam, and the warm-start apply step.
+//
+// MFMA: v_mfma_f32_16x16x16_bf16 (wave64).  Lane l = 16 g + r holds
+//   A[m = r][k = 4g + i], B[k = 4g + i][n = r], D[m = 4g + i][n = r], i = 0..3.
+// The forward runs transposed, Z^T = W^T H^T (M = units, N = 16 rows), so the
+// D fragments of one layer are exactly the B fragments of the next and no
+// data moves between layers.  Backward: dH^T = W dZ^T keeps that layout; the
+// weight gradients (K = rows) read both operands transposed through a small
+// per-wave LDS tile.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "mlp.hpp"
+
+namespace xpbd {
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC0u;   // NaN stays NaN
+    u += 0x7FFFu + ((u >> 16) & 1u);                        // round to nearest, ties to even
+    return uint16_t(u >> 16);
+}
+
+__device__ __forceinline__ bf16x4 pack4(f32x4 v) {
+    bf16x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = short(f2bf(v[i]));
+    return r;
+}
+
+__device__ __forceinline__ bf16x4 ld4(const uint16_t* p) { return *reinterpret_cast<const bf16x4*>(p); }
+
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// GELU, tanh form, and its derivative
+constexpr float GELU_C = 0.7978845608028654f;   // sqrt(2 / pi)
+constexpr float GELU_A = 0.044715f;
+__device__ __forceinline__ float gelu(float x) {
+    const float u = GELU_C * (x + GELU_A * x * x * x);
+    return 0.5f * x * (1.0f + tanhf(u));
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+    const float x2 = x * x;
+    const float t = tanhf(GELU_C * (x + GELU_A * x2 * x));
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * GELU_C * (1.0f + 3.0f * GELU_A * x2);
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float sum16(float v) {   // over the 16 lanes sharing l / 16
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 8);
+    return v;
+}
+
+__device__ __forceinline__ void stage_weights(const uint16_t* bf, uint16_t* W) {
+    const uint4* src = reinterpret_cast<const uint4*>(bf);
+    uint4* dst = reinterpret_cast<uint4*>(W);
+    for (uint32_t q = threadIdx.x; q < uint32_t(MLP_BF / 8); q += blockDim.x) dst[q] = src[q];
+    __syncthreads();
+}
+
+struct TileAct {
+    f32x4 z1[4], z2[4], o;
+    bf16x4 h1[4], h2[4];
+};
+
+// Forward of one 16-row tile.  x[s]: B fragments of X^T (features 16s..16s+15).
+__device__ __forceinline__ void tile_forward(const uint16_t* W, const float* P, const bf16x4 x[2], TileAct& t,
+                                             int r, int g) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) acc = mfma(ld4(W + BF_W1T + (16 * m + r) * MLP_IN + 16 * s + 4 * g), x[s], acc);
+        f32x4 h;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            acc[i] += P[OFF_B1 + 16 * m + 4 * g + i];
+            h[i] = gelu(acc[i]);
+        }
+        t.z1[m] = acc;
+        t.h1[m] = pack4(h);
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc = mfma(ld4(W + BF_W2T + (16 * m + r) * MLP_HID + 16 * s + 4 * g), t.h1[s], acc);
+        f32x4 h;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            acc[i] += P[OFF_B2 + 16 * m + 4 * g + i];
+            h[i] = gelu(acc[i]);
+        }
+        t.z2[m] = acc;
+        t.h2[m] = pack4(h);
+    }
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc = mfma(ld4(W + BF_W3T + r * MLP_HID + 16 * s + 4 * g), t.h2[s], acc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] += P[OFF_B3 + 4 * g + i];
+    t.o = acc;   // lanes g = 0: outputs 0..3 of row r
+}
+
+__device__ __forceinline__ void load_x(const uint16_t* feat, uint32_t row, bool valid, int g, bf16x4 x[2]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (valid) {
+            x[s] = ld4(feat + size_t(row) * MLP_IN + 16 * s + 4 * g);
+        } else {
+            x[s] = bf16x4{0, 0, 0, 0};
+        }
+    }
+}
+
+// ---------------------------------------------------------------- features
+__global__ __launch_bounds__(256) void k_warm_features(WarmFeatArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const float4 pi = a.xp[i], ri = a.rest4[i], ax = a.vaux[i], vi = a.vel[i];
+    const float ih = ax.w;
+    float f[MLP_IN];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int32_t j = a.nbr8[size_t(i) * 8 + k];
+        float dx = 0.f, dy = 0.f, dz = 0.f;
+        if (j >= 0) {
+            const float4 pj = a.xp[j], rj = a.rest4[j];
+            dx = ((pj.x - pi.x) - (rj.x - ri.x)) * ih;
+            dy = ((pj.y - pi.y) - (rj.y - ri.y)) * ih;
+            dz = ((pj.z - pi.z) - (rj.z - ri.z)) * ih;
+        }
+        f[3 * k] = dx;
+        f[3 * k + 1] = dy;
+        f[3 * k + 2] = dz;
+    }
+    const float sv = a.dt * ih;
+    f[24] = vi.x * sv;
+    f[25] = vi.y * sv;
+    f[26] = vi.z * sv;
+    f[27] = ax.x;
+    f[28] = ax.y;
+    f[29] = ax.z;
+    f[30] = 0.f;
+    f[31] = 0.f;
+    uint32_t packed[MLP_IN / 2];
+#pragma unroll
+    for (int q = 0; q < MLP_IN / 2; ++q) packed[q] = uint32_t(f2bf(f[2 * q])) | (uint32_t(f2bf(f[2 * q + 1])) << 16);
+    uint4* dst = reinterpret_cast<uint4*>(a.feat + size_t(i) * MLP_IN);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
+}
+
+__global__ void k_warm_targets(const float4* xp, const float4* xc, const float4* vaux, uint32_t n, float* t) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = xp[i], c = xc[i];
+    const float ih = vaux[i].w;
+    t[3 * size_t(i)] = (c.x - p.x) * ih;
+    t[3 * size_t(i) + 1] = (c.y - p.y) * ih;
+    t[3 * size_t(i) + 2] = (c.z - p.z) * ih;
+}
+
+// ---------------------------------------------------------------- forward
+__global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* feat, uint32_t rows, const float* P,
+                                                     const uint16_t* bf, float* out) {
+    __shared__ __align__(16) uint16_t W[MLP_BF];
+    stage_weights(bf, W);
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t tile = wave; tile * 16 < rows; tile += nwaves) {
+        const uint32_t row = tile * 16 + r;
+        const bool valid = row < rows;
+        bf16x4 x[2];
+        load_x(feat, row, valid, g, x);
+        TileAct t;
+        tile_forward(W, P, x, t, r, g);
+        if (g == 0 && valid) {
+            out[3 * size_t(row)] = t.o[0];
+            out[3 * size_t(row) + 1] = t.o[1];
+            out[3 * size_t(row) + 2] = t.o[2];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_warm_apply(const uint16_t* feat, const float4* xp, const float4* vaux,
+                                                    const float* P, const uint16_t* bf, float inv_out_scale,
+                                                    float4* x0, uint32_t n) {
+    __shared__ __align__(16) uint16_t W[MLP_BF];
+    stage_weights(bf, W);
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t tile = wave; tile * 16 < n; tile += nwaves) {
+        const uint32_t row = tile * 16 + r;
+        const bool valid = row < n;
+        bf16x4 x[2];
+        load_x(feat, row, valid, g, x);
+        TileAct t;
+        tile_forward(W, P, x, t, r, g);
+        if (g == 0 && valid) {
+            const float4 p = xp[row];
+            float4 q = p;
+            const float ih = vaux[row].w;
+            if (p.w > 0.f && ih > 0.f) {
+                const float sc = inv_out_scale / ih;   // spacing / out_scale
+                q.x = p.x + t.o[0] * sc;
+                q.y = p.y + t.o[1] * sc;
+                q.z = p.z + t.o[2] * sc;
+            }
+            x0[row] = q;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- training
+constexpr int TR_X = 0;                       // [32][16]
+constexpr int TR_H1 = TR_X + MLP_IN * 16;      // [64][16]
+constexpr int TR_H2 = TR_H1 + MLP_HID * 16;
+constexpr int TR_D1 = TR_H2 + MLP_HID * 16;
+constexpr int TR_D2 = TR_D1 + MLP_HID * 16;
+constexpr int TR_DO = TR_D2 + MLP_HID * 16;   // [16][16]
+constexpr int TR_WORDS = TR_DO + MLP_OUTP * 16;
+
+__device__ __forceinline__ void st_t(uint16_t* base, int m, bf16x4 v, int r, int g) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) base[(16 * m + 4 * g + i) * 16 + r] = uint16_t(v[i]);
+}
+__device__ __forceinline__ bf16x4 ld_t(const uint16_t* base, int m, int r, int g) {
+    return ld4(base + (16 * m + r) * 16 + 4 * g);
+}
+
+__global__ __launch_bounds__(256) void k_mlp_train(const uint16_t* feat, const float* targets, uint32_t rows,
+                                                   float tscale, float inv_count, const float* P,
+                                                   const uint16_t* bf, float* partial) {
+    __shared__ __align__(16) uint16_t W[MLP_BF];
+    __shared__ __align__(16) uint16_t tr_all[4][TR_WORDS];
+    stage_weights(bf, W);
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4, wv = threadIdx.x >> 6;
+    const uint32_t wave = blockIdx.x * 4 + wv, nwaves = gridDim.x * 4;
+    uint16_t* tr = tr_all[wv];
+
+    const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
+    f32x4 gw1[2][4], gw2[4][4], gw3[4], gb1[4], gb2[4], gb3 = z4;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        gw3[a] = gb1[a] = gb2[a] = z4;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            gw2[a][b] = z4;
+            if (a < 2) gw1[a][b] = z4;
+        }
+    }
+    float lsum = 0.f;
+
+    for (uint32_t tile = wave; tile * 16 < rows; tile += nwaves) {
+        const uint32_t row = tile * 16 + r;
+        const bool valid = row < rows;
+        bf16x4 x[2];
+        load_x(feat, row, valid, g, x);
+        TileAct t;
+        tile_forward(W, P, x, t, r, g);
+
+        f32x4 dO = z4;
+        if (g == 0 && valid) {
+#pragma unroll
+            for (int i = 0; i < MLP_OUT; ++i) {
+                const float d = t.o[i] - targets[3 * size_t(row) + i] * tscale;
+                lsum += d * d;
+                dO[i] = 2.0f * d * inv_count;
+            }
+        }
+        const bf16x4 dOb = pack4(dO);
+        gb3 += dO;
+
+        f32x4 dz2[4];
+        bf16x4 dz2b[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            f32x4 acc = mfma(ld4(W + BF_W3N + (16 * m + r) * MLP_OUTP + 4 * g), dOb, z4);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i] *= gelu_grad(t.z2[m][i]);
+            dz2[m] = acc;
+            dz2b[m] = pack4(acc);
+            gb2[m] += acc;
+        }
+        bf16x4 dz1b[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            f32x4 acc = z4;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) acc = mfma(ld4(W + BF_W2N + (16 * m + r) * MLP_HID + 16 * s + 4 * g), dz2b[s], acc);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i] *= gelu_grad(t.z1[m][i]);
+            dz1b[m] = pack4(acc);
+            gb1[m] += acc;
+        }
+        (void)dz2;
+
+        // operands of the weight gradients, transposed through LDS
+#pragma unroll
+        for (int s = 0; s < 2; ++s) st_t(tr + TR_X, s, x[s], r, g);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            st_t(tr + TR_H1, m, t.h1[m], r, g);
+            st_t(tr + TR_H2, m, t.h2[m], r, g);
+            st_t(tr + TR_D1, m, dz1b[m], r, g);
+            st_t(tr + TR_D2, m, dz2b[m], r, g);
+        }
+        st_t(tr + TR_DO, 0, dOb, r, g);
+        wave_sync();
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const bf16x4 d1 = ld_t(tr + TR_D1, b, r, g);
+            const bf16x4 d2 = ld_t(tr + TR_D2, b, r, g);
+#pragma unroll
+            for (int a = 0; a < 2; ++a) gw1[a][b] = mfma(ld_t(tr + TR_X, a, r, g), d1, gw1[a][b]);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) gw2[a][b] = mfma(ld_t(tr + TR_H1, a, r, g), d2, gw2[a][b]);
+        }
+        const bf16x4 dob = ld_t(tr + TR_DO, 0, r, g);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) gw3[a] = mfma(ld_t(tr + TR_H2, a, r, g), dob, gw3[a]);
+        wave_sync();
+    }
+
+    float* out = partial + size_t(wave) * MLP_PSTRIDE;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (a < 2) out[OFF_W1 + (16 * a + 4 * g + i) * MLP_HID + 16 * b + r] = gw1[a][b][i];
+                out[OFF_W2 + (16 * a + 4 * g + i) * MLP_HID + 16 * b + r] = gw2[a][b][i];
+            }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out[OFF_W3 + (16 * a + 4 * g + i) * MLP_OUTP + r] = gw3[a][i];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float s1 = sum16(gb1[m][i]);
+            const float s2 = sum16(gb2[m][i]);
+            if (r == 0) {
+                out[OFF_B1 + 16 * m + 4 * g + i] = s1;
+                out[OFF_B2 + 16 * m + 4 * g + i] = s2;
+            }
+        }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float s3 = sum16(gb3[i]);
+        if (r == 0) out[OFF_B3 + 4 * g + i] = s3;
+    }
+    float l = lsum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) l += __shfl_xor(l, o);
+    if (lane == 0) out[MLP_PARAMS] = l;
+}
+
+__device__ __forceinline__ bool fixed_zero(int p) {
+    if (p >= OFF_W1 + MLP_FEAT * MLP_HID && p < OFF_B1) return true;           // padded inputs
+    if (p >= OFF_W3 && p < OFF_B3) return (p - OFF_W3) % MLP_OUTP >= MLP_OUT;   // padded outputs
+    if (p >= OFF_B3) return p - OFF_B3 >= MLP_OUT;
+    return false;
+}
+
+__device__ __forceinline__ void pack_param(int p, float val, uint16_t* bf) {
+    const uint16_t b = f2bf(val);
+    if (p < OFF_B1) {
+        const int f = p / MLP_HID, h = p % MLP_HID;
+        bf[BF_W1T + h * MLP_IN + f] = b;
+    } else if (p >= OFF_W2 && p < OFF_B2) {
+        const int q = p - OFF_W2, h1 = q / MLP_HID, h2 = q % MLP_HID;
+        bf[BF_W2T + h2 * MLP_HID + h1] = b;
+        bf[BF_W2N + h1 * MLP_HID + h2] = b;
+    } else if (p >= OFF_W3 && p < OFF_B3) {
+        const int q = p - OFF_W3, h2 = q / MLP_OUTP, o = q % MLP_OUTP;
+        bf[BF_W3T + o * MLP_HID + h2] = b;
+        bf[BF_W3N + h2 * MLP_OUTP + o] = b;
+    }
+}
+
+__global__ void k_mlp_reduce_adam(const float* partial, uint32_t nwaves, float* P, float* m, float* v, uint16_t* bf,
+                                  float* grad_out, float* loss_out, float lr, float bc1, float bc2) {
+    const int p = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p > MLP_PARAMS) return;
+    float s = 0.f;
+    for (uint32_t w = 0; w < nwaves; ++w) s += partial[size_t(w) * MLP_PSTRIDE + p];
+    if (p == MLP_PARAMS) {
+        if (loss_out) *loss_out = s;
+        return;
+    }
+    if (fixed_zero(p)) s = 0.f;
+    if (grad_out) grad_out[p] = s;
+    if (lr > 0.f) {
+        constexpr float B1 = 0.9f, B2 = 0.999f, EPS = 1e-8f;
+        const float mm = B1 * m[p] + (1.f - B1) * s;
+        const float vv = B2 * v[p] + (1.f - B2) * s * s;
+        m[p] = mm;
+        v[p] = vv;
+        float w = P[p] - lr * (mm / bc1) / (sqrtf(vv / bc2) + EPS);
+        if (fixed_zero(p)) w = 0.f;
+        P[p] = w;
+        pack_param(p, w, bf);
+    }
+}
+
+__global__ void k_mlp_pack(const float* P, uint16_t* bf) {
+    const int p = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p < MLP_PARAMS) pack_param(p, P[p], bf);
+}
+
+}  // namespace
+
+void launch_warm_features(const WarmFeatArgs& a, hipStream_t s) {
+    if (a.n == 0) return;
+    hipLaunchKernelGGL(k_warm_features, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);
+}
+
+void launch_warm_targets(const float4* xp, const float4* xconv, const float4* vaux, uint32_t n, float* t,
+                         hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_warm_targets, dim3(cdiv(n, 256)), dim3(256), 0, s, xp, xconv, vaux, n, t);
+}
+
+void launch_mlp_forward(const uint16_t* feat, uint32_t rows, const float* P, const uint16_t* bf, float* out,
+                        hipStream_t s) {
+    if (rows == 0) return;
+    const uint32_t blocks = std::min<uint32_t>(cdiv(cdiv(rows, 16), 4), 2048);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(256), 0, s, feat, rows, P, bf, out);
+}
+
+void launch_mlp_train(const uint16_t* feat, const float* targets, uint32_t rows, float target_scale,
+                      const float* P, const uint16_t* bf, float* partial, hipStream_t s) {
+    const float inv_count = rows ? 1.0f / (3.0f * float(rows)) : 0.f;
+    hipLaunchKernelGGL(k_mlp_train, dim3(MLP_TRAIN_BLOCKS), dim3(256), 0, s, feat, targets, rows, target_scale,
+                       inv_count, P, bf, partial);
+}
+
+void launch_mlp_reduce_adam(const float* partial, float* P, float* m, float* v, uint16_t* bf, float* grad_out,
+                            float* loss_out, float lr, float beta1_t, float beta2_t, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(cdiv(MLP_PARAMS + 1, 256)), dim3(256), 0, s, partial,
+                       MLP_TRAIN_WAVES, P, m, v, bf, grad_out, loss_out, lr, 1.0f - beta1_t, 1.0f - beta2_t);
+}
+
+void launch_mlp_pack(const float* P, uint16_t* bf, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3(cdiv(MLP_PARAMS, 256)), dim3(256), 0, s, P, bf);
+}
+
+void launch_warm_apply(const uint16_t* feat, const float4* xp, const float4* vaux, const float* P,
+                       const uint16_t* bf, float inv_out_scale, float4* x0, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    const uint32_t blocks = std::min<uint32_t>(cdiv(cdiv(n, 16), 4), 2048);
+    hipLaunchKernelGGL(k_warm_apply, dim3(blocks), dim3(256), 0, s, feat, xp, vaux, P, bf, inv_out_scale, x0, n);
+}
+
+}  // namespace xpbd

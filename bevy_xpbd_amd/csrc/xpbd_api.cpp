@@ -21,11 +21,26 @@
 #include "../../include/xpbd.h"
 #include "kernels.hpp"
 #include "mesh.hpp"
+#include "mlp.hpp"
 
 using namespace xpbd;
 
 struct xpbd_mesh {
     Mesh m;
+};
+
+// The warm-start network: f32 master parameters, Adam state, bf16 operand
+// copies and training scratch, all on one device and stream.
+struct xpbd_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    float *P = nullptr, *m = nullptr, *v = nullptr, *partial = nullptr, *grad = nullptr, *loss = nullptr;
+    uint32_t loss_cap = 0;
+    uint16_t* bf = nullptr;
+    float out_scale = 1.0f;
+    uint32_t step = 0;
+    std::string msg;
 };
 
 namespace {
@@ -92,6 +107,18 @@ struct xpbd_cloth {
     uint32_t jacobi_block = 256, remap = 1, fused_predict = 0;
     uint32_t gs_block = 256, gs_blocks = 0;
     uint32_t pj_blocks = 0;        // persistent Jacobi grid (co-resident blocks)
+
+    // learned warm start (grid meshes): 8 grid neighbours and per-vertex
+    // feature constants, built at create when the mesh carries cloth columns
+    bool warm_ok = false;
+    int32_t* nbr8 = nullptr;
+    float4* vaux = nullptr;
+    uint16_t* feat = nullptr;      // n x 32 bf16 features (allocated on first use)
+    float4* W0 = nullptr;          // warm-start iterate
+    uint32_t* warm_flag = nullptr; // [0] fallback flag, [1] warm substeps, [2] fallbacks
+    xpbd_mlp* warm = nullptr;
+    uint32_t warm_iters = 2;
+    float warm_threshold = 1e-4f;
 
     // instrumented frames (xpbd_profile_frames): events around every launch of
     // one kernel kind, enqueued eagerly
@@ -267,6 +294,57 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         }
         F = h->A;
         nparts = h->gs_blocks;
+    } else if (h->warm) {
+        // learned warm start: predict -> features -> x0 = xp + MLP -> warm
+        // iterations; the relative residual of the result gates a plain
+        // fallback substep (predict + I iterations) whose kernels do nothing
+        // when the warm result is accepted.
+        xpbd_mlp* m = h->warm;
+        JacobiArgs a = jacobi_args(h, p);
+        launch_predict(h->X, h->V, h->A, p, s);
+        WarmFeatArgs fa{};
+        fa.xp = h->A;
+        fa.vel = h->V;
+        fa.rest4 = h->rest4;
+        fa.nbr8 = h->nbr8;
+        fa.vaux = h->vaux;
+        fa.dt = h->u.dt;
+        fa.n = h->n;
+        fa.feat = h->feat;
+        launch_warm_features(fa, s);
+        launch_warm_apply(h->feat, h->A, h->vaux, m->P, m->bf, 1.0f / m->out_scale, h->W0, h->n, s);
+        float4* cur = h->W0;
+        float4* nxt = h->B;
+        for (uint32_t k = 0; k < h->warm_iters; ++k) {
+            a.src = cur;
+            a.dst = nxt;
+            ProbeScope ps(h, XPBD_KERNEL_JACOBI_ITER);
+            launch_jacobi(0, a, h->jacobi_block, s);
+            std::swap(cur, nxt);
+        }
+        launch_zero_u32(reinterpret_cast<uint32_t*>(h->resid), 2, s);
+        launch_residual(cur, h->gs_ij, h->gs_rc, h->e, h->resid, s);
+        launch_warm_flag(h->resid, h->rest_sq_sum, h->warm_threshold, h->warm_flag, s);
+        // fallback ping-pong between A and the buffer not holding the warm result
+        float4* other = (cur == h->W0) ? h->B : h->W0;
+        JacobiArgs g = jacobi_args(h, p);
+        g.gate = h->warm_flag;
+        launch_predict_gated(h->X, h->V, h->A, p, h->warm_flag, s);
+        float4* fc = h->A;
+        float4* fn = other;
+        for (uint32_t k = 0; k < iters; ++k) {
+            g.src = fc;
+            g.dst = fn;
+            launch_jacobi(0, g, h->jacobi_block, s);
+            std::swap(fc, fn);
+        }
+        launch_copy4_unless(cur, fc, h->n, h->warm_flag, s);
+        if (collision) {
+            F = fc;
+            if (h->nclothes == 1) nparts = launch_aabb_of(F, h->n, h->aabb, s);
+        } else {
+            launch_finalize(fc, h->X, h->V, h->has_ground ? h->ground : nullptr, p, s);
+        }
     } else if (h->cfg.persistent_jacobi) {
         // predict and all iterations in one launch; the result lands in X,V
         // (no collision) or in A (even I) / B (odd I) for the collision pass.
@@ -631,6 +709,75 @@ void apply_tuning(xpbd_cloth* h) {
     h->pj_blocks = std::max(1u, std::min(persistent_max_blocks(h->nbpk != nullptr, h->cfg.fast_division != 0), groups));
 }
 
+// Grid neighbours and feature constants of the warm-start network.  Needs a
+// grid mesh (cloth_cols): vertex i of cloth c sits at (i - off_c) % cols,
+// (i - off_c) / cols.  Neighbour slots: +x, -x, +y, -y, (+x,+y), (-x,-y),
+// (+x,-y), (-x,+y).  Per vertex: scaled log10 compliance of its stretch,
+// shear and bend constraints (classified by grid offset of the partner) and
+// 1 / spacing (rest length of a stretch partner).
+float scaled_log_compliance(float c) { return (std::log10(std::max(c, 1e-10f)) + 5.5f) / 2.5f; }
+
+int build_warm_topology(xpbd_cloth* h, const Mesh& m) {
+    h->warm_ok = false;
+    const size_t C = m.cloth_cols.size();
+    if (C == 0 || C + 1 != m.cloth_offsets.size()) return XPBD_OK;
+    const size_t n = h->n;
+    std::vector<int32_t> nbr(8 * n, -1);
+    std::vector<float> aux(4 * n, 0.f);
+    static const int DX[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+    static const int DY[8] = {0, 0, 1, -1, 1, -1, -1, 1};
+    for (size_t c = 0; c < C; ++c) {
+        const uint32_t off = m.cloth_offsets[c], end = m.cloth_offsets[c + 1], cols = m.cloth_cols[c];
+        if (cols == 0 || (end - off) % cols != 0) return XPBD_OK;
+        const int rows = int((end - off) / cols), ic = int(cols);
+        for (uint32_t i = off; i < end; ++i) {
+            const int col = int((i - off) % cols), row = int((i - off) / cols);
+            for (int k = 0; k < 8; ++k) {
+                const int nc = col + DX[k], nr = row + DY[k];
+                if (nc >= 0 && nc < ic && nr >= 0 && nr < rows) nbr[8 * i + k] = int32_t(off + uint32_t(nr * ic + nc));
+            }
+            float cs = 0.f, csh = 0.f, cb = 0.f, hs = 0.f;
+            bool fs = false, fsh = false, fb = false;
+            for (uint32_t q = m.neighbor_offsets[i]; q < m.neighbor_offsets[i + 1]; ++q) {
+                const uint32_t j = m.neighbor_other[q];
+                if (j < off || j >= end) continue;
+                const int dc = std::abs(int((j - off) % cols) - col), dr = std::abs(int((j - off) / cols) - row);
+                if (!fs && dc + dr == 1) {
+                    fs = true;
+                    cs = m.neighbor_compliance[q];
+                    hs = m.neighbor_rest_len[q];
+                } else if (!fsh && dc == 1 && dr == 1) {
+                    fsh = true;
+                    csh = m.neighbor_compliance[q];
+                } else if (!fb && ((dc == 2 && dr == 0) || (dc == 0 && dr == 2))) {
+                    fb = true;
+                    cb = m.neighbor_compliance[q];
+                }
+            }
+            aux[4 * i] = scaled_log_compliance(cs);
+            aux[4 * i + 1] = scaled_log_compliance(csh);
+            aux[4 * i + 2] = scaled_log_compliance(cb);
+            aux[4 * i + 3] = hs > 0.f ? 1.0f / hs : 0.f;
+        }
+    }
+    TRY(dalloc(h, &h->nbr8, 8 * n));
+    TRY(dalloc(h, &h->vaux, n));
+    TRY(upload(h, h->nbr8, nbr.data(), nbr.size()));
+    TRY(upload(h, reinterpret_cast<float*>(h->vaux), aux.data(), aux.size()));
+    h->warm_ok = true;
+    return XPBD_OK;
+}
+
+int ensure_warm_buffers(xpbd_cloth* h) {
+    if (!h->feat) TRY(dalloc(h, &h->feat, size_t(h->n) * MLP_IN));
+    if (!h->W0) TRY(dalloc(h, &h->W0, h->n));
+    if (!h->warm_flag) {
+        TRY(dalloc(h, &h->warm_flag, 4));
+        HIPC(h, hipMemsetAsync(h->warm_flag, 0, 4 * sizeof(uint32_t), h->stream));
+    }
+    return XPBD_OK;
+}
+
 int autotune(xpbd_cloth* h) {
     if (h->n == 0 || h->e == 0) return XPBD_OK;
     const DevParams p = dev_params(h);
@@ -749,6 +896,7 @@ static const void* mesh_field(const Mesh& m, int f, uint64_t& count) {
         case XPBD_MESH_BATCH_OFFSETS: count = m.batch_offsets.size(); return m.batch_offsets.data();
         case XPBD_MESH_GROUND: count = m.ground.size(); return m.ground.data();
         case XPBD_MESH_CLOTH_OFFSETS: count = m.cloth_offsets.size(); return m.cloth_offsets.data();
+        case XPBD_MESH_CLOTH_COLS: count = m.cloth_cols.size(); return m.cloth_cols.data();
         default: count = 0; return nullptr;
     }
 }
@@ -843,6 +991,8 @@ XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, cons
         h->own_stream = true;
     }
     int r = create_device(h, m);
+    if (r != XPBD_OK) return r;
+    r = build_warm_topology(h, m);
     if (r != XPBD_OK) return r;
     r = upload_state(h, h->init_pos4.data(), nullptr);
     if (r != XPBD_OK) return r;
@@ -1146,6 +1296,278 @@ XPBD_API int xpbd_profile_frames(xpbd_cloth* h, uint32_t frames, int which, floa
     TRY(check_device_flag(h));
     *avg_ms = count ? float(total / count) : 0.f;
     if (launches) *launches = count;
+    return XPBD_OK;
+}
+
+// ---------------------------------------------------------------- warm start
+}  // extern "C"
+
+namespace {
+
+int mfail(xpbd_mlp* m, int code, const std::string& msg) {
+    if (m) m->msg = msg;
+    return code;
+}
+
+#define MHIPC(m, expr)                                                                                       \
+    do {                                                                                                     \
+        hipError_t _e = (expr);                                                                              \
+        if (_e != hipSuccess)                                                                                \
+            return mfail((m), _e == hipErrorOutOfMemory ? XPBD_E_NOMEM : XPBD_E_HIP,                       \
+                         std::string(#expr) + ": " + hipGetErrorString(_e));                                \
+    } while (0)
+
+template <class T>
+int malloc_mlp(xpbd_mlp* m, T** p, size_t count) {
+    void* q = nullptr;
+    MHIPC(m, hipMalloc(&q, std::max<size_t>(count * sizeof(T), 256)));
+    m->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return XPBD_OK;
+}
+
+// He-style initial weights from a seeded SplitMix64 + Box-Muller stream;
+// biases zero; padded rows / columns zero.
+void mlp_init_params(uint64_t seed, std::vector<float>& P) {
+    P.assign(MLP_PARAMS, 0.f);
+    uint64_t st = seed;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    auto normal = [&]() {
+        const double u1 = (double(next() >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+        const double u2 = double(next() >> 11) * (1.0 / 9007199254740992.0);
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    for (int f = 0; f < MLP_FEAT; ++f)
+        for (int j = 0; j < MLP_HID; ++j) P[OFF_W1 + f * MLP_HID + j] = float(normal() * std::sqrt(2.0 / MLP_FEAT));
+    for (int i = 0; i < MLP_HID; ++i)
+        for (int j = 0; j < MLP_HID; ++j) P[OFF_W2 + i * MLP_HID + j] = float(normal() * std::sqrt(2.0 / MLP_HID));
+    for (int i = 0; i < MLP_HID; ++i)
+        for (int o = 0; o < MLP_OUT; ++o) P[OFF_W3 + i * MLP_OUTP + o] = float(normal() * std::sqrt(1.0 / MLP_HID));
+}
+
+bool mlp_param_fixed_zero(int p) {
+    if (p >= OFF_W1 + MLP_FEAT * MLP_HID && p < OFF_B1) return true;
+    if (p >= OFF_W3 && p < OFF_B3) return (p - OFF_W3) % MLP_OUTP >= MLP_OUT;
+    if (p >= OFF_B3) return p - OFF_B3 >= MLP_OUT;
+    return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+XPBD_API int xpbd_mlp_create(uint64_t seed, xpbd_mlp** out) {
+    if (!out) return XPBD_E_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return XPBD_E_NO_DEVICE;
+    std::unique_ptr<xpbd_mlp, void (*)(xpbd_mlp*)> guard(new (std::nothrow) xpbd_mlp(), xpbd_mlp_destroy);
+    xpbd_mlp* m = guard.get();
+    if (!m) return XPBD_E_NOMEM;
+    if (hipGetDevice(&m->device) != hipSuccess) return XPBD_E_HIP;
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) return XPBD_E_HIP;
+    TRY(malloc_mlp(m, &m->P, MLP_PARAMS));
+    TRY(malloc_mlp(m, &m->m, MLP_PARAMS));
+    TRY(malloc_mlp(m, &m->v, MLP_PARAMS));
+    TRY(malloc_mlp(m, &m->grad, MLP_PARAMS));
+    TRY(malloc_mlp(m, &m->bf, MLP_BF));
+    TRY(malloc_mlp(m, &m->partial, size_t(MLP_TRAIN_WAVES) * MLP_PSTRIDE));
+    m->loss_cap = 1;
+    TRY(malloc_mlp(m, &m->loss, m->loss_cap));
+    std::vector<float> P;
+    mlp_init_params(seed, P);
+    MHIPC(m, hipMemcpy(m->P, P.data(), P.size() * sizeof(float), hipMemcpyHostToDevice));
+    MHIPC(m, hipMemset(m->m, 0, MLP_PARAMS * sizeof(float)));
+    MHIPC(m, hipMemset(m->v, 0, MLP_PARAMS * sizeof(float)));
+    MHIPC(m, hipMemset(m->bf, 0, MLP_BF * sizeof(uint16_t)));
+    launch_mlp_pack(m->P, m->bf, m->stream);
+    MHIPC(m, hipStreamSynchronize(m->stream));
+    *out = guard.release();
+    return XPBD_OK;
+}
+
+XPBD_API void xpbd_mlp_destroy(xpbd_mlp* m) {
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (void* p : m->allocs) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+XPBD_API const char* xpbd_mlp_last_error(const xpbd_mlp* m) { return m ? m->msg.c_str() : "null handle"; }
+
+XPBD_API int xpbd_mlp_get_params(xpbd_mlp* m, float* params, float* out_scale) {
+    if (!m || !params) return XPBD_E_INVALID;
+    MHIPC(m, hipStreamSynchronize(m->stream));
+    MHIPC(m, hipMemcpy(params, m->P, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost));
+    if (out_scale) *out_scale = m->out_scale;
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_mlp_set_params(xpbd_mlp* m, const float* params, float out_scale) {
+    if (!m || !params || !(out_scale > 0.f)) return XPBD_E_INVALID;
+    std::vector<float> P(params, params + MLP_PARAMS);
+    for (int p = 0; p < MLP_PARAMS; ++p)
+        if (mlp_param_fixed_zero(p)) P[p] = 0.f;
+    MHIPC(m, hipStreamSynchronize(m->stream));
+    MHIPC(m, hipMemcpy(m->P, P.data(), MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice));
+    MHIPC(m, hipMemset(m->m, 0, MLP_PARAMS * sizeof(float)));
+    MHIPC(m, hipMemset(m->v, 0, MLP_PARAMS * sizeof(float)));
+    m->step = 0;
+    m->out_scale = out_scale;
+    launch_mlp_pack(m->P, m->bf, m->stream);
+    MHIPC(m, hipStreamSynchronize(m->stream));
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_mlp_forward(xpbd_mlp* m, const void* d_features, uint32_t rows, void* d_out) {
+    if (!m || (rows && (!d_features || !d_out))) return XPBD_E_INVALID;
+    launch_mlp_forward(static_cast<const uint16_t*>(d_features), rows, m->P, m->bf, static_cast<float*>(d_out),
+                       m->stream);
+    MHIPC(m, hipGetLastError());
+    MHIPC(m, hipStreamSynchronize(m->stream));
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_mlp_gradient(xpbd_mlp* m, const void* d_features, const void* d_targets, uint32_t rows,
+                               float* grad, float* loss) {
+    if (!m || !grad || rows == 0 || !d_features || !d_targets) return XPBD_E_INVALID;
+    launch_mlp_train(static_cast<const uint16_t*>(d_features), static_cast<const float*>(d_targets), rows,
+                     m->out_scale, m->P, m->bf, m->partial, m->stream);
+    launch_mlp_reduce_adam(m->partial, m->P, m->m, m->v, m->bf, m->grad, m->loss, 0.f, 1.f, 1.f, m->stream);
+    MHIPC(m, hipGetLastError());
+    MHIPC(m, hipMemcpyAsync(grad, m->grad, MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    float sq = 0.f;
+    MHIPC(m, hipMemcpyAsync(&sq, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    MHIPC(m, hipStreamSynchronize(m->stream));
+    if (loss) *loss = sq / (3.0f * float(rows));
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_mlp_train(xpbd_mlp* m, const void* d_features, const void* d_targets, uint32_t rows,
+                            uint32_t steps, uint32_t batch, float lr, float* loss_curve) {
+    if (!m || !d_features || !d_targets || rows == 0 || batch == 0 || !(lr > 0.f)) return XPBD_E_INVALID;
+    batch = std::min(batch, rows);
+    if (steps > m->loss_cap) {
+        float* q = nullptr;
+        TRY(malloc_mlp(m, &q, steps));
+        m->loss = q;
+        m->loss_cap = steps;
+    }
+    const uint16_t* X = static_cast<const uint16_t*>(d_features);
+    const float* T = static_cast<const float*>(d_targets);
+    const uint32_t nb = rows / batch;   // whole minibatches per epoch (rows in their given order)
+    double b1t = std::pow(0.9, double(m->step)), b2t = std::pow(0.999, double(m->step));
+    for (uint32_t k = 0; k < steps; ++k) {
+        const uint32_t start = (k % nb) * batch;
+        ++m->step;
+        b1t *= 0.9;
+        b2t *= 0.999;
+        // cosine decay from lr to lr / 10 over the call
+        const float lrk = lr * (0.1f + 0.9f * 0.5f * (1.0f + std::cos(3.14159265f * float(k) / float(steps))));
+        launch_mlp_train(X + size_t(start) * MLP_IN, T + size_t(start) * 3, batch, m->out_scale, m->P, m->bf,
+                         m->partial, m->stream);
+        launch_mlp_reduce_adam(m->partial, m->P, m->m, m->v, m->bf, nullptr, m->loss + k, lrk, float(b1t),
+                               float(b2t), m->stream);
+    }
+    MHIPC(m, hipGetLastError());
+    if (loss_curve) {
+        std::vector<float> sq(steps);
+        MHIPC(m, hipMemcpyAsync(sq.data(), m->loss, steps * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+        MHIPC(m, hipStreamSynchronize(m->stream));
+        for (uint32_t k = 0; k < steps; ++k) loss_curve[k] = sq[k] / (3.0f * float(batch));
+    }
+    MHIPC(m, hipStreamSynchronize(m->stream));
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_warm_sample(xpbd_cloth* h, uint32_t iterations, void* d_features, void* d_targets) {
+    if (!h || !d_features || !d_targets) return XPBD_E_INVALID;
+    if (!h->warm_ok) return fail(h, XPBD_E_UNSUPPORTED, "warm start needs a grid mesh (cloth columns)");
+    if (h->n == 0) return XPBD_OK;
+    hipStream_t s = h->stream;
+    const DevParams p = dev_params(h);
+    launch_predict(h->X, h->V, h->sP, p, s);   // predicted positions, kept for the targets
+    launch_predict(h->X, h->V, h->A, p, s);
+    WarmFeatArgs fa{};
+    fa.xp = h->sP;
+    fa.vel = h->V;
+    fa.rest4 = h->rest4;
+    fa.nbr8 = h->nbr8;
+    fa.vaux = h->vaux;
+    fa.dt = h->u.dt;
+    fa.n = h->n;
+    fa.feat = static_cast<uint16_t*>(d_features);
+    launch_warm_features(fa, s);
+    JacobiArgs a = jacobi_args(h, p);
+    float4* cur = h->A;
+    float4* nxt = h->B;
+    for (uint32_t k = 0; k < iterations; ++k) {
+        a.src = cur;
+        a.dst = nxt;
+        launch_jacobi(0, a, h->jacobi_block, s);
+        std::swap(cur, nxt);
+    }
+    launch_warm_targets(h->sP, cur, h->vaux, h->n, static_cast<float*>(d_targets), s);
+    HIPC(h, hipGetLastError());
+    HIPC(h, hipStreamSynchronize(s));
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_warm_features(xpbd_cloth* h, void* d_features) {
+    if (!h || !d_features) return XPBD_E_INVALID;
+    if (!h->warm_ok) return fail(h, XPBD_E_UNSUPPORTED, "warm start needs a grid mesh (cloth columns)");
+    if (h->n == 0) return XPBD_OK;
+    const DevParams p = dev_params(h);
+    launch_predict(h->X, h->V, h->sP, p, h->stream);
+    WarmFeatArgs fa{};
+    fa.xp = h->sP;
+    fa.vel = h->V;
+    fa.rest4 = h->rest4;
+    fa.nbr8 = h->nbr8;
+    fa.vaux = h->vaux;
+    fa.dt = h->u.dt;
+    fa.n = h->n;
+    fa.feat = static_cast<uint16_t*>(d_features);
+    launch_warm_features(fa, h->stream);
+    HIPC(h, hipGetLastError());
+    HIPC(h, hipStreamSynchronize(h->stream));
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_set_warm_start(xpbd_cloth* h, xpbd_mlp* m, uint32_t warm_iterations, float residual_threshold) {
+    if (!h) return XPBD_E_INVALID;
+    if (m) {
+        if (!h->warm_ok) return fail(h, XPBD_E_UNSUPPORTED, "warm start needs a grid mesh (cloth columns)");
+        if (h->cfg.solver != XPBD_SOLVER_JACOBI) return fail(h, XPBD_E_UNSUPPORTED, "warm start is for the Jacobi solver");
+        if (!(residual_threshold >= 0.f)) return XPBD_E_INVALID;
+        TRY(ensure_warm_buffers(h));
+        if (m->device != h->device) return fail(h, XPBD_E_INVALID, "network and cloth on different devices");
+        HIPC(h, hipStreamSynchronize(m->stream));
+    }
+    h->warm = m;
+    h->warm_iters = warm_iterations;
+    h->warm_threshold = residual_threshold;
+    drop_graph(h);
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_warm_stats(xpbd_cloth* h, uint32_t* substeps, uint32_t* fallbacks, uint32_t reset) {
+    if (!h) return XPBD_E_INVALID;
+    uint32_t f[4] = {0, 0, 0, 0};
+    if (h->warm_flag) {
+        HIPC(h, hipMemcpyAsync(f, h->warm_flag, sizeof(f), hipMemcpyDeviceToHost, h->stream));
+        HIPC(h, hipStreamSynchronize(h->stream));
+        if (reset) HIPC(h, hipMemset(h->warm_flag, 0, sizeof(f)));
+    }
+    if (substeps) *substeps = f[1];
+    if (fallbacks) *fallbacks = f[2];
     return XPBD_OK;
 }
 

@@ -70,7 +70,9 @@ enum {
     XPBD_MESH_BATCH_OFFSETS = 16,          /* u32 x (B+1) colour batch starts   */
     XPBD_MESH_GROUND = 17,                 /* u32 x N  1 = ground plane y=0     */
     XPBD_MESH_CLOTH_OFFSETS = 18,          /* u32 x (C+1) first vertex of cloth */
-    XPBD_MESH_NUM_FIELDS = 19
+    XPBD_MESH_CLOTH_COLS = 19,             /* u32 x C grid columns per cloth
+                                              (empty for meshes that are not grids) */
+    XPBD_MESH_NUM_FIELDS = 20
 };
 
 /* grid_cloth_hanging (mesh_prep.rs:363-406). */
@@ -247,6 +249,51 @@ XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t*
  * out[1] = the first such b (bits); out[2] = mismatches of the quotient
  * against '/' over `pairs` pseudo-random pairs, out[3..4] = first (a, b). */
 XPBD_API int xpbd_selftest_division(int exp_lo, int exp_hi, uint64_t pairs, uint64_t seed, uint32_t* out5);
+
+/* ---------------------------------------------------------------------------
+ * Learned warm start (no reference counterpart).  A per-vertex MLP
+ * (30 -> 64 -> 64 -> 3, GELU; BF16 weights and activations on MFMA, FP32
+ * accumulate) predicts the converged constraint correction of a substep from
+ * the predicted positions; the substep then runs `warm_iterations` Jacobi
+ * iterations from that start, and falls back to the plain substep (predict +
+ * I iterations) when the relative residual of the result exceeds the
+ * threshold.  Features (per vertex, 30): the 8 grid neighbours' rest-relative
+ * displacements / spacing (24), velocity * dt / spacing (3), scaled log10
+ * stretch / shear / bend compliance (3).  Grid meshes only (cloth columns).
+ * ------------------------------------------------------------------------- */
+typedef struct xpbd_mlp xpbd_mlp;
+#define XPBD_MLP_PARAMS 7312   /* f32 master parameters, layout in mlp.hpp */
+
+/* Seeded He-style initialisation; the network lives on the current device. */
+XPBD_API int xpbd_mlp_create(uint64_t seed, xpbd_mlp** out);
+XPBD_API void xpbd_mlp_destroy(xpbd_mlp* m);
+XPBD_API const char* xpbd_mlp_last_error(const xpbd_mlp* m);
+/* Master parameters (host, XPBD_MLP_PARAMS floats) and the output scale
+ * (model output = correction / spacing * out_scale).  set resets Adam. */
+XPBD_API int xpbd_mlp_get_params(xpbd_mlp* m, float* params, float* out_scale);
+XPBD_API int xpbd_mlp_set_params(xpbd_mlp* m, const float* params, float out_scale);
+/* Forward of `rows` feature rows (device, rows x 32 bf16) -> device rows x 3 f32. */
+XPBD_API int xpbd_mlp_forward(xpbd_mlp* m, const void* d_features, uint32_t rows, void* d_out);
+/* Gradient of the mean squared error against device targets (rows x 3 f32,
+ * correction / spacing; scaled by out_scale inside) -> host grad
+ * (XPBD_MLP_PARAMS floats) and the loss. */
+XPBD_API int xpbd_mlp_gradient(xpbd_mlp* m, const void* d_features, const void* d_targets, uint32_t rows,
+                               float* grad, float* loss);
+/* `steps` Adam steps on consecutive minibatches of `batch` rows (rows in the
+ * order given; cosine decay lr -> lr/10); loss_curve: host, `steps` floats or NULL. */
+XPBD_API int xpbd_mlp_train(xpbd_mlp* m, const void* d_features, const void* d_targets, uint32_t rows,
+                            uint32_t steps, uint32_t batch, float lr, float* loss_curve);
+/* Training pairs from the cloth's current state (state unchanged): features
+ * of the predicted positions (device, N x 32 bf16) and the targets (device,
+ * N x 3 f32) = (result of `iterations` plain Jacobi iterations - predicted) / spacing. */
+XPBD_API int xpbd_warm_sample(xpbd_cloth* h, uint32_t iterations, void* d_features, void* d_targets);
+/* Features of the predicted positions only. */
+XPBD_API int xpbd_warm_features(xpbd_cloth* h, void* d_features);
+/* Attach (m != NULL) or detach the network.  The network must outlive the
+ * attachment; attach again after changing its output scale. */
+XPBD_API int xpbd_set_warm_start(xpbd_cloth* h, xpbd_mlp* m, uint32_t warm_iterations, float residual_threshold);
+/* Substeps run with the warm start and how many fell back; reset != 0 zeroes them. */
+XPBD_API int xpbd_warm_stats(xpbd_cloth* h, uint32_t* substeps, uint32_t* fallbacks, uint32_t reset);
 
 /* Library info. */
 XPBD_API int xpbd_abi_version(void);

@@ -1,0 +1,172 @@
+"""The learned warm start on the GPU, through the C ABI, against the CPU
+oracle of the network (oracle/mlp_oracle.py) and the plain solver.
+
+- features of a predicted state: bit-exact (bf16) against the oracle;
+- BF16-MFMA forward and backward: within the tolerance the bf16 rounding
+  points leave (the oracle rounds at the same points, sums in float64);
+- the warm-start substep: with a zero network and I warm iterations it is
+  the plain substep bit for bit; with threshold 0 every substep falls back
+  to the plain substep, bit for bit.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from _helpers import bits_equal, oracle_params
+from oracle import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def small_batch(X):
+    descs = [X.SheetDesc.make(cols=32, rows=32, spacing=1 / 31, pins=1, jitter=1e-3, seed=s,
+                              stretch_compliance=c) for s, c in ((1, 0.0), (2, 1e-7))]
+    descs.append(X.SheetDesc.make(cols=40, rows=24, spacing=0.02, layout=1, height=1.0, angle=0.7, pins=3,
+                                  jitter=1e-3, seed=3, shear_compliance=1e-4, bend_compliance=1e-2))
+    return X.ClothMeshData.batch(descs)
+
+
+def make(X, mesh, I=10, use_graph=0):
+    cfg = X.ClothSimConfig.default(solve_substeps=8, solve_inner_iterations=I, use_graph=use_graph)
+    u = X.ClothSimUniforms.default(X.SOLVER_JACOBI, 8, coll_scale=0.0)
+    return X.ClothComputePlugin(mesh, cfg, u), cfg, u
+
+
+def predicted(O, X, f, pos, vel, u, cfg):
+    """Predicted positions of the state (pos, vel): the oracle's predict, i.e.
+    one substep with zero iterations and no constraints would be wrong; use
+    the product's predict through the features' own definition instead:
+    x~ = x + dt * v', v' = v + g dt (speed-capped), as xpbd_cpu.rs:38-65."""
+    dt = np.float32(u.dt)
+    g = np.asarray(u.gravity[:3], np.float32)
+    v = (vel[:, :3] + g * dt).astype(np.float32)
+    sp = np.sqrt(((v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2]).astype(np.float32))
+    cap = sp > np.float32(12.0)
+    v[cap] = (v[cap] * (np.float32(12.0) / sp[cap])[:, None]).astype(np.float32)
+    x = (pos[:, :3] + v * dt).astype(np.float32)
+    pinned = f["inv_mass"] <= 0
+    x[pinned] = pos[pinned, :3]
+    return x
+
+
+def test_features_bit_exact(O, X):
+    note("case warm features")
+    mesh = small_batch(X)
+    f = mesh.fields()
+    sim, cfg, u = make(X, mesh)
+    sim.step_frames(3)
+    pos, vel = sim.read_state()
+    n = len(pos)
+    d = X.DeviceArray((n, 32), np.uint16)
+    sim.warm_features(d)
+    got = d.numpy().astype(np.uint32) << 16
+    got = got.view(np.float32)
+    nbr, aux = M.grid_topology(f)
+    xp = predicted(O, X, f, pos, vel, u, cfg)
+    want = M.features(xp, vel, f["rest_positions"], nbr, aux, u.dt)
+    assert got.tobytes() == want.tobytes(), np.argwhere(got != want)[:5]
+    sim.close()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_and_gradient_against_oracle(X):
+    note("case mlp forward/gradient")
+    rng = np.random.default_rng(5)
+    rows = 1000
+    P = M.init_params(seed=7)
+    feat = M.bf16(rng.normal(0, 0.5, (rows, 32)).astype(np.float32))
+    feat[:, 30:] = 0
+    T = rng.normal(0, 1, (rows, 3)).astype(np.float32)
+    mlp = X.WarmStartMLP(seed=1)
+    mlp.set_params(P, out_scale=1.7)
+    dfeat = X.DeviceArray.from_numpy((feat.view(np.uint32) >> 16).astype(np.uint16))
+    dT = X.DeviceArray.from_numpy(T)
+    out = mlp.forward(dfeat, rows)
+    ref, _ = M.forward(P, feat, bf16_points=True)
+    note(f"  forward rel err {rel(out, ref):.2e}")
+    assert rel(out, ref) < 2e-3
+    assert np.max(np.abs(out - ref)) < 2e-2 * np.max(np.abs(ref))
+    g, loss = mlp.gradient(dfeat, dT, rows)
+    rloss, rg = M.loss_and_grad(P, feat, T, 1.7, bf16_points=True)
+    note(f"  loss {loss:.6f} vs {rloss:.6f}")
+    assert abs(loss - rloss) <= 1e-3 * abs(rloss)
+    for name, a, b in (("W1", M.OFF_W1, M.OFF_B1), ("b1", M.OFF_B1, M.OFF_W2), ("W2", M.OFF_W2, M.OFF_B2),
+                       ("b2", M.OFF_B2, M.OFF_W3), ("W3", M.OFF_W3, M.OFF_B3), ("b3", M.OFF_B3, M.PARAMS)):
+        e = rel(g[a:b], rg[a:b])
+        note(f"  grad {name}: rel err {e:.2e}")
+        assert e < 1e-2, name
+    assert np.all(g[M.fixed_zero_mask()] == 0)
+    mlp.close()
+
+
+def test_training_reduces_loss(X):
+    note("case mlp training")
+    rng = np.random.default_rng(9)
+    rows = 8192
+    P = M.init_params(seed=3)
+    feat = M.bf16(rng.normal(0, 0.5, (rows, 32)).astype(np.float32))
+    feat[:, 30:] = 0
+    teacher = M.init_params(seed=11)
+    T, _ = M.forward(teacher, feat, bf16_points=False)
+    T = T.astype(np.float32)
+    mlp = X.WarmStartMLP(seed=1)
+    mlp.set_params(P, out_scale=1.0)
+    dfeat = X.DeviceArray.from_numpy((feat.view(np.uint32) >> 16).astype(np.uint16))
+    dT = X.DeviceArray.from_numpy(T)
+    curve = mlp.train(dfeat, dT, rows, steps=400, batch=1024, lr=2e-3)
+    note(f"  loss {curve[0]:.4f} -> {curve[-1]:.4f}")
+    assert curve[-1] < 0.2 * curve[0]
+    mlp.close()
+
+
+def test_zero_network_warm_substep_equals_plain(X):
+    """x0 = xp exactly for a zero network, so I warm iterations reproduce the
+    plain substep bit for bit (features, apply, iteration chain, residual
+    gate, selection and velocity update)."""
+    note("case warm zero network")
+    mesh = small_batch(X)
+    a, _, _ = make(X, mesh, I=6)
+    b, _, _ = make(X, mesh, I=6)
+    mlp = X.WarmStartMLP(seed=1)
+    mlp.set_params(np.zeros(X.MLP_PARAMS, np.float32), out_scale=1.0)
+    a.set_warm_start(mlp, warm_iterations=6, residual_threshold=float("inf"))
+    for k in range(3):
+        a.step_substeps(4, False)
+        b.step_substeps(4, False)
+        pa, va = a.read_state()
+        pb, vb = b.read_state()
+        assert bits_equal(pa, pb) and bits_equal(va, vb), k
+    assert a.warm_stats() == (12, 0)
+    a.close()
+    b.close()
+    mlp.close()
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_threshold_zero_always_falls_back_to_plain(X, use_graph):
+    note(f"case warm fallback graph={use_graph}")
+    mesh = small_batch(X)
+    a, _, _ = make(X, mesh, I=10, use_graph=use_graph)
+    b, _, _ = make(X, mesh, I=10, use_graph=use_graph)
+    mlp = X.WarmStartMLP(seed=2)
+    P, _ = mlp.params()
+    mlp.set_params(P, out_scale=10.0)
+    a.set_warm_start(mlp, warm_iterations=2, residual_threshold=0.0)
+    a.step_frames(2)
+    b.step_frames(2)
+    pa, va = a.read_state()
+    pb, vb = b.read_state()
+    assert bits_equal(pa, pb) and bits_equal(va, vb)
+    assert a.warm_stats() == (16, 16)
+    a.close()
+    b.close()
+    mlp.close()
