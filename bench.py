@@ -74,6 +74,31 @@ def cpu_baseline(mesh_fields, u, cfg, budget_s: float, threads: int):
     return done / el, done, el
 
 
+def learned_warm_start():
+    """Train the warm-start network on the GPU (solver-generated pairs of the
+    32x32 and 256x256 training sheets) and measure what it gains on held-out
+    32x32 stiff sheets: one substep of warm start + 2 iterations against
+    plain Jacobi at several iteration counts (equal-residual speedup), and
+    10 frames behind the residual gate (threshold 1e-4)."""
+    try:
+        from bevy_xpbd_amd import warmstart as W
+        t0 = time.perf_counter()
+        feat, tgt, rows, th = W.generate_dataset(seed=1, frames=(3, 12), target_iterations=200)
+        mlp, curve = W.train(feat, tgt, rows, th, steps=3000, batch=4096, lr=2e-3)
+        res = W.evaluate(mlp, threshold=1e-4)
+        res.update({"training_pairs": rows, "train_steps": 3000, "train_batch": 4096,
+                    "train_loss_first_last": [round(float(curve[0]), 4), round(float(curve[-50:].mean()), 4)],
+                    "seconds": round(time.perf_counter() - t0, 1)})
+        mlp.close()
+        log(f"bench: warm start: warm+2 ~ plain I={res['equivalent_plain_iterations']}, single-substep speedup "
+            f"{res['single_substep_speedup_at_equal_residual']}x; gated fallbacks "
+            f"{res['gated_fallbacks']}/{res['gated_substeps']}")
+        return res
+    except Exception as exc:   # reported, never fatal to the throughput line
+        log(f"bench: warm start evaluation failed: {exc!r}")
+        return {"error": repr(exc)}
+
+
 def build_workload(X, name: str):
     """Returns (mesh, thickness, description, config extras)."""
     if name == "sheet1024":
@@ -105,6 +130,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-warm", action="store_true", help="skip the learned warm start's train + evaluation")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-kernel timing launches (for rocprofv3 counter passes)")
     args = ap.parse_args()
@@ -182,6 +208,10 @@ def main():
     log(f"bench: dominant kernel {rkernel} {r_us:.1f} us = {achieved:.0f} GB/s algorithmic; "
         f"collision sweep {sweep_ms * 1e3:.1f} us; radix pass {radix_ms * 1e3:.1f} us")
 
+    warm = None
+    if not args.no_warm:
+        warm = learned_warm_start()
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -202,6 +232,8 @@ def main():
               "copy_kernel_GBps": round(32 * n / (copy_ms * 1e-3) / 1e9, 1),
               "persistent_predict_plus_iterations_us": round(pers_ms * 1e3, 2)}
     config.update(extra)
+    if warm is not None:
+        config["learned_warm_start"] = warm
     out = {
         "metric": "cloth substeps/s at fixed constraint residual; learned vs plain Jacobi; HBM %",
         "value": round(value, 2),

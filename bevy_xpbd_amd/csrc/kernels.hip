@@ -1057,6 +1057,7 @@ __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
 // ---------------------------------------------------------------------------
 template <bool PACKED, bool FASTDIV>
 __global__ __launch_bounds__(256) void k_jacobi_persistent(JacobiArgs a, PersistArgs q) {
+    if (q.gate && *q.gate == 0u) return;   // uniform: every block leaves before any barrier
     __shared__ float2 lpairs[LDS_PAIRS_MAX];
     const bool lds = stage_pairs<PACKED>(a, lpairs);
     const DevParams& p = a.p;
@@ -1245,6 +1246,17 @@ void launch_predict_gated(const float4* X, const float4* V, float4* P, DevParams
 void launch_copy4_unless(const float4* a, float4* b, uint32_t n, const uint32_t* gate, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_copy4_unless, dim3(cdiv(n, 256)), dim3(256), 0, s, a, b, n, gate);
+}
+
+__global__ void k_warm_count(uint32_t* flag) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        flag[0] = 0u;
+        flag[1] = flag[1] + 1u;
+    }
+}
+
+void launch_warm_count(uint32_t* flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_warm_count, dim3(1), dim3(64), 0, s, flag);
 }
 
 void launch_warm_flag(const double* resid, double rest_sq, float threshold, uint32_t* flag, hipStream_t s) {

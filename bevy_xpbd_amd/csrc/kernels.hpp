@@ -72,6 +72,7 @@ struct PersistArgs {
     uint32_t final_write;
     uint32_t* barrier;
     uint32_t* error;
+    const uint32_t* gate;   // non-null: the launch does nothing unless *gate != 0
 };
 
 // Cell grid of one cloth in the collision broad phase (device memory, 12 words).
@@ -151,6 +152,8 @@ void launch_predict_gated(const float4* X, const float4* V, float4* P, DevParams
 void launch_copy4_unless(const float4* a, float4* b, uint32_t n, const uint32_t* gate, hipStream_t s);
 // flag[0] = sqrt(resid) / sqrt(rest_sq) > threshold; flag[1] += 1; flag[2] += flag[0]
 void launch_warm_flag(const double* resid, double rest_sq, float threshold, uint32_t* flag, hipStream_t s);
+// no gate: count the warm substep only
+void launch_warm_count(uint32_t* flag, hipStream_t s);
 uint32_t persistent_max_blocks(bool packed, bool fastdiv);
 void launch_jacobi_persistent(const JacobiArgs& a, const PersistArgs& q, uint32_t blocks, hipStream_t s);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);

@@ -296,9 +296,9 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         nparts = h->gs_blocks;
     } else if (h->warm) {
         // learned warm start: predict -> features -> x0 = xp + MLP -> warm
-        // iterations; the relative residual of the result gates a plain
-        // fallback substep (predict + I iterations) whose kernels do nothing
-        // when the warm result is accepted.
+        // iterations.  With a finite threshold, the relative residual of the
+        // result gates one persistent plain substep (predict + I iterations)
+        // that does nothing when the warm result is accepted.
         xpbd_mlp* m = h->warm;
         JacobiArgs a = jacobi_args(h, p);
         launch_predict(h->X, h->V, h->A, p, s);
@@ -312,9 +312,10 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         fa.n = h->n;
         fa.feat = h->feat;
         launch_warm_features(fa, s);
-        launch_warm_apply(h->feat, h->A, h->vaux, m->P, m->bf, 1.0f / m->out_scale, h->W0, h->n, s);
-        float4* cur = h->W0;
-        float4* nxt = h->B;
+        // x0 lands where the ping-pong (W0 / sP) ends in W0
+        float4* cur = (h->warm_iters % 2) ? h->sP : h->W0;
+        float4* nxt = (h->warm_iters % 2) ? h->W0 : h->sP;
+        launch_warm_apply(h->feat, h->A, h->vaux, m->P, m->bf, 1.0f / m->out_scale, cur, h->n, s);
         for (uint32_t k = 0; k < h->warm_iters; ++k) {
             a.src = cur;
             a.dst = nxt;
@@ -322,28 +323,34 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
             launch_jacobi(0, a, h->jacobi_block, s);
             std::swap(cur, nxt);
         }
-        launch_zero_u32(reinterpret_cast<uint32_t*>(h->resid), 2, s);
-        launch_residual(cur, h->gs_ij, h->gs_rc, h->e, h->resid, s);
-        launch_warm_flag(h->resid, h->rest_sq_sum, h->warm_threshold, h->warm_flag, s);
-        // fallback ping-pong between A and the buffer not holding the warm result
-        float4* other = (cur == h->W0) ? h->B : h->W0;
-        JacobiArgs g = jacobi_args(h, p);
-        g.gate = h->warm_flag;
-        launch_predict_gated(h->X, h->V, h->A, p, h->warm_flag, s);
-        float4* fc = h->A;
-        float4* fn = other;
-        for (uint32_t k = 0; k < iters; ++k) {
-            g.src = fc;
-            g.dst = fn;
-            launch_jacobi(0, g, h->jacobi_block, s);
-            std::swap(fc, fn);
+        float4* fin = h->W0;
+        if (std::isfinite(h->warm_threshold)) {
+            launch_zero_u32(reinterpret_cast<uint32_t*>(h->resid), 2, s);
+            launch_residual(h->W0, h->gs_ij, h->gs_rc, h->e, h->resid, s);
+            launch_warm_flag(h->resid, h->rest_sq_sum, h->warm_threshold, h->warm_flag, s);
+            launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
+            JacobiArgs g = jacobi_args(h, p);
+            PersistArgs q{};
+            q.X = h->X;
+            q.V = h->V;
+            q.A = h->A;
+            q.B = h->B;
+            q.iterations = iters;
+            q.final_write = 0;
+            q.barrier = h->barrier;
+            q.error = h->err;
+            q.gate = h->warm_flag;
+            launch_jacobi_persistent(g, q, h->pj_blocks, s);
+            fin = (iters % 2 == 0) ? h->A : h->B;
+            launch_copy4_unless(h->W0, fin, h->n, h->warm_flag, s);
+        } else {
+            launch_warm_count(h->warm_flag, s);
         }
-        launch_copy4_unless(cur, fc, h->n, h->warm_flag, s);
         if (collision) {
-            F = fc;
+            F = fin;
             if (h->nclothes == 1) nparts = launch_aabb_of(F, h->n, h->aabb, s);
         } else {
-            launch_finalize(fc, h->X, h->V, h->has_ground ? h->ground : nullptr, p, s);
+            launch_finalize(fin, h->X, h->V, h->has_ground ? h->ground : nullptr, p, s);
         }
     } else if (h->cfg.persistent_jacobi) {
         // predict and all iterations in one launch; the result lands in X,V

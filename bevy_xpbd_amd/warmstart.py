@@ -1,0 +1,239 @@
+"""Training and evaluation of the learned warm start, on the GPU.
+
+- dataset: seeded cloths of the two training shapes (32x32 hanging sheets,
+  two corners pinned, and a 256x256 horizontal sheet, centre pinned, at a
+  seeded rotation), log-uniform compliances, simulated by the plain Jacobi
+  solver; at chosen frames each vertex gives one pair (features of the
+  predicted state, (state after `target_iterations` plain iterations -
+  predicted) / spacing), written straight into device buffers;
+- training: Adam on the BF16 MFMA kernels (WarmStartMLP.train);
+- evaluation: from one state, the relative residual after one substep for
+  the plain solver at several iteration counts and for warm start + k
+  iterations.
+
+No reference counterpart (the reference has no learned component).
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import api as A
+from .workloads import SplitMix64
+
+
+def _log_uniform(rng: SplitMix64, lo: float, hi: float) -> float:
+    a, b = math.log10(lo), math.log10(hi)
+    return 10.0 ** (a + (b - a) * rng.unit())
+
+
+def training_descs(seed: int = 1, small: int = 32, large: int = 1):
+    """`small` 32x32 hanging sheets and `large` 256x256 horizontal sheets."""
+    rng = SplitMix64(seed)
+    descs = []
+    for _ in range(small):
+        d = A.SheetDesc.make(cols=32, rows=32, spacing=1.0 / 31.0, layout=A.LAYOUT_HANGING_XY, height=0.0,
+                             pins=A.PIN_TWO_CORNERS if rng.below(2) == 0 else A.PIN_TOP_ROW)
+        d.jitter, d.seed = 1e-3, rng.next()
+        d.stretch_compliance = 0.0 if rng.below(4) == 0 else _log_uniform(rng, 1e-8, 1e-3)
+        d.shear_compliance = _log_uniform(rng, 1e-8, 1e-3)
+        d.bend_compliance = _log_uniform(rng, 1e-8, 1e-3)
+        descs.append(d)
+    for _ in range(large):
+        d = A.SheetDesc.make(cols=256, rows=256, spacing=1.0 / 255.0, layout=A.LAYOUT_HORIZONTAL_XZ, height=1.0,
+                             angle=2.0 * math.pi * rng.unit(), pins=A.PIN_CENTER)
+        d.jitter, d.seed = 1e-3, rng.next()
+        d.stretch_compliance = _log_uniform(rng, 1e-8, 1e-3)
+        d.shear_compliance = _log_uniform(rng, 1e-8, 1e-3)
+        d.bend_compliance = _log_uniform(rng, 1e-8, 1e-3)
+        descs.append(d)
+    return descs
+
+
+def eval_descs(seed: int = 101, count: int = 4):
+    """The evaluation layout: `count` 256x256 sheets in one batch, hanging
+    (two corners or top row) or horizontal (centre pin), seeded compliances."""
+    rng = SplitMix64(seed)
+    descs = []
+    for k in range(count):
+        if k % 2 == 0:
+            d = A.SheetDesc.make(cols=256, rows=256, spacing=1.0 / 255.0, layout=A.LAYOUT_HANGING_XY, height=0.0,
+                                 pins=A.PIN_TWO_CORNERS if rng.below(2) == 0 else A.PIN_TOP_ROW)
+        else:
+            d = A.SheetDesc.make(cols=256, rows=256, spacing=1.0 / 255.0, layout=A.LAYOUT_HORIZONTAL_XZ,
+                                 height=1.0, angle=2.0 * math.pi * rng.unit(), pins=A.PIN_CENTER)
+        d.jitter, d.seed = 1e-3, rng.next()
+        d.stretch_compliance = _log_uniform(rng, 1e-8, 1e-3)
+        d.shear_compliance = _log_uniform(rng, 1e-8, 1e-3)
+        d.bend_compliance = _log_uniform(rng, 1e-8, 1e-3)
+        descs.append(d)
+    return descs
+
+
+def make_sim(mesh, substeps: int = 8, iterations: int = 10, use_graph: int = 0):
+    cfg = A.ClothSimConfig.default(solve_substeps=substeps, solve_inner_iterations=iterations, use_graph=use_graph)
+    u = A.ClothSimUniforms.default(A.SOLVER_JACOBI, substeps, coll_scale=0.0)
+    return A.ClothComputePlugin(mesh, cfg, u)
+
+
+def generate_dataset(seed: int = 1, frames=(3, 12), target_iterations: int = 200, small: int = 32,
+                     large: int = 1, log=None):
+    """Pairs from the plain solver: returns (features DeviceArray rows x 32
+    bf16, targets DeviceArray rows x 3 f32, rows).  Rows are shuffled."""
+    mesh = A.ClothMeshData.batch(training_descs(seed, small, large))
+    sim = make_sim(mesh)
+    n = sim.n
+    rows = n * len(frames)
+    feat = A.DeviceArray((rows, A.MLP_IN), np.uint16)
+    tgt = A.DeviceArray((rows, 3), np.float32)
+    done = 0
+    for k, f in enumerate(frames):
+        sim.step_frames(f - done)
+        done = f
+        fk = A.DeviceArray((n, A.MLP_IN), np.uint16)
+        tk = A.DeviceArray((n, 3), np.float32)
+        sim.warm_sample(target_iterations, fk, tk)
+        feat.upload(fk.numpy(), k * n * A.MLP_IN * 2)
+        tgt.upload(tk.numpy(), k * n * 3 * 4)
+        if log:
+            log(f"warmstart: sampled frame {f}: {n} pairs")
+    sim.close()
+    # shuffle the rows once (the trainer walks consecutive minibatches)
+    perm = np.random.default_rng(seed).permutation(rows)
+    fh, th = feat.numpy()[perm], tgt.numpy()[perm]
+    feat.upload(fh)
+    tgt.upload(th)
+    return feat, tgt, rows, th
+
+
+def train(feat, tgt, rows: int, targets_host=None, seed: int = 1, steps: int = 3000, batch: int = 4096,
+          lr: float = 2e-3, log=None):
+    """A trained WarmStartMLP; the output scale is 1 / std of the targets."""
+    th = targets_host if targets_host is not None else tgt.numpy()
+    std = float(np.std(th))
+    scale = 1.0 / std if std > 0 else 1.0
+    mlp = A.WarmStartMLP(seed=seed)
+    p, _ = mlp.params()
+    mlp.set_params(p, out_scale=scale)
+    t0 = time.perf_counter()
+    curve = mlp.train(feat, tgt, rows, steps=steps, batch=batch, lr=lr)
+    if log:
+        log(f"warmstart: trained {steps} steps x {batch} rows in {time.perf_counter() - t0:.2f}s, "
+            f"loss {curve[0]:.4f} -> {float(np.mean(curve[-50:])):.4f} (scaled units; target std {std:.3e})")
+    return mlp, curve
+
+
+def substep_residuals(mesh, state, iterations_list, mlp=None, warm_iterations=(2,), threshold=float("inf")):
+    """Relative residual after one substep from `state` (pos, vel) for the
+    plain solver at each iteration count and, with `mlp`, for warm start + k
+    iterations.  Returns {"plain": {I: r}, "warm": {k: r}}."""
+    pos, vel = state
+    out = {"plain": {}, "warm": {}}
+    for it in iterations_list:
+        sim = make_sim(mesh, iterations=it)
+        sim.write_state(pos, vel)
+        sim.step_substeps(1, False)
+        out["plain"][it] = sim.residual()
+        sim.close()
+    if mlp is not None:
+        for k in warm_iterations:
+            sim = make_sim(mesh, iterations=10)
+            sim.set_warm_start(mlp, warm_iterations=k, residual_threshold=threshold)
+            sim.write_state(pos, vel)
+            sim.step_substeps(1, False)
+            out["warm"][k] = sim.residual()
+            sim.close()
+    return out
+
+
+def stiff_eval_descs(seed: int = 77, count: int = 16):
+    """Held-out sheets of the first training shape: 32x32 hanging, inextensible
+    stretch (compliance 0), shear 1e-6, bend 1e-4, two corners or top row."""
+    rng = SplitMix64(seed)
+    descs = []
+    for _ in range(count):
+        d = A.SheetDesc.make(cols=32, rows=32, spacing=1.0 / 31.0, layout=A.LAYOUT_HANGING_XY, height=0.0,
+                             pins=A.PIN_TWO_CORNERS if rng.below(2) == 0 else A.PIN_TOP_ROW)
+        d.jitter, d.seed = 1e-3, rng.next()
+        d.stretch_compliance, d.shear_compliance, d.bend_compliance = 0.0, 1e-6, 1e-4
+        descs.append(d)
+    return descs
+
+
+def equivalent_iterations(plain: dict, r: float) -> float:
+    """Plain iteration count reaching residual r, by log-linear interpolation
+    of the measured (iterations, residual) points (clamped to their range)."""
+    pts = sorted(plain.items())
+    if r >= pts[0][1]:
+        return float(pts[0][0])
+    for (i0, r0), (i1, r1) in zip(pts, pts[1:]):
+        if r1 <= r <= r0:
+            if r0 == r1:
+                return float(i0)
+            t = (math.log(r0) - math.log(r)) / (math.log(r0) - math.log(r1))
+            return float(i0 + t * (i1 - i0))
+    return float(pts[-1][0])
+
+
+def substep_time_ms(mesh, iterations: int, mlp=None, warm_iterations: int = 2, threshold: float = float("inf"),
+                    frames: int = 10) -> float:
+    """Mean wall time of one substep (hipGraph replay of whole frames)."""
+    sim = make_sim(mesh, iterations=iterations, use_graph=1)
+    if mlp is not None:
+        sim.set_warm_start(mlp, warm_iterations=warm_iterations, residual_threshold=threshold)
+    sim.step_frames(2)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step_frames(frames)
+    sim.sync()
+    el = time.perf_counter() - t0
+    sim.close()
+    return 1e3 * el / (frames * 8)
+
+
+def evaluate(mlp, descs=None, start_frames: int = 2, plain_iterations=(2, 4, 10, 20, 50, 100),
+             warm_iterations: int = 2, threshold: float = 1e-4):
+    """The gain of the warm start on held-out cloths.
+
+    Single substep from a plain-solver state: residual of plain at each
+    iteration count and of warm start + `warm_iterations`; the plain
+    iteration count of equal residual; the time per substep of both at that
+    equality.  Multi-step: over 10 frames with the residual gate at
+    `threshold`, how many substeps fell back, and the resulting rate."""
+    mesh = A.ClothMeshData.batch(descs if descs is not None else stiff_eval_descs())
+    sim = make_sim(mesh)
+    sim.step_frames(start_frames)
+    state = sim.read_state()
+    sim.close()
+    r = substep_residuals(mesh, state, plain_iterations, mlp, warm_iterations=(warm_iterations,))
+    rw = r["warm"][warm_iterations]
+    eq = equivalent_iterations(r["plain"], rw)
+    i_eq = max(1, int(round(eq)))
+    t_plain = substep_time_ms(mesh, i_eq)
+    t_warm = substep_time_ms(mesh, 10, mlp, warm_iterations, float("inf"))
+    gated = make_sim(mesh, use_graph=1)
+    gated.set_warm_start(mlp, warm_iterations=warm_iterations, residual_threshold=threshold)
+    gated.step_frames(2)
+    gated.sync()
+    gated.warm_stats(reset=True)
+    t0 = time.perf_counter()
+    gated.step_frames(10)
+    gated.sync()
+    tg = time.perf_counter() - t0
+    sub, fb = gated.warm_stats()
+    gated.close()
+    return {
+        "particles": mesh.num_particles,
+        "residual_plain": {str(k): v for k, v in r["plain"].items()},
+        f"residual_warm_plus_{warm_iterations}": rw,
+        "equivalent_plain_iterations": round(eq, 1),
+        "substep_ms_plain_at_equal_residual": round(t_plain, 4),
+        "substep_ms_warm": round(t_warm, 4),
+        "single_substep_speedup_at_equal_residual": round(t_plain / t_warm, 2),
+        "gated_threshold": threshold,
+        "gated_substeps": sub,
+        "gated_fallbacks": fb,
+        "gated_substeps_per_s": round(80 / tg, 1),
+    }
