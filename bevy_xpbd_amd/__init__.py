@@ -26,6 +26,7 @@ from .api import (  # noqa: F401
     derive_collision_grid,
     lib,
     selftest_division,
+    selftest_sqrt,
     KERNEL_JACOBI_ITER,
     KERNEL_COLLISION_SWEEP,
     KERNEL_RADIX_PASS,

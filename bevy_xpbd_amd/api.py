@@ -167,6 +167,7 @@ def lib():
             "xpbd_profile_frames": (i32, [vp, u32, i32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
             "xpbd_debug_collision_keys": (i32, [vp, f32, vp, vp, vp]),
             "xpbd_selftest_division": (i32, [i32, i32, u64, u64, vp]),
+            "xpbd_selftest_sqrt": (i32, [i32, i32, vp]),
             "xpbd_mlp_create": (i32, [u64, pp]),
             "xpbd_mlp_destroy": (None, [vp]),
             "xpbd_mlp_last_error": (C.c_char_p, [vp]),
@@ -559,6 +560,14 @@ class WarmStartMLP:
             self.close()
         except Exception:
             pass
+
+
+def selftest_sqrt(exp_lo: int = -100, exp_hi: int = 127) -> dict:
+    """Device check of the fast square root against sqrtf, exhaustive over
+    the positive normals with exponents in [exp_lo, exp_hi] and 0."""
+    out = np.zeros(2, np.uint32)
+    _check(lib().xpbd_selftest_sqrt(exp_lo, exp_hi, _ptr(out)), "xpbd_selftest_sqrt")
+    return {"sqrt_mismatches": int(out[0]), "first_bad": int(out[1])}
 
 
 def selftest_division(exp_lo: int = -60, exp_hi: int = 60, pairs: int = 1 << 28, seed: int = 1) -> dict:

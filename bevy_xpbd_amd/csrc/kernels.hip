@@ -125,12 +125,32 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
 // ---------------------------------------------------------------------------
 // Correctly rounded division without the general div_scale/div_fixup
 // expansion, valid where no operand or result leaves the normal range.
+// Fast correctly rounded division and square root for "tame" iterates.
+//
 // y = RN(1/b) from the hardware reciprocal and one FMA Newton step; then
 // q = RN(a*y), r = a - b*q (exact with FMA), RN(q + r*y) = RN(a/b)
-// (Markstein's theorem: y the correctly rounded reciprocal, q within one ulp).
-// That the refined y equals RN(1/b) for every significand is checked
-// exhaustively on the device by xpbd_selftest_division.  Off by default: the
-// range guards cost more than the shorter division saves in the gather.
+// (Markstein's theorem: y the correctly rounded reciprocal, q within one ulp,
+// no intermediate under- or overflow).  sqrt_rn: the hardware square root is
+// within one ulp; of its two neighbours, the one whose product with it lies on
+// the other side of x is taken (Tuckerman's rounding test, the residuals
+// x - s_prev*s and x - s_next*s exact with FMA).  Both are checked on the
+// device by xpbd_selftest_division / xpbd_selftest_sqrt (reciprocal and
+// square root exhaustively over the exponents used, the quotient on random
+// operands across the operand ranges below).
+//
+// When they apply: every coordinate of the slices an entry reads is 0 or of
+// magnitude in [2^-50, 2^20] (tracked per slice on the device), and the host
+// has checked inverse masses <= 2^40, compliance / dt^2 <= 2^40 and rest
+// lengths 0 or in [2^-30, 2^30].  Then:
+// - a difference component is 0 or in [2^-73, 2^21];
+// - a squared length below 2^-100 gives len < 2^-50, skipped by the 1e-8
+//   test on either path, so it is taken as 0 (the hardware root is within one
+//   ulp from about 2^-101 up; it is not below); above, the root is exact;
+// - past the 1e-8 test, len is in [1e-8, 2^22]; -C is 0 or in [2^-54, 2^30]
+//   (a multiple of the smaller ulp of len and rest); the weight sum is in
+//   [1e-8, 2^42];
+// - every quotient is 0 or in [2^-96, 2^57] and every Markstein
+//   intermediate normal.
 __device__ __forceinline__ float rcp_rn(float b) {
     const float y0 = __builtin_amdgcn_rcpf(b);
     const float e = __builtin_fmaf(-b, y0, 1.0f);
@@ -142,22 +162,60 @@ __device__ __forceinline__ float div_rn(float a, float b, float y) {
     const float v = __builtin_fmaf(r, y, q);
     return q == 0.f ? q : v;
 }
-// |x| in [2^-60, 2^60] or x == 0: inside the range where div_rn is exact for
-// the quotients this file forms (|a/b| <= 2^60, all intermediates normal).
-__device__ __forceinline__ bool div_safe(float x) {
-    const float ax = fabsf(x);
-    return ax == 0.f || (ax >= 8.67361738e-19f && ax <= 1.15292150e+18f);
+__device__ __forceinline__ float sqrt_rn(float x) {   // x == 0 or normal positive
+    float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
+    s = rd <= 0.f ? sd : s;
+    s = ru > 0.f ? su : s;
+    return s;
+}
+constexpr float TAME_LO = 8.881784197001252e-16f;   // 2^-50
+constexpr float TAME_HI = 1048576.0f;               // 2^20
+constexpr float SQRT_FAST_MIN = 7.888609052210118e-31f;   // 2^-100
+__device__ __forceinline__ bool tame_coord(float c) {
+    const float a = fabsf(c);
+    return a == 0.f || (a >= TAME_LO && a <= TAME_HI);
+}
+__device__ __forceinline__ bool tame3(float4 v) { return tame_coord(v.x) && tame_coord(v.y) && tame_coord(v.z); }
+
+// Slice-tameness byte of the wavefront's 64 outputs (every lane of the wave
+// must reach this; `mine` = false only for a written value out of range).
+__device__ __forceinline__ void write_slice_tame(uint8_t* out, uint32_t slice, bool in_range, bool mine) {
+    const int all = __all(mine ? 1 : 0);
+    if (out && in_range && (threadIdx.x & 63) == 0) out[slice] = all ? 1u : 0u;
+}
+
+// All slices holding neighbours of `slice` tame in the input iterate
+// (wavefront-uniform).  Ranges wider than 128 slices never qualify.
+__device__ __forceinline__ bool neighbourhood_tame(const uint8_t* tame, const uint2* srange, uint32_t slice,
+                                                   uint32_t nslices) {
+    if (!tame || slice >= nslices) return false;
+    const uint2 rg = srange[slice];
+    const uint32_t lane = threadIdx.x & 63;
+    bool ok = rg.x <= rg.y && rg.y - rg.x < 128u;
+    if (ok) {
+        const uint32_t k0 = rg.x + lane, k1 = rg.x + lane + 64u;
+        if (k0 <= rg.y && tame[k0] == 0) ok = false;
+        if (k1 <= rg.y && tame[k1] == 0) ok = false;
+    }
+    return __all(ok ? 1 : 0) != 0;
 }
 
 // One constraint entry of the gather: accumulates (n * w_i) * dlambda in the
-// reference's order of operations (xpbd_cpu.rs:159-182, 206-222).
-template <bool FASTDIV>
+// reference's order of operations (xpbd_cpu.rs:159-182, 206-222).  FAST: the
+// input is tame (see above) and the shorter correctly rounded operations are
+// exact; otherwise the compiler's general ones.
+template <bool FAST>
 __device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float rest, float comp, const DevParams& p,
                                              float& ax, float& ay, float& az) {
     const float wi = pi.w, wj = pj.w;
     if (wj <= 0.f && wi <= 0.f) return;
     float gx = pi.x - pj.x, gy = pi.y - pj.y, gz = pi.z - pj.z;
-    const float len = len3(gx, gy, gz);
+    const float dot = (gx * gx + gy * gy) + gz * gz;
+    const float len = FAST ? (dot < SQRT_FAST_MIN ? 0.f : sqrt_rn(dot)) : sqrtf(dot);
     if (len < 1e-8f) return;
     const float c = len - rest;
     const float alpha_t = comp * p.inv_dt_sq;
@@ -165,8 +223,7 @@ __device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float rest, f
     if (wsum < 1e-8f) return;
     const float num = -c - alpha_t * 0.0f;
     float dlam;
-    if (FASTDIV && div_safe(len) && div_safe(gx) && div_safe(gy) && div_safe(gz) && div_safe(wsum) &&
-        div_safe(num)) {
+    if (FAST) {
         const float yl = rcp_rn(len);
         gx = div_rn(gx, len, yl);
         gy = div_rn(gy, len, yl);
@@ -285,7 +342,13 @@ __global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
     const uint32_t tile = a.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t slice = tile * (blockDim.x >> 6) + (threadIdx.x >> 6);
     bool valid;
-    const float4 out = jacobi_particle<MODE, PACKED, FASTDIV>(a, lpairs, lds, slice, threadIdx.x & 63, valid);
+    // FASTDIV: the fast operations where the slices this wavefront reads are
+    // proven tame (wavefront-uniform)
+    const bool fast = FASTDIV && neighbourhood_tame(a.tame_in, a.srange, slice, a.p.nslices);
+    const float4 out = fast ? jacobi_particle<MODE, PACKED, true>(a, lpairs, lds, slice, threadIdx.x & 63, valid)
+                            : jacobi_particle<MODE, PACKED, false>(a, lpairs, lds, slice, threadIdx.x & 63, valid);
+    if (!(MODE & JM_FINAL) && a.tame_out)
+        write_slice_tame(a.tame_out, slice, slice < a.p.nslices, !valid || tame3(out));
     if (MODE & JM_AABB) {
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         aabb_fold(valid, out.x, out.y, out.z, lo, hi);
@@ -857,10 +920,15 @@ __global__ void k_coll_gather(const float4* F, const uint32_t* sids, uint32_t n,
 }
 
 // Standalone predict (xpbd_cpu.rs:251-264): P = predict(X, V).
-__global__ void k_predict(const float4* X, const float4* V, float4* P, DevParams p) {
+__global__ void k_predict(const float4* X, const float4* V, float4* P, DevParams p, uint8_t* tame_out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
-    P[i] = predict(X[i], V[i], i, p);
+    bool t = true;
+    if (i < p.n) {
+        const float4 q = predict(X[i], V[i], i, p);
+        P[i] = q;
+        t = tame3(q);
+    }
+    if (tame_out) write_slice_tame(tame_out, i >> 6, i < p.n, t);   // blocks are whole slices
 }
 
 __global__ void k_predict_gated(const float4* X, const float4* V, float4* P, DevParams p, const uint32_t* gate) {
@@ -1080,10 +1148,10 @@ __global__ __launch_bounds__(256) void k_jacobi_persistent(JacobiArgs a, Persist
             if (last && q.final_write) {
                 b.dst = q.X;
                 b.vout = q.V;
-                jacobi_particle<JM_FINAL, PACKED, FASTDIV>(b, lpairs, lds, slice, threadIdx.x & 63, valid);
+                jacobi_particle<JM_FINAL, PACKED, false>(b, lpairs, lds, slice, threadIdx.x & 63, valid);
             } else {
                 b.dst = nxt;
-                jacobi_particle<0, PACKED, FASTDIV>(b, lpairs, lds, slice, threadIdx.x & 63, valid);
+                jacobi_particle<0, PACKED, false>(b, lpairs, lds, slice, threadIdx.x & 63, valid);
             }
         }
         if (!last) gsync.sync();
@@ -1210,9 +1278,11 @@ __global__ void k_selftest_div(uint64_t n, uint64_t seed, uint32_t* bad, uint32_
     for (uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += uint64_t(gridDim.x) * blockDim.x) {
         const uint32_t ua = mix32(seed * 0x9E3779B97F4A7C15ull + 2 * k);
         const uint32_t ub = mix32(seed * 0x9E3779B97F4A7C15ull + 2 * k + 1);
-        // exponents spread over [-40, 40] (safe range), random significands and signs
-        const float a = __uint_as_float((ua & 0x807FFFFFu) | (uint32_t(87 + (ua >> 24) % 81) << 23));
-        const float b = __uint_as_float((ub & 0x807FFFFFu) | (uint32_t(87 + (ub >> 24) % 81) << 23));
+        // numerator exponents [-74, 32], denominator exponents [-28, 43] (the
+        // operand ranges of the gather's divisions on tame iterates), random
+        // significands and signs
+        const float a = __uint_as_float((ua & 0x807FFFFFu) | (uint32_t(53 + (ua >> 24) % 107) << 23));
+        const float b = __uint_as_float((ub & 0x807FFFFFu) | (uint32_t(99 + (ub >> 24) % 72) << 23));
         const float q = div_rn(a, b, rcp_rn(b));
         const float ref = a / b;
         if (__float_as_uint(q) != __float_as_uint(ref)) {
@@ -1224,6 +1294,24 @@ __global__ void k_selftest_div(uint64_t n, uint64_t seed, uint32_t* bad, uint32_
     }
 }
 
+__global__ void k_selftest_sqrt(int exp_lo, int exp_hi, uint32_t* bad, uint32_t* first) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (1u << 23)) return;
+    for (int e = exp_lo; e <= exp_hi; ++e) {
+        const float x = __uint_as_float((uint32_t(e + 127) << 23) | m);
+        const float y = sqrt_rn(x);
+        const float ref = sqrtf(x);
+        if (__float_as_uint(y) != __float_as_uint(ref)) {
+            if (atomicAdd(bad, 1u) == 0) *first = __float_as_uint(x);
+        }
+    }
+    if (m == 0 && __float_as_uint(sqrt_rn(0.0f)) != 0u) atomicAdd(bad, 1u);
+}
+
+void launch_selftest_sqrt(int exp_lo, int exp_hi, uint32_t* bad, uint32_t* first, hipStream_t s) {
+    hipLaunchKernelGGL(k_selftest_sqrt, dim3((1u << 23) / 256), dim3(256), 0, s, exp_lo, exp_hi, bad, first);
+}
+
 void launch_selftest_rcp(int exp_lo, int exp_hi, uint32_t* bad, uint32_t* first, hipStream_t s) {
     hipLaunchKernelGGL(k_selftest_rcp, dim3((1u << 23) / 256), dim3(256), 0, s, exp_lo, exp_hi, bad, first);
 }
@@ -1232,9 +1320,20 @@ void launch_selftest_div(uint64_t n, uint64_t seed, uint32_t* bad, uint32_t* fir
     hipLaunchKernelGGL(k_selftest_div, dim3(8192), dim3(256), 0, s, n, seed, bad, first);
 }
 
-void launch_predict(const float4* X, const float4* V, float4* P, DevParams p, hipStream_t s) {
+void launch_predict(const float4* X, const float4* V, float4* P, DevParams p, hipStream_t s, uint8_t* tame_out) {
     if (p.n == 0) return;
-    hipLaunchKernelGGL(k_predict, dim3(cdiv(p.n, 256)), dim3(256), 0, s, X, V, P, p);
+    hipLaunchKernelGGL(k_predict, dim3(cdiv(p.n, 256)), dim3(256), 0, s, X, V, P, p, tame_out);
+}
+
+__global__ void k_tame_scan(const float4* P, uint32_t n, uint8_t* slice_tame) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool t = i >= n || tame3(P[i]);
+    write_slice_tame(slice_tame, i >> 6, i < n, t);
+}
+
+void launch_tame_scan(const float4* P, uint32_t n, uint8_t* slice_tame, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_tame_scan, dim3(cdiv(n, 256)), dim3(256), 0, s, P, n, slice_tame);
 }
 
 void launch_predict_gated(const float4* X, const float4* V, float4* P, DevParams p, const uint32_t* gate,

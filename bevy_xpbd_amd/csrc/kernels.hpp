@@ -58,6 +58,14 @@ struct JacobiArgs {
     uint32_t fastdiv;    // 1: shared-reciprocal correctly rounded divisions
     uint32_t num_pairs;  // packed: size of sell.pairs
     const uint32_t* gate;   // non-null: the launch does nothing unless *gate != 0
+    // fast-division proof (FASTDIV kernels), per slice of 64 particles:
+    // tame_in[s] != 0 when every coordinate of slice s of the input iterate is
+    // 0 or of magnitude in [2^-50, 2^20]; srange[s] = first and last slice
+    // holding a neighbour of slice s (x > y: too wide, never fast).  The launch
+    // writes tame_out[s] for the slices it produces.
+    const uint8_t* tame_in;
+    uint8_t* tame_out;
+    const uint2* srange;
 };
 
 // Persistent Jacobi substep: predict X,V -> A, iterate A/B; the last iterate
@@ -126,7 +134,8 @@ struct GsArgs {
 // ---- launchers (kernels.hip) ----
 uint32_t jacobi_blocks(uint32_t nslices, uint32_t block);
 uint32_t launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s);
-void launch_predict(const float4* X, const float4* V, float4* P, DevParams p, hipStream_t s);
+void launch_predict(const float4* X, const float4* V, float4* P, DevParams p, hipStream_t s,
+                    uint8_t* tame_out = nullptr);
 void launch_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p,
                      hipStream_t s);
 void launch_coll_grid(const float* part, uint32_t nparts, CollGrid* grid, float cell_size, hipStream_t s);
@@ -145,6 +154,8 @@ void launch_coll_sweep(const CollArgs& a, hipStream_t s);
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
 uint32_t gs_max_blocks(uint32_t block);
 void launch_copy4(const float4* a, float4* b, uint32_t n, hipStream_t s);
+// per-slice tame bytes of P (1: every coordinate 0 or of magnitude in [2^-50, 2^20])
+void launch_tame_scan(const float4* P, uint32_t n, uint8_t* slice_tame, hipStream_t s);
 // gated variants for the warm-start fallback: run only if *gate != 0
 void launch_predict_gated(const float4* X, const float4* V, float4* P, DevParams p, const uint32_t* gate,
                           hipStream_t s);
@@ -164,5 +175,6 @@ void launch_residual(const float4* X, const uint2* ij, const float2* rc, uint32_
 uint32_t launch_aabb_of(const float4* P, uint32_t n, float* part, hipStream_t s);
 void launch_selftest_rcp(int exp_lo, int exp_hi, uint32_t* bad, uint32_t* first, hipStream_t s);
 void launch_selftest_div(uint64_t n, uint64_t seed, uint32_t* bad, uint32_t* first, hipStream_t s);
+void launch_selftest_sqrt(int exp_lo, int exp_hi, uint32_t* bad, uint32_t* first, hipStream_t s);
 
 }  // namespace xpbd

@@ -108,6 +108,13 @@ struct xpbd_cloth {
     uint32_t gs_block = 256, gs_blocks = 0;
     uint32_t pj_blocks = 0;        // persistent Jacobi grid (co-resident blocks)
 
+    // fast correctly rounded division / sqrt in the Jacobi gather: the static
+    // inputs' ranges (host-checked) and per-buffer "tame" flags of A and B
+    float max_inv_mass = 0.f, max_comp = 0.f;
+    bool rest_tame = true;
+    uint8_t *tameA = nullptr, *tameB = nullptr;   // per slice: 1 = coordinates in range
+    uint2* srange = nullptr;                      // per slice: first / last neighbour slice
+
     // learned warm start (grid meshes): 8 grid neighbours and per-vertex
     // feature constants, built at create when the mesh carries cloth columns
     bool warm_ok = false;
@@ -193,6 +200,23 @@ DevParams dev_params(const xpbd_cloth* h) {
     return p;
 }
 
+// The shorter correctly rounded operations are exact for tame iterates when
+// inverse masses <= 2^40, compliance / dt^2 <= 2^40 and every rest length is
+// 0 or in [2^-30, 2^30] (the ranges the kernels' proof assumes).
+bool fast_division_active(const xpbd_cloth* h) {
+    if (!h->cfg.fast_division) return false;
+    const float two40 = 1099511627776.0f;
+    const float inv_dt_sq = h->u.inv_dt * h->u.inv_dt;
+    return h->rest_tame && std::isfinite(h->max_inv_mass) && h->max_inv_mass <= two40 &&
+           std::isfinite(inv_dt_sq) && double(h->max_comp) * double(inv_dt_sq) <= double(two40);
+}
+
+uint8_t* tame_slot(xpbd_cloth* h, const float4* buf) {
+    if (buf == h->A) return h->tameA;
+    if (buf == h->B) return h->tameB;
+    return nullptr;
+}
+
 JacobiArgs jacobi_args(const xpbd_cloth* h, const DevParams& p) {
     JacobiArgs a{};
     a.sell = Sell{h->soff, h->swid, h->nbj, h->nbr, h->nbc, h->nbpk, h->pairs};
@@ -200,8 +224,9 @@ JacobiArgs jacobi_args(const xpbd_cloth* h, const DevParams& p) {
     a.aabb = h->aabb;
     a.p = p;
     a.remap = h->remap;
-    a.fastdiv = h->cfg.fast_division ? 1u : 0u;
+    a.fastdiv = fast_division_active(h) ? 1u : 0u;
     a.num_pairs = h->num_pairs;
+    a.srange = h->srange;
     return a;
 }
 
@@ -379,13 +404,17 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         // predict -> A, then iterations ping-pong A/B; the last one writes the
         // state in place (no collision) or the iterate F the collision reads.
         JacobiArgs a = jacobi_args(h, p);
-        launch_predict(h->X, h->V, h->A, p, s);
+        const bool fast = a.fastdiv != 0;
+        launch_predict(h->X, h->V, h->A, p, s, fast ? tame_slot(h, h->A) : nullptr);
         float4* cur = h->A;
         float4* nxt = h->B;
         for (uint32_t k = 0; k < iters; ++k) {
             a.src = cur;
+            a.tame_in = fast ? tame_slot(h, cur) : nullptr;
+            a.tame_out = nullptr;
             if (k + 1 < iters) {
                 a.dst = nxt;
+                a.tame_out = fast ? tame_slot(h, nxt) : nullptr;
                 ProbeScope ps(h, XPBD_KERNEL_JACOBI_ITER);
                 launch_jacobi(0, a, h->jacobi_block, s);
                 std::swap(cur, nxt);
@@ -601,6 +630,19 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(upload(h, h->nbj, sj.data(), sj.size()));
     TRY(upload(h, h->nbr, sr.data(), sr.size()));
     TRY(upload(h, h->nbc, sc.data(), sc.size()));
+    {   // neighbour slice range of every slice (its own slice included)
+        std::vector<uint2> rg(swid.size());
+        for (size_t sl = 0; sl < swid.size(); ++sl) {
+            uint32_t lo = uint32_t(sl), hi = uint32_t(sl);
+            for (uint32_t e = soff[sl]; e < soff[sl + 1]; ++e) {
+                lo = std::min(lo, sj[e] / 64);
+                hi = std::max(hi, sj[e] / 64);
+            }
+            rg[sl] = make_uint2(lo, hi);
+        }
+        TRY(dalloc(h, &h->srange, rg.size()));
+        TRY(upload(h, h->srange, rg.data(), rg.size()));
+    }
     if (!h->cfg.plain_adjacency) {
         std::vector<uint32_t> pk;
         std::vector<float2> pairs;
@@ -686,6 +728,20 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
 
     TRY(dalloc(h, &h->barrier, GRID_SYNC_WORDS));
     TRY(dalloc(h, &h->err, 4));
+    TRY(dalloc(h, &h->tameA, h->nslices));
+    TRY(dalloc(h, &h->tameB, h->nslices));
+    HIPC(h, hipMemset(h->tameA, 0, h->nslices));
+    HIPC(h, hipMemset(h->tameB, 0, h->nslices));
+    // static ranges of the fast-division proof
+    h->max_inv_mass = 0.f;
+    for (float w : m.inv_mass) h->max_inv_mass = std::isfinite(w) ? std::max(h->max_inv_mass, w) : INFINITY;
+    h->max_comp = 0.f;
+    h->rest_tame = true;
+    for (size_t k = 0; k < m.constraint_comp.size(); ++k) {
+        const float c = m.constraint_comp[k], r = m.constraint_rest[k];
+        h->max_comp = std::isfinite(c) ? std::max(h->max_comp, std::fabs(c)) : INFINITY;
+        if (!(r == 0.f || (r >= 9.313225746154785e-10f && r <= 1073741824.0f))) h->rest_tame = false;   // [2^-30, 2^30]
+    }
     TRY(dalloc(h, &h->resid, 2));
     HIPC(h, hipMemset(h->err, 0, 16));
     HIPC(h, hipMemset(h->barrier, 0, GRID_SYNC_WORDS * sizeof(uint32_t)));
@@ -801,11 +857,20 @@ int autotune(xpbd_cloth* h) {
             a.remap = remap;
             a.src = h->A;
             a.dst = h->B;
+            if (a.fastdiv) {   // the production path: slice flags of both iterates
+                launch_tame_scan(h->A, h->n, tame_slot(h, h->A), h->stream);
+                a.tame_in = tame_slot(h, h->A);
+                a.tame_out = tame_slot(h, h->B);
+            }
             launch_jacobi(0, a, block, h->stream);   // warm-up
             (void)hipEventRecord(e0, h->stream);
             for (uint32_t r = 0; r < reps; ++r) {
                 a.src = a.dst;
                 a.dst = (a.dst == h->A) ? h->B : h->A;
+                if (a.fastdiv) {
+                    a.tame_in = tame_slot(h, a.src);
+                    a.tame_out = tame_slot(h, a.dst);
+                }
                 launch_jacobi(0, a, block, h->stream);
             }
             (void)hipEventRecord(e1, h->stream);
@@ -964,6 +1029,7 @@ XPBD_API void xpbd_sim_config_default(xpbd_sim_config* out) {
     c.autotune = 0;
     c.jacobi_block = 0;
     c.fused_predict = 0;
+    c.fast_division = 1;
     *out = c;
 }
 
@@ -1160,6 +1226,12 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
     JacobiArgs a = jacobi_args(h, p);
     a.src = h->A;
     a.dst = h->B;
+    if (a.fastdiv) {   // as in a substep: slice flags of the input and output iterates
+        launch_tame_scan(h->A, h->n, tame_slot(h, h->A), s);
+        launch_tame_scan(h->B, h->n, tame_slot(h, h->B), s);
+        a.tame_in = tame_slot(h, h->A);
+        a.tame_out = tame_slot(h, h->B);
+    }
     auto prep_collision = [&]() {
         const uint32_t np = h->nclothes == 1 ? launch_aabb_of(h->A, h->n, h->aabb, s) : 0u;
         enqueue_grids(h, h->A, np, coll_cell(h), s);
@@ -1211,6 +1283,10 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
                 launch_jacobi(0, a, h->jacobi_block, s);
                 a.src = a.dst;
                 a.dst = (a.dst == h->A) ? h->B : h->A;
+                if (a.fastdiv) {
+                    a.tame_in = tame_slot(h, a.src);
+                    a.tame_out = tame_slot(h, a.dst);
+                }
                 break;
             case XPBD_KERNEL_COLLISION_SWEEP: launch_coll_sweep(c, s); break;
             case XPBD_KERNEL_RADIX_PASS:
@@ -1596,6 +1672,31 @@ XPBD_API int xpbd_selftest_division(int exp_lo, int exp_hi, uint64_t pairs, uint
             hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
             rc = XPBD_E_HIP;
         std::memcpy(out5, h, 5 * sizeof(uint32_t));
+    }
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(d);
+    return rc;
+}
+
+XPBD_API int xpbd_selftest_sqrt(int exp_lo, int exp_hi, uint32_t* out2) {
+    if (!out2 || exp_lo > exp_hi || exp_lo < -126 || exp_hi > 127) return XPBD_E_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return XPBD_E_NO_DEVICE;
+    uint32_t* d = nullptr;
+    if (hipMalloc(&d, 4 * sizeof(uint32_t)) != hipSuccess) return XPBD_E_NOMEM;
+    hipStream_t s = nullptr;
+    int rc = XPBD_OK;
+    if (hipMemset(d, 0, 4 * sizeof(uint32_t)) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        rc = XPBD_E_HIP;
+    } else {
+        launch_selftest_sqrt(exp_lo, exp_hi, d, d + 1, s);
+        uint32_t h[4] = {0};
+        if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess ||
+            hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = XPBD_E_HIP;
+        out2[0] = h[0];
+        out2[1] = h[1];
     }
     if (s) (void)hipStreamDestroy(s);
     (void)hipFree(d);

@@ -136,10 +136,14 @@ typedef struct xpbd_sim_config {
     uint32_t fused_predict;               /* 1: predict inside the first iteration */
     uint32_t plain_adjacency;             /* 1: 12-byte adjacency entries even when the
                                              4-byte packed form fits (read at create) */
-    uint32_t fast_division;               /* 1: shared-reciprocal division (Markstein
-                                             step, range-guarded) in the Jacobi gather
-                                             instead of the compiler's general f32
-                                             division; both correctly rounded */
+    uint32_t fast_division;               /* 1 (default): in the Jacobi gather, the
+                                             shorter correctly rounded division
+                                             (Markstein step) and square root
+                                             (Tuckerman test) wherever their range
+                                             conditions are proven (static inputs on
+                                             the host, iterate coordinates on the
+                                             device); the compiler's general
+                                             operations elsewhere.  Same results. */
     uint32_t xcd_remap;                   /* Jacobi block order: 0 = default/autotuned,
                                              1 = hardware order, 2 = XCD-contiguous tiles */
     uint32_t persistent_jacobi;           /* 1: predict + all I Jacobi iterations of a
@@ -249,6 +253,10 @@ XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t*
  * out[1] = the first such b (bits); out[2] = mismatches of the quotient
  * against '/' over `pairs` pseudo-random pairs, out[3..4] = first (a, b). */
 XPBD_API int xpbd_selftest_division(int exp_lo, int exp_hi, uint64_t pairs, uint64_t seed, uint32_t* out5);
+/* Device self-test of the fast square root: every positive normal float with
+ * exponent in [exp_lo, exp_hi] (and 0) against the correctly rounded sqrtf.
+ * out2 = {mismatches, first mismatching input (bits)}. */
+XPBD_API int xpbd_selftest_sqrt(int exp_lo, int exp_hi, uint32_t* out2);
 
 /* ---------------------------------------------------------------------------
  * Learned warm start (no reference counterpart).  A per-vertex MLP
