@@ -780,6 +780,12 @@ __device__ __forceinline__ float dist2_of(float4 a, float4 b) {
     return (dx * dx + dy * dy) + dz * dz;
 }
 
+__device__ __forceinline__ float clamp_apply(float v) {
+    if (v < -COLLISION_APPLY_CLAMP) v = -COLLISION_APPLY_CLAMP;
+    if (v > COLLISION_APPLY_CLAMP) v = COLLISION_APPLY_CLAMP;
+    return v;
+}
+
 __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     const uint32_t n = a.p.n;
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -860,63 +866,129 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
             if (idx >= n) done = true;
         }
     }
-    float ax = 0.f, ay = 0.f, az = 0.f;
-    if (count <= CMAX) {
-#pragma unroll
-        for (int q = 0; q < CMAX; ++q)
-            if (q < count) {
-                ax = ax + lx[q];
-                ay = ay + ly[q];
-                az = az + lz[q];
-            }
-    } else {
-        uint32_t last = 0;
-        bool has_last = false;
-        for (;;) {
-            uint32_t best = EMPTY_KEY;
-            float bx = 0.f, by = 0.f, bz = 0.f;
-#pragma unroll
-            for (int r = 0; r < 9; ++r) {
-                if (rows[r].start == EMPTY_KEY) continue;
-                for (uint32_t idx = rows[r].start; idx < n; ++idx) {
-                    const uint32_t key = a.skeys[idx];
-                    if (key > rows[r].kmax) break;
-                    if (key < rows[r].kmin || idx == s) continue;
-                    const float4 fj = a.sP[idx];
-                    const float d2 = dist2_of(fi, fj);
-                    if (d2 > th2 || d2 < 1e-18f) continue;
-                    const uint32_t j = a.sids[idx];
-                    if (j >= best || (has_last && j <= last)) continue;
-                    float cx, cy, cz;
-                    if (collision_contrib(i, j, fi, fj, a.rest4, a.p, cx, cy, cz)) {
-                        best = j;
-                        bx = cx;
-                        by = cy;
-                        bz = cz;
-                    }
-                }
-            }
-            if (best == EMPTY_KEY) break;
-            ax = ax + bx;
-            ay = ay + by;
-            az = az + bz;
-            last = best;
-            has_last = true;
-        }
+    if (count > CMAX) {   // resolved by k_coll_sweep_overflow
+        const uint32_t k = atomicAdd(&a.ovf[0], 1u);
+        if (k < n) a.ovf[2 + k] = s;   // at most n entries per launch
+        return;
     }
-    auto cl = [](float v) {
-        if (v < -COLLISION_APPLY_CLAMP) v = -COLLISION_APPLY_CLAMP;
-        if (v > COLLISION_APPLY_CLAMP) v = COLLISION_APPLY_CLAMP;
-        return v;
-    };
-    const float4 nx = make_float4(fi.x + cl(ax), fi.y + cl(ay), fi.z + cl(az), fi.w);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll
+    for (int q = 0; q < CMAX; ++q)
+        if (q < count) {
+            ax = ax + lx[q];
+            ay = ay + ly[q];
+            az = az + lz[q];
+        }
+    const float4 nx = make_float4(fi.x + clamp_apply(ax), fi.y + clamp_apply(ay), fi.z + clamp_apply(az), fi.w);
     finish_particle(nx, a.prev[i], i, a.ground, a.p, a.xout, a.vout);
 }
 
-// Positions permuted into sorted (cell) order: sP[s] = F[sids[s]].
-__global__ void k_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float4* sP) {
+// Particles with more than CMAX contacts.  One pass over the 27 cells
+// collects up to OVF_MAX contacts into a list kept sorted by partner id in
+// LDS (one list per thread); more than that falls back to repeated selection
+// of the next-smallest partner.  Either way the sum runs in ascending partner
+// id, the brute-force loop's order.  The list's count is cleared by
+// k_coll_gather, which runs before every sweep.
+constexpr int OVF_MAX = 32;
+constexpr uint32_t OVF_BLOCK = 64;
+
+__global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
+    __shared__ uint32_t lj[OVF_MAX][OVF_BLOCK];
+    __shared__ float4 lc[OVF_MAX][OVF_BLOCK];
+    const uint32_t n = a.p.n;
+    const uint32_t total = min(__hip_atomic_load(&a.ovf[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), n);
+    const float th2 = a.p.thickness * a.p.thickness;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t k = blockIdx.x * blockDim.x + t; k < total; k += gridDim.x * blockDim.x) {
+        const uint32_t s = a.ovf[2 + k];
+        const float4 fi = a.sP[s];
+        const uint32_t i = a.sids[s];
+        const CollGrid g = a.grids[a.cloth_of ? a.cloth_of[i] : 0u];
+        uint32_t c[3];
+        cell_of(fi, g, c);
+        RowRange rows[9];
+        sweep_rows(a, g, c, rows);
+        int cnt = 0;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            if (rows[r].start == EMPTY_KEY) continue;
+            for (uint32_t idx = rows[r].start; idx < n; ++idx) {
+                const uint32_t key = a.skeys[idx];
+                if (key > rows[r].kmax) break;
+                if (key < rows[r].kmin || idx == s) continue;
+                const float4 fj = a.sP[idx];
+                const float d2 = dist2_of(fi, fj);
+                if (d2 > th2 || d2 < 1e-18f) continue;
+                const uint32_t j = a.sids[idx];
+                float cx, cy, cz;
+                if (!collision_contrib(i, j, fi, fj, a.rest4, a.p, cx, cy, cz)) continue;
+                if (cnt < OVF_MAX) {   // insertion into the sorted list
+                    int w = cnt;
+                    while (w > 0 && lj[w - 1][t] > j) {
+                        lj[w][t] = lj[w - 1][t];
+                        lc[w][t] = lc[w - 1][t];
+                        --w;
+                    }
+                    lj[w][t] = j;
+                    lc[w][t] = make_float4(cx, cy, cz, 0.f);
+                }
+                ++cnt;
+            }
+        }
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        if (cnt <= OVF_MAX) {
+            for (int q = 0; q < cnt; ++q) {
+                const float4 v = lc[q][t];
+                ax = ax + v.x;
+                ay = ay + v.y;
+                az = az + v.z;
+            }
+        } else {
+            uint32_t last = 0;
+            bool has_last = false;
+            for (;;) {
+                uint32_t best = EMPTY_KEY;
+                float bx = 0.f, by = 0.f, bz = 0.f;
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    if (rows[r].start == EMPTY_KEY) continue;
+                    for (uint32_t idx = rows[r].start; idx < n; ++idx) {
+                        const uint32_t key = a.skeys[idx];
+                        if (key > rows[r].kmax) break;
+                        if (key < rows[r].kmin || idx == s) continue;
+                        const float4 fj = a.sP[idx];
+                        const float d2 = dist2_of(fi, fj);
+                        if (d2 > th2 || d2 < 1e-18f) continue;
+                        const uint32_t j = a.sids[idx];
+                        if (j >= best || (has_last && j <= last)) continue;
+                        float cx, cy, cz;
+                        if (collision_contrib(i, j, fi, fj, a.rest4, a.p, cx, cy, cz)) {
+                            best = j;
+                            bx = cx;
+                            by = cy;
+                            bz = cz;
+                        }
+                    }
+                }
+                if (best == EMPTY_KEY) break;
+                ax = ax + bx;
+                ay = ay + by;
+                az = az + bz;
+                last = best;
+                has_last = true;
+            }
+        }
+        const float4 nx = make_float4(fi.x + clamp_apply(ax), fi.y + clamp_apply(ay), fi.z + clamp_apply(az), fi.w);
+        finish_particle(nx, a.prev[i], i, a.ground, a.p, a.xout, a.vout);
+    }
+}
+
+// Positions permuted into sorted (cell) order: sP[s] = F[sids[s]].  Also
+// clears the count of the sweep's overflow list (ovf[0]).
+__global__ void k_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float4* sP, uint32_t* ovf) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n) sP[s] = F[sids[s]];
+    if (ovf && s == 0) ovf[0] = 0u;
 }
 
 // Standalone predict (xpbd_cpu.rs:251-264): P = predict(X, V).
@@ -1425,14 +1497,16 @@ void launch_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t
     hipLaunchKernelGGL(k_coll_table, dim3(cdiv(n, 256)), dim3(256), 0, s, skeys, n, table, mask);
 }
 
-void launch_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float4* sP, hipStream_t s) {
+void launch_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float4* sP, uint32_t* ovf,
+                        hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_coll_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, F, sids, n, sP);
+    hipLaunchKernelGGL(k_coll_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, F, sids, n, sP, ovf);
 }
 
 void launch_coll_sweep(const CollArgs& a, hipStream_t s) {
     if (a.p.n == 0) return;
     hipLaunchKernelGGL(k_coll_sweep, dim3(cdiv(a.p.n, 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_coll_sweep_overflow, dim3(std::min(cdiv(a.p.n, OVF_BLOCK), 2048u)), dim3(OVF_BLOCK), 0, s, a);
 }
 
 uint32_t persistent_max_blocks(bool packed, bool fastdiv) {

@@ -106,6 +106,9 @@ struct CollArgs {
     const uint2* table;
     uint32_t table_mask;
     DevParams p;
+    // particles with more than CMAX contacts, resolved by a second kernel:
+    // [0] count (cleared by launch_coll_gather), [2..] sorted indices; n + 2 words
+    uint32_t* ovf;
 };
 
 // Words of the grid-barrier state of a persistent launch (zeroed before each
@@ -149,7 +152,9 @@ uint32_t radix_hist_words(uint32_t n);
 void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
                        uint32_t shift, uint32_t* hist, hipStream_t s);
 void launch_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask, hipStream_t s);
-void launch_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float4* sP, hipStream_t s);
+// sP = F in sorted order; clears the sweep's overflow count ovf[0]
+void launch_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float4* sP, uint32_t* ovf,
+                        hipStream_t s);
 void launch_coll_sweep(const CollArgs& a, hipStream_t s);
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
 uint32_t gs_max_blocks(uint32_t block);

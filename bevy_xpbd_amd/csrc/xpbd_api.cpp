@@ -88,6 +88,7 @@ struct xpbd_cloth {
     uint32_t *keys = nullptr, *ids = nullptr, *keys2 = nullptr, *ids2 = nullptr, *hist = nullptr;
     uint2* table = nullptr;
     uint32_t table_slots = 0;
+    uint32_t* ovf = nullptr;       // collision overflow list (count, done, n sorted indices)
     float* aabb = nullptr;         // bounds partials, 8 floats per producing block / cloth
     uint32_t aabb_slots = 0;
     CollGrid* grids = nullptr;     // one per cloth
@@ -266,7 +267,7 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->table_slots, s);
     launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
     launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, s);
-    launch_coll_gather(F, h->ids, h->n, h->sP, s);
+    launch_coll_gather(F, h->ids, h->n, h->sP, h->ovf, s);
     CollArgs c{};
     c.F = F;
     c.sP = h->sP;
@@ -282,6 +283,7 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     c.table = h->table;
     c.table_mask = h->table_slots - 1;
     c.p = p;
+    c.ovf = h->ovf;
     ProbeScope ps(h, XPBD_KERNEL_COLLISION_SWEEP);
     launch_coll_sweep(c, s);
 }
@@ -689,6 +691,8 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     // sheet puts nearly every particle in its own group), load <= 1/2
     h->table_slots = next_pow2(uint32_t(std::max<size_t>(2 * n, 64)));
     TRY(dalloc(h, &h->table, h->table_slots));
+    TRY(dalloc(h, &h->ovf, size_t(n) + 2));
+    HIPC(h, hipMemset(h->ovf, 0, (size_t(n) + 2) * sizeof(uint32_t)));
     h->aabb_slots = std::max<uint32_t>({h->nslices, (h->n + 255) / 256, 4096u, uint32_t(m.cloth_offsets.size()), 1u});
     TRY(dalloc(h, &h->aabb, size_t(h->aabb_slots) * 8));
     {
@@ -1238,7 +1242,7 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         launch_coll_keys(h->A, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->table_slots, s);
         launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
         launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, s);
-        launch_coll_gather(h->A, h->ids, h->n, h->sP, s);
+        launch_coll_gather(h->A, h->ids, h->n, h->sP, h->ovf, s);
     };
     CollArgs c{};
     if (which == XPBD_KERNEL_COLLISION_SWEEP || which == XPBD_KERNEL_RADIX_PASS) {
@@ -1257,6 +1261,7 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         c.table = h->table;
         c.table_mask = h->table_slots - 1;
         c.p = p;
+        c.ovf = h->ovf;
     }
     GsArgs g{};
     if (which == XPBD_KERNEL_GS_SUBSTEP) {
@@ -1288,7 +1293,10 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
                     a.tame_out = tame_slot(h, a.dst);
                 }
                 break;
-            case XPBD_KERNEL_COLLISION_SWEEP: launch_coll_sweep(c, s); break;
+            case XPBD_KERNEL_COLLISION_SWEEP:
+                launch_zero_u32(h->ovf, 1, s);   // the gather clears it before a substep's sweep
+                launch_coll_sweep(c, s);
+                break;
             case XPBD_KERNEL_RADIX_PASS:
                 launch_radix_pass(h->keys, h->ids, h->keys2, h->ids2, h->n, 0, h->hist, s);
                 break;
