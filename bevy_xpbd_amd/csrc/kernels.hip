@@ -695,6 +695,21 @@ __device__ __forceinline__ uint32_t table_find(const uint2* table, uint32_t mask
     return EMPTY_KEY;
 }
 
+// Lookup whose first probe `e` (the entry at slot_of(g)) was loaded by the
+// caller: most lookups end there; a collision continues at the next slot.
+__device__ __forceinline__ uint32_t table_resolve(const uint2* table, uint32_t mask, uint32_t g, uint2 e) {
+    if (e.x == g) return e.y;
+    if (e.x == EMPTY_KEY) return EMPTY_KEY;
+    uint32_t h = (slot_of(g, mask) + 1) & mask;
+    for (uint32_t probe = 1; probe <= mask; ++probe) {
+        const uint2 f = table[h];
+        if (f.x == g) return f.y;
+        if (f.x == EMPTY_KEY) return EMPTY_KEY;
+        h = (h + 1) & mask;
+    }
+    return EMPTY_KEY;
+}
+
 // Pair kernel of self_collision_resolve (xpbd_cpu.rs:84-121) for the pair
 // (min(i,j), max(i,j)); returns the contribution to particle i.  The rest
 // positions are read only for pairs inside the contact distance.
@@ -763,11 +778,24 @@ __device__ __forceinline__ void sweep_rows(const CollArgs& a, const CollGrid& g,
         g0[r] = rows[r].kmin >> GROUP_SHIFT;
         g1[r] = rows[r].kmax >> GROUP_SHIFT;
     }
+    // first probes of all rows issued together (independent loads), then
+    // resolved; the same for the rows whose first group is empty
+    // (unconditional loads, slot 0 for absent rows, so no branch splits them)
+    uint2 e[9];
 #pragma unroll
-    for (int r = 0; r < 9; ++r) st[r] = g0[r] != EMPTY_KEY ? table_find(a.table, a.table_mask, g0[r]) : EMPTY_KEY;
+    for (int r = 0; r < 9; ++r) e[r] = a.table[g0[r] != EMPTY_KEY ? slot_of(g0[r], a.table_mask) : 0u];
 #pragma unroll
     for (int r = 0; r < 9; ++r)
-        if (st[r] == EMPTY_KEY && g1[r] != g0[r] && g1[r] != EMPTY_KEY) st[r] = table_find(a.table, a.table_mask, g1[r]);
+        st[r] = g0[r] != EMPTY_KEY ? table_resolve(a.table, a.table_mask, g0[r], e[r]) : EMPTY_KEY;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const bool second = st[r] == EMPTY_KEY && g1[r] != g0[r] && g1[r] != EMPTY_KEY;
+        g1[r] = second ? g1[r] : EMPTY_KEY;
+        e[r] = a.table[second ? slot_of(g1[r], a.table_mask) : 0u];
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+        if (g1[r] != EMPTY_KEY) st[r] = table_resolve(a.table, a.table_mask, g1[r], e[r]);
 #pragma unroll
     for (int r = 0; r < 9; ++r) rows[r].start = st[r];
 }

@@ -403,12 +403,33 @@ __device__ __forceinline__ void pack_param(int p, float val, uint16_t* bf) {
     }
 }
 
-__global__ void k_mlp_reduce_adam(const float* partial, uint32_t nwaves, float* P, float* m, float* v, uint16_t* bf,
-                                  float* grad_out, float* loss_out, float lr, float bc1, float bc2) {
-    const int p = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (p > MLP_PARAMS) return;
+// 64 parameters per block (one per lane), the partials split over 16
+// waves; each wave sums its share with four independent accumulators, then
+// the waves' sums are added in a fixed order: deterministic, and ~16x the
+// parallelism of one thread per parameter.
+constexpr uint32_t RED_WAVES = 16;
+
+__global__ __launch_bounds__(64 * RED_WAVES) void k_mlp_reduce_adam(const float* partial, uint32_t nwaves, float* P,
+                                                                   float* m, float* v, uint16_t* bf, float* grad_out,
+                                                                   float* loss_out, float lr, float bc1, float bc2) {
+    __shared__ float part[RED_WAVES][64];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const int p = int(blockIdx.x * 64u + lane);
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    if (p <= MLP_PARAMS) {
+        uint32_t w = wv;
+        for (; w + 3u * RED_WAVES < nwaves; w += 4u * RED_WAVES) {
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) acc[u] += partial[size_t(w + u * RED_WAVES) * MLP_PSTRIDE + p];
+        }
+        for (; w < nwaves; w += RED_WAVES) acc[0] += partial[size_t(w) * MLP_PSTRIDE + p];
+    }
+    part[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    __syncthreads();
+    if (wv != 0 || p > MLP_PARAMS) return;
     float s = 0.f;
-    for (uint32_t w = 0; w < nwaves; ++w) s += partial[size_t(w) * MLP_PSTRIDE + p];
+#pragma unroll
+    for (uint32_t k = 0; k < RED_WAVES; ++k) s += part[k][lane];
     if (p == MLP_PARAMS) {
         if (loss_out) *loss_out = s;
         return;
@@ -462,7 +483,7 @@ void launch_mlp_train(const uint16_t* feat, const float* targets, uint32_t rows,
 
 void launch_mlp_reduce_adam(const float* partial, float* P, float* m, float* v, uint16_t* bf, float* grad_out,
                             float* loss_out, float lr, float beta1_t, float beta2_t, hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(cdiv(MLP_PARAMS + 1, 256)), dim3(256), 0, s, partial,
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(cdiv(MLP_PARAMS + 1, 64)), dim3(64 * RED_WAVES), 0, s, partial,
                        MLP_TRAIN_WAVES, P, m, v, bf, grad_out, loss_out, lr, 1.0f - beta1_t, 1.0f - beta2_t);
 }
 
