@@ -650,12 +650,16 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
     }
 }
 
-// Hash table: group of 4 consecutive x-cells (key >> 2) -> first sorted
+// Hash table: group of 8 consecutive x-cells (key >> 3) -> first sorted
 // index of the group.  A 3-cell row [kmin, kmax] spans one or two groups, so
 // one probe (rarely two) finds where its particles start; one insert per
-// occupied group keeps the atomic traffic ~4x below one per cell.
-constexpr uint32_t GROUP_SHIFT = 2;
-// Slot of a group key: the low 3 bits stay (8 consecutive groups = 32 cells
+// occupied group keeps the atomic traffic ~8x below one per cell.  The table
+// has >= 4 slots per particle (>= 32 per occupied group on a dense sheet):
+// a wave waits for its longest probe chain, and on a 1024x1024 sheet halving
+// the slots (or the group) lengthened those chains enough to cost 1.5-2.5x in
+// the sweep; the skipped cells before kmin cost less than the extra probes.
+constexpr uint32_t GROUP_SHIFT = 3;
+// Slot of a group key: the low 3 bits stay (8 consecutive groups = 64 cells
 // share one 64-byte line, so threads walking the sorted order probe the same
 // lines); the block index g >> 3 is placed by Fibonacci hashing, taking the
 // HIGH bits of (g >> 3) * 2^32/phi.  Those depend on every key bit (a batch
@@ -711,19 +715,20 @@ __device__ __forceinline__ uint32_t table_resolve(const uint2* table, uint32_t m
 }
 
 // Pair kernel of self_collision_resolve (xpbd_cpu.rs:84-121) for the pair
-// (min(i,j), max(i,j)); returns the contribution to particle i.  The rest
-// positions are read only for pairs inside the contact distance.
-__device__ __forceinline__ bool collision_contrib(uint32_t i, uint32_t j, float4 fi, float4 fj, const float4* rest4,
-                                                  const DevParams& p, float& cx, float& cy, float& cz) {
+// (min(i,j), max(i,j)); returns the contribution to particle i.  fi/fj carry
+// the positions (xyz), wi/wj the inverse masses, ri/rj the rest positions.
+__device__ __forceinline__ bool collision_contrib(uint32_t i, uint32_t j, float4 fi, float wi, float4 fj, float wj,
+                                                  float4 ri, float4 rj, const DevParams& p, float& cx, float& cy,
+                                                  float& cz) {
     const bool i_lo = i < j;
     const float4 pa = i_lo ? fi : fj, pb = i_lo ? fj : fi;
-    const float wa = pa.w, wb = pb.w;
+    const float wa = i_lo ? wi : wj, wb = i_lo ? wj : wi;
     if (wa <= 0.f && wb <= 0.f) return false;
     const float th2 = p.thickness * p.thickness;
     float dx = pb.x - pa.x, dy = pb.y - pa.y, dz = pb.z - pa.z;
     const float dist2 = (dx * dx + dy * dy) + dz * dz;
     if (dist2 > th2 || dist2 < 1e-18f) return false;
-    const float4 ra = rest4[i_lo ? i : j], rb = rest4[i_lo ? j : i];
+    const float4 ra = i_lo ? ri : rj, rb = i_lo ? rj : ri;
     const float ex = rb.x - ra.x, ey = rb.y - ra.y, ez = rb.z - ra.z;
     const float rest2 = (ex * ex + ey * ey) + ez * ez;
     if (dist2 > rest2) return false;
@@ -755,7 +760,7 @@ __device__ __forceinline__ bool collision_contrib(uint32_t i, uint32_t j, float4
 // CMAX contacts in a list sorted by partner id (registers); a particle with
 // more contacts falls back to repeated selection of the next-smallest partner.
 constexpr int CMAX = 4;
-constexpr uint32_t SB = 4;   // candidates fetched per step of a row scan
+constexpr uint32_t SB = 2;   // candidates fetched per step of a row scan (more costs occupancy)
 
 struct RowRange {
     uint32_t start, kmin, kmax;
@@ -818,10 +823,11 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     const uint32_t n = a.p.n;
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const float4 fi = a.sP[s];
+    const float4 fi = a.sP[s];    // xyz, w = cell key bits
+    const float4 ri = a.sR[s];    // rest xyz, w = inverse mass
     const uint32_t i = a.sids[s];
-    if (fi.w <= 0.f) {
-        if (a.xout != a.prev) a.xout[i] = fi;    // timing runs write scratch
+    if (ri.w <= 0.f) {
+        if (a.xout != a.prev) a.xout[i] = make_float4(fi.x, fi.y, fi.z, ri.w);   // timing runs write scratch
         return;
     }
     const CollGrid g = a.grids[a.cloth_of ? a.cloth_of[i] : 0u];
@@ -854,9 +860,8 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
             float4 ff[SB];
 #pragma unroll
             for (uint32_t u = 0; u < SB; ++u) {
-                const uint32_t q = min(idx + u, n - 1);
-                kk[u] = idx + u < n ? a.skeys[q] : EMPTY_KEY;
-                ff[u] = a.sP[q];
+                ff[u] = a.sP[min(idx + u, n - 1)];
+                kk[u] = idx + u < n ? __float_as_uint(ff[u].w) : EMPTY_KEY;
             }
 #pragma unroll
             for (uint32_t u = 0; u < SB; ++u) {
@@ -871,8 +876,9 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
                 const float d2 = dist2_of(fi, fj);
                 if (d2 > th2 || d2 < 1e-18f) continue;
                 const uint32_t j = a.sids[q];
+                const float4 rj = a.sR[q];
                 float cx, cy, cz;
-                if (!collision_contrib(i, j, fi, fj, a.rest4, a.p, cx, cy, cz)) continue;
+                if (!collision_contrib(i, j, fi, ri.w, fj, rj.w, ri, rj, a.p, cx, cy, cz)) continue;
                 ++count;
                 if (count > CMAX) continue;
                 // insert at the end, then bubble down to keep ascending partner ids
@@ -907,7 +913,7 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
             ay = ay + ly[q];
             az = az + lz[q];
         }
-    const float4 nx = make_float4(fi.x + clamp_apply(ax), fi.y + clamp_apply(ay), fi.z + clamp_apply(az), fi.w);
+    const float4 nx = make_float4(fi.x + clamp_apply(ax), fi.y + clamp_apply(ay), fi.z + clamp_apply(az), ri.w);
     finish_particle(nx, a.prev[i], i, a.ground, a.p, a.xout, a.vout);
 }
 
@@ -930,6 +936,7 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
     for (uint32_t k = blockIdx.x * blockDim.x + t; k < total; k += gridDim.x * blockDim.x) {
         const uint32_t s = a.ovf[2 + k];
         const float4 fi = a.sP[s];
+        const float4 ri = a.sR[s];
         const uint32_t i = a.sids[s];
         const CollGrid g = a.grids[a.cloth_of ? a.cloth_of[i] : 0u];
         uint32_t c[3];
@@ -941,15 +948,16 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
         for (int r = 0; r < 9; ++r) {
             if (rows[r].start == EMPTY_KEY) continue;
             for (uint32_t idx = rows[r].start; idx < n; ++idx) {
-                const uint32_t key = a.skeys[idx];
+                const float4 fj = a.sP[idx];
+                const uint32_t key = __float_as_uint(fj.w);
                 if (key > rows[r].kmax) break;
                 if (key < rows[r].kmin || idx == s) continue;
-                const float4 fj = a.sP[idx];
                 const float d2 = dist2_of(fi, fj);
                 if (d2 > th2 || d2 < 1e-18f) continue;
                 const uint32_t j = a.sids[idx];
+                const float4 rj = a.sR[idx];
                 float cx, cy, cz;
-                if (!collision_contrib(i, j, fi, fj, a.rest4, a.p, cx, cy, cz)) continue;
+                if (!collision_contrib(i, j, fi, ri.w, fj, rj.w, ri, rj, a.p, cx, cy, cz)) continue;
                 if (cnt < OVF_MAX) {   // insertion into the sorted list
                     int w = cnt;
                     while (w > 0 && lj[w - 1][t] > j) {
@@ -981,16 +989,17 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
                 for (int r = 0; r < 9; ++r) {
                     if (rows[r].start == EMPTY_KEY) continue;
                     for (uint32_t idx = rows[r].start; idx < n; ++idx) {
-                        const uint32_t key = a.skeys[idx];
+                        const float4 fj = a.sP[idx];
+                        const uint32_t key = __float_as_uint(fj.w);
                         if (key > rows[r].kmax) break;
                         if (key < rows[r].kmin || idx == s) continue;
-                        const float4 fj = a.sP[idx];
                         const float d2 = dist2_of(fi, fj);
                         if (d2 > th2 || d2 < 1e-18f) continue;
                         const uint32_t j = a.sids[idx];
                         if (j >= best || (has_last && j <= last)) continue;
+                        const float4 rj = a.sR[idx];
                         float cx, cy, cz;
-                        if (collision_contrib(i, j, fi, fj, a.rest4, a.p, cx, cy, cz)) {
+                        if (collision_contrib(i, j, fi, ri.w, fj, rj.w, ri, rj, a.p, cx, cy, cz)) {
                             best = j;
                             bx = cx;
                             by = cy;
@@ -1006,16 +1015,25 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
                 has_last = true;
             }
         }
-        const float4 nx = make_float4(fi.x + clamp_apply(ax), fi.y + clamp_apply(ay), fi.z + clamp_apply(az), fi.w);
+        const float4 nx = make_float4(fi.x + clamp_apply(ax), fi.y + clamp_apply(ay), fi.z + clamp_apply(az), ri.w);
         finish_particle(nx, a.prev[i], i, a.ground, a.p, a.xout, a.vout);
     }
 }
 
 // Positions permuted into sorted (cell) order: sP[s] = F[sids[s]].  Also
 // clears the count of the sweep's overflow list (ovf[0]).
-__global__ void k_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float4* sP, uint32_t* ovf) {
+// Sorted copies for the sweep: sP = (position, key bits), sR = (rest
+// position, inverse mass), so one 16-byte load gives a candidate's key and
+// position, and one more its rest position and mass.
+__global__ void k_coll_gather(const float4* F, const float4* rest4, const uint32_t* sids, const uint32_t* skeys,
+                              uint32_t n, float4* sP, float4* sR, uint32_t* ovf) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) sP[s] = F[sids[s]];
+    if (s < n) {
+        const uint32_t i = sids[s];
+        const float4 f = F[i], r = rest4[i];
+        sP[s] = make_float4(f.x, f.y, f.z, __uint_as_float(skeys[s]));
+        sR[s] = make_float4(r.x, r.y, r.z, f.w);
+    }
     if (ovf && s == 0) ovf[0] = 0u;
 }
 
@@ -1525,10 +1543,11 @@ void launch_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t
     hipLaunchKernelGGL(k_coll_table, dim3(cdiv(n, 256)), dim3(256), 0, s, skeys, n, table, mask);
 }
 
-void launch_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float4* sP, uint32_t* ovf,
+void launch_coll_gather(const float4* F, const float4* rest4, const uint32_t* sids, const uint32_t* skeys, uint32_t n,
+                        float4* sP, float4* sR, uint32_t* ovf,
                         hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_coll_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, F, sids, n, sP, ovf);
+    hipLaunchKernelGGL(k_coll_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, F, rest4, sids, skeys, n, sP, sR, ovf);
 }
 
 void launch_coll_sweep(const CollArgs& a, hipStream_t s) {

@@ -93,8 +93,8 @@ struct CollGrid {
 
 struct CollArgs {
     const float4* F;     // positions after the constraint iterations
-    const float4* sP;    // F permuted into sorted order
-    const float4* rest4;
+    const float4* sP;    // sorted order: position xyz, w = cell key bits
+    const float4* sR;    // sorted order: rest position xyz, w = inverse mass
     const float4* prev;  // X (substep start) for the velocity update
     float4* xout;
     float4* vout;
@@ -152,9 +152,9 @@ uint32_t radix_hist_words(uint32_t n);
 void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
                        uint32_t shift, uint32_t* hist, hipStream_t s);
 void launch_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask, hipStream_t s);
-// sP = F in sorted order; clears the sweep's overflow count ovf[0]
-void launch_coll_gather(const float4* F, const uint32_t* sids, uint32_t n, float4* sP, uint32_t* ovf,
-                        hipStream_t s);
+// sorted copies for the sweep (sP, sR); also clears the overflow count
+void launch_coll_gather(const float4* F, const float4* rest4, const uint32_t* sids, const uint32_t* skeys, uint32_t n,
+                        float4* sP, float4* sR, uint32_t* ovf, hipStream_t s);
 void launch_coll_sweep(const CollArgs& a, hipStream_t s);
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
 uint32_t gs_max_blocks(uint32_t block);

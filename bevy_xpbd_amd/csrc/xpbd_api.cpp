@@ -75,7 +75,8 @@ struct xpbd_cloth {
 
     std::vector<void*> allocs;
     float4 *X = nullptr, *V = nullptr, *A = nullptr, *B = nullptr, *V2 = nullptr, *rest4 = nullptr;
-    float4* sP = nullptr;   // collision: positions in sorted order
+    float4* sP = nullptr;   // collision: positions + cell key in sorted order
+    float4* sR = nullptr;   // collision: rest positions + inverse mass in sorted order
     uint32_t *soff = nullptr, *swid = nullptr, *nbj = nullptr;
     float *nbr = nullptr, *nbc = nullptr;
     uint32_t* nbpk = nullptr;       // packed adjacency (nullptr: plain layout in use)
@@ -267,11 +268,11 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->table_slots, s);
     launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
     launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, s);
-    launch_coll_gather(F, h->ids, h->n, h->sP, h->ovf, s);
+    launch_coll_gather(F, h->rest4, h->ids, h->keys, h->n, h->sP, h->sR, h->ovf, s);
     CollArgs c{};
     c.F = F;
     c.sP = h->sP;
-    c.rest4 = h->rest4;
+    c.sR = h->sR;
     c.prev = h->X;
     c.xout = xout;
     c.vout = vout;
@@ -582,7 +583,7 @@ int validate_mesh(const Mesh& m, std::string& why) {
     const size_t n = m.num_particles();
     if (m.positions.size() != 3 * n || m.rest_positions.size() != 3 * n) return why = "position arrays", XPBD_E_INVALID;
     if (m.neighbor_offsets.size() != n + 1) return why = "neighbor_offsets size", XPBD_E_INVALID;
-    if (n >= 0xFFFFFFF0u) return why = "too many particles", XPBD_E_INVALID;
+    if (n > (1u << 29)) return why = "too many particles (at most 2^29)", XPBD_E_INVALID;
     for (size_t k = 0; k < m.num_constraints(); ++k)
         if (m.constraint_i[k] >= n || m.constraint_j[k] >= n || m.constraint_i[k] == m.constraint_j[k])
             return why = "constraint endpoint out of range", XPBD_E_INVALID;
@@ -607,6 +608,7 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(dalloc(h, &h->V2, n));
     TRY(dalloc(h, &h->rest4, n));
     TRY(dalloc(h, &h->sP, n));
+    TRY(dalloc(h, &h->sR, n));
     std::vector<float> rest4(4 * n);
     h->init_pos4.assign(4 * n, 0.f);
     for (size_t i = 0; i < n; ++i) {
@@ -687,9 +689,12 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(dalloc(h, &h->keys2, n));
     TRY(dalloc(h, &h->ids2, n));
     TRY(dalloc(h, &h->hist, radix_hist_words(uint32_t(n))));
-    // one entry per occupied 4-cell group: at most n groups (a stretched
-    // sheet puts nearly every particle in its own group), load <= 1/2
-    h->table_slots = next_pow2(uint32_t(std::max<size_t>(2 * n, 64)));
+    // one entry per occupied 8-cell group: at most n groups (a stretched
+    // sheet puts nearly every particle in its own group), so load <= 1/4;
+    // >= 4 slots per particle keeps the sweep's probe chains short (the
+    // hash table notes in kernels.hip).  next_pow2 caps at 2^31 slots, which
+    // still leaves load <= 1/4 up to the particle limit checked at creation.
+    h->table_slots = next_pow2(uint32_t(std::min<size_t>(std::max<size_t>(4 * size_t(n), 64), 0x80000000u)));
     TRY(dalloc(h, &h->table, h->table_slots));
     TRY(dalloc(h, &h->ovf, size_t(n) + 2));
     HIPC(h, hipMemset(h->ovf, 0, (size_t(n) + 2) * sizeof(uint32_t)));
@@ -1242,14 +1247,14 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         launch_coll_keys(h->A, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->table_slots, s);
         launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
         launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, s);
-        launch_coll_gather(h->A, h->ids, h->n, h->sP, h->ovf, s);
+        launch_coll_gather(h->A, h->rest4, h->ids, h->keys, h->n, h->sP, h->sR, h->ovf, s);
     };
     CollArgs c{};
     if (which == XPBD_KERNEL_COLLISION_SWEEP || which == XPBD_KERNEL_RADIX_PASS) {
         prep_collision();
         c.F = h->A;
         c.sP = h->sP;
-        c.rest4 = h->rest4;
+        c.sR = h->sR;
         c.prev = h->X;
         c.xout = h->B;
         c.vout = h->V2;
