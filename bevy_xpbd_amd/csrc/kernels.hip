@@ -513,24 +513,58 @@ __global__ void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, 
 }
 
 // ---- stable LSD radix sort of (key, id), 8-bit digits, 4 passes ----------
+// A pass is three kernels: per-tile digit counts, an exclusive scan of the
+// counts over tiles (per digit), and a stable scatter.  A tile is 2048 keys;
+// each of its 4 wavefronts owns a contiguous quarter (512 keys, 8 rounds of
+// 64), so ranks in input order are (wavefront, round, lane) and need no block
+// barrier per round.  Lanes with equal digits are found by matching the 8
+// digit bits with ballots; one leader per distinct digit updates the
+// wavefront's own counters (distinct addresses, no atomics).
 constexpr uint32_t RADIX_BLOCK = 256;
-constexpr uint32_t RADIX_ITEMS = 4;
+constexpr uint32_t RADIX_WAVES = RADIX_BLOCK / 64;
+constexpr uint32_t RADIX_ITEMS = 8;
 constexpr uint32_t RADIX_TILE = RADIX_BLOCK * RADIX_ITEMS;
+constexpr uint32_t RADIX_WAVE_SPAN = 64 * RADIX_ITEMS;   // keys owned by one wavefront
+
+// Lanes (among `valid` ones) whose digit equals this lane's.
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (uint32_t b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+    }
+    return peers;
+}
 
 __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const uint32_t* keys, uint32_t n, uint32_t shift,
                                                           uint32_t* hist, uint32_t ntiles) {
-    __shared__ uint32_t h[256];
-    const uint32_t t = threadIdx.x;
-    h[t] = 0;
+    __shared__ uint32_t h[RADIX_WAVES][256];
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+#pragma unroll
+    for (uint32_t w = 0; w < RADIX_WAVES; ++w) h[w][t] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * RADIX_TILE;
-#pragma unroll 4
+    const uint32_t base = blockIdx.x * RADIX_TILE + wave * RADIX_WAVE_SPAN + lane;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t kk[RADIX_ITEMS];
+#pragma unroll
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
-        const uint32_t idx = base + r * RADIX_BLOCK + t;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+        const uint32_t idx = base + r * 64;
+        kk[r] = idx < n ? keys[idx] : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+        const bool valid = base + r * 64 < n;
+        const uint32_t d = (kk[r] >> shift) & 255u;
+        const uint64_t peers = digit_peers(d, valid);
+        if (valid && (peers & lt) == 0) h[wave][d] += uint32_t(__popcll(peers));
     }
     __syncthreads();
-    hist[t * ntiles + blockIdx.x] = h[t];
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < RADIX_WAVES; ++w) c += h[w][t];
+    hist[t * ntiles + blockIdx.x] = c;
 }
 
 // Exclusive scan of 256 values held one per thread (block of 256).
@@ -568,82 +602,76 @@ __global__ __launch_bounds__(256) void k_radix_scan_tiles(uint32_t* hist, uint32
     if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
 }
 
-// Stable scatter.  Each tile is first sorted by digit in LDS (ranks in input
-// order: round by round; inside a round, wave by wave; inside a wave, by lane,
-// from a 64-lane ballot match on the 8 digit bits), then written out in LDS
-// order, so each digit's run of the tile leaves as contiguous stores.
+// Stable scatter: the tile is ranked in input order (see above), placed in
+// LDS in digit order, and written out in that order, so each digit's run of
+// the tile leaves as contiguous stores.
 __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* kin, const uint32_t* iin,
                                                              uint32_t* kout, uint32_t* iout, uint32_t n,
                                                              uint32_t shift, const uint32_t* hist,
                                                              uint32_t ntiles, const uint32_t* dtot) {
-    constexpr uint32_t NW = RADIX_BLOCK / 64;
-    __shared__ uint32_t gbase[256];      // global position of this tile's first element of digit d
-    __shared__ uint32_t lstart[256];     // tile-local start of digit d
-    __shared__ uint32_t run[256];        // running local count of digit d
-    __shared__ uint32_t wcnt[NW][256];
+    __shared__ uint32_t wcnt[RADIX_WAVES][256];   // per-wavefront digit counts, then its start in the tile
+    __shared__ int32_t gbase[256];                 // global position of local sorted position 0 of digit d
     __shared__ uint32_t skey[RADIX_TILE];
     __shared__ uint32_t sid[RADIX_TILE];
     __shared__ uint32_t lds[4];
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t tile0 = blockIdx.x * RADIX_TILE;
     const uint32_t count = min(RADIX_TILE, n - tile0);
-    // tile-local histogram
-    run[t] = 0;
-    for (uint32_t w = 0; w < NW; ++w) wcnt[w][t] = 0;
-    __syncthreads();
-    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
-        const uint32_t k = r * RADIX_BLOCK + t;
-        if (k < count) atomicAdd(&run[(kin[tile0 + k] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    uint32_t tot;
-    const uint32_t mine = run[t];
-    const uint32_t lex = block_exclusive_scan_256(mine, lds, tot);
-    const uint32_t digit_base = block_exclusive_scan_256(dtot[t], lds, tot);
-    lstart[t] = lex;
-    gbase[t] = digit_base + hist[t * ntiles + blockIdx.x];
-    run[t] = 0;
-    __syncthreads();
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
-        const uint32_t k = r * RADIX_BLOCK + t;
-        const bool valid = k < count;
-        const uint32_t key = valid ? kin[tile0 + k] : 0u;
-        const uint32_t id = valid ? iin[tile0 + k] : 0u;
-        const uint32_t d = (key >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (uint32_t b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
-        }
-        const uint32_t rank = __popcll(peers & lt);
-        if (valid && rank == 0) wcnt[wave][d] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = lstart[d] + run[d] + rank;
-            for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][d];
-            skey[pos] = key;
-            sid[pos] = id;
-        }
-        __syncthreads();
-        uint32_t add = 0;
-        for (uint32_t w = 0; w < NW; ++w) {
-            add += wcnt[w][t];
-            wcnt[w][t] = 0;
-        }
-        run[t] += add;
-        __syncthreads();
+    for (uint32_t w = 0; w < RADIX_WAVES; ++w) wcnt[w][t] = 0;
+    __syncthreads();
+    const uint32_t base = wave * RADIX_WAVE_SPAN + lane;   // tile-local index of round 0
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t kk[RADIX_ITEMS], ii[RADIX_ITEMS], rk[RADIX_ITEMS];
+#pragma unroll
+    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+        const uint32_t q = base + r * 64;
+        kk[r] = q < count ? kin[tile0 + q] : 0u;
+        ii[r] = q < count ? iin[tile0 + q] : 0u;
     }
-    // write out in local sorted order: element q of the tile goes to
-    // gbase[d] + (q - lstart[d])
+#pragma unroll
+    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+        const bool valid = base + r * 64 < count;
+        const uint32_t d = (kk[r] >> shift) & 255u;
+        const uint64_t peers = digit_peers(d, valid);
+        const uint32_t before = wcnt[wave][d];   // read by every lane before the leader's update
+        rk[r] = before + uint32_t(__popcll(peers & lt));
+        if (valid && (peers & lt) == 0) wcnt[wave][d] = before + uint32_t(__popcll(peers));
+    }
+    __syncthreads();
+    // digit t: tile-local start, then each wavefront's start within it
+    uint32_t c[RADIX_WAVES], tot_t = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < RADIX_WAVES; ++w) {
+        c[w] = wcnt[w][t];
+        tot_t += c[w];
+    }
+    uint32_t tot;
+    const uint32_t lstart = block_exclusive_scan_256(tot_t, lds, tot);
+    const uint32_t digit_base = block_exclusive_scan_256(dtot[t], lds, tot);
+    gbase[t] = int32_t(digit_base + hist[t * ntiles + blockIdx.x]) - int32_t(lstart);
+    uint32_t run = lstart;
+#pragma unroll
+    for (uint32_t w = 0; w < RADIX_WAVES; ++w) {
+        wcnt[w][t] = run;
+        run += c[w];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+        if (base + r * 64 < count) {
+            const uint32_t pos = wcnt[wave][(kk[r] >> shift) & 255u] + rk[r];
+            skey[pos] = kk[r];
+            sid[pos] = ii[r];
+        }
+    }
+    __syncthreads();
+#pragma unroll
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
         const uint32_t q = r * RADIX_BLOCK + t;
         if (q < count) {
             const uint32_t key = skey[q];
-            const uint32_t d = (key >> shift) & 255u;
-            const uint32_t dst = gbase[d] + (q - lstart[d]);
+            const uint32_t dst = uint32_t(gbase[(key >> shift) & 255u] + int32_t(q));
             kout[dst] = key;
             iout[dst] = sid[q];
         }
