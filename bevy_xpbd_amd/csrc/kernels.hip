@@ -334,8 +334,11 @@ __device__ __forceinline__ bool stage_pairs(const JacobiArgs& a, float2* lpairs)
     return lds;
 }
 
+// At least 8 wavefronts per SIMD (the compiler keeps it at <= 64 VGPRs, no
+// spills): one more wavefront of gathers in flight than the unconstrained 70
+// VGPRs allow; 39.6 -> 36.7 us per iteration on the 1024^2 sheet.
 template <uint32_t MODE, bool PACKED, bool FASTDIV>
-__global__ __launch_bounds__(512) void k_jacobi(JacobiArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_jacobi(JacobiArgs a) {
     if (a.gate && *a.gate == 0u) return;   // uniform over the grid
     __shared__ float2 lpairs[LDS_PAIRS_MAX];
     const bool lds = stage_pairs<PACKED>(a, lpairs);
@@ -499,7 +502,7 @@ __device__ __forceinline__ void cell_of(float4 p, const CollGrid& g, uint32_t c[
 }
 
 __global__ void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of,
-                            uint32_t* keys, uint32_t* ids, uint2* table, uint32_t slots) {
+                            uint32_t* keys, uint32_t* ids) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const CollGrid& g = grids[cloth_of ? cloth_of[i] : 0u];
@@ -508,8 +511,6 @@ __global__ void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, 
         keys[i] = g.key_base + (c[0] + g.dims[0] * (c[1] + g.dims[1] * c[2]));
         ids[i] = i;
     }
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += stride)
-        table[s] = make_uint2(EMPTY_KEY, 0u);
 }
 
 // ---- stable LSD radix sort of (key, id), 8-bit digits, 4 passes ----------
@@ -700,7 +701,11 @@ __device__ __forceinline__ uint32_t slot_of(uint32_t g, uint32_t mask) {
     return ((blk << 3) | (g & 7u)) & mask;
 }
 
-__global__ void k_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask) {
+// Inserts each group at its first sorted index s and records the slot in
+// slot_rec[s], so k_coll_untable can empty exactly the slots it used: the
+// table is all empty between collision passes (filled once at creation) and
+// no pass clears the whole table (256 MB for a 6M-particle batch).
+__global__ void k_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask, uint32_t* slot_rec) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint32_t g = skeys[s] >> GROUP_SHIFT;
@@ -710,10 +715,21 @@ __global__ void k_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, ui
         const uint32_t prev = atomicCAS(&table[h].x, EMPTY_KEY, g);
         if (prev == EMPTY_KEY) {
             table[h].y = s;
+            slot_rec[s] = h;
             return;
         }
         h = (h + 1) & mask;
     }
+    slot_rec[s] = EMPTY_KEY;   // full table (not reachable at load <= 1/4)
+}
+
+__global__ void k_coll_untable(const uint32_t* skeys, uint32_t n, uint2* table, const uint32_t* slot_rec) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t g = skeys[s] >> GROUP_SHIFT;
+    if (s > 0 && (skeys[s - 1] >> GROUP_SHIFT) == g) return;
+    const uint32_t h = slot_rec[s];
+    if (h != EMPTY_KEY) table[h] = make_uint2(EMPTY_KEY, EMPTY_KEY);
 }
 
 __device__ __forceinline__ uint32_t table_find(const uint2* table, uint32_t mask, uint32_t g) {
@@ -1539,10 +1555,10 @@ void launch_coll_grids(const float4* P, const uint32_t* cloth_off, uint32_t nclo
 }
 
 void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
-                      uint32_t* ids, uint2* table, uint32_t slots, hipStream_t s) {
-    const uint32_t work = std::max(n, slots);
-    hipLaunchKernelGGL(k_coll_keys, dim3(std::max(1u, std::min(cdiv(work, 256), 8192u))), dim3(256), 0, s, F, n,
-                       grids, cloth_of, keys, ids, table, slots);
+                      uint32_t* ids, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_coll_keys, dim3(std::min(cdiv(n, 256), 8192u)), dim3(256), 0, s, F, n, grids, cloth_of, keys,
+                       ids);
 }
 
 uint32_t radix_hist_words(uint32_t n) { return 256u * std::max(1u, cdiv(n, RADIX_TILE)) + 256u; }
@@ -1566,9 +1582,15 @@ void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32
     launch_radix_pass(keys_alt, ids_alt, keys, ids, n, 24, hist, s);
 }
 
-void launch_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask, hipStream_t s) {
+void launch_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask, uint32_t* slot_rec,
+                       hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_coll_table, dim3(cdiv(n, 256)), dim3(256), 0, s, skeys, n, table, mask);
+    hipLaunchKernelGGL(k_coll_table, dim3(cdiv(n, 256)), dim3(256), 0, s, skeys, n, table, mask, slot_rec);
+}
+
+void launch_coll_untable(const uint32_t* skeys, uint32_t n, uint2* table, const uint32_t* slot_rec, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_coll_untable, dim3(cdiv(n, 256)), dim3(256), 0, s, skeys, n, table, slot_rec);
 }
 
 void launch_coll_gather(const float4* F, const float4* rest4, const uint32_t* sids, const uint32_t* skeys, uint32_t n,
@@ -1584,6 +1606,16 @@ void launch_coll_sweep(const CollArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_coll_sweep_overflow, dim3(std::min(cdiv(a.p.n, OVF_BLOCK), 2048u)), dim3(OVF_BLOCK), 0, s, a);
 }
 
+// Blocks per CU a grid-barrier launch may use: the occupancy figure, but at
+// most 7 wavefronts per SIMD.  A grid filling all 8 wavefront slots of every
+// SIMD was not fully co-resident on MI355X (its barrier wait expired), so
+// one slot per SIMD stays free.
+int coresident_per_cu(int per_cu, uint32_t block_threads) {
+    const int waves = int((block_threads + 63) / 64);
+    const int cap = std::max(1, (7 * 4) / std::max(1, waves));
+    return std::max(0, std::min(per_cu, cap));
+}
+
 uint32_t persistent_max_blocks(bool packed, bool fastdiv) {
     int dev = 0, per_cu = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
@@ -1594,7 +1626,7 @@ uint32_t persistent_max_blocks(bool packed, bool fastdiv) {
     else e = fastdiv ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jacobi_persistent<false, true>, 256, 0)
                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jacobi_persistent<false, false>, 256, 0);
     if (e != hipSuccess) return 0;
-    return uint32_t(std::max(0, per_cu) * std::max(0, cus));
+    return uint32_t(coresident_per_cu(per_cu, 256) * std::max(0, cus));
 }
 
 void launch_jacobi_persistent(const JacobiArgs& a, const PersistArgs& q, uint32_t blocks, hipStream_t s) {
@@ -1612,7 +1644,7 @@ uint32_t gs_max_blocks(uint32_t block) {
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_substep, int(block), 0) != hipSuccess) return 0;
-    return uint32_t(std::max(0, per_cu) * std::max(0, cus));
+    return uint32_t(coresident_per_cu(per_cu, block) * std::max(0, cus));
 }
 
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s) {
