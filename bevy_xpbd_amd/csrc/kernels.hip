@@ -830,10 +830,14 @@ __device__ __forceinline__ bool collision_contrib(uint32_t i, uint32_t j, float4
 // Narrow phase + apply + post_velocity, one particle per thread in sorted
 // order.  Contributions must be added in ascending partner id — the order the
 // reference's O(N^2) loop adds them (xpbd_cpu.rs:82-123).  One pass over the
-// 3x3 neighbour rows (each a contiguous run of the sorted keys) keeps up to
-// CMAX contacts in a list sorted by partner id (registers); a particle with
-// more contacts falls back to repeated selection of the next-smallest partner.
-constexpr int CMAX = 4;
+// 3x3 neighbour rows (each a contiguous run of the sorted keys) collects up to
+// LMAX candidates inside the contact distance; a particle with more goes to
+// k_coll_sweep_overflow.  Computing each contribution inside the divergent
+// scan made every wavefront walk the contact path whenever one lane had a
+// contact (2.3x the VALU instructions on a folded sheet); collected first,
+// the contributions run in one short convergent loop (frame 65 of the
+// 1024^2 sheet: 146 -> 88 us).
+constexpr int LMAX = 8;   // candidates inside the contact distance kept per particle
 constexpr uint32_t SB = 2;   // candidates fetched per step of a row scan (more costs occupancy)
 
 struct RowRange {
@@ -911,25 +915,20 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     RowRange rows[9];
     sweep_rows(a, g, c, rows);
 
-    // one pass: contacts kept sorted by partner id
-    uint32_t lj[CMAX];
-    float lx[CMAX], ly[CMAX], lz[CMAX];
+    // the scan only collects the sorted indices of candidates inside the
+    // contact distance (cheap, convergent); their partner ids are loaded
+    // together and sorted, and the contributions computed afterwards in
+    // ascending partner id
+    uint32_t lq[LMAX];
 #pragma unroll
-    for (int q = 0; q < CMAX; ++q) {
-        lj[q] = EMPTY_KEY;
-        lx[q] = ly[q] = lz[q] = 0.f;
-    }
-    int count = 0;
-    // rows[] is indexed with compile-time indices only (unrolled loops), so it
-    // stays in registers instead of the per-thread scratch segment
+    for (int k = 0; k < LMAX; ++k) lq[k] = s;
+    int cnt = 0;
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         const uint32_t kmin = rows[r].kmin, kmax = rows[r].kmax;
         uint32_t idx = rows[r].start;
         bool done = idx == EMPTY_KEY;
         while (!done) {
-            // SB candidates' loads issued together (indices clamped in range;
-            // slots past the end count as "beyond the row")
             uint32_t kk[SB];
             float4 ff[SB];
 #pragma unroll
@@ -946,52 +945,61 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
                 }
                 const uint32_t q = idx + u;
                 if (kk[u] < kmin || q == s) continue;
-                const float4 fj = ff[u];
-                const float d2 = dist2_of(fi, fj);
+                const float d2 = dist2_of(fi, ff[u]);
                 if (d2 > th2 || d2 < 1e-18f) continue;
-                const uint32_t j = a.sids[q];
-                const float4 rj = a.sR[q];
-                float cx, cy, cz;
-                if (!collision_contrib(i, j, fi, ri.w, fj, rj.w, ri, rj, a.p, cx, cy, cz)) continue;
-                ++count;
-                if (count > CMAX) continue;
-                // insert at the end, then bubble down to keep ascending partner ids
-                lj[CMAX - 1] = j;
-                lx[CMAX - 1] = cx;
-                ly[CMAX - 1] = cy;
-                lz[CMAX - 1] = cz;
 #pragma unroll
-                for (int w = CMAX - 1; w > 0; --w) {
-                    if (lj[w] < lj[w - 1]) {
-                        uint32_t tj = lj[w]; lj[w] = lj[w - 1]; lj[w - 1] = tj;
-                        float t = lx[w]; lx[w] = lx[w - 1]; lx[w - 1] = t;
-                        t = ly[w]; ly[w] = ly[w - 1]; ly[w - 1] = t;
-                        t = lz[w]; lz[w] = lz[w - 1]; lz[w - 1] = t;
-                    }
-                }
+                for (int k = 0; k < LMAX; ++k) lq[k] = k == cnt ? q : lq[k];
+                ++cnt;
             }
             idx += SB;
             if (idx >= n) done = true;
         }
     }
-    if (count > CMAX) {   // resolved by k_coll_sweep_overflow
+    if (cnt > LMAX) {   // resolved by k_coll_sweep_overflow
         const uint32_t k = atomicAdd(&a.ovf[0], 1u);
         if (k < n) a.ovf[2 + k] = s;   // at most n entries per launch
         return;
     }
     float ax = 0.f, ay = 0.f, az = 0.f;
+    if (cnt > 0) {
+        uint32_t lj[LMAX];
 #pragma unroll
-    for (int q = 0; q < CMAX; ++q)
-        if (q < count) {
-            ax = ax + lx[q];
-            ay = ay + ly[q];
-            az = az + lz[q];
+        for (int k = 0; k < LMAX; ++k) {
+            const uint32_t j = a.sids[lq[k]];   // unused slots read this particle's own id
+            lj[k] = k < cnt ? j : EMPTY_KEY;
         }
+        // Batcher odd-even merge network on 8 (partner ids are distinct)
+#define XPBD_CX(x, y)                                   \
+        if (lj[y] < lj[x]) {                            \
+            const uint32_t tj = lj[x]; lj[x] = lj[y]; lj[y] = tj; \
+            const uint32_t tq = lq[x]; lq[x] = lq[y]; lq[y] = tq; \
+        }
+        XPBD_CX(0, 1) XPBD_CX(2, 3) XPBD_CX(4, 5) XPBD_CX(6, 7)
+        XPBD_CX(0, 2) XPBD_CX(1, 3) XPBD_CX(4, 6) XPBD_CX(5, 7)
+        XPBD_CX(1, 2) XPBD_CX(5, 6)
+        XPBD_CX(0, 4) XPBD_CX(1, 5) XPBD_CX(2, 6) XPBD_CX(3, 7)
+        XPBD_CX(2, 4) XPBD_CX(3, 5)
+        XPBD_CX(1, 2) XPBD_CX(3, 4) XPBD_CX(5, 6)
+#undef XPBD_CX
+#pragma unroll
+        for (int k = 0; k < LMAX; ++k) {
+            if (k < cnt) {
+                const float4 fj = a.sP[lq[k]];
+                const float4 rj = a.sR[lq[k]];
+                float cx, cy, cz;
+                if (collision_contrib(i, lj[k], fi, ri.w, fj, rj.w, ri, rj, a.p, cx, cy, cz)) {
+                    ax = ax + cx;
+                    ay = ay + cy;
+                    az = az + cz;
+                }
+            }
+        }
+    }
     const float4 nx = make_float4(fi.x + clamp_apply(ax), fi.y + clamp_apply(ay), fi.z + clamp_apply(az), ri.w);
     finish_particle(nx, a.prev[i], i, a.ground, a.p, a.xout, a.vout);
 }
 
-// Particles with more than CMAX contacts.  One pass over the 27 cells
+// Particles with more than LMAX candidates.  One pass over the 27 cells
 // collects up to OVF_MAX contacts into a list kept sorted by partner id in
 // LDS (one list per thread); more than that falls back to repeated selection
 // of the next-smallest partner.  Either way the sum runs in ascending partner

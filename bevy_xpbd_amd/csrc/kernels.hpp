@@ -106,7 +106,7 @@ struct CollArgs {
     const uint2* table;
     uint32_t table_mask;
     DevParams p;
-    // particles with more than CMAX contacts, resolved by a second kernel:
+    // particles with more than LMAX candidates, resolved by a second kernel:
     // [0] count (cleared by launch_coll_gather), [2..] sorted indices; n + 2 words
     uint32_t* ovf;
 };
