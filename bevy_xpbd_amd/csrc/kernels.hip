@@ -531,10 +531,16 @@ __device__ __forceinline__ void cell_of(float4 p, const CollGrid& g, uint32_t c[
     }
 }
 
+// Cell keys of the current positions.  Also empties the group-table slots
+// the previous collision pass filled (slot_rec holds, per sorted index, the
+// slot its group was inserted at, or EMPTY_KEY): the table is empty again
+// before this pass inserts.
 __global__ void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of,
-                            uint32_t* keys, uint32_t* ids) {
+                            uint32_t* keys, uint32_t* ids, uint2* table, const uint32_t* slot_rec) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t h = slot_rec[i];
+        if (h != EMPTY_KEY) table[h] = make_uint2(EMPTY_KEY, EMPTY_KEY);
         const CollGrid& g = grids[cloth_of ? cloth_of[i] : 0u];
         uint32_t c[3];
         cell_of(F[i], g, c);
@@ -729,37 +735,6 @@ __device__ __forceinline__ uint32_t slot_of(uint32_t g, uint32_t mask) {
     const uint32_t bits = __popc(mask >> 3);   // log2(slots / 8); the table has >= 64 slots
     const uint32_t blk = ((g >> 3) * 2654435761u) >> (32u - bits);
     return ((blk << 3) | (g & 7u)) & mask;
-}
-
-// Inserts each group at its first sorted index s and records the slot in
-// slot_rec[s], so k_coll_untable can empty exactly the slots it used: the
-// table is all empty between collision passes (filled once at creation) and
-// no pass clears the whole table (256 MB for a 6M-particle batch).
-__global__ void k_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask, uint32_t* slot_rec) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t g = skeys[s] >> GROUP_SHIFT;
-    if (s > 0 && (skeys[s - 1] >> GROUP_SHIFT) == g) return;
-    uint32_t h = slot_of(g, mask);
-    for (uint32_t probe = 0; probe <= mask; ++probe) {
-        const uint32_t prev = atomicCAS(&table[h].x, EMPTY_KEY, g);
-        if (prev == EMPTY_KEY) {
-            table[h].y = s;
-            slot_rec[s] = h;
-            return;
-        }
-        h = (h + 1) & mask;
-    }
-    slot_rec[s] = EMPTY_KEY;   // full table (not reachable at load <= 1/4)
-}
-
-__global__ void k_coll_untable(const uint32_t* skeys, uint32_t n, uint2* table, const uint32_t* slot_rec) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t g = skeys[s] >> GROUP_SHIFT;
-    if (s > 0 && (skeys[s - 1] >> GROUP_SHIFT) == g) return;
-    const uint32_t h = slot_rec[s];
-    if (h != EMPTY_KEY) table[h] = make_uint2(EMPTY_KEY, EMPTY_KEY);
 }
 
 __device__ __forceinline__ uint32_t table_find(const uint2* table, uint32_t mask, uint32_t g) {
@@ -1106,17 +1081,35 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
 // clears the count of the sweep's overflow list (ovf[0]).
 // Sorted copies for the sweep: sP = (position, key bits), sR = (rest
 // position, inverse mass), so one 16-byte load gives a candidate's key and
-// position, and one more its rest position and mass.
+// position, and one more its rest position and mass.  The first sorted index
+// of each group is inserted into the group table (slot recorded in slot_rec,
+// EMPTY_KEY elsewhere, for k_coll_keys to empty next pass).  Also clears the
+// sweep's overflow count.
 __global__ void k_coll_gather(const float4* F, const float4* rest4, const uint32_t* sids, const uint32_t* skeys,
-                              uint32_t n, float4* sP, float4* sR, uint32_t* ovf) {
+                              uint32_t n, float4* sP, float4* sR, uint32_t* ovf, uint2* table, uint32_t mask,
+                              uint32_t* slot_rec) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) {
-        const uint32_t i = sids[s];
-        const float4 f = F[i], r = rest4[i];
-        sP[s] = make_float4(f.x, f.y, f.z, __uint_as_float(skeys[s]));
-        sR[s] = make_float4(r.x, r.y, r.z, f.w);
-    }
     if (ovf && s == 0) ovf[0] = 0u;
+    if (s >= n) return;
+    const uint32_t i = sids[s];
+    const uint32_t key = skeys[s];
+    const float4 f = F[i], r = rest4[i];
+    sP[s] = make_float4(f.x, f.y, f.z, __uint_as_float(key));
+    sR[s] = make_float4(r.x, r.y, r.z, f.w);
+    uint32_t rec = EMPTY_KEY;
+    const uint32_t g = key >> GROUP_SHIFT;
+    if (s == 0 || (skeys[s - 1] >> GROUP_SHIFT) != g) {
+        uint32_t h = slot_of(g, mask);
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+            if (atomicCAS(&table[h].x, EMPTY_KEY, g) == EMPTY_KEY) {
+                table[h].y = s;
+                rec = h;
+                break;
+            }
+            h = (h + 1) & mask;
+        }
+    }
+    slot_rec[s] = rec;
 }
 
 // Standalone predict (xpbd_cpu.rs:251-264): P = predict(X, V).
@@ -1593,10 +1586,10 @@ void launch_coll_grids(const float4* P, const uint32_t* cloth_off, uint32_t nclo
 }
 
 void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
-                      uint32_t* ids, hipStream_t s) {
+                      uint32_t* ids, uint2* table, const uint32_t* slot_rec, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_coll_keys, dim3(std::min(cdiv(n, 256), 8192u)), dim3(256), 0, s, F, n, grids, cloth_of, keys,
-                       ids);
+                       ids, table, slot_rec);
 }
 
 uint32_t radix_hist_words(uint32_t n) { return 256u * std::max(1u, cdiv(n, RADIX_TILE)) + 256u; }
@@ -1620,22 +1613,12 @@ void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32
     launch_radix_pass(keys_alt, ids_alt, keys, ids, n, 24, hist, s);
 }
 
-void launch_coll_table(const uint32_t* skeys, uint32_t n, uint2* table, uint32_t mask, uint32_t* slot_rec,
-                       hipStream_t s) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_coll_table, dim3(cdiv(n, 256)), dim3(256), 0, s, skeys, n, table, mask, slot_rec);
-}
-
-void launch_coll_untable(const uint32_t* skeys, uint32_t n, uint2* table, const uint32_t* slot_rec, hipStream_t s) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_coll_untable, dim3(cdiv(n, 256)), dim3(256), 0, s, skeys, n, table, slot_rec);
-}
-
 void launch_coll_gather(const float4* F, const float4* rest4, const uint32_t* sids, const uint32_t* skeys, uint32_t n,
-                        float4* sP, float4* sR, uint32_t* ovf,
+                        float4* sP, float4* sR, uint32_t* ovf, uint2* table, uint32_t mask, uint32_t* slot_rec,
                         hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_coll_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, F, rest4, sids, skeys, n, sP, sR, ovf);
+    hipLaunchKernelGGL(k_coll_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, F, rest4, sids, skeys, n, sP, sR, ovf, table,
+                       mask, slot_rec);
 }
 
 void launch_coll_sweep(const CollArgs& a, hipStream_t s) {

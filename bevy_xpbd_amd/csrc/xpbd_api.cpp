@@ -77,7 +77,7 @@ struct xpbd_cloth {
     float4 *X = nullptr, *V = nullptr, *A = nullptr, *B = nullptr, *V2 = nullptr, *rest4 = nullptr;
     float4* sP = nullptr;   // collision: positions + cell key in sorted order
     float4* sR = nullptr;   // collision: rest positions + inverse mass in sorted order
-    uint32_t* slot_rec = nullptr;   // collision: table slot of each group, at its first sorted index
+    uint32_t* slot_rec = nullptr;   // collision: table slot filled at each sorted index (EMPTY_KEY: none)
     uint32_t *soff = nullptr, *swid = nullptr, *nbj = nullptr;
     float *nbr = nullptr, *nbc = nullptr;
     uint32_t* nbpk = nullptr;       // packed adjacency (nullptr: plain layout in use)
@@ -266,10 +266,10 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
                        float4* vout) {
     hipStream_t s = h->stream;
     enqueue_grids(h, F, nparts, coll_cell(h), s);
-    launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, s);
+    launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, s);
     launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
-    launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, h->slot_rec, s);
-    launch_coll_gather(F, h->rest4, h->ids, h->keys, h->n, h->sP, h->sR, h->ovf, s);
+    launch_coll_gather(F, h->rest4, h->ids, h->keys, h->n, h->sP, h->sR, h->ovf, h->table, h->table_slots - 1,
+                       h->slot_rec, s);
     CollArgs c{};
     c.F = F;
     c.sP = h->sP;
@@ -286,11 +286,8 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     c.table_mask = h->table_slots - 1;
     c.p = p;
     c.ovf = h->ovf;
-    {
-        ProbeScope ps(h, XPBD_KERNEL_COLLISION_SWEEP);
-        launch_coll_sweep(c, s);
-    }
-    launch_coll_untable(h->keys, h->n, h->table, h->slot_rec, s);
+    ProbeScope ps(h, XPBD_KERNEL_COLLISION_SWEEP);
+    launch_coll_sweep(c, s);
 }
 
 // One substep (cloth_jacobi.rs:702-883 / cloth_compute.rs:1313-1488).  The
@@ -702,6 +699,7 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(dalloc(h, &h->table, h->table_slots));
     HIPC(h, hipMemset(h->table, 0xFF, size_t(h->table_slots) * sizeof(uint2)));   // every slot empty
     TRY(dalloc(h, &h->slot_rec, n));
+    HIPC(h, hipMemset(h->slot_rec, 0xFF, size_t(n) * sizeof(uint32_t)));   // no slot recorded
     TRY(dalloc(h, &h->ovf, size_t(n) + 2));
     HIPC(h, hipMemset(h->ovf, 0, (size_t(n) + 2) * sizeof(uint32_t)));
     h->aabb_slots = std::max<uint32_t>({h->nslices, (h->n + 255) / 256, 4096u, uint32_t(m.cloth_offsets.size()), 1u});
@@ -1250,10 +1248,10 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
     auto prep_collision = [&]() {
         const uint32_t np = h->nclothes == 1 ? launch_aabb_of(h->A, h->n, h->aabb, s) : 0u;
         enqueue_grids(h, h->A, np, coll_cell(h), s);
-        launch_coll_keys(h->A, h->n, h->grids, h->cloth_of, h->keys, h->ids, s);
+        launch_coll_keys(h->A, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, s);
         launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
-        launch_coll_table(h->keys, h->n, h->table, h->table_slots - 1, h->slot_rec, s);
-        launch_coll_gather(h->A, h->rest4, h->ids, h->keys, h->n, h->sP, h->sR, h->ovf, s);
+        launch_coll_gather(h->A, h->rest4, h->ids, h->keys, h->n, h->sP, h->sR, h->ovf, h->table, h->table_slots - 1,
+                           h->slot_rec, s);
     };
     CollArgs c{};
     if (which == XPBD_KERNEL_COLLISION_SWEEP || which == XPBD_KERNEL_RADIX_PASS) {
@@ -1343,9 +1341,6 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
     hipError_t er = hipEventRecord(e0, s);
     for (uint32_t r = 0; r < reps; ++r) one();
     if (er == hipSuccess) er = hipEventRecord(e1, s);
-    // leave the group table empty for the next collision pass
-    if (which == XPBD_KERNEL_COLLISION_SWEEP || which == XPBD_KERNEL_RADIX_PASS)
-        launch_coll_untable(h->keys, h->n, h->table, h->slot_rec, s);
     HIPC(h, er);
     HIPC(h, hipEventSynchronize(e1));
     float ms = 0.f;
@@ -1732,7 +1727,7 @@ XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t*
     hipStream_t s = h->stream;
     const uint32_t np = h->nclothes == 1 ? launch_aabb_of(h->X, h->n, h->aabb, s) : 0u;
     enqueue_grids(h, h->X, np, cell_size, s);
-    launch_coll_keys(h->X, h->n, h->grids, h->cloth_of, h->keys, h->ids, s);
+    launch_coll_keys(h->X, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, s);
     launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
     HIPC(h, hipGetLastError());
     HIPC(h, hipStreamSynchronize(s));
