@@ -4,6 +4,8 @@
   bit-exact against the f32 oracle's frozen trajectory (tests/golden), and
   within the stated trajectory tolerance of the f64 oracle (1e-4 m; the f32
   solver itself is 1.8e-5 m away after 120 frames).
+- 256x256 horizontal sheet (config 2), both solvers, 10 frames; the
+  512-cloth batch (config 4), one frame: bit-exact against the f32 oracle.
 - The error against the f64 oracle is printed for each config run here.
 """
 import hashlib
@@ -72,4 +74,70 @@ def test_config3_one_frame_error_against_f64(O, X):
     x64, _ = s.frames(f["positions"].astype(np.float64), np.zeros(f["positions"].shape), p, 8, 1, 1,
                       dtype=np.float64)
     note(f"  config3 frame 1: max |GPU f32 - oracle f64| = {max_abs(gp, x64):.3e} m")
+    sim.close()
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_config2_ten_frames_error_against_f64(O, X, solver):
+    """256x256 horizontal sheet (centre pin, seeded rotation, self-collision),
+    S=4, I=8, frames replayed from the hipGraph: bit-exact against the f32
+    oracle after 1 and 10 frames; the f64 error is reported and bounded.  The
+    drape is contact-rich and chaotic: the f32 and f64 oracles themselves are
+    ~5e-7 m apart after 1 frame and 6e-6 m (Jacobi) / 2e-3 m (GS) after 10,
+    but 0.76 m (Jacobi) after 30, so a long-horizon f64 tolerance would only
+    state the chaos."""
+    note(f"case config2 ten frames solver={solver}")
+    O.set_threads(16)
+    h = 1 / 255
+    rng = np.random.default_rng(2)
+    kw = dict(cols=256, rows=256, spacing=h, layout=1, height=1.0, angle=float(rng.uniform(0, 2 * np.pi)), pins=3)
+    mesh = X.ClothMeshData.sheet(**kw)
+    cfg = X.ClothSimConfig.default(solve_substeps=4, solve_inner_iterations=8, solver=solver, use_graph=1)
+    u = X.ClothSimUniforms.default(solver, 4, thickness=2 * (0.4 * h), jacobi_omega=0.32 if solver == 0 else 1.0)
+    sim = X.ClothComputePlugin(mesh, cfg, u)
+    f = mesh.fields()
+    s = O.Solver(f)
+    p = oracle_params(O, X, u, cfg, collision=True)
+    x32, v32 = f["positions"].copy(), np.zeros_like(f["positions"])
+    x64, v64 = x32.astype(np.float64), np.zeros(x32.shape)
+    done = 0
+    for frames, bound in ((1, 2e-6), (10, 1e-2)):
+        sim.step_frames(frames - done)
+        x32, v32 = s.frames(x32, v32, p, 4, 1, frames - done)
+        x64, v64 = s.frames(x64, v64, p, 4, 1, frames - done, dtype=np.float64)
+        done = frames
+        gp, _ = sim.read_state()
+        assert bits_equal(gp, x32), f"frame {frames}: GPU differs from the f32 oracle"
+        err = max_abs(gp, x64)
+        note(f"  config2 solver={solver} frame {frames}: max |GPU f32 - oracle f64| = {err:.3e} m")
+        assert err < bound
+    sim.close()
+
+
+def test_config4_batch_one_frame_error_against_f64(O, X):
+    """The 512-cloth uneven batch (seed 4), one frame of 8 substeps x 10
+    Jacobi iterations with per-cloth self-collision, from the hipGraph:
+    bit-exact against the f32 oracle; the f64 error is reported."""
+    from bevy_xpbd_amd.workloads import batch_descs
+    note("case config4 batch one frame")
+    O.set_threads(16)
+    descs, _ = batch_descs(count=512, seed=4, spacing=0.01)
+    mesh = X.ClothMeshData.batch(descs)
+    cfg = X.ClothSimConfig.default(solve_substeps=8, solve_inner_iterations=10, use_graph=1)
+    u = X.ClothSimUniforms.default(X.SOLVER_JACOBI, 8, thickness=2 * (0.4 * 0.01))
+    sim = X.ClothComputePlugin(mesh, cfg, u)
+    f = mesh.fields()
+    s = O.Solver(f)
+    p = oracle_params(O, X, u, cfg, collision=True)
+    sim.step_frames(1)
+    gp, _ = sim.read_state()
+    note("  oracle f32")
+    x32, _ = s.frames(f["positions"].copy(), np.zeros_like(f["positions"]), p, 8, 1, 1)
+    assert bits_equal(gp, x32)
+    note("  oracle f64")
+    x64, _ = s.frames(f["positions"].astype(np.float64), np.zeros(f["positions"].shape), p, 8, 1, 1,
+                      dtype=np.float64)
+    err = max_abs(gp, x64)
+    note(f"  config4 frame 1: max |GPU f32 - oracle f64| = {err:.3e} m")
+    assert err < 1e-4
     sim.close()
