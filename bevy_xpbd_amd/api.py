@@ -86,6 +86,7 @@ class ClothSimConfig(C.Structure):
         ("collision_cell", C.c_float), ("use_graph", C.c_uint32), ("autotune", C.c_uint32),
         ("jacobi_block", C.c_uint32), ("fused_predict", C.c_uint32), ("plain_adjacency", C.c_uint32),
         ("fast_division", C.c_uint32), ("xcd_remap", C.c_uint32), ("persistent_jacobi", C.c_uint32),
+        ("grid_tiles", C.c_uint32),
     ]
 
     @classmethod
@@ -166,6 +167,7 @@ def lib():
             "xpbd_time_kernel": (i32, [vp, i32, u32, C.POINTER(C.c_float)]),
             "xpbd_profile_frames": (i32, [vp, u32, i32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
             "xpbd_debug_collision_keys": (i32, [vp, f32, vp, vp, vp]),
+            "xpbd_get_layout": (i32, [vp, vp]),
             "xpbd_selftest_division": (i32, [i32, i32, u64, u64, vp]),
             "xpbd_selftest_sqrt": (i32, [i32, i32, vp]),
             "xpbd_mlp_create": (i32, [u64, pp]),
@@ -420,6 +422,15 @@ class ClothComputePlugin:
         _check(lib().xpbd_warm_stats(self._h, C.byref(sub), C.byref(fb), 1 if reset else 0), "xpbd_warm_stats",
                self._h)
         return sub.value, fb.value
+
+    def layout(self) -> dict:
+        """Adjacency layout chosen at create: "plain", "packed" or "grid_tiles"
+        (with the tile grid's columns and rows), and whether the fast
+        correctly rounded division is active."""
+        out = (C.c_uint32 * 4)()
+        _check(lib().xpbd_get_layout(self._h, out), "xpbd_get_layout", self._h)
+        return {"adjacency": ("plain", "packed", "grid_tiles")[out[0]], "tile_cols": int(out[1]),
+                "tile_rows": int(out[2]), "fast_division": bool(out[3])}
 
     def debug_collision_keys(self, cell_size: float, num_cloths: int = 1):
         grid = np.zeros((num_cloths, 12), np.uint32)

@@ -389,6 +389,115 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
 }
 
+// ---------------------------------------------------------------------------
+// Grid-tiled Jacobi iteration (one grid cloth, columns a multiple of 64).  A
+// block owns a 64 x JT_ROWS tile of the grid (one wavefront per row = one
+// 64-particle slice) and first stages the tile's positions with a 2-particle
+// halo (bend constraints reach 2) in LDS with coalesced loads; every
+// neighbour gather then reads LDS instead of L2.  The entries are the same
+// CSR entries in the same order (their offsets re-encoded for the tile), so
+// the sums, and the results, are those of k_jacobi.
+// ---------------------------------------------------------------------------
+template <uint32_t MODE, bool FAST>
+__device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, const float2* lpairs, bool lds,
+                                                        const float4* tl, uint32_t slice, uint32_t lane,
+                                                        uint32_t lidx, bool row_ok, bool& valid) {
+    const uint32_t i = slice * 64 + lane;
+    const DevParams& p = a.p;
+    valid = row_ok && slice < p.nslices && i < p.n;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 out = pi;
+    if (!valid) return out;
+    pi = tl[lidx];
+    out = pi;
+    const float wi = pi.w;
+    if (wi > 0.f) {
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        const uint32_t base = a.sell.soff[slice] + lane;
+        const uint32_t width = a.sell.swid[slice];
+        uint32_t k0 = 0;
+        if (width >= JU) {
+            // the next chunk's entry words load while this chunk computes
+            uint32_t wn[JU];
+#pragma unroll
+            for (uint32_t u = 0; u < JU; ++u) wn[u] = __builtin_nontemporal_load(a.sell.tpacked + base + u * 64);
+            for (; k0 + JU <= width; k0 += JU) {
+                uint32_t wc[JU];
+#pragma unroll
+                for (uint32_t u = 0; u < JU; ++u) wc[u] = wn[u];
+                if (k0 + 2 * JU <= width) {
+#pragma unroll
+                    for (uint32_t u = 0; u < JU; ++u)
+                        wn[u] = __builtin_nontemporal_load(a.sell.tpacked + base + (k0 + JU + u) * 64);
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < JU; ++u) {
+                    const float4 pj = tl[int32_t(lidx) + int32_t(int16_t(wc[u] & 0xFFFFu))];
+                    const float2 rc = lds ? lpairs[wc[u] >> 16] : a.sell.pairs[wc[u] >> 16];
+                    jacobi_entry<FAST>(pi, pj, rc.x, rc.y, p, ax, ay, az);
+                }
+            }
+        }
+        for (; k0 < width; ++k0) {
+            const uint32_t w = __builtin_nontemporal_load(a.sell.tpacked + base + k0 * 64);
+            const float4 pj = tl[int32_t(lidx) + int32_t(int16_t(w & 0xFFFFu))];
+            const float2 rc = lds ? lpairs[w >> 16] : a.sell.pairs[w >> 16];
+            jacobi_entry<FAST>(pi, pj, rc.x, rc.y, p, ax, ay, az);
+        }
+        float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
+        const float ml = len3(dx, dy, dz);
+        if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
+            const float sc = JACOBI_CORRECTION_CAP / ml;
+            dx = dx * sc;
+            dy = dy * sc;
+            dz = dz * sc;
+        }
+        out = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, wi);
+    }
+    if (MODE & JM_FINAL) {
+        if (pi.w > 0.f) finish_particle(out, a.dst[i], i, a.ground, p, a.dst, a.vout);
+    } else {
+        a.dst[i] = out;
+    }
+    return out;
+}
+
+template <uint32_t MODE, bool FASTDIV>
+__global__ __launch_bounds__(64 * JT_ROWS) __attribute__((amdgpu_waves_per_eu(8))) void k_jacobi_tiled(JacobiArgs a) {
+    if (a.gate && *a.gate == 0u) return;   // uniform over the grid
+    __shared__ float2 lpairs[LDS_PAIRS_MAX];
+    __shared__ float4 tl[JT_H * JT_W];
+    const uint32_t cols = a.sell.tcols, rows = a.sell.trows, ntx = cols / 64;
+    const uint32_t b = a.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t tx = b % ntx, ty = b / ntx;
+    const int32_t r0 = int32_t(ty * JT_ROWS) - 2, c0 = int32_t(tx * 64) - 2;
+    for (uint32_t q = threadIdx.x; q < JT_H * JT_W; q += blockDim.x) {
+        const int32_t r = r0 + int32_t(q / JT_W), c = c0 + int32_t(q % JT_W);
+        if (r >= 0 && r < int32_t(rows) && c >= 0 && c < int32_t(cols)) {
+            const uint32_t id = uint32_t(r) * cols + uint32_t(c);
+            tl[q] = (MODE & JM_PREDICT) ? predict(a.src[id], a.vel[id], id, a.p) : a.src[id];
+        }
+    }
+    const bool lds = stage_pairs<true>(a, lpairs);   // its barrier also publishes the tile
+    if (!lds) __syncthreads();
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t row = ty * JT_ROWS + wave;
+    const bool row_ok = row < rows;
+    const uint32_t slice = row_ok ? row * ntx + tx : a.p.nslices;
+    const uint32_t lidx = (wave + 2) * JT_W + lane + 2;
+    bool valid;
+    const bool fast = FASTDIV && neighbourhood_tame(a.tame_in, a.srange, slice, a.p.nslices);
+    const float4 out = fast ? jacobi_particle_tiled<MODE, true>(a, lpairs, lds, tl, slice, lane, lidx, row_ok, valid)
+                            : jacobi_particle_tiled<MODE, false>(a, lpairs, lds, tl, slice, lane, lidx, row_ok, valid);
+    if (!(MODE & JM_FINAL) && a.tame_out)
+        write_slice_tame(a.tame_out, slice, slice < a.p.nslices, !valid || tame3(out));
+    if (MODE & JM_AABB) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        aabb_fold(valid, out.x, out.y, out.z, lo, hi);
+        block_aabb_store(lo, hi, a.aabb, b);
+    }
+}
+
 // Substep end without a collision pass when the last iterate is in F.
 __global__ void k_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1426,7 +1535,29 @@ static void launch_jacobi_t(uint32_t mode, const JacobiArgs& a, uint32_t blocks,
     }
 }
 
+template <bool FASTDIV>
+static void launch_jacobi_tiled_t(uint32_t mode, const JacobiArgs& a, uint32_t blocks, hipStream_t s) {
+    const dim3 g(blocks), b(64 * JT_ROWS);
+    switch (mode) {
+        case 0: hipLaunchKernelGGL((k_jacobi_tiled<0, FASTDIV>), g, b, 0, s, a); break;
+        case JM_PREDICT: hipLaunchKernelGGL((k_jacobi_tiled<JM_PREDICT, FASTDIV>), g, b, 0, s, a); break;
+        case JM_FINAL: hipLaunchKernelGGL((k_jacobi_tiled<JM_FINAL, FASTDIV>), g, b, 0, s, a); break;
+        case JM_AABB: hipLaunchKernelGGL((k_jacobi_tiled<JM_AABB, FASTDIV>), g, b, 0, s, a); break;
+        case JM_PREDICT | JM_AABB:
+            hipLaunchKernelGGL((k_jacobi_tiled<JM_PREDICT | JM_AABB, FASTDIV>), g, b, 0, s, a);
+            break;
+        default: break;
+    }
+}
+
 uint32_t launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s) {
+    if (a.sell.tpacked && a.sell.tcols >= 64 && a.sell.tcols % 64 == 0) {   // grid tiles: one 64 x 8 tile per block
+        const uint32_t blocks = (a.sell.tcols / 64) * cdiv(a.sell.trows, JT_ROWS);
+        if (blocks == 0) return 0;
+        if (a.fastdiv) launch_jacobi_tiled_t<true>(mode, a, blocks, s);
+        else launch_jacobi_tiled_t<false>(mode, a, blocks, s);
+        return blocks;
+    }
     const uint32_t blocks = jacobi_blocks(a.p.nslices, block);
     if (blocks == 0) return 0;
     if (a.sell.packed) {

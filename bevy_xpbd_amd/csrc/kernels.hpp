@@ -37,7 +37,18 @@ struct Sell {
     // looked up in a table of the mesh's distinct pairs; padding is 0.
     const uint32_t* packed;
     const float2* pairs;
+    // Grid tiles (one grid cloth, tcols a multiple of 64): the packed entries
+    // with the low 16 bits replaced by the neighbour's offset in the block's
+    // LDS tile (dy * JT_W + dx); nullptr: no tiled path
+    const uint32_t* tpacked;
+    uint32_t tcols, trows;
 };
+
+// LDS tile of the grid-tiled Jacobi: JT_ROWS rows of 64 particles (one
+// wavefront each) and a 2-particle halo on every side
+constexpr uint32_t JT_ROWS = 8;
+constexpr uint32_t JT_W = 64 + 4;
+constexpr uint32_t JT_H = JT_ROWS + 4;
 
 enum JacobiMode : uint32_t {
     JM_PREDICT = 1,   // inputs are the substep-start state; predict on the fly

@@ -81,6 +81,8 @@ struct xpbd_cloth {
     uint32_t *soff = nullptr, *swid = nullptr, *nbj = nullptr;
     float *nbr = nullptr, *nbc = nullptr;
     uint32_t* nbpk = nullptr;       // packed adjacency (nullptr: plain layout in use)
+    uint32_t* nbtile = nullptr;     // packed adjacency with LDS-tile offsets (nullptr: no grid tiles)
+    uint32_t tile_cols = 0, tile_rows = 0;
     float2* pairs = nullptr;
     uint32_t num_pairs = 0;
     uint8_t* ground = nullptr;
@@ -222,7 +224,7 @@ uint8_t* tame_slot(xpbd_cloth* h, const float4* buf) {
 
 JacobiArgs jacobi_args(const xpbd_cloth* h, const DevParams& p) {
     JacobiArgs a{};
-    a.sell = Sell{h->soff, h->swid, h->nbj, h->nbr, h->nbc, h->nbpk, h->pairs};
+    a.sell = Sell{h->soff, h->swid, h->nbj, h->nbr, h->nbc, h->nbpk, h->pairs, h->nbtile, h->tile_cols, h->tile_rows};
     a.ground = h->has_ground ? h->ground : nullptr;
     a.aabb = h->aabb;
     a.p = p;
@@ -580,6 +582,37 @@ bool pack_sell(const Mesh& m, const std::vector<uint32_t>& soff, std::vector<uin
     return true;
 }
 
+// Grid tiles: for one grid cloth with a column count C that is a multiple of
+// 64 (so a 64-particle slice is 64 consecutive columns of one row), each
+// packed entry's neighbour offset j - i = dy * C + dx (|dx|, |dy| <= 2, the
+// neighbour inside the cloth) is re-encoded as dy * JT_W + dx, its offset in
+// the block's LDS tile.  False (no tiled path) if any entry does not fit.
+bool tile_sell(const Mesh& m, const std::vector<uint32_t>& soff, const std::vector<uint32_t>& packed,
+               std::vector<uint32_t>& tiled, uint32_t& cols, uint32_t& rows) {
+    const size_t n = m.num_particles();
+    if (m.cloth_offsets.size() != 2 || m.cloth_cols.size() != 1) return false;
+    const int64_t c = m.cloth_cols[0];
+    if (c < 64 || c % 64 != 0 || n % size_t(c) != 0 || n == 0) return false;
+    tiled.assign(packed.size(), 0u);
+    for (size_t i = 0; i < n; ++i) {
+        const size_t s = i / 64, l = i % 64;
+        const uint32_t d = m.neighbor_offsets[i + 1] - m.neighbor_offsets[i];
+        const int64_t col = int64_t(i) % c, row = int64_t(i) / c;
+        for (uint32_t k = 0; k < d; ++k) {
+            const uint32_t w = packed[soff[s] + size_t(k) * 64 + l];
+            const int64_t delta = int64_t(int16_t(uint16_t(w & 0xFFFFu)));
+            const int64_t j = int64_t(i) + delta, jr = j / c, jc = j % c;
+            const int64_t dy = jr - row, dx = jc - col;
+            if (dx < -2 || dx > 2 || dy < -2 || dy > 2) return false;
+            const int64_t off = dy * int64_t(JT_W) + dx;
+            tiled[soff[s] + size_t(k) * 64 + l] = (w & 0xFFFF0000u) | (uint32_t(off) & 0xFFFFu);
+        }
+    }
+    cols = uint32_t(c);
+    rows = uint32_t(n / size_t(c));
+    return true;
+}
+
 int validate_mesh(const Mesh& m, std::string& why) {
     const size_t n = m.num_particles();
     if (m.positions.size() != 3 * n || m.rest_positions.size() != 3 * n) return why = "position arrays", XPBD_E_INVALID;
@@ -657,6 +690,14 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
             TRY(upload(h, h->nbpk, pk.data(), pk.size()));
             TRY(upload(h, h->pairs, pairs.data(), pairs.size()));
             h->num_pairs = uint32_t(pairs.size());
+            std::vector<uint32_t> tk;
+            uint32_t tc = 0, tr = 0;
+            if (h->cfg.grid_tiles && tile_sell(m, soff, pk, tk, tc, tr)) {
+                TRY(dalloc(h, &h->nbtile, tk.size()));
+                TRY(upload(h, h->nbtile, tk.data(), tk.size()));
+                h->tile_cols = tc;
+                h->tile_rows = tr;
+            }
         }
     }
 
@@ -1043,6 +1084,7 @@ XPBD_API void xpbd_sim_config_default(xpbd_sim_config* out) {
     c.jacobi_block = 0;
     c.fused_predict = 0;
     c.fast_division = 1;
+    c.grid_tiles = 1;
     *out = c;
 }
 
@@ -1719,6 +1761,15 @@ XPBD_API int xpbd_selftest_sqrt(int exp_lo, int exp_hi, uint32_t* out2) {
     if (s) (void)hipStreamDestroy(s);
     (void)hipFree(d);
     return rc;
+}
+
+XPBD_API int xpbd_get_layout(xpbd_cloth* h, uint32_t* out4) {
+    if (!h || !out4) return XPBD_E_INVALID;
+    out4[0] = h->nbtile ? XPBD_ADJ_GRID_TILES : (h->nbpk ? XPBD_ADJ_PACKED : XPBD_ADJ_PLAIN);
+    out4[1] = h->tile_cols;
+    out4[2] = h->tile_rows;
+    out4[3] = fast_division_active(h) ? 1u : 0u;
+    return XPBD_OK;
 }
 
 XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t* grid_out, uint32_t* sorted_keys,

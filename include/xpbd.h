@@ -149,6 +149,11 @@ typedef struct xpbd_sim_config {
     uint32_t persistent_jacobi;           /* 1: predict + all I Jacobi iterations of a
                                              substep in one co-resident launch with grid
                                              barriers (same results) */
+    uint32_t grid_tiles;                  /* 1 (default): a single grid cloth whose
+                                             column count is a multiple of 64 runs the
+                                             Jacobi gather on 64x8 tiles staged in LDS
+                                             with a 2-particle halo (read at create;
+                                             same results) */
 } xpbd_sim_config;
 
 /* ClothSimUniforms (cloth_compute.rs:182-202). */
@@ -245,6 +250,15 @@ XPBD_API int xpbd_profile_frames(xpbd_cloth* h, uint32_t frames, int which, floa
  * inv_cell (floats), dims[3], key base (u32). */
 XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t* grid_out,
                                        uint32_t* sorted_keys, uint32_t* sorted_ids);
+
+/* Layout the handle chose at create (no reference counterpart; for tests and
+ * tools): out4 = {adjacency: 0 plain 12-byte entries, 1 packed 4-byte entries,
+ * 2 packed entries on LDS grid tiles; grid-tile columns; grid-tile rows;
+ * 1 if the fast correctly rounded division is active}. */
+#define XPBD_ADJ_PLAIN 0
+#define XPBD_ADJ_PACKED 1
+#define XPBD_ADJ_GRID_TILES 2
+XPBD_API int xpbd_get_layout(xpbd_cloth* h, uint32_t* out4);
 
 /* Device self-test of the shared-reciprocal division used by the Jacobi
  * gather (no reference counterpart; a numerics check of this library):

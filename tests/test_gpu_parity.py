@@ -22,23 +22,26 @@ def note(msg):
 
 
 def make(X, mesh, solver=0, S=8, I=10, coll_stride=1, use_graph=0, fused_predict=0, jacobi_block=0,
-         plain_adjacency=0, fast_division=0, persistent_jacobi=0, **uk):
+         plain_adjacency=0, fast_division=0, persistent_jacobi=0, grid_tiles=1, **uk):
     cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=I, solver=solver,
                                    collision_every_n_substeps=coll_stride, use_graph=use_graph,
                                    fused_predict=fused_predict, jacobi_block=jacobi_block,
                                    plain_adjacency=plain_adjacency, fast_division=fast_division,
-                                   persistent_jacobi=persistent_jacobi)
+                                   persistent_jacobi=persistent_jacobi, grid_tiles=grid_tiles)
     u = X.ClothSimUniforms.default(solver, S, **uk)
     return X.ClothComputePlugin(mesh, cfg, u), cfg, u
 
 
 def run_pair(O, X, mesh_kw=None, mesh=None, solver=0, S=8, I=10, substeps=8, collision=False, fused_predict=0,
-             jacobi_block=0, plain_adjacency=0, fast_division=0, persistent_jacobi=0, **uk):
+             jacobi_block=0, plain_adjacency=0, fast_division=0, persistent_jacobi=0, grid_tiles=1, layout=None,
+             **uk):
     mesh = mesh or X.ClothMeshData.sheet(**mesh_kw)
     f = mesh.fields()
     sim, cfg, u = make(X, mesh, solver, S, I, fused_predict=fused_predict, jacobi_block=jacobi_block,
                        plain_adjacency=plain_adjacency, fast_division=fast_division,
-                       persistent_jacobi=persistent_jacobi, **uk)
+                       persistent_jacobi=persistent_jacobi, grid_tiles=grid_tiles, **uk)
+    if layout is not None:
+        assert sim.layout()["adjacency"] == layout, sim.layout()
     s = O.Solver(f)
     p = oracle_params(O, X, u, cfg, collision=collision)
     pos = f["positions"].copy()
@@ -96,6 +99,33 @@ def test_jacobi_block_sizes(O, X, block, fast_division):
     note(f"case block={block} fast_division={fast_division}")
     run_pair(O, X, dict(cols=70, rows=33, spacing=0.03, pins=2, jitter=1e-3, seed=8), S=8, I=5, substeps=3,
              collision=True, jacobi_block=block, fast_division=fast_division)
+
+
+@pytest.mark.parametrize("cols,rows,fast,fused", [(64, 37, 0, 0), (64, 37, 1, 0), (128, 16, 1, 1), (128, 16, 0, 1)])
+def test_grid_tiles_bit_exact(O, X, cols, rows, fast, fused):
+    """Grid-tiled Jacobi (64 x 8 LDS tiles with a 2-particle halo): columns a
+    multiple of 64, a partial last band of tiles (37 rows), fused predict
+    (staged predicted positions), both division forms, with collision."""
+    note(f"case grid tiles {cols}x{rows} fast={fast} fused={fused}")
+    kw = dict(cols=cols, rows=rows, spacing=0.02, pins=2, jitter=1e-3, seed=cols + rows)
+    run_pair(O, X, kw, S=8, I=5, substeps=3, collision=True, fast_division=fast, fused_predict=fused,
+             thickness=0.016, layout="grid_tiles")
+    run_pair(O, X, kw, S=8, I=5, substeps=1, collision=True, fast_division=fast, fused_predict=fused,
+             thickness=0.016, grid_tiles=0, layout="packed")
+
+
+def test_grid_tiles_only_where_they_fit(X):
+    """Tiles need one grid cloth whose column count is a multiple of 64."""
+    note("case grid tiles eligibility")
+    cases = [(dict(cols=64, rows=9), "grid_tiles"), (dict(cols=70, rows=9), "packed"),
+             (dict(cols=32, rows=32), "packed")]
+    for kw, want in cases:
+        sim, _, _ = make(X, X.ClothMeshData.sheet(spacing=0.02, pins=2, **kw))
+        assert sim.layout()["adjacency"] == want, (kw, sim.layout())
+        sim.close()
+    sim, _, _ = make(X, X.ClothMeshData.sheet(cols=64, rows=9, spacing=0.02, pins=2), plain_adjacency=1)
+    assert sim.layout()["adjacency"] == "plain"
+    sim.close()
 
 
 def test_collision_dense_cluster_overflow_path(O, X):
