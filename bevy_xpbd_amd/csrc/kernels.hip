@@ -138,14 +138,17 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
 // square root exhaustively over the exponents used, the quotient on random
 // operands across the operand ranges below).
 //
-// When they apply: every coordinate of the slices an entry reads is 0 or of
-// magnitude in [2^-50, 2^20] (tracked per slice on the device), and the host
-// has checked inverse masses <= 2^40, compliance / dt^2 <= 2^40 and rest
-// lengths 0 or in [2^-30, 2^30].  Then:
-// - a difference component is 0 or in [2^-73, 2^21];
-// - a squared length below 2^-100 gives len < 2^-50, skipped by the 1e-8
-//   test on either path, so it is taken as 0 (the hardware root is within one
-//   ulp from about 2^-101 up; it is not below); above, the root is exact;
+// When they apply: every coordinate of the slices an entry reads is +0 or of
+// magnitude in [2^-50, 2^20] (tracked per slice on the device; -0 is not
+// tame), and the host has checked inverse masses <= 2^40, compliance / dt^2
+// <= 2^40 and rest lengths 0 or in [2^-30, 2^30].  Then:
+// - a difference component is +0 (never -0: x - y = -0 needs x = -0) or of
+//   magnitude in [2^-73, 2^21], so its quotient by len needs no sign-of-zero
+//   fix-up (div_rn_nz);
+// - a squared length below 2^-100 has a root below 2^-50, which the hardware
+//   root (within one ulp from about 2^-101 up, not exact below) keeps far
+//   under 1e-8: the entry is skipped by the 1e-8 test on either path; above,
+//   the root is exact;
 // - past the 1e-8 test, len is in [1e-8, 2^22]; -C is 0 or in [2^-54, 2^30]
 //   (a multiple of the smaller ulp of len and rest); the weight sum is in
 //   [1e-8, 2^42];
@@ -162,7 +165,13 @@ __device__ __forceinline__ float div_rn(float a, float b, float y) {
     const float v = __builtin_fmaf(r, y, q);
     return q == 0.f ? q : v;
 }
-__device__ __forceinline__ float sqrt_rn(float x) {   // x == 0 or normal positive
+// a != -0 (see above): a = +0 gives q = +0, r = +0 and +0; otherwise q != 0
+__device__ __forceinline__ float div_rn_nz(float a, float b, float y) {
+    const float q = a * y;
+    const float r = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(r, y, q);
+}
+__device__ __forceinline__ float sqrt_rn(float x) {   // exact for x == 0 or x >= 2^-100
     float s = __builtin_amdgcn_sqrtf(x);
     const float sd = __uint_as_float(__float_as_uint(s) - 1u);
     const float su = __uint_as_float(__float_as_uint(s) + 1u);
@@ -174,10 +183,9 @@ __device__ __forceinline__ float sqrt_rn(float x) {   // x == 0 or normal positi
 }
 constexpr float TAME_LO = 8.881784197001252e-16f;   // 2^-50
 constexpr float TAME_HI = 1048576.0f;               // 2^20
-constexpr float SQRT_FAST_MIN = 7.888609052210118e-31f;   // 2^-100
 __device__ __forceinline__ bool tame_coord(float c) {
     const float a = fabsf(c);
-    return a == 0.f || (a >= TAME_LO && a <= TAME_HI);
+    return __float_as_uint(c) == 0u || (a >= TAME_LO && a <= TAME_HI);
 }
 __device__ __forceinline__ bool tame3(float4 v) { return tame_coord(v.x) && tame_coord(v.y) && tame_coord(v.z); }
 
@@ -205,29 +213,28 @@ __device__ __forceinline__ bool neighbourhood_tame(const uint8_t* tame, const ui
 }
 
 // One constraint entry of the gather: accumulates (n * w_i) * dlambda in the
-// reference's order of operations (xpbd_cpu.rs:159-182, 206-222).  FAST: the
-// input is tame (see above) and the shorter correctly rounded operations are
-// exact; otherwise the compiler's general ones.
+// reference's order of operations (xpbd_cpu.rs:159-182, 206-222).  Called only
+// for w_i > 0, so the reference's both-pinned skip never applies.  pr is the
+// entry's pair as staged: (rest, alpha~ = compliance / dt^2, alpha~ * 0).
+// FAST: the input is tame (see above) and the shorter correctly rounded
+// operations are exact; otherwise the compiler's general ones.
 template <bool FAST>
-__device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float rest, float comp, const DevParams& p,
-                                             float& ax, float& ay, float& az) {
+__device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float4 pr, float& ax, float& ay, float& az) {
     const float wi = pi.w, wj = pj.w;
-    if (wj <= 0.f && wi <= 0.f) return;
     float gx = pi.x - pj.x, gy = pi.y - pj.y, gz = pi.z - pj.z;
     const float dot = (gx * gx + gy * gy) + gz * gz;
-    const float len = FAST ? (dot < SQRT_FAST_MIN ? 0.f : sqrt_rn(dot)) : sqrtf(dot);
+    const float len = FAST ? sqrt_rn(dot) : sqrtf(dot);
     if (len < 1e-8f) return;
-    const float c = len - rest;
-    const float alpha_t = comp * p.inv_dt_sq;
-    const float wsum = wi + wj + alpha_t;
+    const float c = len - pr.x;
+    const float wsum = wi + wj + pr.y;
     if (wsum < 1e-8f) return;
-    const float num = -c - alpha_t * 0.0f;
+    const float num = -c - pr.z;
     float dlam;
     if (FAST) {
         const float yl = rcp_rn(len);
-        gx = div_rn(gx, len, yl);
-        gy = div_rn(gy, len, yl);
-        gz = div_rn(gz, len, yl);
+        gx = div_rn_nz(gx, len, yl);
+        gy = div_rn_nz(gy, len, yl);
+        gz = div_rn_nz(gz, len, yl);
         dlam = div_rn(num, wsum, rcp_rn(wsum));
     } else {
         gx = gx / len;
@@ -240,27 +247,45 @@ __device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float rest, f
     az = az + (gz * wi) * dlam;
 }
 
+// (rest, compliance) -> the staged pair (rest, alpha~, alpha~ * 0): the same
+// float operations the reference does per constraint, done once per pair.
+__device__ __forceinline__ float4 stage_pair(float2 rc, const DevParams& p) {
+    const float alpha_t = rc.y * p.inv_dt_sq;
+    return make_float4(rc.x, alpha_t, alpha_t * 0.0f, 0.f);
+}
+
 // Entries are processed JU at a time: all adjacency loads of the chunk, then
 // all neighbour gathers, then the arithmetic in entry order — several load
 // chains in flight per wavefront.  Padding entries point at the particle
 // itself (zero length: skipped by the reference's len < 1e-8 rule), so no
 // load is conditional.  The slice width is wavefront-uniform.
 constexpr uint32_t JU = 6;
-constexpr uint32_t LDS_PAIRS_MAX = 1024;   // (rest, compliance) table staged in LDS up to this size
+constexpr uint32_t LDS_PAIRS_MAX = 1024;   // pair table staged in LDS up to this size
+
+// The staged pair of packed entry word w.  Two LDS table forms: (rest,
+// compliance) as given (fewer registers: the per-slice kernels at the 64-VGPR
+// cap), or already staged (two fewer operations per entry: the tiled kernel).
+__device__ __forceinline__ float4 entry_pair(const JacobiArgs& a, const float2* lpairs, bool lds, uint32_t w) {
+    return stage_pair(lds ? lpairs[w >> 16] : a.sell.pairs[w >> 16], a.p);
+}
+__device__ __forceinline__ float4 entry_pair(const JacobiArgs& a, const float4* lpairs, bool lds, uint32_t w) {
+    return lds ? lpairs[w >> 16] : stage_pair(a.sell.pairs[w >> 16], a.p);
+}
+__device__ __forceinline__ void to_lds(float2& dst, float2 rc, const DevParams&) { dst = rc; }
+__device__ __forceinline__ void to_lds(float4& dst, float2 rc, const DevParams& p) { dst = stage_pair(rc, p); }
 
 template <bool PACKED>
 __device__ __forceinline__ void load_entry(const JacobiArgs& a, const float2* lpairs, bool lds, uint32_t i, uint32_t e,
-                                           uint32_t& j, float& rest, float& comp) {
+                                           uint32_t& j, float4& pr) {
     if (PACKED) {
         const uint32_t w = __builtin_nontemporal_load(a.sell.packed + e);
         j = uint32_t(int32_t(i) + int32_t(int16_t(w & 0xFFFFu)));
-        const float2 rc = lds ? lpairs[w >> 16] : a.sell.pairs[w >> 16];
-        rest = rc.x;
-        comp = rc.y;
+        pr = entry_pair(a, lpairs, lds, w);
     } else {
         j = __builtin_nontemporal_load(a.sell.j + e);
-        rest = __builtin_nontemporal_load(a.sell.rest + e);
-        comp = __builtin_nontemporal_load(a.sell.comp + e);
+        const float rest = __builtin_nontemporal_load(a.sell.rest + e);
+        const float comp = __builtin_nontemporal_load(a.sell.comp + e);
+        pr = stage_pair(make_float2(rest, comp), a.p);
     }
 }
 
@@ -309,32 +334,29 @@ __device__ __forceinline__ float4 jacobi_particle(const JacobiArgs& a, const flo
                     pp[u] = a.src[jj[u]];
                 }
 #pragma unroll
-                for (uint32_t u = 0; u < JU; ++u) {
-                    const float2 rc = lds ? lpairs[wc[u] >> 16] : a.sell.pairs[wc[u] >> 16];
-                    jacobi_entry<FASTDIV>(pi, pp[u], rc.x, rc.y, p, ax, ay, az);
-                }
+                for (uint32_t u = 0; u < JU; ++u) jacobi_entry<FASTDIV>(pi, pp[u], entry_pair(a, lpairs, lds, wc[u]), ax, ay, az);
             }
         }
         for (; k0 + JU <= width; k0 += JU) {
             uint32_t jj[JU];
-            float rr[JU], cc[JU];
+            float4 pr[JU];
             float4 pp[JU];
 #pragma unroll
-            for (uint32_t u = 0; u < JU; ++u) load_entry<PACKED>(a, lpairs, lds, i, base + (k0 + u) * 64, jj[u], rr[u], cc[u]);
+            for (uint32_t u = 0; u < JU; ++u) load_entry<PACKED>(a, lpairs, lds, i, base + (k0 + u) * 64, jj[u], pr[u]);
 #pragma unroll
             for (uint32_t u = 0; u < JU; ++u) {
                 if (MODE & JM_PREDICT) pp[u] = predict(a.src[jj[u]], a.vel[jj[u]], jj[u], p);
                 else pp[u] = a.src[jj[u]];
             }
 #pragma unroll
-            for (uint32_t u = 0; u < JU; ++u) jacobi_entry<FASTDIV>(pi, pp[u], rr[u], cc[u], p, ax, ay, az);
+            for (uint32_t u = 0; u < JU; ++u) jacobi_entry<FASTDIV>(pi, pp[u], pr[u], ax, ay, az);
         }
         for (; k0 < width; ++k0) {
             uint32_t j;
-            float rest, comp;
-            load_entry<PACKED>(a, lpairs, lds, i, base + k0 * 64, j, rest, comp);
+            float4 pr;
+            load_entry<PACKED>(a, lpairs, lds, i, base + k0 * 64, j, pr);
             const float4 pj = (MODE & JM_PREDICT) ? predict(a.src[j], a.vel[j], j, p) : a.src[j];
-            jacobi_entry<FASTDIV>(pi, pj, rest, comp, p, ax, ay, az);
+            jacobi_entry<FASTDIV>(pi, pj, pr, ax, ay, az);
         }
         float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
         const float ml = len3(dx, dy, dz);
@@ -354,11 +376,11 @@ __device__ __forceinline__ float4 jacobi_particle(const JacobiArgs& a, const flo
     return out;
 }
 
-template <bool PACKED>
-__device__ __forceinline__ bool stage_pairs(const JacobiArgs& a, float2* lpairs) {
+template <bool PACKED, typename PT>
+__device__ __forceinline__ bool stage_pairs(const JacobiArgs& a, PT* lpairs) {
     const bool lds = PACKED && a.num_pairs <= LDS_PAIRS_MAX;
     if (lds) {
-        for (uint32_t q = threadIdx.x; q < a.num_pairs; q += blockDim.x) lpairs[q] = a.sell.pairs[q];
+        for (uint32_t q = threadIdx.x; q < a.num_pairs; q += blockDim.x) to_lds(lpairs[q], a.sell.pairs[q], a.p);
         __syncthreads();
     }
     return lds;
@@ -399,7 +421,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
 // the sums, and the results, are those of k_jacobi.
 // ---------------------------------------------------------------------------
 template <uint32_t MODE, bool FAST>
-__device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, const float2* lpairs, bool lds,
+__device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, const float4* lpairs, bool lds,
                                                         const float4* tl, uint32_t slice, uint32_t lane,
                                                         uint32_t lidx, bool row_ok, bool& valid) {
     const uint32_t i = slice * 64 + lane;
@@ -433,16 +455,14 @@ __device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, con
 #pragma unroll
                 for (uint32_t u = 0; u < JU; ++u) {
                     const float4 pj = tl[int32_t(lidx) + int32_t(int16_t(wc[u] & 0xFFFFu))];
-                    const float2 rc = lds ? lpairs[wc[u] >> 16] : a.sell.pairs[wc[u] >> 16];
-                    jacobi_entry<FAST>(pi, pj, rc.x, rc.y, p, ax, ay, az);
+                    jacobi_entry<FAST>(pi, pj, entry_pair(a, lpairs, lds, wc[u]), ax, ay, az);
                 }
             }
         }
         for (; k0 < width; ++k0) {
             const uint32_t w = __builtin_nontemporal_load(a.sell.tpacked + base + k0 * 64);
             const float4 pj = tl[int32_t(lidx) + int32_t(int16_t(w & 0xFFFFu))];
-            const float2 rc = lds ? lpairs[w >> 16] : a.sell.pairs[w >> 16];
-            jacobi_entry<FAST>(pi, pj, rc.x, rc.y, p, ax, ay, az);
+            jacobi_entry<FAST>(pi, pj, entry_pair(a, lpairs, lds, w), ax, ay, az);
         }
         float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
         const float ml = len3(dx, dy, dz);
@@ -465,7 +485,7 @@ __device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, con
 template <uint32_t MODE, bool FASTDIV>
 __global__ __launch_bounds__(64 * JT_ROWS) __attribute__((amdgpu_waves_per_eu(8))) void k_jacobi_tiled(JacobiArgs a) {
     if (a.gate && *a.gate == 0u) return;   // uniform over the grid
-    __shared__ float2 lpairs[LDS_PAIRS_MAX];
+    __shared__ float4 lpairs[LDS_PAIRS_MAX];
     __shared__ float4 tl[JT_H * JT_W];
     const uint32_t cols = a.sell.tcols, rows = a.sell.trows, ntx = cols / 64;
     const uint32_t b = a.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
