@@ -260,27 +260,34 @@ __device__ __forceinline__ float4 stage_pair(float2 rc, const DevParams& p) {
 // itself (zero length: skipped by the reference's len < 1e-8 rule), so no
 // load is conditional.  The slice width is wavefront-uniform.
 constexpr uint32_t JU = 6;
-constexpr uint32_t LDS_PAIRS_MAX = 1024;   // pair table staged in LDS up to this size
+constexpr uint32_t LDS_PAIRS_MAX = 2048;   // (rest, compliance) table staged in LDS up to this size
+constexpr uint32_t JT_PAIRS_MAX = 1024;    // the tiled kernel's staged table (float4)
 
 // The staged pair of packed entry word w.  Two LDS table forms: (rest,
 // compliance) as given (fewer registers: the per-slice kernels at the 64-VGPR
 // cap), or already staged (two fewer operations per entry: the tiled kernel).
-__device__ __forceinline__ float4 entry_pair(const JacobiArgs& a, const float2* lpairs, bool lds, uint32_t w) {
-    return stage_pair(lds ? lpairs[w >> 16] : a.sell.pairs[w >> 16], a.p);
+// LDSP: the table is in LDS (a compile-time choice: a run-time one made the
+// compiler merge both into generic flat loads waited on per entry).
+template <bool LDSP>
+__device__ __forceinline__ float4 entry_pair(const JacobiArgs& a, const float2* lpairs, uint32_t w) {
+    if constexpr (LDSP) return stage_pair(lpairs[w >> 16], a.p);
+    else return stage_pair(a.sell.pairs[w >> 16], a.p);
 }
-__device__ __forceinline__ float4 entry_pair(const JacobiArgs& a, const float4* lpairs, bool lds, uint32_t w) {
-    return lds ? lpairs[w >> 16] : stage_pair(a.sell.pairs[w >> 16], a.p);
+template <bool LDSP>
+__device__ __forceinline__ float4 entry_pair(const JacobiArgs& a, const float4* lpairs, uint32_t w) {
+    if constexpr (LDSP) return lpairs[w >> 16];
+    else return stage_pair(a.sell.pairs[w >> 16], a.p);
 }
 __device__ __forceinline__ void to_lds(float2& dst, float2 rc, const DevParams&) { dst = rc; }
 __device__ __forceinline__ void to_lds(float4& dst, float2 rc, const DevParams& p) { dst = stage_pair(rc, p); }
 
-template <bool PACKED>
-__device__ __forceinline__ void load_entry(const JacobiArgs& a, const float2* lpairs, bool lds, uint32_t i, uint32_t e,
+template <bool PACKED, bool LDSP>
+__device__ __forceinline__ void load_entry(const JacobiArgs& a, const float2* lpairs, uint32_t i, uint32_t e,
                                            uint32_t& j, float4& pr) {
     if (PACKED) {
         const uint32_t w = __builtin_nontemporal_load(a.sell.packed + e);
         j = uint32_t(int32_t(i) + int32_t(int16_t(w & 0xFFFFu)));
-        pr = entry_pair(a, lpairs, lds, w);
+        pr = entry_pair<LDSP>(a, lpairs, w);
     } else {
         j = __builtin_nontemporal_load(a.sell.j + e);
         const float rest = __builtin_nontemporal_load(a.sell.rest + e);
@@ -292,9 +299,9 @@ __device__ __forceinline__ void load_entry(const JacobiArgs& a, const float2* lp
 // One particle of a Jacobi iteration (the body shared by the per-iteration
 // kernel and the persistent one).  Returns the new position; writes it (or,
 // with JM_FINAL, the new state and velocity).
-template <uint32_t MODE, bool PACKED, bool FASTDIV>
-__device__ __forceinline__ float4 jacobi_particle(const JacobiArgs& a, const float2* lpairs, bool lds,
-                                                  uint32_t slice, uint32_t lane, bool& valid) {
+template <uint32_t MODE, bool PACKED, bool FASTDIV, bool LDSP>
+__device__ __forceinline__ float4 jacobi_particle(const JacobiArgs& a, const float2* lpairs, uint32_t slice,
+                                                  uint32_t lane, bool& valid) {
     const uint32_t i = slice * 64 + lane;
     const DevParams& p = a.p;
     valid = slice < p.nslices && i < p.n;
@@ -334,7 +341,7 @@ __device__ __forceinline__ float4 jacobi_particle(const JacobiArgs& a, const flo
                     pp[u] = a.src[jj[u]];
                 }
 #pragma unroll
-                for (uint32_t u = 0; u < JU; ++u) jacobi_entry<FASTDIV>(pi, pp[u], entry_pair(a, lpairs, lds, wc[u]), ax, ay, az);
+                for (uint32_t u = 0; u < JU; ++u) jacobi_entry<FASTDIV>(pi, pp[u], entry_pair<LDSP>(a, lpairs, wc[u]), ax, ay, az);
             }
         }
         for (; k0 + JU <= width; k0 += JU) {
@@ -342,7 +349,7 @@ __device__ __forceinline__ float4 jacobi_particle(const JacobiArgs& a, const flo
             float4 pr[JU];
             float4 pp[JU];
 #pragma unroll
-            for (uint32_t u = 0; u < JU; ++u) load_entry<PACKED>(a, lpairs, lds, i, base + (k0 + u) * 64, jj[u], pr[u]);
+            for (uint32_t u = 0; u < JU; ++u) load_entry<PACKED, LDSP>(a, lpairs, i, base + (k0 + u) * 64, jj[u], pr[u]);
 #pragma unroll
             for (uint32_t u = 0; u < JU; ++u) {
                 if (MODE & JM_PREDICT) pp[u] = predict(a.src[jj[u]], a.vel[jj[u]], jj[u], p);
@@ -354,7 +361,7 @@ __device__ __forceinline__ float4 jacobi_particle(const JacobiArgs& a, const flo
         for (; k0 < width; ++k0) {
             uint32_t j;
             float4 pr;
-            load_entry<PACKED>(a, lpairs, lds, i, base + k0 * 64, j, pr);
+            load_entry<PACKED, LDSP>(a, lpairs, i, base + k0 * 64, j, pr);
             const float4 pj = (MODE & JM_PREDICT) ? predict(a.src[j], a.vel[j], j, p) : a.src[j];
             jacobi_entry<FASTDIV>(pi, pj, pr, ax, ay, az);
         }
@@ -376,9 +383,9 @@ __device__ __forceinline__ float4 jacobi_particle(const JacobiArgs& a, const flo
     return out;
 }
 
-template <bool PACKED, typename PT>
+template <bool PACKED, uint32_t MAXP, typename PT>
 __device__ __forceinline__ bool stage_pairs(const JacobiArgs& a, PT* lpairs) {
-    const bool lds = PACKED && a.num_pairs <= LDS_PAIRS_MAX;
+    const bool lds = PACKED && a.num_pairs <= MAXP;
     if (lds) {
         for (uint32_t q = threadIdx.x; q < a.num_pairs; q += blockDim.x) to_lds(lpairs[q], a.sell.pairs[q], a.p);
         __syncthreads();
@@ -393,15 +400,21 @@ template <uint32_t MODE, bool PACKED, bool FASTDIV>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_jacobi(JacobiArgs a) {
     if (a.gate && *a.gate == 0u) return;   // uniform over the grid
     __shared__ float2 lpairs[LDS_PAIRS_MAX];
-    const bool lds = stage_pairs<PACKED>(a, lpairs);
+    const bool lds = stage_pairs<PACKED, LDS_PAIRS_MAX>(a, lpairs);
     const uint32_t tile = a.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t slice = tile * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
     bool valid;
     // FASTDIV: the fast operations where the slices this wavefront reads are
-    // proven tame (wavefront-uniform)
+    // proven tame (wavefront-uniform); lds: uniform over the launch
     const bool fast = FASTDIV && neighbourhood_tame(a.tame_in, a.srange, slice, a.p.nslices);
-    const float4 out = fast ? jacobi_particle<MODE, PACKED, true>(a, lpairs, lds, slice, threadIdx.x & 63, valid)
-                            : jacobi_particle<MODE, PACKED, false>(a, lpairs, lds, slice, threadIdx.x & 63, valid);
+    float4 out;
+    if (PACKED && lds)
+        out = fast ? jacobi_particle<MODE, PACKED, true, true>(a, lpairs, slice, lane, valid)
+                   : jacobi_particle<MODE, PACKED, false, true>(a, lpairs, slice, lane, valid);
+    else
+        out = fast ? jacobi_particle<MODE, PACKED, true, false>(a, lpairs, slice, lane, valid)
+                   : jacobi_particle<MODE, PACKED, false, false>(a, lpairs, slice, lane, valid);
     if (!(MODE & JM_FINAL) && a.tame_out)
         write_slice_tame(a.tame_out, slice, slice < a.p.nslices, !valid || tame3(out));
     if (MODE & JM_AABB) {
@@ -420,9 +433,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
 // CSR entries in the same order (their offsets re-encoded for the tile), so
 // the sums, and the results, are those of k_jacobi.
 // ---------------------------------------------------------------------------
-template <uint32_t MODE, bool FAST>
-__device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, const float4* lpairs, bool lds,
-                                                        const float4* tl, uint32_t slice, uint32_t lane,
+template <uint32_t MODE, bool FAST, bool LDSP>
+__device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, const float4* lpairs, const float4* tl, uint32_t slice, uint32_t lane,
                                                         uint32_t lidx, bool row_ok, bool& valid) {
     const uint32_t i = slice * 64 + lane;
     const DevParams& p = a.p;
@@ -455,14 +467,14 @@ __device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, con
 #pragma unroll
                 for (uint32_t u = 0; u < JU; ++u) {
                     const float4 pj = tl[int32_t(lidx) + int32_t(int16_t(wc[u] & 0xFFFFu))];
-                    jacobi_entry<FAST>(pi, pj, entry_pair(a, lpairs, lds, wc[u]), ax, ay, az);
+                    jacobi_entry<FAST>(pi, pj, entry_pair<LDSP>(a, lpairs, wc[u]), ax, ay, az);
                 }
             }
         }
         for (; k0 < width; ++k0) {
             const uint32_t w = __builtin_nontemporal_load(a.sell.tpacked + base + k0 * 64);
             const float4 pj = tl[int32_t(lidx) + int32_t(int16_t(w & 0xFFFFu))];
-            jacobi_entry<FAST>(pi, pj, entry_pair(a, lpairs, lds, w), ax, ay, az);
+            jacobi_entry<FAST>(pi, pj, entry_pair<LDSP>(a, lpairs, w), ax, ay, az);
         }
         float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
         const float ml = len3(dx, dy, dz);
@@ -485,7 +497,7 @@ __device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, con
 template <uint32_t MODE, bool FASTDIV>
 __global__ __launch_bounds__(64 * JT_ROWS) __attribute__((amdgpu_waves_per_eu(8))) void k_jacobi_tiled(JacobiArgs a) {
     if (a.gate && *a.gate == 0u) return;   // uniform over the grid
-    __shared__ float4 lpairs[LDS_PAIRS_MAX];
+    __shared__ float4 lpairs[JT_PAIRS_MAX];
     __shared__ float4 tl[JT_H * JT_W];
     const uint32_t cols = a.sell.tcols, rows = a.sell.trows, ntx = cols / 64;
     const uint32_t b = a.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -498,7 +510,7 @@ __global__ __launch_bounds__(64 * JT_ROWS) __attribute__((amdgpu_waves_per_eu(8)
             tl[q] = (MODE & JM_PREDICT) ? predict(a.src[id], a.vel[id], id, a.p) : a.src[id];
         }
     }
-    const bool lds = stage_pairs<true>(a, lpairs);   // its barrier also publishes the tile
+    const bool lds = stage_pairs<true, JT_PAIRS_MAX>(a, lpairs);   // its barrier also publishes the tile
     if (!lds) __syncthreads();
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t row = ty * JT_ROWS + wave;
@@ -507,8 +519,13 @@ __global__ __launch_bounds__(64 * JT_ROWS) __attribute__((amdgpu_waves_per_eu(8)
     const uint32_t lidx = (wave + 2) * JT_W + lane + 2;
     bool valid;
     const bool fast = FASTDIV && neighbourhood_tame(a.tame_in, a.srange, slice, a.p.nslices);
-    const float4 out = fast ? jacobi_particle_tiled<MODE, true>(a, lpairs, lds, tl, slice, lane, lidx, row_ok, valid)
-                            : jacobi_particle_tiled<MODE, false>(a, lpairs, lds, tl, slice, lane, lidx, row_ok, valid);
+    float4 out;
+    if (lds)
+        out = fast ? jacobi_particle_tiled<MODE, true, true>(a, lpairs, tl, slice, lane, lidx, row_ok, valid)
+                   : jacobi_particle_tiled<MODE, false, true>(a, lpairs, tl, slice, lane, lidx, row_ok, valid);
+    else
+        out = fast ? jacobi_particle_tiled<MODE, true, false>(a, lpairs, tl, slice, lane, lidx, row_ok, valid)
+                   : jacobi_particle_tiled<MODE, false, false>(a, lpairs, tl, slice, lane, lidx, row_ok, valid);
     if (!(MODE & JM_FINAL) && a.tame_out)
         write_slice_tame(a.tame_out, slice, slice < a.p.nslices, !valid || tame3(out));
     if (MODE & JM_AABB) {
@@ -1449,7 +1466,7 @@ template <bool PACKED, bool FASTDIV>
 __global__ __launch_bounds__(256) void k_jacobi_persistent(JacobiArgs a, PersistArgs q) {
     if (q.gate && *q.gate == 0u) return;   // uniform: every block leaves before any barrier
     __shared__ float2 lpairs[LDS_PAIRS_MAX];
-    const bool lds = stage_pairs<PACKED>(a, lpairs);
+    const bool lds = stage_pairs<PACKED, LDS_PAIRS_MAX>(a, lpairs);
     const DevParams& p = a.p;
     GridSync gsync;
     gsync.init(q.barrier, q.error);
@@ -1467,13 +1484,16 @@ __global__ __launch_bounds__(256) void k_jacobi_persistent(JacobiArgs a, Persist
         for (uint32_t gi = blockIdx.x; gi < groups; gi += gridDim.x) {
             const uint32_t slice = gi * spb + (threadIdx.x >> 6);
             bool valid;
+            const uint32_t lane = threadIdx.x & 63;
             if (last && q.final_write) {
                 b.dst = q.X;
                 b.vout = q.V;
-                jacobi_particle<JM_FINAL, PACKED, false>(b, lpairs, lds, slice, threadIdx.x & 63, valid);
+                if (PACKED && lds) jacobi_particle<JM_FINAL, PACKED, false, true>(b, lpairs, slice, lane, valid);
+                else jacobi_particle<JM_FINAL, PACKED, false, false>(b, lpairs, slice, lane, valid);
             } else {
                 b.dst = nxt;
-                jacobi_particle<0, PACKED, false>(b, lpairs, lds, slice, threadIdx.x & 63, valid);
+                if (PACKED && lds) jacobi_particle<0, PACKED, false, true>(b, lpairs, slice, lane, valid);
+                else jacobi_particle<0, PACKED, false, false>(b, lpairs, slice, lane, valid);
             }
         }
         if (!last) gsync.sync();
