@@ -260,7 +260,8 @@ __device__ __forceinline__ float4 stage_pair(float2 rc, const DevParams& p) {
 // itself (zero length: skipped by the reference's len < 1e-8 rule), so no
 // load is conditional.  The slice width is wavefront-uniform.
 constexpr uint32_t JU = 6;
-constexpr uint32_t LDS_PAIRS_MAX = 2048;   // (rest, compliance) table staged in LDS up to this size
+constexpr uint32_t LDS_PAIRS_MAX = 2048;   // (rest, compliance) table staged in LDS up to this size (static)
+constexpr uint32_t LDS_PAIRS_DYN_MAX = 4096;   // k_jacobi: table sized to the mesh's pairs (dynamic LDS, <= 32 KB)
 constexpr uint32_t JT_PAIRS_MAX = 1024;    // the tiled kernel's staged table (float4)
 
 // The staged pair of packed entry word w.  Two LDS table forms: (rest,
@@ -399,8 +400,9 @@ __device__ __forceinline__ bool stage_pairs(const JacobiArgs& a, PT* lpairs) {
 template <uint32_t MODE, bool PACKED, bool FASTDIV>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_jacobi(JacobiArgs a) {
     if (a.gate && *a.gate == 0u) return;   // uniform over the grid
-    __shared__ float2 lpairs[LDS_PAIRS_MAX];
-    const bool lds = stage_pairs<PACKED, LDS_PAIRS_MAX>(a, lpairs);
+    extern __shared__ float2 jpairs_dyn[];   // num_pairs entries (launch-time size)
+    float2* lpairs = jpairs_dyn;
+    const bool lds = stage_pairs<PACKED, LDS_PAIRS_DYN_MAX>(a, lpairs);
     const uint32_t tile = a.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t slice = tile * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63;
@@ -1557,19 +1559,16 @@ uint32_t jacobi_blocks(uint32_t nslices, uint32_t block) { return cdiv(nslices, 
 
 template <bool PACKED, bool FASTDIV>
 static void launch_jacobi_t(uint32_t mode, const JacobiArgs& a, uint32_t blocks, uint32_t block, hipStream_t s) {
+    // the pair table's LDS, as k_jacobi decides it (stage_pairs)
+    const size_t sh = (PACKED && a.num_pairs <= LDS_PAIRS_DYN_MAX) ? size_t(a.num_pairs) * sizeof(float2) : 0;
+    const dim3 g(blocks), b(block);
     switch (mode) {
-        case 0: hipLaunchKernelGGL((k_jacobi<0, PACKED, FASTDIV>), dim3(blocks), dim3(block), 0, s, a); break;
-        case JM_PREDICT:
-            hipLaunchKernelGGL((k_jacobi<JM_PREDICT, PACKED, FASTDIV>), dim3(blocks), dim3(block), 0, s, a);
-            break;
-        case JM_FINAL:
-            hipLaunchKernelGGL((k_jacobi<JM_FINAL, PACKED, FASTDIV>), dim3(blocks), dim3(block), 0, s, a);
-            break;
-        case JM_AABB:
-            hipLaunchKernelGGL((k_jacobi<JM_AABB, PACKED, FASTDIV>), dim3(blocks), dim3(block), 0, s, a);
-            break;
+        case 0: hipLaunchKernelGGL((k_jacobi<0, PACKED, FASTDIV>), g, b, sh, s, a); break;
+        case JM_PREDICT: hipLaunchKernelGGL((k_jacobi<JM_PREDICT, PACKED, FASTDIV>), g, b, sh, s, a); break;
+        case JM_FINAL: hipLaunchKernelGGL((k_jacobi<JM_FINAL, PACKED, FASTDIV>), g, b, sh, s, a); break;
+        case JM_AABB: hipLaunchKernelGGL((k_jacobi<JM_AABB, PACKED, FASTDIV>), g, b, sh, s, a); break;
         case JM_PREDICT | JM_AABB:
-            hipLaunchKernelGGL((k_jacobi<JM_PREDICT | JM_AABB, PACKED, FASTDIV>), dim3(blocks), dim3(block), 0, s, a);
+            hipLaunchKernelGGL((k_jacobi<JM_PREDICT | JM_AABB, PACKED, FASTDIV>), g, b, sh, s, a);
             break;
         default: break;
     }
