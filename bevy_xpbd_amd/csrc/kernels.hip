@@ -961,7 +961,10 @@ __device__ __forceinline__ bool collision_contrib(uint32_t i, uint32_t j, float4
 // the contributions run in one short convergent loop (frame 65 of the
 // 1024^2 sheet: 146 -> 88 us).
 constexpr int LMAX = 8;   // candidates inside the contact distance kept per particle
-constexpr uint32_t SB = 2;   // candidates fetched per step of a row scan (more costs occupancy)
+// candidates loaded per window of a row scan: 5 measured best on the 1024^2
+// sheet and the 512-cloth batch (4..12 tried; wider windows cost occupancy,
+// narrower ones more window steps)
+constexpr uint32_t SW = 5;
 
 struct RowRange {
     uint32_t start, kmin, kmax;
@@ -1042,42 +1045,39 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     // contact distance (cheap, convergent); their partner ids are loaded
     // together and sorted, and the contributions computed afterwards in
     // ascending partner id
-    uint32_t lq[LMAX];
-#pragma unroll
-    for (int k = 0; k < LMAX; ++k) lq[k] = s;
-    int cnt = 0;
+    // each row is read in fixed windows of SW candidates, checked with
+    // predication; hits go to a per-thread LDS list (slot LMAX absorbs the
+    // misses, so every candidate does one unconditional store)
+    __shared__ uint32_t cand[LMAX + 1][256];
+    const uint32_t t = threadIdx.x;
+    uint32_t cnt = 0;
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
+        if (rows[r].start == EMPTY_KEY) continue;
         const uint32_t kmin = rows[r].kmin, kmax = rows[r].kmax;
         uint32_t idx = rows[r].start;
-        bool done = idx == EMPTY_KEY;
-        while (!done) {
-            uint32_t kk[SB];
-            float4 ff[SB];
+        for (;;) {
+            float4 ff[SW];
 #pragma unroll
-            for (uint32_t u = 0; u < SB; ++u) {
-                ff[u] = a.sP[min(idx + u, n - 1)];
-                kk[u] = idx + u < n ? __float_as_uint(ff[u].w) : EMPTY_KEY;
-            }
+            for (uint32_t u = 0; u < SW; ++u) ff[u] = a.sP[min(idx + u, n - 1)];
+            uint32_t last = 0;
 #pragma unroll
-            for (uint32_t u = 0; u < SB; ++u) {
-                if (done) break;
-                if (kk[u] > kmax) {
-                    done = true;
-                    break;
-                }
+            for (uint32_t u = 0; u < SW; ++u) {
                 const uint32_t q = idx + u;
-                if (kk[u] < kmin || q == s) continue;
+                const uint32_t key = q < n ? __float_as_uint(ff[u].w) : EMPTY_KEY;
                 const float d2 = dist2_of(fi, ff[u]);
-                if (d2 > th2 || d2 < 1e-18f) continue;
-#pragma unroll
-                for (int k = 0; k < LMAX; ++k) lq[k] = k == cnt ? q : lq[k];
-                ++cnt;
+                const bool hit = key >= kmin && key <= kmax && q != s && !(d2 > th2 || d2 < 1e-18f);
+                cand[hit ? min(cnt, uint32_t(LMAX)) : uint32_t(LMAX)][t] = q;
+                cnt += hit ? 1u : 0u;
+                last = key;
             }
-            idx += SB;
-            if (idx >= n) done = true;
+            if (last > kmax || idx + SW >= n) break;   // the row ends inside this window
+            idx += SW;
         }
     }
+    uint32_t lq[LMAX];
+#pragma unroll
+    for (int k = 0; k < LMAX; ++k) lq[k] = k < int(cnt) ? cand[k][t] : s;
     if (cnt > LMAX) {   // resolved by k_coll_sweep_overflow
         const uint32_t k = atomicAdd(&a.ovf[0], 1u);
         if (k < n) a.ovf[2 + k] = s;   // at most n entries per launch
