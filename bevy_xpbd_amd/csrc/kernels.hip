@@ -679,24 +679,6 @@ __device__ __forceinline__ void cell_of(float4 p, const CollGrid& g, uint32_t c[
     }
 }
 
-// Cell keys of the current positions.  Also empties the group-table slots
-// the previous collision pass filled (slot_rec holds, per sorted index, the
-// slot its group was inserted at, or EMPTY_KEY): the table is empty again
-// before this pass inserts.
-__global__ void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of,
-                            uint32_t* keys, uint32_t* ids, uint2* table, const uint32_t* slot_rec) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t h = slot_rec[i];
-        if (h != EMPTY_KEY) table[h] = make_uint2(EMPTY_KEY, EMPTY_KEY);
-        const CollGrid& g = grids[cloth_of ? cloth_of[i] : 0u];
-        uint32_t c[3];
-        cell_of(F[i], g, c);
-        keys[i] = g.key_base + (c[0] + g.dims[0] * (c[1] + g.dims[1] * c[2]));
-        ids[i] = i;
-    }
-}
-
 // ---- stable LSD radix sort of (key, id), 8-bit digits, 4 passes ----------
 // A pass is three kernels: per-tile digit counts, an exclusive scan of the
 // counts over tiles (per digit), and a stable scatter.  A tile is 2048 keys;
@@ -752,6 +734,49 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const uint32_t* keys
     hist[t * ntiles + blockIdx.x] = c;
 }
 
+// Cell keys of the current positions, laid out as the sort's tiles (one
+// block per 2048 keys), with the sort's first per-tile digit counts (pass 0,
+// the low 8 bits) taken on the way, so that pass needs no histogram launch.
+// Also empties the group-table slots the previous collision pass filled
+// (slot_rec holds, per sorted index, the slot its group was inserted at, or
+// EMPTY_KEY): the table is empty again before this pass inserts.
+__global__ __launch_bounds__(RADIX_BLOCK) void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grids,
+                                                         const uint32_t* cloth_of, uint32_t* keys, uint32_t* ids,
+                                                         uint2* table, const uint32_t* slot_rec, uint32_t* hist,
+                                                         uint32_t ntiles) {
+    __shared__ uint32_t h[RADIX_WAVES][256];
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+#pragma unroll
+    for (uint32_t w = 0; w < RADIX_WAVES; ++w) h[w][t] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * RADIX_TILE + wave * RADIX_WAVE_SPAN + lane;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+        const uint32_t i = base + r * 64;
+        const bool valid = i < n;
+        uint32_t key = 0;
+        if (valid) {
+            const uint32_t sl = slot_rec[i];
+            if (sl != EMPTY_KEY) table[sl] = make_uint2(EMPTY_KEY, EMPTY_KEY);
+            const CollGrid& g = grids[cloth_of ? cloth_of[i] : 0u];
+            uint32_t c[3];
+            cell_of(F[i], g, c);
+            key = g.key_base + (c[0] + g.dims[0] * (c[1] + g.dims[1] * c[2]));
+            keys[i] = key;
+            ids[i] = i;
+        }
+        const uint32_t d = key & 255u;
+        const uint64_t peers = digit_peers(d, valid);
+        if (valid && (peers & lt) == 0) h[wave][d] += uint32_t(__popcll(peers));
+    }
+    __syncthreads();
+    uint32_t cnt = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < RADIX_WAVES; ++w) cnt += h[w][t];
+    hist[t * ntiles + blockIdx.x] = cnt;
+}
+
 // Exclusive scan of 256 values held one per thread (block of 256).
 __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* lds, uint32_t& total) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -771,20 +796,23 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_
 }
 
 // Per digit (one block each): exclusive prefix of the per-tile counts over
-// tiles, in place; the digit's total goes to dtot[d].
+// tiles, in place; the digit's total goes to dtot[d].  Each thread scans a
+// contiguous run of tiles serially, then one block scan joins the runs.
 __global__ __launch_bounds__(256) void k_radix_scan_tiles(uint32_t* hist, uint32_t ntiles, uint32_t* dtot) {
     __shared__ uint32_t lds[4];
     uint32_t* row = hist + size_t(blockIdx.x) * ntiles;
-    uint32_t carry = 0;
-    for (uint32_t b = 0; b < ntiles; b += 256) {
-        const uint32_t k = b + threadIdx.x;
-        const uint32_t v = k < ntiles ? row[k] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_scan_256(v, lds, tot);
-        if (k < ntiles) row[k] = carry + ex;
-        carry += tot;
+    const uint32_t per = (ntiles + 255) / 256;
+    const uint32_t b = threadIdx.x * per, e = min(b + per, ntiles);
+    uint32_t sum = 0;
+    for (uint32_t k = b; k < e; ++k) sum += row[k];
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan_256(sum, lds, tot);
+    for (uint32_t k = b; k < e; ++k) {
+        const uint32_t v = row[k];
+        row[k] = run;
+        run += v;
     }
-    if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
+    if (threadIdx.x == 0) dtot[blockIdx.x] = tot;
 }
 
 // Stable scatter: the tile is ranked in input order (see above), placed in
@@ -1756,28 +1784,29 @@ void launch_coll_grids(const float4* P, const uint32_t* cloth_off, uint32_t nclo
 }
 
 void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
-                      uint32_t* ids, uint2* table, const uint32_t* slot_rec, hipStream_t s) {
+                      uint32_t* ids, uint2* table, const uint32_t* slot_rec, uint32_t* hist, hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_coll_keys, dim3(std::min(cdiv(n, 256), 8192u)), dim3(256), 0, s, F, n, grids, cloth_of, keys,
-                       ids, table, slot_rec);
+    const uint32_t ntiles = std::max(1u, cdiv(n, RADIX_TILE));
+    hipLaunchKernelGGL(k_coll_keys, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, F, n, grids, cloth_of, keys, ids, table,
+                       slot_rec, hist, ntiles);
 }
 
 uint32_t radix_hist_words(uint32_t n) { return 256u * std::max(1u, cdiv(n, RADIX_TILE)) + 256u; }
 
 void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
-                       uint32_t shift, uint32_t* hist, hipStream_t s) {
+                       uint32_t shift, uint32_t* hist, hipStream_t s, bool hist_done) {
     const uint32_t ntiles = std::max(1u, cdiv(n, RADIX_TILE));
     uint32_t* dtot = hist + 256u * ntiles;
-    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, n, shift, hist, ntiles);
+    if (!hist_done) hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, n, shift, hist, ntiles);
     hipLaunchKernelGGL(k_radix_scan_tiles, dim3(256), dim3(256), 0, s, hist, ntiles, dtot);
     hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, iin, kout, iout, n, shift, hist,
                        ntiles, dtot);
 }
 
 void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,
-                       uint32_t* hist, hipStream_t s) {
+                       uint32_t* hist, hipStream_t s, bool first_hist_done) {
     if (n == 0) return;
-    launch_radix_pass(keys, ids, keys_alt, ids_alt, n, 0, hist, s);
+    launch_radix_pass(keys, ids, keys_alt, ids_alt, n, 0, hist, s, first_hist_done);
     launch_radix_pass(keys_alt, ids_alt, keys, ids, n, 8, hist, s);
     launch_radix_pass(keys, ids, keys_alt, ids_alt, n, 16, hist, s);
     launch_radix_pass(keys_alt, ids_alt, keys, ids, n, 24, hist, s);

@@ -155,15 +155,16 @@ void launch_finalize(const float4* F, float4* X, float4* V, const uint8_t* groun
 void launch_coll_grid(const float* part, uint32_t nparts, CollGrid* grid, float cell_size, hipStream_t s);
 void launch_coll_grids(const float4* P, const uint32_t* cloth_off, uint32_t nclothes, float* part, float cell_size,
                        uint32_t local_bits, CollGrid* grids, hipStream_t s);
-// cell keys (and ids = identity); also empties the group-table slots the
-// previous pass recorded in slot_rec
+// cell keys (and ids = identity) plus the sort's pass-0 tile counts in hist;
+// also empties the group-table slots the previous pass recorded in slot_rec
 void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
-                      uint32_t* ids, uint2* table, const uint32_t* slot_rec, hipStream_t s);
+                      uint32_t* ids, uint2* table, const uint32_t* slot_rec, uint32_t* hist, hipStream_t s);
+// first_hist_done: hist already holds pass 0's tile counts (launch_coll_keys)
 void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,
-                       uint32_t* hist, hipStream_t s);
+                       uint32_t* hist, hipStream_t s, bool first_hist_done = false);
 uint32_t radix_hist_words(uint32_t n);
 void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
-                       uint32_t shift, uint32_t* hist, hipStream_t s);
+                       uint32_t shift, uint32_t* hist, hipStream_t s, bool hist_done = false);
 // sorted copies for the sweep (sP, sR), group-table inserts (slots recorded
 // in slot_rec, n words; the table must be empty on entry), overflow count clear
 void launch_coll_gather(const float4* F, const float4* rest4, const uint32_t* sids, const uint32_t* skeys, uint32_t n,

@@ -268,8 +268,8 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
                        float4* vout) {
     hipStream_t s = h->stream;
     enqueue_grids(h, F, nparts, coll_cell(h), s);
-    launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, s);
-    launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
+    launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist, s);
+    launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true);
     launch_coll_gather(F, h->rest4, h->ids, h->keys, h->n, h->sP, h->sR, h->ovf, h->table, h->table_slots - 1,
                        h->slot_rec, s);
     CollArgs c{};
@@ -1290,8 +1290,8 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
     auto prep_collision = [&]() {
         const uint32_t np = h->nclothes == 1 ? launch_aabb_of(h->A, h->n, h->aabb, s) : 0u;
         enqueue_grids(h, h->A, np, coll_cell(h), s);
-        launch_coll_keys(h->A, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, s);
-        launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
+        launch_coll_keys(h->A, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist, s);
+        launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true);
         launch_coll_gather(h->A, h->rest4, h->ids, h->keys, h->n, h->sP, h->sR, h->ovf, h->table, h->table_slots - 1,
                            h->slot_rec, s);
     };
@@ -1778,8 +1778,8 @@ XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t*
     hipStream_t s = h->stream;
     const uint32_t np = h->nclothes == 1 ? launch_aabb_of(h->X, h->n, h->aabb, s) : 0u;
     enqueue_grids(h, h->X, np, cell_size, s);
-    launch_coll_keys(h->X, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, s);
-    launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s);
+    launch_coll_keys(h->X, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist, s);
+    launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true);
     HIPC(h, hipGetLastError());
     HIPC(h, hipStreamSynchronize(s));
     HIPC(h, hipMemcpy(grid_out, h->grids, size_t(h->nclothes) * sizeof(CollGrid), hipMemcpyDeviceToHost));
