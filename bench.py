@@ -190,13 +190,13 @@ def main():
     b_sub = algorithmic_bytes_per_substep(n, e, I, True, table_slots)
     substep_gbs = b_sub * value / 1e9
     traffic = None
-    if args.workload == "sheet1024":
-        tpath = os.path.join(ROOT, "profiles", "jacobi_iter_1024_traffic.json")
-        if os.path.exists(tpath):
-            try:
-                traffic = json.load(open(tpath)).get("bytes_per_launch")
-            except Exception:
-                traffic = None
+    tname = {"sheet1024": "jacobi_iter_1024_traffic.json", "batch512": "jacobi_iter_batch512_traffic.json"}
+    tpath = os.path.join(ROOT, "profiles", tname[args.workload])
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
     residual = sim.residual()
     sim.close()
     if jac_launches:
