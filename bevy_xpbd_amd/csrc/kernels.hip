@@ -505,22 +505,27 @@ __global__ __launch_bounds__(64 * JT_ROWS) __attribute__((amdgpu_waves_per_eu(8)
     const uint32_t b = a.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t tx = b % ntx, ty = b / ntx;
     const int32_t r0 = int32_t(ty * JT_ROWS) - 2, c0 = int32_t(tx * 64) - 2;
+    bool tame = true;
     for (uint32_t q = threadIdx.x; q < JT_H * JT_W; q += blockDim.x) {
         const int32_t r = r0 + int32_t(q / JT_W), c = c0 + int32_t(q % JT_W);
         if (r >= 0 && r < int32_t(rows) && c >= 0 && c < int32_t(cols)) {
             const uint32_t id = uint32_t(r) * cols + uint32_t(c);
-            tl[q] = (MODE & JM_PREDICT) ? predict(a.src[id], a.vel[id], id, a.p) : a.src[id];
+            const float4 v = (MODE & JM_PREDICT) ? predict(a.src[id], a.vel[id], id, a.p) : a.src[id];
+            tl[q] = v;
+            tame = tame && tame3(v);
         }
     }
-    const bool lds = stage_pairs<true, JT_PAIRS_MAX>(a, lpairs);   // its barrier also publishes the tile
-    if (!lds) __syncthreads();
+    const bool lds = stage_pairs<true, JT_PAIRS_MAX>(a, lpairs);
+    // The block's entries read only staged positions, so the fast operations
+    // need only these to be tame (predicted ones, in JM_PREDICT): one
+    // block-wide AND, which is also the barrier that publishes the tile.
+    const bool fast = FASTDIV && __syncthreads_and(tame ? 1 : 0) != 0;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t row = ty * JT_ROWS + wave;
     const bool row_ok = row < rows;
     const uint32_t slice = row_ok ? row * ntx + tx : a.p.nslices;
     const uint32_t lidx = (wave + 2) * JT_W + lane + 2;
     bool valid;
-    const bool fast = FASTDIV && neighbourhood_tame(a.tame_in, a.srange, slice, a.p.nslices);
     float4 out;
     if (lds)
         out = fast ? jacobi_particle_tiled<MODE, true, true>(a, lpairs, tl, slice, lane, lidx, row_ok, valid)

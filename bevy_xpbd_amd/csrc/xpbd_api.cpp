@@ -820,7 +820,9 @@ int upload_state(xpbd_cloth* h, const float* pos4, const float* vel4) {
 void apply_tuning(xpbd_cloth* h) {
     if (h->cfg.jacobi_block) h->jacobi_block = h->cfg.jacobi_block;
     if (h->cfg.xcd_remap) h->remap = h->cfg.xcd_remap == 2 ? 1u : 0u;
-    h->fused_predict = h->cfg.fused_predict ? 1u : 0u;
+    // 0: automatic, on where the grid-tiled kernel runs (it stages the
+    // predicted positions and proves their range itself); 1: on; 2: off
+    h->fused_predict = (h->cfg.fused_predict == 1 || (h->cfg.fused_predict == 0 && h->nbtile)) ? 1u : 0u;
     // persistent Jacobi: every block must be resident at once (grid barrier)
     const uint32_t groups = std::max(1u, (h->nslices + 3) / 4);
     h->pj_blocks = std::max(1u, std::min(persistent_max_blocks(h->nbpk != nullptr, h->cfg.fast_division != 0), groups));

@@ -133,7 +133,9 @@ typedef struct xpbd_sim_config {
     uint32_t use_graph;                   /* 1: replay frames from a hipGraph */
     uint32_t autotune;                    /* 1: time launch shapes at create */
     uint32_t jacobi_block;                /* Jacobi workgroup size 64..512; 0 = 256 or autotuned */
-    uint32_t fused_predict;               /* 1: predict inside the first iteration */
+    uint32_t fused_predict;               /* predict inside the first iteration:
+                                             0 (default) where grid tiles run,
+                                             1 always, 2 never (same results) */
     uint32_t plain_adjacency;             /* 1: 12-byte adjacency entries even when the
                                              4-byte packed form fits (read at create) */
     uint32_t fast_division;               /* 1 (default): in the Jacobi gather, the

@@ -101,11 +101,13 @@ def test_jacobi_block_sizes(O, X, block, fast_division):
              collision=True, jacobi_block=block, fast_division=fast_division)
 
 
-@pytest.mark.parametrize("cols,rows,fast,fused", [(64, 37, 0, 0), (64, 37, 1, 0), (128, 16, 1, 1), (128, 16, 0, 1)])
+@pytest.mark.parametrize("cols,rows,fast,fused", [(64, 37, 0, 0), (64, 37, 1, 0), (128, 16, 1, 1), (128, 16, 0, 1),
+                                                (64, 37, 1, 2), (128, 16, 0, 2)])
 def test_grid_tiles_bit_exact(O, X, cols, rows, fast, fused):
     """Grid-tiled Jacobi (64 x 8 LDS tiles with a 2-particle halo): columns a
-    multiple of 64, a partial last band of tiles (37 rows), fused predict
-    (staged predicted positions), both division forms, with collision."""
+    multiple of 64, a partial last band of tiles (37 rows), predict fused
+    into the first iteration (0: automatic = on for tiles, 1: on) or separate
+    (2), both division forms, with collision."""
     note(f"case grid tiles {cols}x{rows} fast={fast} fused={fused}")
     kw = dict(cols=cols, rows=rows, spacing=0.02, pins=2, jitter=1e-3, seed=cols + rows)
     run_pair(O, X, kw, S=8, I=5, substeps=3, collision=True, fast_division=fast, fused_predict=fused,
