@@ -648,26 +648,33 @@ __global__ __launch_bounds__(1024) void k_coll_grid(const float* part, uint32_t 
 }
 
 // Many cloths: one block per cloth reduces its particle range.
-__global__ __launch_bounds__(256) void k_cloth_aabb(const float4* P, const uint32_t* cloth_off, float* part) {
-    const uint32_t c = blockIdx.x;
-    const uint32_t b = cloth_off[c], e = cloth_off[c + 1];
+// Per-cloth bounds of a batch in balanced pieces: block b reduces particles
+// [span[b].x, span[b].y) of one cloth (at most CB_SPAN of them) into partial
+// b; the grid kernel then folds each cloth's partials.  (One block per cloth
+// let the largest cloths set the time; atomics on ~3000 words serialised.)
+constexpr uint32_t CB_SPAN = 4096;
+
+__global__ __launch_bounds__(256) void k_cloth_aabb(const float4* P, const uint2* span, float* part) {
+    const uint2 r = span[blockIdx.x];
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+    for (uint32_t i = r.x + threadIdx.x; i < r.y; i += blockDim.x) {
         const float4 v = P[i];
         aabb_fold(true, v.x, v.y, v.z, lo, hi);
     }
-    block_aabb_store(lo, hi, part, c);
+    block_aabb_store(lo, hi, part, blockIdx.x);
 }
 
-__global__ void k_coll_grids(const float* part, uint32_t nclothes, float cell_size, uint32_t local_bits,
-                             CollGrid* grids) {
+// Grids from the partials of each cloth (pieces [pfirst[c], pfirst[c+1])).
+__global__ void k_coll_grids(const float* part, const uint32_t* pfirst, uint32_t nclothes, float cell_size,
+                             uint32_t local_bits, CollGrid* grids) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nclothes) return;
-    float lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = part[size_t(c) * 8 + a];
-        hi[a] = part[size_t(c) * 8 + 3 + a];
-    }
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t k = pfirst[c]; k < pfirst[c + 1]; ++k)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], part[size_t(k) * 8 + a]);
+            hi[a] = fmaxf(hi[a], part[size_t(k) * 8 + 3 + a]);
+        }
     const uint64_t cap = (local_bits >= 32 ? (1ull << 32) : (1ull << local_bits)) - 2;
     grid_from_bounds(lo, hi, cell_size, cap, local_bits >= 32 ? 0u : (c << local_bits), &grids[c]);
 }
@@ -1781,12 +1788,15 @@ void launch_coll_grid(const float* part, uint32_t nparts, CollGrid* grid, float 
     hipLaunchKernelGGL(k_coll_grid, dim3(1), dim3(1024), 0, s, part, nparts, grid, cell_size);
 }
 
-void launch_coll_grids(const float4* P, const uint32_t* cloth_off, uint32_t nclothes, float* part, float cell_size,
-                       uint32_t local_bits, CollGrid* grids, hipStream_t s) {
-    hipLaunchKernelGGL(k_cloth_aabb, dim3(nclothes), dim3(256), 0, s, P, cloth_off, part);
-    hipLaunchKernelGGL(k_coll_grids, dim3(cdiv(nclothes, 64)), dim3(64), 0, s, part, nclothes, cell_size, local_bits,
-                       grids);
+void launch_coll_grids(const float4* P, const uint2* span, uint32_t nspans, const uint32_t* pfirst,
+                       uint32_t nclothes, float* part, float cell_size, uint32_t local_bits, CollGrid* grids,
+                       hipStream_t s) {
+    if (nspans) hipLaunchKernelGGL(k_cloth_aabb, dim3(nspans), dim3(256), 0, s, P, span, part);
+    hipLaunchKernelGGL(k_coll_grids, dim3(cdiv(nclothes, 64)), dim3(64), 0, s, part, pfirst, nclothes, cell_size,
+                       local_bits, grids);
 }
+
+uint32_t cloth_span_max() { return CB_SPAN; }
 
 void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
                       uint32_t* ids, uint2* table, const uint32_t* slot_rec, uint32_t* hist, hipStream_t s) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace xpbd {
 
@@ -153,8 +154,13 @@ void launch_predict(const float4* X, const float4* V, float4* P, DevParams p, hi
 void launch_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p,
                      hipStream_t s);
 void launch_coll_grid(const float* part, uint32_t nparts, CollGrid* grid, float cell_size, hipStream_t s);
-void launch_coll_grids(const float4* P, const uint32_t* cloth_off, uint32_t nclothes, float* part, float cell_size,
-                       uint32_t local_bits, CollGrid* grids, hipStream_t s);
+// per-cloth bounds of P: piece b reduces particles [span[b].x, span[b].y)
+// of one cloth into part[8b..]; cloth c owns pieces [pfirst[c], pfirst[c+1]);
+// then the grids
+void launch_coll_grids(const float4* P, const uint2* span, uint32_t nspans, const uint32_t* pfirst,
+                       uint32_t nclothes, float* part, float cell_size, uint32_t local_bits, CollGrid* grids,
+                       hipStream_t s);
+uint32_t cloth_span_max();   // particles per piece at most
 // cell keys (and ids = identity) plus the sort's pass-0 tile counts in hist;
 // also empties the group-table slots the previous pass recorded in slot_rec
 void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,

@@ -95,6 +95,9 @@ struct xpbd_cloth {
     uint32_t* ovf = nullptr;       // collision overflow list (count, done, n sorted indices)
     float* aabb = nullptr;         // bounds partials, 8 floats per producing block / cloth
     uint32_t aabb_slots = 0;
+    uint2* cspan = nullptr;         // batches: bounds pieces (particle range of one cloth each)
+    uint32_t* cpfirst = nullptr;    // batches: first piece of each cloth (nclothes + 1)
+    uint32_t ncspans = 0;
     CollGrid* grids = nullptr;     // one per cloth
     uint32_t nclothes = 1, local_bits = 32;
     uint32_t* cloth_of = nullptr;  // particle -> cloth (nullptr for one cloth)
@@ -246,7 +249,8 @@ float coll_cell(const xpbd_cloth* h) {
 // its last iteration wrote; many cloths get one bounds block each.
 void enqueue_grids(xpbd_cloth* h, const float4* F, uint32_t nparts, float cell, hipStream_t s) {
     if (h->nclothes == 1) launch_coll_grid(h->aabb, nparts, h->grids, cell, s);
-    else launch_coll_grids(F, h->cloth_off, h->nclothes, h->aabb, cell, h->local_bits, h->grids, s);
+    else launch_coll_grids(F, h->cspan, h->ncspans, h->cpfirst, h->nclothes, h->aabb, cell, h->local_bits, h->grids,
+                           s);
 }
 
 // Instrumented frames: bracket a launch of kind `which` with a pair of events.
@@ -744,6 +748,25 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(dalloc(h, &h->ovf, size_t(n) + 2));
     HIPC(h, hipMemset(h->ovf, 0, (size_t(n) + 2) * sizeof(uint32_t)));
     h->aabb_slots = std::max<uint32_t>({h->nslices, (h->n + 255) / 256, 4096u, uint32_t(m.cloth_offsets.size()), 1u});
+    {   // bounds pieces of a batch: each cloth cut into ranges of at most cloth_span_max() particles
+        std::vector<uint2> sp;
+        std::vector<uint32_t> pf;
+        const size_t nc = m.cloth_offsets.size() >= 2 ? m.cloth_offsets.size() - 1 : 0;
+        for (size_t c = 0; c < nc; ++c) {
+            pf.push_back(uint32_t(sp.size()));
+            for (uint32_t b = m.cloth_offsets[c]; b < m.cloth_offsets[c + 1]; b += cloth_span_max())
+                sp.push_back(make_uint2(b, std::min(b + cloth_span_max(), m.cloth_offsets[c + 1])));
+        }
+        pf.push_back(uint32_t(sp.size()));
+        h->aabb_slots = std::max<uint32_t>(h->aabb_slots, uint32_t(sp.size()));
+        h->ncspans = uint32_t(sp.size());
+        if (!sp.empty()) {
+            TRY(dalloc(h, &h->cspan, sp.size()));
+            TRY(upload(h, h->cspan, sp.data(), sp.size()));
+        }
+        TRY(dalloc(h, &h->cpfirst, pf.size()));
+        TRY(upload(h, h->cpfirst, pf.data(), pf.size()));
+    }
     TRY(dalloc(h, &h->aabb, size_t(h->aabb_slots) * 8));
     {
         std::vector<uint32_t> offs = m.cloth_offsets;
