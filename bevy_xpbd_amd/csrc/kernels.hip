@@ -925,17 +925,6 @@ __device__ __forceinline__ uint32_t slot_of(uint32_t g, uint32_t mask) {
     return ((blk << 3) | (g & 7u)) & mask;
 }
 
-__device__ __forceinline__ uint32_t table_find(const uint2* table, uint32_t mask, uint32_t g) {
-    uint32_t h = slot_of(g, mask);
-    for (uint32_t probe = 0; probe <= mask; ++probe) {
-        const uint2 e = table[h];
-        if (e.x == g) return e.y;
-        if (e.x == EMPTY_KEY) return EMPTY_KEY;
-        h = (h + 1) & mask;
-    }
-    return EMPTY_KEY;
-}
-
 // Lookup whose first probe `e` (the entry at slot_of(g)) was loaded by the
 // caller: most lookups end there; a collision continues at the next slot.
 __device__ __forceinline__ uint32_t table_resolve(const uint2* table, uint32_t mask, uint32_t g, uint2 e) {
