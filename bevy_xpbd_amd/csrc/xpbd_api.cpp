@@ -962,12 +962,55 @@ int autotune(xpbd_cloth* h) {
                 best_remap = remap;
             }
         }
+    h->jacobi_block = best_block;
+    h->remap = best_remap;
+    // Coloured GS: the grid size of the persistent substep.  Every block takes
+    // part in each of the I x colours grid barriers, whose cost grows with the
+    // grid, while fewer blocks leave more constraints per thread per colour;
+    // the best point is measured (same results for every grid: each colour's
+    // constraints are vertex-disjoint).
+    if (h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL && h->e && h->gs_blocks > 64) {
+        GsArgs g{};
+        g.X = h->X;
+        g.V = h->V;
+        g.W = h->A;
+        g.xout = h->B;
+        g.vout = h->V2;
+        g.ij = h->gs_ij;
+        g.rc = h->gs_rc;
+        g.batch_offsets = h->gs_off;
+        g.num_batches = h->num_batches;
+        g.iterations = std::max(h->cfg.solve_inner_iterations, 1u);
+        g.barrier = h->barrier;
+        g.error = h->err;
+        g.aabb = h->aabb;
+        g.final_write = 1;
+        g.p = p;
+        float gbest = 1e30f;
+        uint32_t gbest_blocks = h->gs_blocks;
+        for (uint32_t b = h->gs_blocks; b >= 64; b /= 2) {
+            launch_zero_u32(h->barrier, GRID_SYNC_WORDS, h->stream);
+            launch_gs(g, b, h->gs_block, h->stream);   // warm-up
+            (void)hipEventRecord(e0, h->stream);
+            for (uint32_t r = 0; r < 3; ++r) {
+                launch_zero_u32(h->barrier, GRID_SYNC_WORDS, h->stream);
+                launch_gs(g, b, h->gs_block, h->stream);
+            }
+            (void)hipEventRecord(e1, h->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < gbest) {
+                gbest = ms;
+                gbest_blocks = b;
+            }
+        }
+        h->gs_blocks = gbest_blocks;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     HIPC(h, hipGetLastError());
-    h->jacobi_block = best_block;
-    h->remap = best_remap;
-    return XPBD_OK;
+    return check_device_flag(h);
 }
 
 }  // namespace
@@ -1105,7 +1148,7 @@ XPBD_API void xpbd_sim_config_default(xpbd_sim_config* out) {
     c.solver = XPBD_SOLVER_JACOBI;
     c.collision_cell = 0.f;
     c.use_graph = 1;
-    c.autotune = 0;
+    c.autotune = 1;
     c.jacobi_block = 0;
     c.fused_predict = 0;
     c.fast_division = 1;

@@ -131,7 +131,9 @@ typedef struct xpbd_sim_config {
     uint32_t solver;                      /* XPBD_SOLVER_*                   */
     float collision_cell;                 /* hash cell side [m]; 0 = thickness */
     uint32_t use_graph;                   /* 1: replay frames from a hipGraph */
-    uint32_t autotune;                    /* 1: time launch shapes at create */
+    uint32_t autotune;                    /* 1 (default): time launch shapes at create
+                                             (Jacobi block size and order, coloured GS
+                                             grid size); same results either way */
     uint32_t jacobi_block;                /* Jacobi workgroup size 64..512; 0 = 256 or autotuned */
     uint32_t fused_predict;               /* predict inside the first iteration:
                                              0 (default) where grid tiles run,

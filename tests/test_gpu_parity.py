@@ -22,24 +22,24 @@ def note(msg):
 
 
 def make(X, mesh, solver=0, S=8, I=10, coll_stride=1, use_graph=0, fused_predict=0, jacobi_block=0,
-         plain_adjacency=0, fast_division=0, persistent_jacobi=0, grid_tiles=1, **uk):
+         plain_adjacency=0, fast_division=0, persistent_jacobi=0, grid_tiles=1, autotune=0, **uk):
     cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=I, solver=solver,
                                    collision_every_n_substeps=coll_stride, use_graph=use_graph,
                                    fused_predict=fused_predict, jacobi_block=jacobi_block,
                                    plain_adjacency=plain_adjacency, fast_division=fast_division,
-                                   persistent_jacobi=persistent_jacobi, grid_tiles=grid_tiles)
+                                   persistent_jacobi=persistent_jacobi, grid_tiles=grid_tiles, autotune=autotune)
     u = X.ClothSimUniforms.default(solver, S, **uk)
     return X.ClothComputePlugin(mesh, cfg, u), cfg, u
 
 
 def run_pair(O, X, mesh_kw=None, mesh=None, solver=0, S=8, I=10, substeps=8, collision=False, fused_predict=0,
              jacobi_block=0, plain_adjacency=0, fast_division=0, persistent_jacobi=0, grid_tiles=1, layout=None,
-             **uk):
+             autotune=0, **uk):
     mesh = mesh or X.ClothMeshData.sheet(**mesh_kw)
     f = mesh.fields()
     sim, cfg, u = make(X, mesh, solver, S, I, fused_predict=fused_predict, jacobi_block=jacobi_block,
                        plain_adjacency=plain_adjacency, fast_division=fast_division,
-                       persistent_jacobi=persistent_jacobi, grid_tiles=grid_tiles, **uk)
+                       persistent_jacobi=persistent_jacobi, grid_tiles=grid_tiles, autotune=autotune, **uk)
     if layout is not None:
         assert sim.layout()["adjacency"] == layout, sim.layout()
     s = O.Solver(f)
@@ -355,6 +355,16 @@ def test_config2_256_collision_substeps_bit_exact(O, X, solver):
     rng = np.random.default_rng(2)
     kw = dict(cols=256, rows=256, spacing=h, layout=1, height=1.0, angle=float(rng.uniform(0, 2 * np.pi)), pins=3)
     run_pair(O, X, kw, solver=solver, S=4, I=8, substeps=3, collision=True, thickness=2 * (0.4 * h),
+             jacobi_omega=0.32 if solver == 0 else 1.0)
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_autotuned_launch_shapes_bit_exact(O, X, solver):
+    """Autotuned launch shapes (Jacobi block and block order; the coloured
+    GS grid size) give the same results as the defaults."""
+    note(f"case autotune solver={solver}")
+    kw = dict(cols=128, rows=96, spacing=0.01, pins=2, jitter=1e-3, seed=21)
+    run_pair(O, X, kw, solver=solver, S=8, I=6, substeps=3, collision=True, thickness=0.008, autotune=1,
              jacobi_omega=0.32 if solver == 0 else 1.0)
 
 
