@@ -1308,21 +1308,87 @@ __global__ void k_predict_gated(const float4* X, const float4* V, float4* P, Dev
     P[i] = predict(X[i], V[i], i, p);
 }
 
-__global__ void k_copy4_unless(const float4* a, float4* b, uint32_t n, const uint32_t* gate) {
-    if (*gate != 0u) return;
+__global__ void k_copy4_if(const float4* a, float4* b, uint32_t n, const uint32_t* gate) {
+    if (*gate == 0u) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = a[i];
 }
 
-// Warm-start acceptance: relative residual against the threshold, plus the
-// substep / fallback counters.  One thread.
-__global__ void k_warm_flag(const double* resid, double rest_sq, float threshold, uint32_t* flag) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const double rel = rest_sq > 0.0 ? sqrt(resid[0]) / sqrt(rest_sq) : 0.0;
-    const uint32_t f = rel > double(threshold) ? 1u : 0u;
-    flag[0] = f;
-    flag[1] = flag[1] + 1u;
-    flag[2] = flag[2] + f;
+// Warm-start acceptance in one launch.  The relative residual ||C|| / ||rest||
+// of X over the constraints (ij, rest; ordered so that neighbouring threads
+// load neighbouring vertices): C in f32 per constraint (hardware
+// sqrt, about 1 ulp: a gate statistic, not the reported residual), summed in
+// f32 per thread and in double across threads.  Block sums meet in
+// WG_SLOTS accumulators (block b in slot b mod WG_SLOTS, each slot on its
+// own 256-byte line, so that same-address atomics form short parallel
+// chains); the last block of a slot passes the slot total on to the final
+// accumulator, and the last slot's last block compares the total with the
+// threshold -- a NaN residual falls back -- sets flag[0] (1: fall back),
+// counts the substep (flag[1]) and the fallback (flag[2]), and clears the
+// grid-barrier words of the fallback launch.  Every accumulator and counter
+// is reset by the block that reads it last, so the next launch finds zeros.
+constexpr uint32_t WG_SLOTS = 32;
+constexpr uint32_t WG_LINE = 64;   // words per slot: (sum as double, count)
+
+__device__ __forceinline__ bool wg_arrive(uint32_t* slot, double v, uint32_t arrivals, double* total) {
+    double* sum = reinterpret_cast<double*>(slot);
+    __hip_atomic_fetch_add(sum, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the add has been performed before this block is counted
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if (__hip_atomic_fetch_add(slot + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 != arrivals) return false;
+    *total = __hip_atomic_load(sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(sum, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(slot + 2, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_warm_gate(const float4* X, const uint2* ij, const float* rest, uint32_t e,
+                                                  double rest_sq, float threshold, uint32_t* acc, uint32_t* flag,
+                                                  uint32_t* barrier) {
+    float sf = 0.f;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < e; k += gridDim.x * blockDim.x) {
+        const uint2 q = ij[k];
+        const float4 a = X[q.x], b = X[q.y];
+        const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+        const float c = __builtin_amdgcn_sqrtf((dx * dx + dy * dy) + dz * dz) - rest[k];
+        sf += c * c;
+    }
+    double sd = sf;
+    for (int off = 32; off > 0; off >>= 1) sd += __shfl_xor(sd, off, 64);
+    __shared__ double red[4];
+    __shared__ uint32_t last;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sd;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t slot = blockIdx.x % WG_SLOTS;
+        const uint32_t in_slot = (gridDim.x - slot + WG_SLOTS - 1) / WG_SLOTS;
+        const uint32_t slots = min(gridDim.x, WG_SLOTS);
+        double tot = 0.0;
+        last = 0u;
+        if (wg_arrive(acc + WG_LINE * slot, (red[0] + red[1]) + (red[2] + red[3]), in_slot, &tot) &&
+            wg_arrive(acc + WG_LINE * WG_SLOTS, tot, slots, &tot)) {
+            const double rel = rest_sq > 0.0 ? sqrt(tot) / sqrt(rest_sq) : 0.0;
+            const uint32_t f = rel <= double(threshold) ? 0u : 1u;
+            flag[0] = f;
+            flag[1] = flag[1] + 1u;
+            flag[2] = flag[2] + f;
+            last = 1u;
+        }
+    }
+    __syncthreads();
+    if (!last) return;
+    for (uint32_t w = threadIdx.x; w < GRID_SYNC_WORDS; w += blockDim.x) barrier[w] = 0u;
+}
+
+// The state update from the warm result F, or from alt when the gate chose
+// the fallback (*gate != 0).
+__global__ void k_finalize_sel(const float4* F, const float4* alt, const uint32_t* gate, float4* X, float4* V,
+                               const uint8_t* ground, DevParams p) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const float4 f = (*gate != 0u ? alt : F)[i];
+    if (f.w <= 0.f) return;
+    finish_particle(f, X[i], i, ground, p, X, V);
 }
 
 // ---------------------------------------------------------------------------
@@ -1734,24 +1800,24 @@ void launch_predict_gated(const float4* X, const float4* V, float4* P, DevParams
     hipLaunchKernelGGL(k_predict_gated, dim3(cdiv(p.n, 256)), dim3(256), 0, s, X, V, P, p, gate);
 }
 
-void launch_copy4_unless(const float4* a, float4* b, uint32_t n, const uint32_t* gate, hipStream_t s) {
+void launch_copy4_if(const float4* a, float4* b, uint32_t n, const uint32_t* gate, hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_copy4_unless, dim3(cdiv(n, 256)), dim3(256), 0, s, a, b, n, gate);
+    hipLaunchKernelGGL(k_copy4_if, dim3(cdiv(n, 256)), dim3(256), 0, s, a, b, n, gate);
 }
 
-__global__ void k_warm_count(uint32_t* flag) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        flag[0] = 0u;
-        flag[1] = flag[1] + 1u;
-    }
+uint32_t warm_gate_words() { return WG_LINE * (WG_SLOTS + 1); }
+
+void launch_warm_gate(const float4* X, const uint2* ij, const float* rest, uint32_t e, double rest_sq,
+                      float threshold, uint32_t* acc, uint32_t* flag, uint32_t* barrier, hipStream_t s) {
+    const uint32_t blocks = std::max(1u, std::min(cdiv(e, 256 * 4), 2048u));
+    hipLaunchKernelGGL(k_warm_gate, dim3(blocks), dim3(256), 0, s, X, ij, rest, e, rest_sq, threshold, acc, flag,
+                       barrier);
 }
 
-void launch_warm_count(uint32_t* flag, hipStream_t s) {
-    hipLaunchKernelGGL(k_warm_count, dim3(1), dim3(64), 0, s, flag);
-}
-
-void launch_warm_flag(const double* resid, double rest_sq, float threshold, uint32_t* flag, hipStream_t s) {
-    hipLaunchKernelGGL(k_warm_flag, dim3(1), dim3(64), 0, s, resid, rest_sq, threshold, flag);
+void launch_finalize_sel(const float4* F, const float4* alt, const uint32_t* gate, float4* X, float4* V,
+                         const uint8_t* ground, DevParams p, hipStream_t s) {
+    if (p.n == 0) return;
+    hipLaunchKernelGGL(k_finalize_sel, dim3(cdiv(p.n, 256)), dim3(256), 0, s, F, alt, gate, X, V, ground, p);
 }
 
 void launch_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p, hipStream_t s) {

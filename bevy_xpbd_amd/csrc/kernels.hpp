@@ -185,12 +185,18 @@ void launch_tame_scan(const float4* P, uint32_t n, uint8_t* slice_tame, hipStrea
 // gated variants for the warm-start fallback: run only if *gate != 0
 void launch_predict_gated(const float4* X, const float4* V, float4* P, DevParams p, const uint32_t* gate,
                           hipStream_t s);
-// copy a -> b only if *gate == 0
-void launch_copy4_unless(const float4* a, float4* b, uint32_t n, const uint32_t* gate, hipStream_t s);
-// flag[0] = sqrt(resid) / sqrt(rest_sq) > threshold; flag[1] += 1; flag[2] += flag[0]
-void launch_warm_flag(const double* resid, double rest_sq, float threshold, uint32_t* flag, hipStream_t s);
-// no gate: count the warm substep only
-void launch_warm_count(uint32_t* flag, hipStream_t s);
+// copy a -> b only if *gate != 0
+void launch_copy4_if(const float4* a, float4* b, uint32_t n, const uint32_t* gate, hipStream_t s);
+// warm-start gate: flag[0] = relative residual of X (f32 per constraint) >
+// threshold or NaN; flag[1] += 1; flag[2] += flag[0]; clears barrier
+// (GRID_SYNC_WORDS).  acc: warm_gate_words() words (256-byte aligned), zero
+// before the first launch; each launch leaves them zero
+uint32_t warm_gate_words();
+void launch_warm_gate(const float4* X, const uint2* ij, const float* rest, uint32_t e, double rest_sq,
+                      float threshold, uint32_t* acc, uint32_t* flag, uint32_t* barrier, hipStream_t s);
+// finalize from F, or from alt if *gate != 0
+void launch_finalize_sel(const float4* F, const float4* alt, const uint32_t* gate, float4* X, float4* V,
+                         const uint8_t* ground, DevParams p, hipStream_t s);
 uint32_t persistent_max_blocks(bool packed, bool fastdiv);
 void launch_jacobi_persistent(const JacobiArgs& a, const PersistArgs& q, uint32_t blocks, hipStream_t s);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);

@@ -23,18 +23,12 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC0u;   // NaN stays NaN
-    u += 0x7FFFu + ((u >> 16) & 1u);                        // round to nearest, ties to even
-    return uint16_t(u >> 16);
-}
+// f32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f)); }
 
+typedef __bf16 bf16v4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ bf16x4 pack4(f32x4 v) {
-    bf16x4 r;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) r[i] = short(f2bf(v[i]));
-    return r;
+    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bf16v4));
 }
 
 __device__ __forceinline__ bf16x4 ld4(const uint16_t* p) { return *reinterpret_cast<const bf16x4*>(p); }
@@ -43,17 +37,28 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// GELU, tanh form, and its derivative
+// GELU, tanh form, and its derivative, written with s = sigmoid(2u) =
+// (1 + tanh u) / 2, u = C (x + A x^3): GELU(x) = x s, and since
+// 1 - tanh^2 u = 4 s (1 - s), GELU'(x) = s + x s (1 - s) (2C + 6CA x^2).
+// exp(-2u) = 2^(x (K1 + K2 x^2)) with K1 = -2C log2(e), K2 = A K1, through
+// the hardware exp2 and reciprocal (about 1e-6 relative, far inside the bf16
+// rounding the activations get next).  Large |x|: exp2 -> inf or 0, so
+// s -> 0 or 1 and GELU -> 0 or x.
 constexpr float GELU_C = 0.7978845608028654f;   // sqrt(2 / pi)
 constexpr float GELU_A = 0.044715f;
-__device__ __forceinline__ float gelu(float x) {
-    const float u = GELU_C * (x + GELU_A * x * x * x);
-    return 0.5f * x * (1.0f + tanhf(u));
+constexpr float GELU_K1 = float(-2.0 * 0.7978845608028654 * 1.4426950408889634);
+constexpr float GELU_K2 = float(-2.0 * 0.7978845608028654 * 1.4426950408889634 * 0.044715);
+__device__ __forceinline__ float sigmoid_2u(float x, float x2) {
+    const float t = __builtin_fmaf(GELU_K2, x2, GELU_K1);
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t * x));
 }
+__device__ __forceinline__ float gelu(float x) { return x * sigmoid_2u(x, x * x); }
 __device__ __forceinline__ float gelu_grad(float x) {
     const float x2 = x * x;
-    const float t = tanhf(GELU_C * (x + GELU_A * x2 * x));
-    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * GELU_C * (1.0f + 3.0f * GELU_A * x2);
+    const float s = sigmoid_2u(x, x2);
+    const float q = __builtin_fmaf(6.0f * GELU_C * GELU_A, x2, 2.0f * GELU_C);
+    const float d = __builtin_fmaf(-s, s, s);   // s (1 - s)
+    return __builtin_fmaf(x * d, q, s);
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -70,12 +75,20 @@ __device__ __forceinline__ float sum16(float v) {   // over the 16 lanes sharing
     return v;
 }
 
-__device__ __forceinline__ void stage_weights(const uint16_t* bf, uint16_t* W) {
+// LDS copy of the biases: b1 [0, 64), b2 [64, 128), b3 [128, 144)
+constexpr int LB_B1 = 0, LB_B2 = MLP_HID, LB_B3 = 2 * MLP_HID, LB_WORDS = 2 * MLP_HID + MLP_OUTP;
+
+__device__ __forceinline__ void stage_weights(const uint16_t* bf, const float* P, uint16_t* W, float* B) {
     const uint4* src = reinterpret_cast<const uint4*>(bf);
     uint4* dst = reinterpret_cast<uint4*>(W);
     for (uint32_t q = threadIdx.x; q < uint32_t(MLP_BF / 8); q += blockDim.x) dst[q] = src[q];
+    for (uint32_t q = threadIdx.x; q < uint32_t(LB_WORDS); q += blockDim.x) {
+        B[q] = q < uint32_t(LB_B2) ? P[OFF_B1 + q] : q < uint32_t(LB_B3) ? P[OFF_B2 + q - LB_B2] : P[OFF_B3 + q - LB_B3];
+    }
     __syncthreads();
 }
+
+__device__ __forceinline__ f32x4 ld_b4(const float* B) { return *reinterpret_cast<const f32x4*>(B); }
 
 struct TileAct {
     f32x4 z1[4], z2[4], o;
@@ -83,41 +96,34 @@ struct TileAct {
 };
 
 // Forward of one 16-row tile.  x[s]: B fragments of X^T (features 16s..16s+15).
-__device__ __forceinline__ void tile_forward(const uint16_t* W, const float* P, const bf16x4 x[2], TileAct& t,
+// The bias initialises the accumulator (z = b + W^T h).
+__device__ __forceinline__ void tile_forward(const uint16_t* W, const float* B, const bf16x4 x[2], TileAct& t,
                                              int r, int g) {
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        f32x4 acc = ld_b4(B + LB_B1 + 16 * m + 4 * g);
 #pragma unroll
         for (int s = 0; s < 2; ++s) acc = mfma(ld4(W + BF_W1T + (16 * m + r) * MLP_IN + 16 * s + 4 * g), x[s], acc);
         f32x4 h;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            acc[i] += P[OFF_B1 + 16 * m + 4 * g + i];
-            h[i] = gelu(acc[i]);
-        }
+        for (int i = 0; i < 4; ++i) h[i] = gelu(acc[i]);
         t.z1[m] = acc;
         t.h1[m] = pack4(h);
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        f32x4 acc = ld_b4(B + LB_B2 + 16 * m + 4 * g);
 #pragma unroll
         for (int s = 0; s < 4; ++s) acc = mfma(ld4(W + BF_W2T + (16 * m + r) * MLP_HID + 16 * s + 4 * g), t.h1[s], acc);
         f32x4 h;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            acc[i] += P[OFF_B2 + 16 * m + 4 * g + i];
-            h[i] = gelu(acc[i]);
-        }
+        for (int i = 0; i < 4; ++i) h[i] = gelu(acc[i]);
         t.z2[m] = acc;
         t.h2[m] = pack4(h);
     }
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    f32x4 acc = ld_b4(B + LB_B3 + 4 * g);
 #pragma unroll
     for (int s = 0; s < 4; ++s) acc = mfma(ld4(W + BF_W3T + r * MLP_HID + 16 * s + 4 * g), t.h2[s], acc);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] += P[OFF_B3 + 4 * g + i];
     t.o = acc;   // lanes g = 0: outputs 0..3 of row r
 }
 
@@ -133,26 +139,31 @@ __device__ __forceinline__ void load_x(const uint16_t* feat, uint32_t row, bool 
 }
 
 // ---------------------------------------------------------------- features
-__global__ __launch_bounds__(256) void k_warm_features(WarmFeatArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+// Grid-neighbour feature of vertex i: (x_j - x_i) - (r_j - r_i), over the
+// spacing; 0 where the neighbour is absent (j < 0).  Branch-free (an absent
+// neighbour loads vertex i itself), so that every gather of a vertex is in
+// flight at once.
+__device__ __forceinline__ void nbr_delta(const WarmFeatArgs& a, uint32_t i, int32_t j, float4 pi, float4 ri,
+                                          float ih, float d[3]) {
+    const bool has = j >= 0;
+    const uint32_t jj = has ? uint32_t(j) : i;
+    const float4 pj = a.xp[jj], rj = a.rest4[jj];
+    d[0] = has ? ((pj.x - pi.x) - (rj.x - ri.x)) * ih : 0.f;
+    d[1] = has ? ((pj.y - pi.y) - (rj.y - ri.y)) * ih : 0.f;
+    d[2] = has ? ((pj.z - pi.z) - (rj.z - ri.z)) * ih : 0.f;
+}
+
+// The 32 bf16 features of vertex i, packed in pairs (feature 2q in the low
+// half of packed[q]).
+__device__ __forceinline__ void row_features(const WarmFeatArgs& a, uint32_t i, uint32_t packed[MLP_IN / 2]) {
     const float4 pi = a.xp[i], ri = a.rest4[i], ax = a.vaux[i], vi = a.vel[i];
     const float ih = ax.w;
+    const int4* nb = reinterpret_cast<const int4*>(a.nbr8 + size_t(i) * 8);
+    const int4 n0 = nb[0], n1 = nb[1];
+    const int32_t js[8] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
     float f[MLP_IN];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int32_t j = a.nbr8[size_t(i) * 8 + k];
-        float dx = 0.f, dy = 0.f, dz = 0.f;
-        if (j >= 0) {
-            const float4 pj = a.xp[j], rj = a.rest4[j];
-            dx = ((pj.x - pi.x) - (rj.x - ri.x)) * ih;
-            dy = ((pj.y - pi.y) - (rj.y - ri.y)) * ih;
-            dz = ((pj.z - pi.z) - (rj.z - ri.z)) * ih;
-        }
-        f[3 * k] = dx;
-        f[3 * k + 1] = dy;
-        f[3 * k + 2] = dz;
-    }
+    for (int k = 0; k < 8; ++k) nbr_delta(a, i, js[k], pi, ri, ih, f + 3 * k);
     const float sv = a.dt * ih;
     f[24] = vi.x * sv;
     f[25] = vi.y * sv;
@@ -162,9 +173,15 @@ __global__ __launch_bounds__(256) void k_warm_features(WarmFeatArgs a) {
     f[29] = ax.z;
     f[30] = 0.f;
     f[31] = 0.f;
-    uint32_t packed[MLP_IN / 2];
 #pragma unroll
     for (int q = 0; q < MLP_IN / 2; ++q) packed[q] = uint32_t(f2bf(f[2 * q])) | (uint32_t(f2bf(f[2 * q + 1])) << 16);
+}
+
+__global__ __launch_bounds__(256) void k_warm_features(WarmFeatArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    uint32_t packed[MLP_IN / 2];
+    row_features(a, i, packed);
     uint4* dst = reinterpret_cast<uint4*>(a.feat + size_t(i) * MLP_IN);
 #pragma unroll
     for (int q = 0; q < 4; ++q) dst[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
@@ -184,7 +201,8 @@ __global__ void k_warm_targets(const float4* xp, const float4* xc, const float4*
 __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* feat, uint32_t rows, const float* P,
                                                      const uint16_t* bf, float* out) {
     __shared__ __align__(16) uint16_t W[MLP_BF];
-    stage_weights(bf, W);
+    __shared__ __align__(16) float Bs[LB_WORDS];
+    stage_weights(bf, P, W, Bs);
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
@@ -194,7 +212,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* feat, uint3
         bf16x4 x[2];
         load_x(feat, row, valid, g, x);
         TileAct t;
-        tile_forward(W, P, x, t, r, g);
+        tile_forward(W, Bs, x, t, r, g);
         if (g == 0 && valid) {
             out[3 * size_t(row)] = t.o[0];
             out[3 * size_t(row) + 1] = t.o[1];
@@ -203,33 +221,114 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* feat, uint3
     }
 }
 
-__global__ __launch_bounds__(256) void k_warm_apply(const uint16_t* feat, const float4* xp, const float4* vaux,
-                                                    const float* P, const uint16_t* bf, float inv_out_scale,
-                                                    float4* x0, uint32_t n) {
-    __shared__ __align__(16) uint16_t W[MLP_BF];
-    stage_weights(bf, W);
-    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t tile = wave; tile * 16 < n; tile += nwaves) {
-        const uint32_t row = tile * 16 + r;
-        const bool valid = row < n;
-        bf16x4 x[2];
-        load_x(feat, row, valid, g, x);
-        TileAct t;
-        tile_forward(W, P, x, t, r, g);
-        if (g == 0 && valid) {
-            const float4 p = xp[row];
-            float4 q = p;
-            const float ih = vaux[row].w;
-            if (p.w > 0.f && ih > 0.f) {
-                const float sc = inv_out_scale / ih;   // spacing / out_scale
-                q.x = p.x + t.o[0] * sc;
-                q.y = p.y + t.o[1] * sc;
-                q.z = p.z + t.o[2] * sc;
-            }
-            x0[row] = q;
+// A fragments of the three layers for lane 16 g + r, held in registers by
+// an inference wave for all its tiles (56 VGPRs): no LDS traffic (and no
+// bank conflicts of the 64 B / 128 B weight rows) per tile.
+struct WFrag {
+    bf16x4 w1[4][2], w2[4][4], w3[4];
+};
+
+__device__ __forceinline__ void load_wfrag(const uint16_t* bf, int r, int g, WFrag& f) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) f.w1[m][s] = ld4(bf + BF_W1T + (16 * m + r) * MLP_IN + 16 * s + 4 * g);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) f.w2[m][s] = ld4(bf + BF_W2T + (16 * m + r) * MLP_HID + 16 * s + 4 * g);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) f.w3[s] = ld4(bf + BF_W3T + r * MLP_HID + 16 * s + 4 * g);
+}
+
+// Forward of one tile from register-held weights (the same products, in the
+// same order, as tile_forward); o: lanes g = 0 hold outputs 0..3 of row r.
+__device__ __forceinline__ f32x4 tile_infer(const WFrag& wf, const float* B, const bf16x4 x[2], int g) {
+    bf16x4 h1[4], h2[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        f32x4 acc = ld_b4(B + LB_B1 + 16 * m + 4 * g);
+#pragma unroll
+        for (int s = 0; s < 2; ++s) acc = mfma(wf.w1[m][s], x[s], acc);
+        f32x4 h;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[i] = gelu(acc[i]);
+        h1[m] = pack4(h);
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        f32x4 acc = ld_b4(B + LB_B2 + 16 * m + 4 * g);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc = mfma(wf.w2[m][s], h1[s], acc);
+        f32x4 h;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[i] = gelu(acc[i]);
+        h2[m] = pack4(h);
+    }
+    f32x4 acc = ld_b4(B + LB_B3 + 4 * g);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc = mfma(wf.w3[s], h2[s], acc);
+    return acc;
+}
+
+// Warm start of one substep: features of the predicted state, the forward,
+// and x0 = xp + spacing * MLP / out_scale (pinned vertices keep xp), with no
+// feature buffer in HBM.  Each pass of a block takes 256 vertices: thread t
+// builds the features of vertex base + t (row_features, as k_warm_features
+// does) into an LDS row, then each wave runs the forward of 4 of the 16
+// tiles straight from LDS.
+constexpr uint32_t WI_ROWS = 256;            // vertices per block pass (= threads)
+constexpr uint32_t WI_STRIDE = MLP_IN + 4;   // bf16 per LDS row: 72 B, spreads rows over the banks
+
+__global__ __launch_bounds__(256) void k_warm_infer(WarmFeatArgs a, const float* P, const uint16_t* bf,
+                                                    float inv_out_scale, float4* x0, uint32_t* count) {
+    if (count && blockIdx.x == 0 && threadIdx.x == 0) {   // ungated: accepted, one more warm substep
+        count[0] = 0u;
+        count[1] = count[1] + 1u;
+    }
+    __shared__ __align__(16) float Bs[LB_WORDS];
+    __shared__ __align__(16) uint16_t F[WI_ROWS * WI_STRIDE];
+    for (uint32_t q = threadIdx.x; q < uint32_t(LB_WORDS); q += blockDim.x) {
+        Bs[q] = q < uint32_t(LB_B2) ? P[OFF_B1 + q] : q < uint32_t(LB_B3) ? P[OFF_B2 + q - LB_B2] : P[OFF_B3 + q - LB_B3];
+    }
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4, wv = threadIdx.x >> 6;
+    WFrag wf;
+    load_wfrag(bf, r, g, wf);
+    for (uint32_t base = blockIdx.x * WI_ROWS; base < a.n; base += gridDim.x * WI_ROWS) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t packed[MLP_IN / 2];
+        if (i < a.n) {
+            row_features(a, i, packed);
+        } else {
+#pragma unroll
+            for (int q = 0; q < MLP_IN / 2; ++q) packed[q] = 0u;
         }
+        uint2* frow = reinterpret_cast<uint2*>(F + threadIdx.x * WI_STRIDE);
+#pragma unroll
+        for (int q = 0; q < MLP_IN / 4; ++q) frow[q] = make_uint2(packed[2 * q], packed[2 * q + 1]);
+        __syncthreads();   // (the first pass: also the bias copy)
+#pragma unroll 1
+        for (uint32_t tt = 0; tt < 4; ++tt) {
+            const uint32_t lt = 4 * wv + tt;   // tile of the pass
+            if (base + 16 * lt >= a.n) break;
+            const uint32_t row = base + 16 * lt + r;
+            bf16x4 x[2];
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2) x[s2] = ld4(F + (16 * lt + r) * WI_STRIDE + 16 * s2 + 4 * g);
+            const f32x4 o = tile_infer(wf, Bs, x, g);
+            if (g == 0 && row < a.n) {
+                const float4 p = a.xp[row];
+                const float ih = a.vaux[row].w;
+                float4 q = p;
+                if (p.w > 0.f && ih > 0.f) {
+                    const float sc = inv_out_scale / ih;   // spacing / out_scale
+                    q.x = p.x + o[0] * sc;
+                    q.y = p.y + o[1] * sc;
+                    q.z = p.z + o[2] * sc;
+                }
+                x0[row] = q;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -255,7 +354,8 @@ __global__ __launch_bounds__(256) void k_mlp_train(const uint16_t* feat, const f
                                                    const uint16_t* bf, float* partial) {
     __shared__ __align__(16) uint16_t W[MLP_BF];
     __shared__ __align__(16) uint16_t tr_all[4][TR_WORDS];
-    stage_weights(bf, W);
+    __shared__ __align__(16) float Bs[LB_WORDS];
+    stage_weights(bf, P, W, Bs);
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4, wv = threadIdx.x >> 6;
     const uint32_t wave = blockIdx.x * 4 + wv, nwaves = gridDim.x * 4;
     uint16_t* tr = tr_all[wv];
@@ -279,7 +379,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(const uint16_t* feat, const f
         bf16x4 x[2];
         load_x(feat, row, valid, g, x);
         TileAct t;
-        tile_forward(W, P, x, t, r, g);
+        tile_forward(W, Bs, x, t, r, g);
 
         f32x4 dO = z4;
         if (g == 0 && valid) {
@@ -491,11 +591,11 @@ void launch_mlp_pack(const float* P, uint16_t* bf, hipStream_t s) {
     hipLaunchKernelGGL(k_mlp_pack, dim3(cdiv(MLP_PARAMS, 256)), dim3(256), 0, s, P, bf);
 }
 
-void launch_warm_apply(const uint16_t* feat, const float4* xp, const float4* vaux, const float* P,
-                       const uint16_t* bf, float inv_out_scale, float4* x0, uint32_t n, hipStream_t s) {
-    if (n == 0) return;
-    const uint32_t blocks = std::min<uint32_t>(cdiv(cdiv(n, 16), 4), 2048);
-    hipLaunchKernelGGL(k_warm_apply, dim3(blocks), dim3(256), 0, s, feat, xp, vaux, P, bf, inv_out_scale, x0, n);
+void launch_warm_infer(const WarmFeatArgs& a, const float* P, const uint16_t* bf, float inv_out_scale, float4* x0,
+                       uint32_t* count, hipStream_t s) {
+    if (a.n == 0) return;
+    const uint32_t blocks = std::min<uint32_t>(cdiv(a.n, WI_ROWS), 2048);
+    hipLaunchKernelGGL(k_warm_infer, dim3(blocks), dim3(256), 0, s, a, P, bf, inv_out_scale, x0, count);
 }
 
 }  // namespace xpbd

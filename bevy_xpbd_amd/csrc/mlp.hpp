@@ -65,8 +65,10 @@ void launch_mlp_reduce_adam(const float* partial, float* P, float* m, float* v, 
                             float* loss_out, float lr, float beta1_t, float beta2_t, hipStream_t s);
 // bf16 operand copies from the f32 master parameters
 void launch_mlp_pack(const float* P, uint16_t* bf, hipStream_t s);
-// warm start: x0 = xp + spacing * MLP(feat) / out_scale (pinned vertices keep xp)
-void launch_warm_apply(const uint16_t* feat, const float4* xp, const float4* vaux, const float* P,
-                       const uint16_t* bf, float inv_out_scale, float4* x0, uint32_t n, hipStream_t s);
+// warm start of one substep: features of a.xp (a.feat unused), the forward,
+// x0 = xp + spacing * MLP / out_scale (pinned vertices keep xp); count
+// (nullable): count[0] = 0, count[1] += 1
+void launch_warm_infer(const WarmFeatArgs& a, const float* P, const uint16_t* bf, float inv_out_scale, float4* x0,
+                       uint32_t* count, hipStream_t s);
 
 }  // namespace xpbd

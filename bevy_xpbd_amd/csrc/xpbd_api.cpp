@@ -88,6 +88,10 @@ struct xpbd_cloth {
     uint8_t* ground = nullptr;
     uint2* gs_ij = nullptr;
     float2* gs_rc = nullptr;
+    // the constraints again, ordered by lower endpoint (neighbouring loads):
+    // the warm-start gate's residual
+    uint2* near_ij = nullptr;
+    float* near_rest = nullptr;
     uint32_t* gs_off = nullptr;
     uint32_t *keys = nullptr, *ids = nullptr, *keys2 = nullptr, *ids2 = nullptr, *hist = nullptr;
     uint2* table = nullptr;
@@ -128,9 +132,9 @@ struct xpbd_cloth {
     bool warm_ok = false;
     int32_t* nbr8 = nullptr;
     float4* vaux = nullptr;
-    uint16_t* feat = nullptr;      // n x 32 bf16 features (allocated on first use)
     float4* W0 = nullptr;          // warm-start iterate
     uint32_t* warm_flag = nullptr; // [0] fallback flag, [1] warm substeps, [2] fallbacks
+    uint32_t* warm_acc = nullptr;  // the gate's accumulators (warm_gate_words())
     xpbd_mlp* warm = nullptr;
     uint32_t warm_iters = 2;
     float warm_threshold = 1e-4f;
@@ -330,7 +334,7 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         F = h->A;
         nparts = h->gs_blocks;
     } else if (h->warm) {
-        // learned warm start: predict -> features -> x0 = xp + MLP -> warm
+        // learned warm start: predict -> (features, MLP) x0 = xp + MLP -> warm
         // iterations.  With a finite threshold, the relative residual of the
         // result gates one persistent plain substep (predict + I iterations)
         // that does nothing when the warm result is accepted.
@@ -345,12 +349,12 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         fa.vaux = h->vaux;
         fa.dt = h->u.dt;
         fa.n = h->n;
-        fa.feat = h->feat;
-        launch_warm_features(fa, s);
         // x0 lands where the ping-pong (W0 / sP) ends in W0
         float4* cur = (h->warm_iters % 2) ? h->sP : h->W0;
         float4* nxt = (h->warm_iters % 2) ? h->W0 : h->sP;
-        launch_warm_apply(h->feat, h->A, h->vaux, m->P, m->bf, 1.0f / m->out_scale, cur, h->n, s);
+        // (without a gate the launch counts the warm substep itself)
+        launch_warm_infer(fa, m->P, m->bf, 1.0f / m->out_scale, cur,
+                          std::isfinite(h->warm_threshold) ? nullptr : h->warm_flag, s);
         for (uint32_t k = 0; k < h->warm_iters; ++k) {
             a.src = cur;
             a.dst = nxt;
@@ -358,12 +362,12 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
             launch_jacobi(0, a, h->jacobi_block, s);
             std::swap(cur, nxt);
         }
-        float4* fin = h->W0;
         if (std::isfinite(h->warm_threshold)) {
-            launch_zero_u32(reinterpret_cast<uint32_t*>(h->resid), 2, s);
-            launch_residual(h->W0, h->gs_ij, h->gs_rc, h->e, h->resid, s);
-            launch_warm_flag(h->resid, h->rest_sq_sum, h->warm_threshold, h->warm_flag, s);
-            launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
+            // gate (one launch, also clears the barrier words), then the
+            // fallback: a persistent plain substep that runs only if the gate
+            // rejected the warm result; the state comes from either
+            launch_warm_gate(h->W0, h->near_ij, h->near_rest, h->e, h->rest_sq_sum, h->warm_threshold,
+                             h->warm_acc, h->warm_flag, h->barrier, s);
             JacobiArgs g = jacobi_args(h, p);
             PersistArgs q{};
             q.X = h->X;
@@ -376,16 +380,20 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
             q.error = h->err;
             q.gate = h->warm_flag;
             launch_jacobi_persistent(g, q, h->pj_blocks, s);
-            fin = (iters % 2 == 0) ? h->A : h->B;
-            launch_copy4_unless(h->W0, fin, h->n, h->warm_flag, s);
-        } else {
-            launch_warm_count(h->warm_flag, s);
-        }
-        if (collision) {
-            F = fin;
+            float4* alt = (iters % 2 == 0) ? h->A : h->B;
+            if (collision) {
+                launch_copy4_if(alt, h->W0, h->n, h->warm_flag, s);
+                F = h->W0;
+                if (h->nclothes == 1) nparts = launch_aabb_of(F, h->n, h->aabb, s);
+            } else {
+                launch_finalize_sel(h->W0, alt, h->warm_flag, h->X, h->V, h->has_ground ? h->ground : nullptr, p,
+                                    s);
+            }
+        } else if (collision) {
+            F = h->W0;
             if (h->nclothes == 1) nparts = launch_aabb_of(F, h->n, h->aabb, s);
         } else {
-            launch_finalize(fin, h->X, h->V, h->has_ground ? h->ground : nullptr, p, s);
+            launch_finalize(h->W0, h->X, h->V, h->has_ground ? h->ground : nullptr, p, s);
         }
     } else if (h->cfg.persistent_jacobi) {
         // predict and all iterations in one launch; the result lands in X,V
@@ -725,6 +733,22 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(dalloc(h, &h->gs_rc, e));
     TRY(upload(h, h->gs_ij, ij.data(), e));
     TRY(upload(h, h->gs_rc, rc.data(), e));
+    {
+        std::vector<uint32_t> ord(e);
+        for (size_t k = 0; k < e; ++k) ord[k] = uint32_t(k);
+        auto lo = [&](uint32_t k) { return std::min(ij[k].x, ij[k].y); };
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return lo(a) < lo(b); });
+        std::vector<uint2> nij(e);
+        std::vector<float> nrest(e);
+        for (size_t k = 0; k < e; ++k) {
+            nij[k] = ij[ord[k]];
+            nrest[k] = rc[ord[k]].x;
+        }
+        TRY(dalloc(h, &h->near_ij, e));
+        TRY(dalloc(h, &h->near_rest, e));
+        TRY(upload(h, h->near_ij, nij.data(), e));
+        TRY(upload(h, h->near_rest, nrest.data(), e));
+    }
     h->batch_offsets = m.batch_offsets;
     h->num_batches = uint32_t(m.batch_offsets.size() - 1);
     TRY(dalloc(h, &h->gs_off, h->batch_offsets.size()));
@@ -911,11 +935,14 @@ int build_warm_topology(xpbd_cloth* h, const Mesh& m) {
 }
 
 int ensure_warm_buffers(xpbd_cloth* h) {
-    if (!h->feat) TRY(dalloc(h, &h->feat, size_t(h->n) * MLP_IN));
     if (!h->W0) TRY(dalloc(h, &h->W0, h->n));
     if (!h->warm_flag) {
         TRY(dalloc(h, &h->warm_flag, 4));
         HIPC(h, hipMemsetAsync(h->warm_flag, 0, 4 * sizeof(uint32_t), h->stream));
+    }
+    if (!h->warm_acc) {
+        TRY(dalloc(h, &h->warm_acc, warm_gate_words()));
+        HIPC(h, hipMemsetAsync(h->warm_acc, 0, warm_gate_words() * sizeof(uint32_t), h->stream));
     }
     return XPBD_OK;
 }

@@ -32,9 +32,9 @@ def small_batch(X):
     return X.ClothMeshData.batch(descs)
 
 
-def make(X, mesh, I=10, use_graph=0):
+def make(X, mesh, I=10, use_graph=0, coll=False):
     cfg = X.ClothSimConfig.default(solve_substeps=8, solve_inner_iterations=I, use_graph=use_graph)
-    u = X.ClothSimUniforms.default(X.SOLVER_JACOBI, 8, coll_scale=0.0)
+    u = X.ClothSimUniforms.default(X.SOLVER_JACOBI, 8, coll_scale=0.38 if coll else 0.0)
     return X.ClothComputePlugin(mesh, cfg, u), cfg, u
 
 
@@ -72,6 +72,43 @@ def test_features_bit_exact(O, X):
     want = M.features(xp, vel, f["rest_positions"], nbr, aux, u.dt)
     assert got.tobytes() == want.tobytes(), np.argwhere(got != want)[:5]
     sim.close()
+
+
+def test_warm_substep_start_is_xp_plus_scaled_forward(O, X):
+    """With 0 warm iterations and no gate the substep's result is x0 itself:
+    it must equal the predicted state plus spacing * forward / out_scale,
+    where forward is the network on the features of that state (both taken
+    through their own entry points), bit for bit; pinned vertices keep the
+    prediction and grounded ones are clamped to y >= 0."""
+    note("case warm x0")
+    mesh = small_batch(X)
+    f = mesh.fields()
+    sim, cfg, u = make(X, mesh)
+    sim.step_frames(2)
+    pos, vel = sim.read_state()
+    n = len(pos)
+    d = X.DeviceArray((n, 32), np.uint16)
+    sim.warm_features(d)
+    out_scale = 2.5
+    mlp = X.WarmStartMLP(seed=1)
+    mlp.set_params(M.init_params(seed=3), out_scale=out_scale)
+    out = mlp.forward(d, n)
+    assert np.abs(out).max() > 1e-3
+    sim.set_warm_start(mlp, warm_iterations=0, residual_threshold=float("inf"))
+    sim.step_substeps(1, False)
+    got, _ = sim.read_state()
+    _, aux = M.grid_topology(f)
+    ih = aux[:, 3].astype(np.float32)
+    xp = predicted(O, X, f, pos, vel, u, cfg)
+    live = (f["inv_mass"] > 0) & (ih > 0)
+    sc = np.float32(np.float32(1.0) / np.float32(out_scale)) / np.where(live, ih, np.float32(1))
+    want = xp.copy()
+    want[live] = (xp[live] + (out[live] * sc[live, None]).astype(np.float32)).astype(np.float32)
+    low = (f["ground"] != 0) & (want[:, 1] < 0)
+    want[low, 1] = 0.0
+    assert bits_equal(got[:, :3], want), np.argwhere(got[:, :3] != want)[:5]
+    sim.close()
+    mlp.close()
 
 
 def rel(a, b):
@@ -128,17 +165,19 @@ def test_training_reduces_loss(X):
     mlp.close()
 
 
-def test_zero_network_warm_substep_equals_plain(X):
+@pytest.mark.parametrize("threshold,use_graph,coll", [(float("inf"), 0, False), (1e30, 0, False), (1e30, 1, False),
+                                                     (float("inf"), 0, True), (1e30, 1, True)])
+def test_zero_network_warm_substep_equals_plain(X, threshold, use_graph, coll):
     """x0 = xp exactly for a zero network, so I warm iterations reproduce the
-    plain substep bit for bit (features, apply, iteration chain, residual
-    gate, selection and velocity update)."""
-    note("case warm zero network")
+    plain substep bit for bit (features, apply, iteration chain, the gate
+    accepting every substep, selection, collision and velocity update)."""
+    note(f"case warm zero network threshold={threshold} graph={use_graph} coll={coll}")
     mesh = small_batch(X)
-    a, _, _ = make(X, mesh, I=6)
-    b, _, _ = make(X, mesh, I=6)
+    a, _, _ = make(X, mesh, I=6, use_graph=use_graph, coll=coll)
+    b, _, _ = make(X, mesh, I=6, use_graph=use_graph, coll=coll)
     mlp = X.WarmStartMLP(seed=1)
     mlp.set_params(np.zeros(X.MLP_PARAMS, np.float32), out_scale=1.0)
-    a.set_warm_start(mlp, warm_iterations=6, residual_threshold=float("inf"))
+    a.set_warm_start(mlp, warm_iterations=6, residual_threshold=threshold)
     for k in range(3):
         a.step_substeps(4, False)
         b.step_substeps(4, False)
@@ -151,12 +190,12 @@ def test_zero_network_warm_substep_equals_plain(X):
     mlp.close()
 
 
-@pytest.mark.parametrize("use_graph", [0, 1])
-def test_threshold_zero_always_falls_back_to_plain(X, use_graph):
-    note(f"case warm fallback graph={use_graph}")
+@pytest.mark.parametrize("use_graph,coll", [(0, False), (1, False), (1, True)])
+def test_threshold_zero_always_falls_back_to_plain(X, use_graph, coll):
+    note(f"case warm fallback graph={use_graph} coll={coll}")
     mesh = small_batch(X)
-    a, _, _ = make(X, mesh, I=10, use_graph=use_graph)
-    b, _, _ = make(X, mesh, I=10, use_graph=use_graph)
+    a, _, _ = make(X, mesh, I=10, use_graph=use_graph, coll=coll)
+    b, _, _ = make(X, mesh, I=10, use_graph=use_graph, coll=coll)
     mlp = X.WarmStartMLP(seed=2)
     P, _ = mlp.params()
     mlp.set_params(P, out_scale=10.0)
@@ -169,4 +208,26 @@ def test_threshold_zero_always_falls_back_to_plain(X, use_graph):
     assert a.warm_stats() == (16, 16)
     a.close()
     b.close()
+    mlp.close()
+
+
+def test_gate_residual_matches_reported_residual(X):
+    """The gate's f32 residual agrees with the reported (f64) one: a threshold
+    just above the reported relative residual of the warm result accepts it,
+    one just below falls back."""
+    note("case warm gate threshold")
+    mesh = small_batch(X)
+    mlp = X.WarmStartMLP(seed=1)
+    mlp.set_params(np.zeros(X.MLP_PARAMS, np.float32), out_scale=1.0)
+    ref, _, _ = make(X, mesh, I=6)
+    ref.step_substeps(1, False)
+    r = ref.residual()
+    ref.close()
+    assert r > 0
+    for scale, want in ((1.0 + 1e-3, 0), (1.0 - 1e-3, 1)):
+        a, _, _ = make(X, mesh, I=6)
+        a.set_warm_start(mlp, warm_iterations=6, residual_threshold=r * scale)
+        a.step_substeps(1, False)
+        assert a.warm_stats() == (1, want), scale
+        a.close()
     mlp.close()
