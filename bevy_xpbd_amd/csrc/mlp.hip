@@ -26,9 +26,14 @@ inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 // f32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32)
 __device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f)); }
 
-typedef __bf16 bf16v4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16v2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+// two v_cvt_pk_bf16_f32, one per pair
 __device__ __forceinline__ bf16x4 pack4(f32x4 v) {
-    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bf16v4));
+    const bf16v2 lo = __builtin_convertvector((f32x2){v[0], v[1]}, bf16v2);
+    const bf16v2 hi = __builtin_convertvector((f32x2){v[2], v[3]}, bf16v2);
+    return __builtin_bit_cast(bf16x4, (u32x2){__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi)});
 }
 
 __device__ __forceinline__ bf16x4 ld4(const uint16_t* p) { return *reinterpret_cast<const bf16x4*>(p); }
@@ -140,17 +145,22 @@ __device__ __forceinline__ void load_x(const uint16_t* feat, uint32_t row, bool 
 
 // ---------------------------------------------------------------- features
 // Grid-neighbour feature of vertex i: (x_j - x_i) - (r_j - r_i), over the
-// spacing; 0 where the neighbour is absent (j < 0).  Branch-free (an absent
-// neighbour loads vertex i itself), so that every gather of a vertex is in
-// flight at once.
+// spacing; +0 where the neighbour is absent (j < 0).  An absent neighbour
+// loads vertex i itself and its feature bits are masked to +0 (not a select,
+// which the compiler turns into a branch around the loads), so that every
+// gather of a vertex is in flight at once.
+__device__ __forceinline__ float keep_if(float v, uint32_t mask) {
+    return __uint_as_float(__float_as_uint(v) & mask);
+}
+
 __device__ __forceinline__ void nbr_delta(const WarmFeatArgs& a, uint32_t i, int32_t j, float4 pi, float4 ri,
                                           float ih, float d[3]) {
-    const bool has = j >= 0;
-    const uint32_t jj = has ? uint32_t(j) : i;
+    const uint32_t mask = uint32_t(~(j >> 31));   // j >= 0: all ones
+    const uint32_t jj = j >= 0 ? uint32_t(j) : i;
     const float4 pj = a.xp[jj], rj = a.rest4[jj];
-    d[0] = has ? ((pj.x - pi.x) - (rj.x - ri.x)) * ih : 0.f;
-    d[1] = has ? ((pj.y - pi.y) - (rj.y - ri.y)) * ih : 0.f;
-    d[2] = has ? ((pj.z - pi.z) - (rj.z - ri.z)) * ih : 0.f;
+    d[0] = keep_if(((pj.x - pi.x) - (rj.x - ri.x)) * ih, mask);
+    d[1] = keep_if(((pj.y - pi.y) - (rj.y - ri.y)) * ih, mask);
+    d[2] = keep_if(((pj.z - pi.z) - (rj.z - ri.z)) * ih, mask);
 }
 
 // The 32 bf16 features of vertex i, packed in pairs (feature 2q in the low
@@ -240,34 +250,49 @@ __device__ __forceinline__ void load_wfrag(const uint16_t* bf, int r, int g, WFr
     for (int s = 0; s < 4; ++s) f.w3[s] = ld4(bf + BF_W3T + r * MLP_HID + 16 * s + 4 * g);
 }
 
-// Forward of one tile from register-held weights (the same products, in the
-// same order, as tile_forward); o: lanes g = 0 hold outputs 0..3 of row r.
-__device__ __forceinline__ f32x4 tile_infer(const WFrag& wf, const float* B, const bf16x4 x[2], int g) {
-    bf16x4 h1[4], h2[4];
+// Forward of NT tiles from register-held weights (the same products, in the
+// same order, as tile_forward; the tiles' chains interleave to hide the MFMA
+// and LDS latencies); o[t]: lanes g = 0 hold outputs 0..3 of row r of tile t.
+template <int NT>
+__device__ __forceinline__ void tile_infer(const WFrag& wf, const float* B, const bf16x4 (&x)[NT][2], int g,
+                                           f32x4 (&o)[NT]) {
+    bf16x4 h1[NT][4], h2[NT][4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        f32x4 acc = ld_b4(B + LB_B1 + 16 * m + 4 * g);
+        const f32x4 b = ld_b4(B + LB_B1 + 16 * m + 4 * g);
 #pragma unroll
-        for (int s = 0; s < 2; ++s) acc = mfma(wf.w1[m][s], x[s], acc);
-        f32x4 h;
+        for (int t = 0; t < NT; ++t) {
+            f32x4 acc = b;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) h[i] = gelu(acc[i]);
-        h1[m] = pack4(h);
+            for (int s = 0; s < 2; ++s) acc = mfma(wf.w1[m][s], x[t][s], acc);
+            f32x4 h;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) h[i] = gelu(acc[i]);
+            h1[t][m] = pack4(h);
+        }
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        f32x4 acc = ld_b4(B + LB_B2 + 16 * m + 4 * g);
+        const f32x4 b = ld_b4(B + LB_B2 + 16 * m + 4 * g);
 #pragma unroll
-        for (int s = 0; s < 4; ++s) acc = mfma(wf.w2[m][s], h1[s], acc);
-        f32x4 h;
+        for (int t = 0; t < NT; ++t) {
+            f32x4 acc = b;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) h[i] = gelu(acc[i]);
-        h2[m] = pack4(h);
+            for (int s = 0; s < 4; ++s) acc = mfma(wf.w2[m][s], h1[t][s], acc);
+            f32x4 h;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) h[i] = gelu(acc[i]);
+            h2[t][m] = pack4(h);
+        }
     }
-    f32x4 acc = ld_b4(B + LB_B3 + 4 * g);
+    const f32x4 b = ld_b4(B + LB_B3 + 4 * g);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) acc = mfma(wf.w3[s], h2[s], acc);
-    return acc;
+    for (int t = 0; t < NT; ++t) {
+        f32x4 acc = b;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc = mfma(wf.w3[s], h2[t][s], acc);
+        o[t] = acc;
+    }
 }
 
 // Warm start of one substep: features of the predicted state, the forward,
@@ -278,8 +303,11 @@ __device__ __forceinline__ f32x4 tile_infer(const WFrag& wf, const float* B, con
 // tiles straight from LDS.
 constexpr uint32_t WI_ROWS = 256;            // vertices per block pass (= threads)
 constexpr uint32_t WI_STRIDE = MLP_IN + 4;   // bf16 per LDS row: 72 B, spreads rows over the banks
+#ifndef WI_NT
+#define WI_NT 1u   // tiles per forward step of a wave (1, 2 or 4)
+#endif
 
-__global__ __launch_bounds__(256) void k_warm_infer(WarmFeatArgs a, const float* P, const uint16_t* bf,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_warm_infer(WarmFeatArgs a, const float* P, const uint16_t* bf,
                                                     float inv_out_scale, float4* x0, uint32_t* count) {
     if (count && blockIdx.x == 0 && threadIdx.x == 0) {   // ungated: accepted, one more warm substep
         count[0] = 0u;
@@ -307,25 +335,31 @@ __global__ __launch_bounds__(256) void k_warm_infer(WarmFeatArgs a, const float*
         for (int q = 0; q < MLP_IN / 4; ++q) frow[q] = make_uint2(packed[2 * q], packed[2 * q + 1]);
         __syncthreads();   // (the first pass: also the bias copy)
 #pragma unroll 1
-        for (uint32_t tt = 0; tt < 4; ++tt) {
-            const uint32_t lt = 4 * wv + tt;   // tile of the pass
+        for (uint32_t tt = 0; tt < 4; tt += WI_NT) {
+            const uint32_t lt = 4 * wv + tt;   // first tile of the step
             if (base + 16 * lt >= a.n) break;
-            const uint32_t row = base + 16 * lt + r;
-            bf16x4 x[2];
+            bf16x4 x[WI_NT][2];
+            f32x4 o[WI_NT];
 #pragma unroll
-            for (int s2 = 0; s2 < 2; ++s2) x[s2] = ld4(F + (16 * lt + r) * WI_STRIDE + 16 * s2 + 4 * g);
-            const f32x4 o = tile_infer(wf, Bs, x, g);
-            if (g == 0 && row < a.n) {
-                const float4 p = a.xp[row];
-                const float ih = a.vaux[row].w;
-                float4 q = p;
-                if (p.w > 0.f && ih > 0.f) {
-                    const float sc = inv_out_scale / ih;   // spacing / out_scale
-                    q.x = p.x + o[0] * sc;
-                    q.y = p.y + o[1] * sc;
-                    q.z = p.z + o[2] * sc;
+            for (int t = 0; t < int(WI_NT); ++t)
+#pragma unroll
+                for (int s2 = 0; s2 < 2; ++s2) x[t][s2] = ld4(F + (16 * (lt + t) + r) * WI_STRIDE + 16 * s2 + 4 * g);
+            tile_infer<WI_NT>(wf, Bs, x, g, o);
+#pragma unroll
+            for (int t = 0; t < int(WI_NT); ++t) {
+                const uint32_t row = base + 16 * (lt + t) + r;
+                if (g == 0 && row < a.n) {
+                    const float4 p = a.xp[row];
+                    const float ih = a.vaux[row].w;
+                    float4 q = p;
+                    if (p.w > 0.f && ih > 0.f) {
+                        const float sc = inv_out_scale / ih;   // spacing / out_scale
+                        q.x = p.x + o[t][0] * sc;
+                        q.y = p.y + o[t][1] * sc;
+                        q.z = p.z + o[t][2] * sc;
+                    }
+                    x0[row] = q;
                 }
-                x0[row] = q;
             }
         }
         __syncthreads();
@@ -594,7 +628,20 @@ void launch_mlp_pack(const float* P, uint16_t* bf, hipStream_t s) {
 void launch_warm_infer(const WarmFeatArgs& a, const float* P, const uint16_t* bf, float inv_out_scale, float4* x0,
                        uint32_t* count, hipStream_t s) {
     if (a.n == 0) return;
-    const uint32_t blocks = std::min<uint32_t>(cdiv(a.n, WI_ROWS), 2048);
+    // one co-resident wave of blocks: each loads the weights once and walks
+    // its passes (more blocks only reload the weights)
+    static uint32_t resident[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (resident[dev] == 0) {
+        int cus = 0, per_cu = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_warm_infer, int(WI_ROWS), 0) == hipSuccess)
+            resident[dev] = uint32_t(std::max(1, cus * per_cu));
+        else
+            resident[dev] = 2048;
+    }
+    const uint32_t blocks = std::min<uint32_t>(cdiv(a.n, WI_ROWS), resident[dev]);
     hipLaunchKernelGGL(k_warm_infer, dim3(blocks), dim3(256), 0, s, a, P, bf, inv_out_scale, x0, count);
 }
 
