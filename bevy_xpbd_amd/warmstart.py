@@ -237,3 +237,45 @@ def evaluate(mlp, descs=None, start_frames: int = 2, plain_iterations=(2, 4, 10,
         "gated_fallbacks": fb,
         "gated_substeps_per_s": round(80 / tg, 1),
     }
+
+
+def evaluate_batch(mlp, count: int = 512, iterations: int = 10, warm_iterations=(3, 4), frames: int = 20,
+                   start_frames: int = 2):
+    """The warm start at scale behind its gate: `count` held-out stiff sheets
+    in one batch.  The plain solver at `iterations` sets the gate: the
+    largest relative residual it leaves over 10 frames.  Then, for each warm
+    iteration count, `frames` frames with the gate at that threshold: the
+    substeps that fell back to the plain substep, the substep rate against
+    the plain one and the residual at the end."""
+    mesh = A.ClothMeshData.batch(stiff_eval_descs(count=count))
+    sim = make_sim(mesh, iterations=iterations, use_graph=1)
+    sim.step_frames(start_frames)
+    res = []
+    for _ in range(10):
+        sim.step_frames(1)
+        res.append(sim.residual())
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step_frames(frames)
+    sim.sync()
+    plain_rate = frames * 8 / (time.perf_counter() - t0)
+    sim.close()
+    threshold = max(res)
+    out = {"sheets": count, "particles": mesh.num_particles, "plain_iterations": iterations,
+           "plain_residual_range": [min(res), threshold], "plain_substeps_per_s": round(plain_rate, 1),
+           "gate_threshold": threshold, "warm": {}}
+    for k in warm_iterations:
+        g = make_sim(mesh, use_graph=1)
+        g.set_warm_start(mlp, warm_iterations=k, residual_threshold=threshold)
+        g.step_frames(start_frames)
+        g.sync()
+        g.warm_stats(reset=True)
+        t0 = time.perf_counter()
+        g.step_frames(frames)
+        g.sync()
+        rate = frames * 8 / (time.perf_counter() - t0)
+        sub, fb = g.warm_stats()
+        out["warm"][str(k)] = {"fallbacks": fb, "substeps": sub, "substeps_per_s": round(rate, 1),
+                               "speedup_vs_plain": round(rate / plain_rate, 3), "final_residual": g.residual()}
+        g.close()
+    return out
