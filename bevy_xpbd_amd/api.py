@@ -163,6 +163,7 @@ def lib():
             "xpbd_write_state": (i32, [vp, vp, vp]),
             "xpbd_publish": (i32, [vp, vp, vp]),
             "xpbd_bind_render_buffers": (i32, [vp, vp, vp]),
+            "xpbd_import_render_buffers_fd": (i32, [vp, i32, u64, u64, i32, u64, u64]),
             "xpbd_residual": (i32, [vp, C.POINTER(C.c_double)]),
             "xpbd_time_kernel": (i32, [vp, i32, u32, C.POINTER(C.c_float)]),
             "xpbd_profile_frames": (i32, [vp, u32, i32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
@@ -383,6 +384,15 @@ class ClothComputePlugin:
     def bind_render_buffers(self, dst_pos_ptr: int | None, dst_nrm_ptr: int | None):
         _check(lib().xpbd_bind_render_buffers(self._h, dst_pos_ptr, dst_nrm_ptr), "xpbd_bind_render_buffers",
                self._h)
+
+    def import_render_buffers_fd(self, pos_fd: int, pos_size: int, pos_offset: int = 0, nrm_fd: int = -1,
+                                 nrm_size: int = 0, nrm_offset: int = 0):
+        """Zero-copy render buffers from exported device memory (POSIX fds,
+        e.g. Vulkan opaque fds of the renderer's position / normal buffers);
+        the library takes the descriptors over on success."""
+        _check(lib().xpbd_import_render_buffers_fd(self._h, int(pos_fd), int(pos_size), int(pos_offset), int(nrm_fd),
+                                                   int(nrm_size), int(nrm_offset)),
+               "xpbd_import_render_buffers_fd", self._h)
 
     def residual(self) -> float:
         r = C.c_double()

@@ -111,6 +111,10 @@ struct xpbd_cloth {
     double* resid = nullptr;
 
     float4 *render_pos = nullptr, *render_nrm = nullptr;
+    // imported render buffers (xpbd_import_render_buffers_fd): the imports
+    // and their mapped ranges, released at the next bind or at destroy
+    hipExternalMemory_t ext_mem[2] = {nullptr, nullptr};
+    void* ext_ptr[2] = {nullptr, nullptr};
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -503,6 +507,18 @@ void drop_graph(xpbd_cloth* h) {
     h->gexec = nullptr;
     h->graph = nullptr;
     h->graph_valid = false;
+}
+
+// Unmap and release imported render buffers (the caller has synchronised).
+void release_imports(xpbd_cloth* h) {
+    for (int k = 0; k < 2; ++k) {
+        if (h->ext_ptr[k]) (void)hipFree(h->ext_ptr[k]);
+        if (h->ext_mem[k]) (void)hipDestroyExternalMemory(h->ext_mem[k]);
+        if (h->render_pos == h->ext_ptr[k] && h->ext_ptr[k]) h->render_pos = nullptr;
+        if (h->render_nrm == h->ext_ptr[k] && h->ext_ptr[k]) h->render_nrm = nullptr;
+        h->ext_ptr[k] = nullptr;
+        h->ext_mem[k] = nullptr;
+    }
 }
 
 int check_device_flag(xpbd_cloth* h) {
@@ -1242,6 +1258,7 @@ XPBD_API void xpbd_destroy(xpbd_cloth* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graph(h);
+    release_imports(h);
     for (void* p : h->allocs) (void)hipFree(p);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -1345,9 +1362,48 @@ XPBD_API int xpbd_publish(xpbd_cloth* h, void* dpos, void* dnrm) {
 
 XPBD_API int xpbd_bind_render_buffers(xpbd_cloth* h, void* dpos, void* dnrm) {
     if (!h) return XPBD_E_INVALID;
+    if (h->stream) HIPC(h, hipStreamSynchronize(h->stream));
+    release_imports(h);
     h->render_pos = static_cast<float4*>(dpos);
     h->render_nrm = static_cast<float4*>(dnrm);
     drop_graph(h);
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_import_render_buffers_fd(xpbd_cloth* h, int pos_fd, uint64_t pos_size, uint64_t pos_offset,
+                                          int nrm_fd, uint64_t nrm_size, uint64_t nrm_offset) {
+    if (!h) return XPBD_E_INVALID;
+    const uint64_t need = uint64_t(h->n) * sizeof(float4);
+    const int fds[2] = {pos_fd, nrm_fd};
+    const uint64_t sizes[2] = {pos_size, nrm_size}, offs[2] = {pos_offset, nrm_offset};
+    for (int k = 0; k < 2; ++k)
+        if (fds[k] >= 0 && (offs[k] > sizes[k] || sizes[k] - offs[k] < need || offs[k] % 16))
+            return fail(h, XPBD_E_INVALID, "render buffer smaller than 16 N bytes past its offset, or misaligned");
+    if (pos_fd < 0 && nrm_fd < 0) return XPBD_E_INVALID;
+    if (h->stream) HIPC(h, hipStreamSynchronize(h->stream));
+    release_imports(h);
+    h->render_pos = h->render_nrm = nullptr;
+    drop_graph(h);
+    for (int k = 0; k < 2; ++k) {
+        if (fds[k] < 0) continue;
+        hipExternalMemoryHandleDesc d{};
+        d.type = hipExternalMemoryHandleTypeOpaqueFd;
+        d.handle.fd = fds[k];
+        d.size = sizes[k];
+        hipError_t e = hipImportExternalMemory(&h->ext_mem[k], &d);
+        if (e == hipSuccess) {
+            hipExternalMemoryBufferDesc b{};
+            b.offset = offs[k];
+            b.size = need;
+            e = hipExternalMemoryGetMappedBuffer(&h->ext_ptr[k], h->ext_mem[k], &b);
+        }
+        if (e != hipSuccess) {
+            release_imports(h);
+            return fail(h, XPBD_E_HIP, std::string("importing a render buffer: ") + hipGetErrorString(e));
+        }
+    }
+    h->render_pos = static_cast<float4*>(h->ext_ptr[0]);
+    h->render_nrm = static_cast<float4*>(h->ext_ptr[1]);
     return XPBD_OK;
 }
 

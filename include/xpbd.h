@@ -226,6 +226,20 @@ XPBD_API int xpbd_publish(xpbd_cloth* h, void* dst_pos_vec4, void* dst_nrm_vec4)
  * reference's normals pass does once per frame (cloth_jacobi.rs:885-913). */
 XPBD_API int xpbd_bind_render_buffers(xpbd_cloth* h, void* dst_pos_vec4, void* dst_nrm_vec4);
 
+/* Zero-copy render buffers: import the renderer's own position and normal
+ * buffers and bind them as xpbd_bind_render_buffers does, so each frame's
+ * publish writes straight into them.  Each buffer comes as a POSIX file
+ * descriptor of exported device memory (Vulkan VK_KHR_external_memory_fd,
+ * VK_EXTERNAL_MEMORY_HANDLE_TYPE_OPAQUE_FD_BIT), the exported allocation's
+ * size, and the buffer's offset in it (16-byte aligned, 16 N bytes needed
+ * past it); fd -1 leaves that buffer unbound.  On success the library owns
+ * the descriptors; the imports are released by the next bind or import and
+ * by xpbd_destroy.  Replaces the wgpu bind group of ClothMatExt
+ * {sim_positions, sim_normals} (cloth_material.rs:14-48) that the
+ * reference's finalize_normals writes (cloth_sim_jacobi.wgsl:527-547). */
+XPBD_API int xpbd_import_render_buffers_fd(xpbd_cloth* h, int pos_fd, uint64_t pos_size, uint64_t pos_offset,
+                                          int nrm_fd, uint64_t nrm_size, uint64_t nrm_offset);
+
 /* Relative constraint residual sqrt(sum C^2)/sqrt(sum rest^2) of the current
  * state (no reference counterpart; the metric the learned warm start is
  * judged by).  Synchronous. */
