@@ -11,7 +11,7 @@ log-uniform compliances, pins from {two corners, top row, none + ground}),
 self-collision within each cloth, same solver settings.  `--persistent` runs
 predict + all iterations of a substep as one persistent launch.
 
-Prints one JSON line (rank 0).  `--gpus N` with N > 1 exits: this build runs
+Prints one JSON line (rank 0).  `--gpus N` with N > 1 exits: the library runs
 on one GPU.
 """
 from __future__ import annotations
@@ -148,7 +148,7 @@ def main():
                     help="only the per-kernel timing launches (for rocprofv3 counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is design-only)", file=sys.stderr)
         sys.exit(2)
 

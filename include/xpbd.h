@@ -242,7 +242,7 @@ XPBD_API int xpbd_import_render_buffers_fd(xpbd_cloth* h, int pos_fd, uint64_t p
 
 /* Relative constraint residual sqrt(sum C^2)/sqrt(sum rest^2) of the current
  * state (no reference counterpart; the metric the learned warm start is
- * judged by).  Synchronous. */
+ * measured and gated by).  Synchronous. */
 XPBD_API int xpbd_residual(xpbd_cloth* h, double* out);
 
 /* Measurement: average duration [ms] of one launch of kernel `which`
