@@ -331,9 +331,10 @@ XPBD_API int xpbd_warm_sample(xpbd_cloth* h, uint32_t iterations, void* d_featur
 XPBD_API int xpbd_warm_features(xpbd_cloth* h, void* d_features);
 /* Attach (m != NULL) or detach the network.  The network must outlive the
  * attachment; attach again after changing its output scale.  A finite
- * threshold computes the relative residual of every warm substep and runs
- * the plain substep (one persistent launch) when it is exceeded; an infinite
- * threshold skips the check. */
+ * threshold computes the relative residual of every warm substep's result
+ * (f32 per constraint, about 1 ulp from xpbd_residual's figure) and runs the
+ * plain substep (one persistent launch) when it exceeds the threshold or is
+ * NaN; an infinite threshold skips the check. */
 XPBD_API int xpbd_set_warm_start(xpbd_cloth* h, xpbd_mlp* m, uint32_t warm_iterations, float residual_threshold);
 /* Substeps run with the warm start and how many fell back; reset != 0 zeroes them. */
 XPBD_API int xpbd_warm_stats(xpbd_cloth* h, uint32_t* substeps, uint32_t* fallbacks, uint32_t reset);
