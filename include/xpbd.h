@@ -332,7 +332,7 @@ XPBD_API int xpbd_warm_features(xpbd_cloth* h, void* d_features);
 /* Attach (m != NULL) or detach the network.  The network must outlive the
  * attachment; attach again after changing its output scale.  A finite
  * threshold computes the relative residual of every warm substep's result
- * (f32 per constraint, about 1 ulp from xpbd_residual's figure) and runs the
+ * (f32 per constraint; within 0.1% of xpbd_residual's figure) and runs the
  * plain substep (one persistent launch) when it exceeds the threshold or is
  * NaN; an infinite threshold skips the check. */
 XPBD_API int xpbd_set_warm_start(xpbd_cloth* h, xpbd_mlp* m, uint32_t warm_iterations, float residual_threshold);
