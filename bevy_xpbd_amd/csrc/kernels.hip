@@ -435,6 +435,19 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
 // CSR entries in the same order (their offsets re-encoded for the tile), so
 // the sums, and the results, are those of k_jacobi.
 // ---------------------------------------------------------------------------
+// A tiled entry word: low 16 bits the neighbour's byte offset in the LDS
+// tile from the particle's own slot (signed), high 16 bits the pair's byte
+// offset in the staged pair table (pair index x 16), so that each address is
+// one add or one shift.
+__device__ __forceinline__ float4 tile_neighbour(const float4* self, uint32_t w) {
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(self) + int32_t(int16_t(w & 0xFFFFu)));
+}
+template <bool LDSP>
+__device__ __forceinline__ float4 tile_pair(const JacobiArgs& a, const float4* lpairs, uint32_t w) {
+    if constexpr (LDSP) return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(lpairs) + (w >> 16));
+    else return stage_pair(a.sell.pairs[w >> 20], a.p);
+}
+
 template <uint32_t MODE, bool FAST, bool LDSP>
 __device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, const float4* lpairs, const float4* tl, uint32_t slice, uint32_t lane,
                                                         uint32_t lidx, bool row_ok, bool& valid) {
@@ -468,15 +481,14 @@ __device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, con
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < JU; ++u) {
-                    const float4 pj = tl[int32_t(lidx) + int32_t(int16_t(wc[u] & 0xFFFFu))];
-                    jacobi_entry<FAST>(pi, pj, entry_pair<LDSP>(a, lpairs, wc[u]), ax, ay, az);
+                    jacobi_entry<FAST>(pi, tile_neighbour(tl + lidx, wc[u]), tile_pair<LDSP>(a, lpairs, wc[u]), ax, ay,
+                                       az);
                 }
             }
         }
         for (; k0 < width; ++k0) {
             const uint32_t w = __builtin_nontemporal_load(a.sell.tpacked + base + k0 * 64);
-            const float4 pj = tl[int32_t(lidx) + int32_t(int16_t(w & 0xFFFFu))];
-            jacobi_entry<FAST>(pi, pj, entry_pair<LDSP>(a, lpairs, w), ax, ay, az);
+            jacobi_entry<FAST>(pi, tile_neighbour(tl + lidx, w), tile_pair<LDSP>(a, lpairs, w), ax, ay, az);
         }
         float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
         const float ml = len3(dx, dy, dz);

@@ -39,8 +39,9 @@ struct Sell {
     const uint32_t* packed;
     const float2* pairs;
     // Grid tiles (one grid cloth, tcols a multiple of 64): the packed entries
-    // with the low 16 bits replaced by the neighbour's offset in the block's
-    // LDS tile (dy * JT_W + dx); nullptr: no tiled path
+    // re-encoded as (pair index x 16) << 16 | (int16) byte offset of the
+    // neighbour in the block's LDS tile ((dy * JT_W + dx) x 16), so both
+    // addresses come from one operation each; nullptr: no tiled path
     const uint32_t* tpacked;
     uint32_t tcols, trows;
 };

@@ -632,8 +632,10 @@ bool tile_sell(const Mesh& m, const std::vector<uint32_t>& soff, const std::vect
             const int64_t j = int64_t(i) + delta, jr = j / c, jc = j % c;
             const int64_t dy = jr - row, dx = jc - col;
             if (dx < -2 || dx > 2 || dy < -2 || dy > 2) return false;
-            const int64_t off = dy * int64_t(JT_W) + dx;
-            tiled[soff[s] + size_t(k) * 64 + l] = (w & 0xFFFF0000u) | (uint32_t(off) & 0xFFFFu);
+            const int64_t off = (dy * int64_t(JT_W) + dx) * int64_t(sizeof(float4));   // bytes
+            const uint32_t pair = w >> 16;
+            if (pair >= 4096u) return false;   // pair x 16 must fit the high half
+            tiled[soff[s] + size_t(k) * 64 + l] = ((pair * uint32_t(sizeof(float4))) << 16) | (uint32_t(off) & 0xFFFFu);
         }
     }
     cols = uint32_t(c);
