@@ -79,9 +79,10 @@ def learned_warm_start():
     32x32 and 256x256 training sheets) and measure what it gains on held-out
     32x32 stiff sheets: one substep of warm start + 2 iterations against
     plain Jacobi at several iteration counts (equal-residual speedup), and
-    10 frames behind the residual gate (threshold 1e-4).  Then a network
-    trained on 20-iteration targets on 512 stiff sheets in one batch, gated
-    at the plain 10-iteration residual (warmstart.evaluate_batch)."""
+    10 frames behind the residual gate (threshold 1e-4).  Then at scale: a
+    network chosen by a gated rollout on validation sheets
+    (warmstart.train_selected), on 512 held-out stiff sheets in one batch,
+    gated at the plain 10-iteration residual (warmstart.evaluate_batch)."""
     try:
         from bevy_xpbd_amd import warmstart as W
         t0 = time.perf_counter()
@@ -95,11 +96,12 @@ def learned_warm_start():
         log(f"bench: warm start: warm+2 ~ plain I={res['equivalent_plain_iterations']}, single-substep speedup "
             f"{res['single_substep_speedup_at_equal_residual']}x; gated fallbacks "
             f"{res['gated_fallbacks']}/{res['gated_substeps']}")
-        # at scale: a network trained on I=20 targets, gated at the plain I=10 residual
-        feat, tgt, rows, th = W.generate_dataset(seed=1, frames=(3, 12), target_iterations=20)
-        mlp, curve = W.train(feat, tgt, rows, th, steps=10000, batch=4096, lr=2e-3)
-        batch = W.evaluate_batch(mlp, count=512)
-        batch.update({"target_iterations": 20, "train_steps": 10000})
+        # at scale: networks on 20/30/40-iteration targets, one kept by a gated
+        # rollout on validation sheets, then gated at the plain I=10 residual
+        # on the held-out 512 sheets
+        mlp, selection = W.train_selected(log=log)
+        batch = W.evaluate_batch(mlp, count=512, warm_iterations=(2, 3, 4))
+        batch.update({"train_steps": 10000, "selection": selection})
         mlp.close()
         res["batch_512_gated"] = batch
         for k, v in batch["warm"].items():

@@ -279,3 +279,55 @@ def evaluate_batch(mlp, count: int = 512, iterations: int = 10, warm_iterations=
                                "speedup_vs_plain": round(rate / plain_rate, 3), "final_residual": g.residual()}
         g.close()
     return out
+
+
+def validation_fallbacks(mlp, count: int = 64, seed: int = 91, iterations: int = 10, warm_iterations: int = 2,
+                         frames: int = 6, start_frames: int = 2):
+    """Fallbacks of the gated warm start over a short rollout of `count`
+    validation sheets (the evaluation family, another seed), the gate set at
+    the plain solver's largest residual there.  Returns (fallbacks, substeps)."""
+    mesh = A.ClothMeshData.batch(stiff_eval_descs(seed=seed, count=count))
+    sim = make_sim(mesh, iterations=iterations, use_graph=1)
+    sim.step_frames(start_frames)
+    res = []
+    for _ in range(frames):
+        sim.step_frames(1)
+        res.append(sim.residual())
+    sim.close()
+    g = make_sim(mesh, use_graph=1)
+    g.set_warm_start(mlp, warm_iterations=warm_iterations, residual_threshold=max(res))
+    g.step_frames(start_frames)
+    g.warm_stats(reset=True)
+    g.step_frames(frames)
+    fb = g.warm_stats()
+    g.close()
+    return fb[1], fb[0]
+
+
+def train_selected(target_iterations=(20, 30, 40), steps: int = 10000, batch: int = 4096, lr: float = 2e-3,
+                   seed: int = 1, log=None):
+    """One network per target-iteration count, kept by rollout validation: the
+    one whose gated warm + 2 falls back least on the validation sheets (ties:
+    the lower final training loss).  The rollout, not the regression loss,
+    decides: networks of equal loss differ widely in how their small errors
+    feed back through the velocity update.  Returns (mlp, report)."""
+    best, best_key, report = None, None, []
+    for ti in target_iterations:
+        feat, tgt, rows, th = generate_dataset(seed=seed, frames=(3, 12), target_iterations=ti)
+        mlp, curve = train(feat, tgt, rows, th, seed=seed, steps=steps, batch=batch, lr=lr)
+        feat.free()
+        tgt.free()
+        fb, sub = validation_fallbacks(mlp)
+        loss = float(np.mean(curve[-50:]))
+        report.append({"target_iterations": ti, "train_loss": round(loss, 4), "validation_fallbacks": fb,
+                       "validation_substeps": sub})
+        if log:
+            log(f"warmstart: targets I={ti}: loss {loss:.4f}, validation fallbacks {fb}/{sub}")
+        key = (fb, loss)
+        if best is None or key < best_key:
+            if best is not None:
+                best.close()
+            best, best_key = mlp, key
+        else:
+            mlp.close()
+    return best, report
