@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include "mlp.hpp"
@@ -630,18 +631,20 @@ void launch_warm_infer(const WarmFeatArgs& a, const float* P, const uint16_t* bf
     if (a.n == 0) return;
     // one co-resident wave of blocks: each loads the weights once and walks
     // its passes (more blocks only reload the weights)
-    static uint32_t resident[64] = {};
+    static std::atomic<uint32_t> resident[64];   // per device, 0 until first queried
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if (resident[dev] == 0) {
+    uint32_t cap = resident[dev].load(std::memory_order_relaxed);
+    if (cap == 0) {
         int cus = 0, per_cu = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_warm_infer, int(WI_ROWS), 0) == hipSuccess)
-            resident[dev] = uint32_t(std::max(1, cus * per_cu));
+            cap = uint32_t(std::max(1, cus * per_cu));
         else
-            resident[dev] = 2048;
+            cap = 2048;
+        resident[dev].store(cap, std::memory_order_relaxed);
     }
-    const uint32_t blocks = std::min<uint32_t>(cdiv(a.n, WI_ROWS), resident[dev]);
+    const uint32_t blocks = std::min<uint32_t>(cdiv(a.n, WI_ROWS), cap);
     hipLaunchKernelGGL(k_warm_infer, dim3(blocks), dim3(256), 0, s, a, P, bf, inv_out_scale, x0, count);
 }
 
