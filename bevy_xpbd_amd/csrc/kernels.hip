@@ -1639,22 +1639,24 @@ __global__ void k_publish(const float4* X, uint32_t n, const uint32_t* tris, con
     else dnrm[i] = make_float4(0.f, 1.f, 0.f, 0.f);
 }
 
-// Sum over constraints of C^2 (C = |x_i - x_j| - rest), in double.
-__global__ __launch_bounds__(256) void k_residual(const float4* X, const uint2* ij, const float2* rc, uint32_t e,
-                                                 double* out) {
+// Sum over constraints of C^2 (C = |x_i - x_j| - rest), in double: one
+// partial per block (no atomics), summed on the host in block order, so the
+// figure is the same on every run.
+__global__ __launch_bounds__(256) void k_residual(const float4* X, const uint2* ij, const float* rest, uint32_t e,
+                                                 double* part) {
     double s = 0.0;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < e; k += gridDim.x * blockDim.x) {
         const uint2 q = ij[k];
         const float4 a = X[q.x], b = X[q.y];
         const double dx = double(a.x) - double(b.x), dy = double(a.y) - double(b.y), dz = double(a.z) - double(b.z);
-        const double c = sqrt((dx * dx + dy * dy) + dz * dz) - double(rc[k].x);
+        const double c = sqrt((dx * dx + dy * dy) + dz * dz) - double(rest[k]);
         s += c * c;
     }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     __shared__ double red[4];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1959,9 +1961,14 @@ void launch_publish(const float4* X, uint32_t n, const uint32_t* tris, const uin
     hipLaunchKernelGGL(k_publish, dim3(cdiv(n, 256)), dim3(256), 0, s, X, n, tris, vt_off, vt_tri, dpos, dnrm);
 }
 
-void launch_residual(const float4* X, const uint2* ij, const float2* rc, uint32_t e, double* out, hipStream_t s) {
-    if (e == 0) return;
-    hipLaunchKernelGGL(k_residual, dim3(std::min(cdiv(e, 256), 2048u)), dim3(256), 0, s, X, ij, rc, e, out);
+uint32_t residual_max_blocks() { return 2048u; }
+
+uint32_t launch_residual(const float4* X, const uint2* ij, const float* rest, uint32_t e, double* part,
+                         hipStream_t s) {
+    if (e == 0) return 0;
+    const uint32_t blocks = std::min(cdiv(e, 256 * 4), residual_max_blocks());
+    hipLaunchKernelGGL(k_residual, dim3(blocks), dim3(256), 0, s, X, ij, rest, e, part);
+    return blocks;
 }
 
 }  // namespace xpbd

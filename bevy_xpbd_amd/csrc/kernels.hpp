@@ -203,8 +203,11 @@ void launch_jacobi_persistent(const JacobiArgs& a, const PersistArgs& q, uint32_
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
 void launch_publish(const float4* X, uint32_t n, const uint32_t* tris, const uint32_t* vt_off,
                     const uint32_t* vt_tri, float4* dpos, float4* dnrm, hipStream_t s);
-void launch_residual(const float4* X, const uint2* ij, const float2* rc, uint32_t e, double* out,
-                     hipStream_t s);
+// sum of C^2 over the constraints (ij, rest) in double: one partial per
+// block into part[0, blocks), blocks <= residual_max_blocks(); returns blocks
+uint32_t residual_max_blocks();
+uint32_t launch_residual(const float4* X, const uint2* ij, const float* rest, uint32_t e, double* part,
+                         hipStream_t s);
 uint32_t launch_aabb_of(const float4* P, uint32_t n, float* part, hipStream_t s);
 void launch_selftest_rcp(int exp_lo, int exp_hi, uint32_t* bad, uint32_t* first, hipStream_t s);
 void launch_selftest_div(uint64_t n, uint64_t seed, uint32_t* bad, uint32_t* first, hipStream_t s);

@@ -108,7 +108,7 @@ struct xpbd_cloth {
     uint32_t* cloth_off = nullptr; // C + 1 vertex offsets
     uint32_t *tris = nullptr, *vt_off = nullptr, *vt_tri = nullptr;
     uint32_t *barrier = nullptr, *err = nullptr;
-    double* resid = nullptr;
+    double* resid = nullptr;       // per-block partial sums of the reported residual
 
     float4 *render_pos = nullptr, *render_nrm = nullptr;
     // imported render buffers (xpbd_import_render_buffers_fd): the imports
@@ -861,7 +861,7 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
         h->max_comp = std::isfinite(c) ? std::max(h->max_comp, std::fabs(c)) : INFINITY;
         if (!(r == 0.f || (r >= 9.313225746154785e-10f && r <= 1073741824.0f))) h->rest_tame = false;   // [2^-30, 2^30]
     }
-    TRY(dalloc(h, &h->resid, 2));
+    TRY(dalloc(h, &h->resid, residual_max_blocks()));
     HIPC(h, hipMemset(h->err, 0, 16));
     HIPC(h, hipMemset(h->barrier, 0, GRID_SYNC_WORDS * sizeof(uint32_t)));
     return XPBD_OK;
@@ -1411,12 +1411,13 @@ XPBD_API int xpbd_import_render_buffers_fd(xpbd_cloth* h, int pos_fd, uint64_t p
 
 XPBD_API int xpbd_residual(xpbd_cloth* h, double* out) {
     if (!h || !out) return XPBD_E_INVALID;
-    double zero[2] = {0.0, 0.0};
-    HIPC(h, hipMemcpyAsync(h->resid, zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
-    launch_residual(h->X, h->gs_ij, h->gs_rc, h->e, h->resid, h->stream);
-    double c2 = 0.0;
-    HIPC(h, hipMemcpyAsync(&c2, h->resid, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    const uint32_t blocks = launch_residual(h->X, h->near_ij, h->near_rest, h->e, h->resid, h->stream);
+    std::vector<double> part(blocks);
+    if (blocks)
+        HIPC(h, hipMemcpyAsync(part.data(), h->resid, blocks * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPC(h, hipStreamSynchronize(h->stream));
+    double c2 = 0.0;
+    for (double v : part) c2 += v;
     *out = h->rest_sq_sum > 0.0 ? std::sqrt(c2) / std::sqrt(h->rest_sq_sum) : 0.0;
     return XPBD_OK;
 }

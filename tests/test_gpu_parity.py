@@ -374,3 +374,15 @@ def test_config3_1024_two_substeps_bit_exact(O, X):
     O.set_threads(16)
     kw = dict(cols=1024, rows=1024, spacing=h, pins=2, jitter=1e-3, seed=3)
     run_pair(O, X, kw, S=8, I=10, substeps=2, collision=True, thickness=2 * (0.4 * h))
+
+
+def test_residual_is_reproducible(X):
+    """The reported residual sums fixed per-block partials in a fixed order
+    (no atomics): the same state gives the same bits on every call."""
+    note("case residual reproducible")
+    mesh = X.ClothMeshData.sheet(cols=96, rows=80, spacing=0.01, pins=2, jitter=1e-3, seed=8)
+    sim, _, _ = make(X, mesh, S=8, I=10)
+    sim.step_frames(3)
+    rs = {sim.residual() for _ in range(5)}
+    assert len(rs) == 1 and next(iter(rs)) > 0, rs
+    sim.close()
