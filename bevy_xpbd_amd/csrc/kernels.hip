@@ -181,13 +181,6 @@ __device__ __forceinline__ float sqrt_rn(float x) {   // exact for x == 0 or x >
     s = ru > 0.f ? su : s;
     return s;
 }
-constexpr float TAME_LO = 8.881784197001252e-16f;   // 2^-50
-constexpr float TAME_HI = 1048576.0f;               // 2^20
-__device__ __forceinline__ bool tame_coord(float c) {
-    const float a = fabsf(c);
-    return __float_as_uint(c) == 0u || (a >= TAME_LO && a <= TAME_HI);
-}
-__device__ __forceinline__ bool tame3(float4 v) { return tame_coord(v.x) && tame_coord(v.y) && tame_coord(v.z); }
 
 // Slice-tameness byte of the wavefront's 64 outputs (every lane of the wave
 // must reach this; `mine` = false only for a written value out of range).

@@ -52,6 +52,17 @@ constexpr uint32_t JT_ROWS = 8;
 constexpr uint32_t JT_W = 64 + 4;
 constexpr uint32_t JT_H = JT_ROWS + 4;
 
+// A tame coordinate is +0 or of magnitude in [2^-50, 2^20]: on slices whose
+// neighbourhood is tame the Jacobi kernels use the short correctly rounded
+// division and square root (exact on that range).
+constexpr float TAME_LO = 8.881784197001252e-16f;   // 2^-50
+constexpr float TAME_HI = 1048576.0f;               // 2^20
+__device__ __forceinline__ bool tame_coord(float c) {
+    const float a = fabsf(c);
+    return __float_as_uint(c) == 0u || (a >= TAME_LO && a <= TAME_HI);
+}
+__device__ __forceinline__ bool tame3(float4 v) { return tame_coord(v.x) && tame_coord(v.y) && tame_coord(v.z); }
+
 enum JacobiMode : uint32_t {
     JM_PREDICT = 1,   // inputs are the substep-start state; predict on the fly
     JM_FINAL = 2,     // write the new state in place + velocity (no collision)

@@ -335,6 +335,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
         for (int q = 0; q < MLP_IN / 4; ++q) frow[q] = make_uint2(packed[2 * q], packed[2 * q + 1]);
         __syncthreads();   // (the first pass: also the bias copy)
+        bool tame_rows = true;   // the rows this lane wrote (64 per wave: one slice)
 #pragma unroll 1
         for (uint32_t tt = 0; tt < 4; tt += WI_NT) {
             const uint32_t lt = 4 * wv + tt;   // first tile of the step
@@ -360,9 +361,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         q.z = p.z + o[t][2] * sc;
                     }
                     x0[row] = q;
+                    tame_rows = tame_rows && tame3(q);
                 }
             }
         }
+        // wave wv wrote rows base + 64 wv .. + 63: the slice's tame byte
+        const uint32_t first = base + 64 * wv;
+        const int all_tame = __all(tame_rows ? 1 : 0);
+        if (a.tame_out && first < a.n && lane == 0) a.tame_out[first >> 6] = all_tame ? 1u : 0u;
         __syncthreads();
     }
 }

@@ -8,6 +8,8 @@
 
 #include <cstdint>
 
+#include "kernels.hpp"
+
 namespace xpbd {
 
 constexpr int MLP_FEAT = 30;   // features used
@@ -46,6 +48,7 @@ struct WarmFeatArgs {
     float dt;
     uint32_t n;
     uint16_t* feat;        // n x 32 bf16
+    uint8_t* tame_out;     // k_warm_infer: per-slice tame bytes of x0 (nullable)
 };
 
 // features of the predicted state

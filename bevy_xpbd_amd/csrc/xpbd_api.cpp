@@ -129,6 +129,7 @@ struct xpbd_cloth {
     float max_inv_mass = 0.f, max_comp = 0.f;
     bool rest_tame = true;
     uint8_t *tameA = nullptr, *tameB = nullptr;   // per slice: 1 = coordinates in range
+    uint8_t *tameW = nullptr, *tameS = nullptr;   // the same for the warm start's W0 / sP iterates
     uint2* srange = nullptr;                      // per slice: first / last neighbour slice
 
     // learned warm start (grid meshes): 8 grid neighbours and per-vertex
@@ -230,6 +231,8 @@ bool fast_division_active(const xpbd_cloth* h) {
 uint8_t* tame_slot(xpbd_cloth* h, const float4* buf) {
     if (buf == h->A) return h->tameA;
     if (buf == h->B) return h->tameB;
+    if (buf == h->W0) return h->tameW;
+    if (buf == h->sP) return h->tameS;
     return nullptr;
 }
 
@@ -356,12 +359,17 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         // x0 lands where the ping-pong (W0 / sP) ends in W0
         float4* cur = (h->warm_iters % 2) ? h->sP : h->W0;
         float4* nxt = (h->warm_iters % 2) ? h->W0 : h->sP;
-        // (without a gate the launch counts the warm substep itself)
+        // (without a gate the launch counts the warm substep itself); x0's
+        // slice tame bytes let the iterations take the short exact division
+        const bool fast = a.fastdiv != 0;
+        fa.tame_out = fast ? tame_slot(h, cur) : nullptr;
         launch_warm_infer(fa, m->P, m->bf, 1.0f / m->out_scale, cur,
                           std::isfinite(h->warm_threshold) ? nullptr : h->warm_flag, s);
         for (uint32_t k = 0; k < h->warm_iters; ++k) {
             a.src = cur;
             a.dst = nxt;
+            a.tame_in = fast ? tame_slot(h, cur) : nullptr;
+            a.tame_out = fast ? tame_slot(h, nxt) : nullptr;
             ProbeScope ps(h, XPBD_KERNEL_JACOBI_ITER);
             launch_jacobi(0, a, h->jacobi_block, s);
             std::swap(cur, nxt);
@@ -954,6 +962,12 @@ int build_warm_topology(xpbd_cloth* h, const Mesh& m) {
 
 int ensure_warm_buffers(xpbd_cloth* h) {
     if (!h->W0) TRY(dalloc(h, &h->W0, h->n));
+    if (!h->tameW) {
+        TRY(dalloc(h, &h->tameW, h->nslices));
+        TRY(dalloc(h, &h->tameS, h->nslices));
+        HIPC(h, hipMemset(h->tameW, 0, h->nslices));
+        HIPC(h, hipMemset(h->tameS, 0, h->nslices));
+    }
     if (!h->warm_flag) {
         TRY(dalloc(h, &h->warm_flag, 4));
         HIPC(h, hipMemsetAsync(h->warm_flag, 0, 4 * sizeof(uint32_t), h->stream));
