@@ -331,3 +331,26 @@ def train_selected(target_iterations=(20, 30, 40), steps: int = 10000, batch: in
         else:
             mlp.close()
     return best, report
+
+
+def main(argv=None):
+    """Train a rollout-selected network and evaluate it behind the gate at
+    scale; prints one JSON line (warmstart.evaluate_batch's record)."""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="learned warm start: train, select, evaluate at scale")
+    ap.add_argument("--sheets", type=int, default=512, help="held-out stiff 32x32 sheets in the batch")
+    ap.add_argument("--frames", type=int, default=20, help="timed frames per warm iteration count")
+    ap.add_argument("--quick", action="store_true", help="one candidate (40-iteration targets)")
+    args = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)   # noqa: E731
+    mlp, selection = train_selected(target_iterations=(40,) if args.quick else (20, 30, 40), log=log)
+    r = evaluate_batch(mlp, count=args.sheets, warm_iterations=(2, 3, 4), frames=args.frames)
+    r["selection"] = selection
+    mlp.close()
+    print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main()
