@@ -221,7 +221,10 @@ __device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float4 pr, fl
     const float c = len - pr.x;
     const float wsum = wi + wj + pr.y;
     if (wsum < 1e-8f) return;
-    const float num = -c - pr.z;
+    // -C - alpha~ * lambda with lambda = 0: pr.z = alpha~ * 0 is +0 whenever
+    // alpha~ is finite (always on fast slices: alpha~ <= 2^40), and x - (+0)
+    // is x for every x, so the fast form drops the subtraction
+    const float num = FAST ? -c : -c - pr.z;
     float dlam;
     if (FAST) {
         const float yl = rcp_rn(len);
