@@ -220,7 +220,9 @@ __device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float4 pr, fl
     if (len < 1e-8f) return;
     const float c = len - pr.x;
     const float wsum = wi + wj + pr.y;
-    if (wsum < 1e-8f) return;
+    // fast slices only run on meshes whose inverse masses are 0 or >= 2^-26
+    // and whose compliances are >= 0: there wsum >= w_i >= 2^-26 > 1e-8
+    if (!FAST && wsum < 1e-8f) return;
     // -C - alpha~ * lambda with lambda = 0: pr.z = alpha~ * 0 is +0 whenever
     // alpha~ is finite (always on fast slices: alpha~ <= 2^40), and x - (+0)
     // is x for every x, so the fast form drops the subtraction

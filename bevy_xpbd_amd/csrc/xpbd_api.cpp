@@ -127,6 +127,7 @@ struct xpbd_cloth {
     // fast correctly rounded division / sqrt in the Jacobi gather: the static
     // inputs' ranges (host-checked) and per-buffer "tame" flags of A and B
     float max_inv_mass = 0.f, max_comp = 0.f;
+    bool mass_floor = false, comp_nonneg = false;   // fast-division preconditions (see fast_division_active)
     bool rest_tame = true;
     uint8_t *tameA = nullptr, *tameB = nullptr;   // per slice: 1 = coordinates in range
     uint8_t *tameW = nullptr, *tameS = nullptr;   // the same for the warm start's W0 / sP iterates
@@ -224,8 +225,9 @@ bool fast_division_active(const xpbd_cloth* h) {
     if (!h->cfg.fast_division) return false;
     const float two40 = 1099511627776.0f;
     const float inv_dt_sq = h->u.inv_dt * h->u.inv_dt;
-    return h->rest_tame && std::isfinite(h->max_inv_mass) && h->max_inv_mass <= two40 &&
-           std::isfinite(inv_dt_sq) && double(h->max_comp) * double(inv_dt_sq) <= double(two40);
+    return h->rest_tame && h->mass_floor && h->comp_nonneg && std::isfinite(h->max_inv_mass) &&
+           h->max_inv_mass <= two40 && std::isfinite(inv_dt_sq) &&
+           double(h->max_comp) * double(inv_dt_sq) <= double(two40);
 }
 
 uint8_t* tame_slot(xpbd_cloth* h, const float4* buf) {
@@ -862,10 +864,15 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     // static ranges of the fast-division proof
     h->max_inv_mass = 0.f;
     for (float w : m.inv_mass) h->max_inv_mass = std::isfinite(w) ? std::max(h->max_inv_mass, w) : INFINITY;
+    h->mass_floor = true;   // every inverse mass 0 or >= 2^-26
+    for (float w : m.inv_mass)
+        if (!(w == 0.f || w >= 1.4901161193847656e-08f)) h->mass_floor = false;
     h->max_comp = 0.f;
     h->rest_tame = true;
+    h->comp_nonneg = true;
     for (size_t k = 0; k < m.constraint_comp.size(); ++k) {
         const float c = m.constraint_comp[k], r = m.constraint_rest[k];
+        if (!(c >= 0.f)) h->comp_nonneg = false;
         h->max_comp = std::isfinite(c) ? std::max(h->max_comp, std::fabs(c)) : INFINITY;
         if (!(r == 0.f || (r >= 9.313225746154785e-10f && r <= 1073741824.0f))) h->rest_tame = false;   // [2^-30, 2^30]
     }
