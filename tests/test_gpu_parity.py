@@ -386,3 +386,23 @@ def test_residual_is_reproducible(X):
     rs = {sim.residual() for _ in range(5)}
     assert len(rs) == 1 and next(iter(rs)) > 0, rs
     sim.close()
+
+
+@pytest.mark.parametrize("case", ["heavy", "negative_compliance", "plain"])
+def test_fast_division_preconditions(O, X, case):
+    """The short operations (and the tests they let the Jacobi entry drop)
+    apply only where their preconditions hold: inverse masses 0 or >= 2^-26,
+    compliances >= 0.  A mesh outside them runs the general path, still bit
+    for bit; a mesh inside runs the fast path, bit for bit."""
+    note(f"case fast preconditions {case}")
+    kw = dict(cols=64, rows=24, spacing=0.02, pins=2, jitter=1e-3, seed=5)
+    if case == "heavy":
+        kw["inv_mass"] = 1e-9
+    elif case == "negative_compliance":
+        kw["bend_compliance"] = -1e-7
+    descs = [X.SheetDesc.make(**kw)]
+    mesh = X.ClothMeshData.batch(descs)
+    sim, _, _ = make(X, mesh, fast_division=1)
+    assert sim.layout()["fast_division"] == (case == "plain"), sim.layout()
+    sim.close()
+    run_pair(O, X, mesh=mesh, I=6, substeps=4, fast_division=1)
