@@ -1067,10 +1067,11 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const float4 fi = a.sP[s];    // xyz, w = cell key bits
-    const float4 ri = a.sR[s];    // rest xyz, w = inverse mass
     const uint32_t i = a.sids[s];
-    if (ri.w <= 0.f) {
-        if (a.xout != a.prev) a.xout[i] = make_float4(fi.x, fi.y, fi.z, ri.w);   // timing runs write scratch
+    const float4 pv = a.prev[i];  // substep start (the velocity update's), w = inverse mass
+    const float wi = pv.w;
+    if (wi <= 0.f) {
+        if (a.xout != a.prev) a.xout[i] = make_float4(fi.x, fi.y, fi.z, wi);   // timing runs write scratch
         return;
     }
     const CollGrid g = a.grids[a.cloth_of ? a.cloth_of[i] : 0u];
@@ -1124,6 +1125,7 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     }
     float ax = 0.f, ay = 0.f, az = 0.f;
     if (cnt > 0) {
+        const float4 ri = a.rest4[i];   // rest xyz (w = inverse mass): contacts only
         uint32_t lj[LMAX];
 #pragma unroll
         for (int k = 0; k < LMAX; ++k) {
@@ -1147,9 +1149,9 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
         for (int k = 0; k < LMAX; ++k) {
             if (k < cnt) {
                 const float4 fj = a.sP[lq[k]];
-                const float4 rj = a.sR[lq[k]];
+                const float4 rj = a.rest4[lj[k]];
                 float cx, cy, cz;
-                if (collision_contrib(i, lj[k], fi, ri.w, fj, rj.w, ri, rj, a.p, cx, cy, cz)) {
+                if (collision_contrib(i, lj[k], fi, wi, fj, rj.w, ri, rj, a.p, cx, cy, cz)) {
                     ax = ax + cx;
                     ay = ay + cy;
                     az = az + cz;
@@ -1157,8 +1159,8 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
             }
         }
     }
-    const float4 nx = make_float4(fi.x + clamp_apply(ax), fi.y + clamp_apply(ay), fi.z + clamp_apply(az), ri.w);
-    finish_particle(nx, a.prev[i], i, a.ground, a.p, a.xout, a.vout);
+    const float4 nx = make_float4(fi.x + clamp_apply(ax), fi.y + clamp_apply(ay), fi.z + clamp_apply(az), wi);
+    finish_particle(nx, pv, i, a.ground, a.p, a.xout, a.vout);
 }
 
 // Particles with more than LMAX candidates.  One pass over the 27 cells
@@ -1180,8 +1182,10 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
     for (uint32_t k = blockIdx.x * blockDim.x + t; k < total; k += gridDim.x * blockDim.x) {
         const uint32_t s = a.ovf[2 + k];
         const float4 fi = a.sP[s];
-        const float4 ri = a.sR[s];
         const uint32_t i = a.sids[s];
+        const float4 pv = a.prev[i];
+        const float4 ri = a.rest4[i];
+        const float wi = pv.w;
         const CollGrid g = a.grids[a.cloth_of ? a.cloth_of[i] : 0u];
         uint32_t c[3];
         cell_of(fi, g, c);
@@ -1199,9 +1203,9 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
                 const float d2 = dist2_of(fi, fj);
                 if (d2 > th2 || d2 < 1e-18f) continue;
                 const uint32_t j = a.sids[idx];
-                const float4 rj = a.sR[idx];
+                const float4 rj = a.rest4[j];
                 float cx, cy, cz;
-                if (!collision_contrib(i, j, fi, ri.w, fj, rj.w, ri, rj, a.p, cx, cy, cz)) continue;
+                if (!collision_contrib(i, j, fi, wi, fj, rj.w, ri, rj, a.p, cx, cy, cz)) continue;
                 if (cnt < OVF_MAX) {   // insertion into the sorted list
                     int w = cnt;
                     while (w > 0 && lj[w - 1][t] > j) {
@@ -1241,9 +1245,9 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
                         if (d2 > th2 || d2 < 1e-18f) continue;
                         const uint32_t j = a.sids[idx];
                         if (j >= best || (has_last && j <= last)) continue;
-                        const float4 rj = a.sR[idx];
+                        const float4 rj = a.rest4[j];
                         float cx, cy, cz;
-                        if (collision_contrib(i, j, fi, ri.w, fj, rj.w, ri, rj, a.p, cx, cy, cz)) {
+                        if (collision_contrib(i, j, fi, wi, fj, rj.w, ri, rj, a.p, cx, cy, cz)) {
                             best = j;
                             bx = cx;
                             by = cy;
@@ -1259,30 +1263,26 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
                 has_last = true;
             }
         }
-        const float4 nx = make_float4(fi.x + clamp_apply(ax), fi.y + clamp_apply(ay), fi.z + clamp_apply(az), ri.w);
-        finish_particle(nx, a.prev[i], i, a.ground, a.p, a.xout, a.vout);
+        const float4 nx = make_float4(fi.x + clamp_apply(ax), fi.y + clamp_apply(ay), fi.z + clamp_apply(az), wi);
+        finish_particle(nx, pv, i, a.ground, a.p, a.xout, a.vout);
     }
 }
 
-// Positions permuted into sorted (cell) order: sP[s] = F[sids[s]].  Also
-// clears the count of the sweep's overflow list (ovf[0]).
-// Sorted copies for the sweep: sP = (position, key bits), sR = (rest
-// position, inverse mass), so one 16-byte load gives a candidate's key and
-// position, and one more its rest position and mass.  The first sorted index
-// of each group is inserted into the group table (slot recorded in slot_rec,
+// The sorted copy for the sweep: sP = (position, key bits), so one 16-byte
+// load gives a candidate's key and position (rest positions and masses are
+// read by particle id, for contacts only).  The first sorted index of each
+// group is inserted into the group table (slot recorded in slot_rec,
 // EMPTY_KEY elsewhere, for k_coll_keys to empty next pass).  Also clears the
 // sweep's overflow count.
-__global__ void k_coll_gather(const float4* F, const float4* rest4, const uint32_t* sids, const uint32_t* skeys,
-                              uint32_t n, float4* sP, float4* sR, uint32_t* ovf, uint2* table, uint32_t mask,
-                              uint32_t* slot_rec) {
+__global__ void k_coll_gather(const float4* F, const uint32_t* sids, const uint32_t* skeys, uint32_t n, float4* sP,
+                              uint32_t* ovf, uint2* table, uint32_t mask, uint32_t* slot_rec) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (ovf && s == 0) ovf[0] = 0u;
     if (s >= n) return;
     const uint32_t i = sids[s];
     const uint32_t key = skeys[s];
-    const float4 f = F[i], r = rest4[i];
+    const float4 f = F[i];
     sP[s] = make_float4(f.x, f.y, f.z, __uint_as_float(key));
-    sR[s] = make_float4(r.x, r.y, r.z, f.w);
     uint32_t rec = EMPTY_KEY;
     const uint32_t g = key >> GROUP_SHIFT;
     if (s == 0 || (skeys[s - 1] >> GROUP_SHIFT) != g) {
@@ -1894,12 +1894,11 @@ void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32
     launch_radix_pass(keys_alt, ids_alt, keys, ids, n, 24, hist, s);
 }
 
-void launch_coll_gather(const float4* F, const float4* rest4, const uint32_t* sids, const uint32_t* skeys, uint32_t n,
-                        float4* sP, float4* sR, uint32_t* ovf, uint2* table, uint32_t mask, uint32_t* slot_rec,
-                        hipStream_t s) {
+void launch_coll_gather(const float4* F, const uint32_t* sids, const uint32_t* skeys, uint32_t n, float4* sP,
+                        uint32_t* ovf, uint2* table, uint32_t mask, uint32_t* slot_rec, hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_coll_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, F, rest4, sids, skeys, n, sP, sR, ovf, table,
-                       mask, slot_rec);
+    hipLaunchKernelGGL(k_coll_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, F, sids, skeys, n, sP, ovf, table, mask,
+                       slot_rec);
 }
 
 void launch_coll_sweep(const CollArgs& a, hipStream_t s) {

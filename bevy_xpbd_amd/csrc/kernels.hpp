@@ -118,7 +118,7 @@ struct CollGrid {
 struct CollArgs {
     const float4* F;     // positions after the constraint iterations
     const float4* sP;    // sorted order: position xyz, w = cell key bits
-    const float4* sR;    // sorted order: rest position xyz, w = inverse mass
+    const float4* rest4; // by particle id: rest position xyz, w = inverse mass
     const float4* prev;  // X (substep start) for the velocity update
     float4* xout;
     float4* vout;
@@ -183,11 +183,11 @@ void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32
 uint32_t radix_hist_words(uint32_t n);
 void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
                        uint32_t shift, uint32_t* hist, hipStream_t s, bool hist_done = false);
-// sorted copies for the sweep (sP, sR), group-table inserts (slots recorded
-// in slot_rec, n words; the table must be empty on entry), overflow count clear
-void launch_coll_gather(const float4* F, const float4* rest4, const uint32_t* sids, const uint32_t* skeys, uint32_t n,
-                        float4* sP, float4* sR, uint32_t* ovf, uint2* table, uint32_t mask, uint32_t* slot_rec,
-                        hipStream_t s);
+// the sorted copy for the sweep (sP = position, key bits), group-table
+// inserts (slots recorded in slot_rec, n words; the table must be empty on
+// entry), overflow count clear
+void launch_coll_gather(const float4* F, const uint32_t* sids, const uint32_t* skeys, uint32_t n, float4* sP,
+                        uint32_t* ovf, uint2* table, uint32_t mask, uint32_t* slot_rec, hipStream_t s);
 void launch_coll_sweep(const CollArgs& a, hipStream_t s);
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
 uint32_t gs_max_blocks(uint32_t block);

@@ -76,7 +76,6 @@ struct xpbd_cloth {
     std::vector<void*> allocs;
     float4 *X = nullptr, *V = nullptr, *A = nullptr, *B = nullptr, *V2 = nullptr, *rest4 = nullptr;
     float4* sP = nullptr;   // collision: positions + cell key in sorted order
-    float4* sR = nullptr;   // collision: rest positions + inverse mass in sorted order
     uint32_t* slot_rec = nullptr;   // collision: table slot filled at each sorted index (EMPTY_KEY: none)
     uint32_t *soff = nullptr, *swid = nullptr, *nbj = nullptr;
     float *nbr = nullptr, *nbc = nullptr;
@@ -287,12 +286,12 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     enqueue_grids(h, F, nparts, coll_cell(h), s);
     launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist, s);
     launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true);
-    launch_coll_gather(F, h->rest4, h->ids, h->keys, h->n, h->sP, h->sR, h->ovf, h->table, h->table_slots - 1,
+    launch_coll_gather(F, h->ids, h->keys, h->n, h->sP, h->ovf, h->table, h->table_slots - 1,
                        h->slot_rec, s);
     CollArgs c{};
     c.F = F;
     c.sP = h->sP;
-    c.sR = h->sR;
+    c.rest4 = h->rest4;
     c.prev = h->X;
     c.xout = xout;
     c.vout = vout;
@@ -682,7 +681,6 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(dalloc(h, &h->V2, n));
     TRY(dalloc(h, &h->rest4, n));
     TRY(dalloc(h, &h->sP, n));
-    TRY(dalloc(h, &h->sR, n));
     std::vector<float> rest4(4 * n);
     h->init_pos4.assign(4 * n, 0.f);
     for (size_t i = 0; i < n; ++i) {
@@ -1467,7 +1465,7 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         enqueue_grids(h, h->A, np, coll_cell(h), s);
         launch_coll_keys(h->A, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist, s);
         launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true);
-        launch_coll_gather(h->A, h->rest4, h->ids, h->keys, h->n, h->sP, h->sR, h->ovf, h->table, h->table_slots - 1,
+        launch_coll_gather(h->A, h->ids, h->keys, h->n, h->sP, h->ovf, h->table, h->table_slots - 1,
                            h->slot_rec, s);
     };
     CollArgs c{};
@@ -1475,7 +1473,7 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         prep_collision();
         c.F = h->A;
         c.sP = h->sP;
-        c.sR = h->sR;
+        c.rest4 = h->rest4;
         c.prev = h->X;
         c.xout = h->B;
         c.vout = h->V2;
