@@ -1578,13 +1578,26 @@ __global__ __launch_bounds__(256) void k_jacobi_persistent(JacobiArgs a, Persist
     GridSync gsync;
     gsync.init(q.barrier, q.error);
     const uint32_t nthreads = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += nthreads) q.A[i] = predict(q.X[i], q.V[i], i, p);
+    // predict (whole wavefronts per 64 particles: the slice tame bytes of A)
+    const bool track = FASTDIV && q.tameA && q.tameB;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < p.n; i0 += nthreads) {
+        const uint32_t i = i0 + threadIdx.x;
+        const bool valid = i < p.n;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (valid) {
+            v = predict(q.X[i], q.V[i], i, p);
+            q.A[i] = v;
+        }
+        if (track) write_slice_tame(q.tameA, i >> 6, valid, !valid || tame3(v));
+    }
     gsync.sync();
     const uint32_t spb = blockDim.x >> 6;
     const uint32_t groups = (p.nslices + spb - 1) / spb;
     JacobiArgs b = a;
     float4* cur = q.A;
     float4* nxt = q.B;
+    uint8_t* tcur = q.tameA;
+    uint8_t* tnxt = q.tameB;
     for (uint32_t k = 0; k < q.iterations; ++k) {
         const bool last = k + 1 == q.iterations;
         b.src = cur;
@@ -1592,21 +1605,38 @@ __global__ __launch_bounds__(256) void k_jacobi_persistent(JacobiArgs a, Persist
             const uint32_t slice = gi * spb + (threadIdx.x >> 6);
             bool valid;
             const uint32_t lane = threadIdx.x & 63;
+            // the short exact operations where every slice this one reads is tame
+            const bool fast = track && neighbourhood_tame(tcur, a.srange, slice, p.nslices);
             if (last && q.final_write) {
                 b.dst = q.X;
                 b.vout = q.V;
-                if (PACKED && lds) jacobi_particle<JM_FINAL, PACKED, false, true>(b, lpairs, slice, lane, valid);
-                else jacobi_particle<JM_FINAL, PACKED, false, false>(b, lpairs, slice, lane, valid);
+                if (PACKED && lds) {
+                    if (fast) jacobi_particle<JM_FINAL, PACKED, true, true>(b, lpairs, slice, lane, valid);
+                    else jacobi_particle<JM_FINAL, PACKED, false, true>(b, lpairs, slice, lane, valid);
+                } else {
+                    if (fast) jacobi_particle<JM_FINAL, PACKED, true, false>(b, lpairs, slice, lane, valid);
+                    else jacobi_particle<JM_FINAL, PACKED, false, false>(b, lpairs, slice, lane, valid);
+                }
             } else {
                 b.dst = nxt;
-                if (PACKED && lds) jacobi_particle<0, PACKED, false, true>(b, lpairs, slice, lane, valid);
-                else jacobi_particle<0, PACKED, false, false>(b, lpairs, slice, lane, valid);
+                float4 out;
+                if (PACKED && lds) {
+                    out = fast ? jacobi_particle<0, PACKED, true, true>(b, lpairs, slice, lane, valid)
+                               : jacobi_particle<0, PACKED, false, true>(b, lpairs, slice, lane, valid);
+                } else {
+                    out = fast ? jacobi_particle<0, PACKED, true, false>(b, lpairs, slice, lane, valid)
+                               : jacobi_particle<0, PACKED, false, false>(b, lpairs, slice, lane, valid);
+                }
+                if (track && !last) write_slice_tame(tnxt, slice, slice < p.nslices, !valid || tame3(out));
             }
         }
         if (!last) gsync.sync();
         float4* t = cur;
         cur = nxt;
         nxt = t;
+        uint8_t* tt = tcur;
+        tcur = tnxt;
+        tnxt = tt;
     }
 }
 
@@ -1914,7 +1944,11 @@ void launch_coll_sweep(const CollArgs& a, hipStream_t s) {
 int coresident_per_cu(int per_cu, uint32_t block_threads) {
     const int waves = int((block_threads + 63) / 64);
     const int cap = std::max(1, (7 * 4) / std::max(1, waves));
-    return std::max(0, std::min(per_cu, cap));
+    int use = std::min(per_cu, cap);
+    // a kernel whose own limit is below 8 wavefronts per SIMD also keeps a
+    // block's worth of slots free rather than filling that limit
+    if (use == per_cu && per_cu * waves < 8 * 4) use = per_cu - 1;
+    return std::max(1, use);
 }
 
 uint32_t persistent_max_blocks(bool packed, bool fastdiv) {

@@ -105,6 +105,10 @@ struct PersistArgs {
     uint32_t* barrier;
     uint32_t* error;
     const uint32_t* gate;   // non-null: the launch does nothing unless *gate != 0
+    // per-slice tame bytes of A and B (both non-null and fast division on:
+    // the short exact operations on slices whose neighbourhood is tame)
+    uint8_t* tameA;
+    uint8_t* tameB;
 };
 
 // Cell grid of one cloth in the collision broad phase (device memory, 12 words).

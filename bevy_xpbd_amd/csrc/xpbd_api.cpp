@@ -392,6 +392,10 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
             q.barrier = h->barrier;
             q.error = h->err;
             q.gate = h->warm_flag;
+            if (g.fastdiv) {
+                q.tameA = h->tameA;
+                q.tameB = h->tameB;
+            }
             launch_jacobi_persistent(g, q, h->pj_blocks, s);
             float4* alt = (iters % 2 == 0) ? h->A : h->B;
             if (collision) {
@@ -423,6 +427,10 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         q.final_write = collision ? 0u : 1u;
         q.barrier = h->barrier;
         q.error = h->err;
+        if (a.fastdiv) {
+            q.tameA = h->tameA;
+            q.tameB = h->tameB;
+        }
         {
             ProbeScope ps(h, XPBD_KERNEL_JACOBI_PERSISTENT);
             launch_jacobi_persistent(a, q, h->pj_blocks, s);
@@ -1541,6 +1549,10 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
                 q.final_write = 0;
                 q.barrier = h->barrier;
                 q.error = h->err;
+                if (pa.fastdiv) {
+                    q.tameA = h->tameA;
+                    q.tameB = h->tameB;
+                }
                 launch_jacobi_persistent(pa, q, h->pj_blocks, s);
                 break;
             }

@@ -75,15 +75,17 @@ def test_jacobi_iteration_count_edge_cases(O, X, iters, fused, plain):
                  substeps=4, collision=coll, fused_predict=fused, plain_adjacency=plain)
 
 
+@pytest.mark.parametrize("fast", [0, 1])
 @pytest.mark.parametrize("plain", [0, 1])
 @pytest.mark.parametrize("iters", [1, 2, 3, 10])
-def test_persistent_jacobi_bit_exact(O, X, iters, plain):
+def test_persistent_jacobi_bit_exact(O, X, iters, plain, fast):
     """Predict and all iterations of a substep in one launch (grid barriers
-    between iterations): the same results as one launch per iteration."""
-    note(f"case persistent iters={iters} plain_adjacency={plain}")
+    between iterations, the slice tame bytes carried across them for the fast
+    division): the same results as one launch per iteration."""
+    note(f"case persistent iters={iters} plain_adjacency={plain} fast={fast}")
     for coll in (False, True):
         run_pair(O, X, dict(cols=70, rows=33, spacing=0.03, pins=2, jitter=1e-3, seed=9), S=8, I=iters,
-                 substeps=3, collision=coll, plain_adjacency=plain, persistent_jacobi=1)
+                 substeps=3, collision=coll, plain_adjacency=plain, persistent_jacobi=1, fast_division=fast)
 
 
 def test_reference_grid_plain_adjacency(O, X):
