@@ -1934,7 +1934,9 @@ void launch_coll_gather(const float4* F, const uint32_t* sids, const uint32_t* s
 void launch_coll_sweep(const CollArgs& a, hipStream_t s) {
     if (a.p.n == 0) return;
     hipLaunchKernelGGL(k_coll_sweep, dim3(cdiv(a.p.n, 256)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_coll_sweep_overflow, dim3(std::min(cdiv(a.p.n, OVF_BLOCK), 2048u)), dim3(OVF_BLOCK), 0, s, a);
+    // overflow is rare: a small grid (it strides over the list) keeps the
+    // usually empty launch short
+    hipLaunchKernelGGL(k_coll_sweep_overflow, dim3(std::min(cdiv(a.p.n, OVF_BLOCK), 256u)), dim3(OVF_BLOCK), 0, s, a);
 }
 
 // Blocks per CU a grid-barrier launch may use: the occupancy figure, but at
