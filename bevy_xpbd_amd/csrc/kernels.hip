@@ -258,6 +258,10 @@ __device__ __forceinline__ float4 stage_pair(float2 rc, const DevParams& p) {
 // itself (zero length: skipped by the reference's len < 1e-8 rule), so no
 // load is conditional.  The slice width is wavefront-uniform.
 constexpr uint32_t JU = 6;
+// The tiled kernel's chunk: its neighbours come from LDS, so shorter chunks
+// (fewer words held in registers) win there: 2 was fastest of 2..6 and 12
+// on the 1024^2 sheet, while the per-slice kernel's global gathers want 6.
+constexpr uint32_t JT_JU = 2;
 constexpr uint32_t LDS_PAIRS_MAX = 2048;   // (rest, compliance) table staged in LDS up to this size (static)
 constexpr uint32_t LDS_PAIRS_DYN_MAX = 4096;   // k_jacobi: table sized to the mesh's pairs (dynamic LDS, <= 32 KB)
 constexpr uint32_t JT_PAIRS_MAX = 1024;    // the tiled kernel's staged table (float4)
@@ -463,22 +467,22 @@ __device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, con
         const uint32_t base = a.sell.soff[slice] + lane;
         const uint32_t width = a.sell.swid[slice];
         uint32_t k0 = 0;
-        if (width >= JU) {
+        if (width >= JT_JU) {
             // the next chunk's entry words load while this chunk computes
-            uint32_t wn[JU];
+            uint32_t wn[JT_JU];
 #pragma unroll
-            for (uint32_t u = 0; u < JU; ++u) wn[u] = __builtin_nontemporal_load(a.sell.tpacked + base + u * 64);
-            for (; k0 + JU <= width; k0 += JU) {
-                uint32_t wc[JU];
+            for (uint32_t u = 0; u < JT_JU; ++u) wn[u] = __builtin_nontemporal_load(a.sell.tpacked + base + u * 64);
+            for (; k0 + JT_JU <= width; k0 += JT_JU) {
+                uint32_t wc[JT_JU];
 #pragma unroll
-                for (uint32_t u = 0; u < JU; ++u) wc[u] = wn[u];
-                if (k0 + 2 * JU <= width) {
+                for (uint32_t u = 0; u < JT_JU; ++u) wc[u] = wn[u];
+                if (k0 + 2 * JT_JU <= width) {
 #pragma unroll
-                    for (uint32_t u = 0; u < JU; ++u)
-                        wn[u] = __builtin_nontemporal_load(a.sell.tpacked + base + (k0 + JU + u) * 64);
+                    for (uint32_t u = 0; u < JT_JU; ++u)
+                        wn[u] = __builtin_nontemporal_load(a.sell.tpacked + base + (k0 + JT_JU + u) * 64);
                 }
 #pragma unroll
-                for (uint32_t u = 0; u < JU; ++u) {
+                for (uint32_t u = 0; u < JT_JU; ++u) {
                     jacobi_entry<FAST>(pi, tile_neighbour(tl + lidx, wc[u]), tile_pair<LDSP>(a, lpairs, wc[u]), ax, ay,
                                        az);
                 }
