@@ -739,7 +739,6 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const uint32_t* keys
     for (uint32_t w = 0; w < RADIX_WAVES; ++w) h[w][t] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * RADIX_TILE + wave * RADIX_WAVE_SPAN + lane;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t kk[RADIX_ITEMS];
 #pragma unroll
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
@@ -750,8 +749,7 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const uint32_t* keys
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
         const bool valid = base + r * 64 < n;
         const uint32_t d = (kk[r] >> shift) & 255u;
-        const uint64_t peers = digit_peers(d, valid);
-        if (valid && (peers & lt) == 0) h[wave][d] += uint32_t(__popcll(peers));
+        if (valid) atomicAdd(&h[wave][d], 1u);   // LDS atomics: cheaper here than ranking the wave
     }
     __syncthreads();
     uint32_t c = 0;
@@ -776,7 +774,6 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_coll_keys(const float4* F, uint
     for (uint32_t w = 0; w < RADIX_WAVES; ++w) h[w][t] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * RADIX_TILE + wave * RADIX_WAVE_SPAN + lane;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
         const uint32_t i = base + r * 64;
@@ -793,8 +790,7 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_coll_keys(const float4* F, uint
             ids[i] = i;
         }
         const uint32_t d = key & 255u;
-        const uint64_t peers = digit_peers(d, valid);
-        if (valid && (peers & lt) == 0) h[wave][d] += uint32_t(__popcll(peers));
+        if (valid) atomicAdd(&h[wave][d], 1u);   // LDS atomics: cheaper here than ranking the wave
     }
     __syncthreads();
     uint32_t cnt = 0;
