@@ -101,6 +101,8 @@ def lib():
         L.om_grid_cloth_hanging.argtypes = [u32, u32, f32]
         L.om_parse_welded_obj.restype = vp
         L.om_parse_welded_obj.argtypes = [C.c_char_p]
+        L.om_assemble.restype = vp
+        L.om_assemble.argtypes = [u32, vp, vp, u32, vp, vp, vp, vp]
         L.om_sheet.restype = vp
         L.om_sheet.argtypes = [C.POINTER(SheetDesc)]
         L.om_batch.restype = vp
@@ -156,6 +158,21 @@ def grid_cloth_hanging(quad_cols: int, quad_rows: int, cell_size: float) -> dict
 
 def parse_welded_obj(text: str) -> dict:
     return _read_mesh(lib().om_parse_welded_obj(text.encode()))
+
+
+def assemble(positions, inv_mass, constraints) -> dict:
+    """finalize_cloth_mesh's colouring, colour sort and CSR scatter applied to
+    the rows `constraints` = [(i, j, rest, compliance), ...] (mesh_prep.rs:153-221,
+    580-635)."""
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    w = np.ascontiguousarray(inv_mass, np.float32)
+    cons = list(constraints)
+    ci = np.array([c[0] for c in cons], np.uint32)
+    cj = np.array([c[1] for c in cons], np.uint32)
+    rest = np.array([c[2] for c in cons], np.float32)
+    comp = np.array([c[3] for c in cons], np.float32)
+    return _read_mesh(lib().om_assemble(len(pos), _ptr(pos), _ptr(w), len(cons), _ptr(ci), _ptr(cj),
+                                        _ptr(rest), _ptr(comp)))
 
 
 def sheet_desc(**kw) -> SheetDesc:

@@ -1134,6 +1134,26 @@ void* om_grid_cloth_hanging(uint32_t quad_cols, uint32_t quad_rows, float cell_s
 
 void* om_parse_welded_obj(const char* text) { return parse_welded_obj(std::string(text ? text : "")); }
 
+// finalize_cloth_mesh's constraint half (mesh_prep.rs:580-635) on a caller's
+// constraint list: the greedy colouring and colour sort
+// (partition_constraints_for_gs_batches, mesh_prep.rs:153-221), then the CSR
+// by cursor scatter.  Positions are the initial and rest pose; no triangles.
+void* om_assemble(uint32_t n, const float* pos3, const float* inv_mass, uint32_t e, const uint32_t* ci,
+                  const uint32_t* cj, const float* rest, const float* comp) {
+    if (n == 0 || !pos3 || !inv_mass || (e && (!ci || !cj || !rest || !comp))) return nullptr;
+    for (uint32_t k = 0; k < e; ++k)
+        if (ci[k] >= n || cj[k] >= n || ci[k] == cj[k]) return nullptr;
+    auto* m = new OMesh();
+    m->positions.assign(pos3, pos3 + 3 * size_t(n));
+    m->rest_positions = m->positions;
+    m->inv_mass.assign(inv_mass, inv_mass + n);
+    m->ground.assign(n, 0);
+    std::vector<Constraint> cons(e);
+    for (uint32_t k = 0; k < e; ++k) cons[k] = {ci[k], cj[k], rest[k], comp[k]};
+    assemble_constraints(*m, n, std::move(cons));
+    return m;
+}
+
 void* om_sheet(const om_sheet_desc* d) {
     if (!d || d->cols < 2 || d->rows < 2) return nullptr;
     return make_batch(d, 1);

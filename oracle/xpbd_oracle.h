@@ -75,6 +75,10 @@ typedef struct om_sheet_desc {
 
 void* om_grid_cloth_hanging(uint32_t quad_cols, uint32_t quad_rows, float cell_size);
 void* om_parse_welded_obj(const char* text);
+/* The colouring + colour sort + CSR scatter of finalize_cloth_mesh applied to
+ * the given constraint rows (mesh_prep.rs:153-221, 580-635); NULL on bad input. */
+void* om_assemble(uint32_t n, const float* pos3, const float* inv_mass, uint32_t e, const uint32_t* ci,
+                  const uint32_t* cj, const float* rest, const float* comp);
 void* om_sheet(const om_sheet_desc* d);
 void* om_batch(const om_sheet_desc* sheets, uint32_t count);
 uint64_t om_count(const void* mesh, int field);

@@ -23,6 +23,34 @@ def oracle_params(O, X, u, cfg, collision=False, coll_mode=None):
     return p
 
 
+def tiny_mesh(pos, inv_mass, cons=(), rest_pos=None, ground=None):
+    """A tiny mesh dict with a CSR built by cursor scatter (mesh_prep.rs:588-634)."""
+    pos = np.asarray(pos, np.float32)
+    n = len(pos)
+    ci = np.array([c[0] for c in cons], np.uint32)
+    cj = np.array([c[1] for c in cons], np.uint32)
+    crest = np.array([c[2] for c in cons], np.float32)
+    ccomp = np.array([c[3] for c in cons], np.float32)
+    counts = np.zeros(n, np.int64)
+    for a, b in zip(ci, cj):
+        counts[a] += 1
+        counts[b] += 1
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+    other = np.zeros(offs[-1], np.uint32)
+    eid = np.zeros(offs[-1], np.uint32)
+    cur = offs[:-1].astype(np.int64).copy()
+    for e, (a, b) in enumerate(zip(ci, cj)):
+        other[cur[a]], eid[cur[a]] = b, e
+        cur[a] += 1
+        other[cur[b]], eid[cur[b]] = a, e
+        cur[b] += 1
+    return {"num_particles": n, "inv_mass": np.asarray(inv_mass, np.float32),
+            "rest_positions": pos if rest_pos is None else np.asarray(rest_pos, np.float32),
+            "positions": pos, "ground": np.zeros(n, np.uint32) if ground is None else np.asarray(ground, np.uint32), "constraint_i": ci, "constraint_j": cj,
+            "constraint_rest": crest, "constraint_comp": ccomp, "neighbor_offsets": offs,
+            "neighbor_other": other, "neighbor_constraint_id": eid}
+
+
 def max_abs(a, b):
     return float(np.max(np.abs(np.asarray(a, np.float64)[:, :3] - np.asarray(b, np.float64)[:, :3])))
 
