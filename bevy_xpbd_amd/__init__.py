@@ -19,6 +19,7 @@ from .api import (  # noqa: F401
     ClothSimControl,
     ClothSimUniforms,
     DeviceArray,
+    MeshDesc,
     SheetDesc,
     WarmStartMLP,
     MLP_PARAMS,
@@ -41,4 +42,11 @@ from .api import (  # noqa: F401
     PIN_TWO_CORNERS,
     PIN_TOP_ROW,
     PIN_CENTER,
+    XPBD_OK,
+    XPBD_E_INVALID,
+    XPBD_E_HIP,
+    XPBD_E_NOMEM,
+    XPBD_E_UNSUPPORTED,
+    XPBD_E_STATE,
+    XPBD_E_NO_DEVICE,
 )

@@ -78,6 +78,19 @@ class SheetDesc(C.Structure):
         return d
 
 
+class MeshDesc(C.Structure):
+    """xpbd_mesh_desc: ClothSimConfig's topology arrays (cloth_compute.rs:142-179)."""
+    _fields_ = [
+        ("num_particles", C.c_uint32), ("num_tris", C.c_uint32), ("num_distance_constraints", C.c_uint32),
+        ("constraint_batch_count", C.c_uint32), ("initial_pos", C.c_void_p), ("rest_pos", C.c_void_p),
+        ("initial_normals", C.c_void_p), ("inv_mass", C.c_void_p), ("constraint_i", C.c_void_p),
+        ("constraint_j", C.c_void_p), ("constraint_rest_len", C.c_void_p), ("constraint_compliance", C.c_void_p),
+        ("constraint_batch_offsets", C.c_void_p), ("tri_indices", C.c_void_p), ("neighbor_offsets", C.c_void_p),
+        ("neighbor_packed", C.c_void_p), ("ground", C.c_void_p), ("num_cloths", C.c_uint32),
+        ("cloth_offsets", C.c_void_p),
+    ]
+
+
 class ClothSimConfig(C.Structure):
     """Scalar fields of ClothSimConfig (cloth_compute.rs:142-179)."""
     _fields_ = [
@@ -143,6 +156,8 @@ def lib():
             "xpbd_mesh_parse_welded_obj": (i32, [C.c_char_p, C.c_size_t, pp]),
             "xpbd_mesh_sheet": (i32, [C.POINTER(SheetDesc), pp]),
             "xpbd_mesh_batch": (i32, [C.POINTER(SheetDesc), u32, pp]),
+            "xpbd_mesh_from_arrays": (i32, [C.POINTER(MeshDesc), pp]),
+            "xpbd_mesh_last_error": (C.c_char_p, []),
             "xpbd_mesh_count": (u64, [vp, i32]),
             "xpbd_mesh_read": (i32, [vp, i32, vp, u64]),
             "xpbd_mesh_destroy": (None, [vp]),
@@ -168,6 +183,7 @@ def lib():
             "xpbd_time_kernel": (i32, [vp, i32, u32, C.POINTER(C.c_float)]),
             "xpbd_profile_frames": (i32, [vp, u32, i32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
             "xpbd_debug_collision_keys": (i32, [vp, f32, vp, vp, vp]),
+            "xpbd_debug_raise_device_error": (i32, [vp]),
             "xpbd_get_layout": (i32, [vp, vp]),
             "xpbd_selftest_division": (i32, [i32, i32, u64, u64, vp]),
             "xpbd_selftest_sqrt": (i32, [i32, i32, vp]),
@@ -241,6 +257,79 @@ class ClothMeshData:
     def batch(cls, descs) -> "ClothMeshData":
         arr = (SheetDesc * len(descs))(*descs)
         return cls._build(lib().xpbd_mesh_batch, arr, len(descs))
+
+    @classmethod
+    def from_arrays(cls, initial_pos, inv_mass, constraint_i=(), constraint_j=(), constraint_rest_len=(),
+                    constraint_compliance=(), rest_pos=None, initial_normals=None, tri_indices=None,
+                    constraint_batch_offsets=None, neighbor_offsets=None, neighbor_packed=None,
+                    ground=None, cloth_offsets=None) -> "ClothMeshData":
+        """A caller's topology in ClothSimConfig's arrays (cloth_compute.rs:142-179):
+        positions as (N, 3) or (N, 4) rows, constraint rows, optional colour
+        batch offsets (None: coloured here as finalize_cloth_mesh does) and
+        optional Jacobi adjacency (neighbor_offsets + neighbor_packed, the
+        vec4 entries of pack_neighbor_gpu; None: built by cursor scatter);
+        cloth_offsets splits the particles into cloths that collide only
+        within themselves (None: one cloth)."""
+        def vec4(a):
+            if a is None:
+                return None
+            a = np.asarray(a, np.float32)
+            out = np.zeros((len(a), 4), np.float32)
+            out[:, :a.shape[1]] = a[:, :4] if a.shape[1] >= 4 else a
+            if a.shape[1] >= 4:
+                out[:, 3] = 0.0
+            return out
+
+        def arr(a, dt):
+            return None if a is None else np.ascontiguousarray(np.asarray(a).reshape(-1), dtype=dt)
+
+        keep = {
+            "initial_pos": vec4(initial_pos), "rest_pos": vec4(rest_pos), "initial_normals": vec4(initial_normals),
+            "inv_mass": arr(inv_mass, np.float32), "constraint_i": arr(constraint_i, np.uint32),
+            "constraint_j": arr(constraint_j, np.uint32), "constraint_rest_len": arr(constraint_rest_len, np.float32),
+            "constraint_compliance": arr(constraint_compliance, np.float32),
+            "constraint_batch_offsets": arr(constraint_batch_offsets, np.uint32),
+            "tri_indices": arr(tri_indices, np.uint32), "neighbor_offsets": arr(neighbor_offsets, np.uint32),
+            "neighbor_packed": arr(neighbor_packed, np.float32), "ground": arr(ground, np.uint32),
+            "cloth_offsets": arr(cloth_offsets, np.uint32),
+        }
+        d = MeshDesc()
+        d.num_particles = len(keep["initial_pos"])
+        d.num_tris = 0 if keep["tri_indices"] is None else len(keep["tri_indices"]) // 3
+        d.num_distance_constraints = len(keep["constraint_i"])
+        bo = keep["constraint_batch_offsets"]
+        d.constraint_batch_count = 0 if bo is None else max(len(bo) - 1, 0)
+        co = keep["cloth_offsets"]
+        d.num_cloths = 0 if co is None else max(len(co) - 1, 0)
+        for k, v in keep.items():
+            setattr(d, k, None if v is None or v.size == 0 else v.ctypes.data)
+        h = C.c_void_p()
+        code = lib().xpbd_mesh_from_arrays(C.byref(d), C.byref(h))
+        if code != XPBD_OK:
+            raise XpbdError(code, "xpbd_mesh_from_arrays", lib().xpbd_mesh_last_error().decode())
+        return cls(h)
+
+    def sim_config_arrays(self) -> dict:
+        """The topology arrays of this mesh as ClothSimConfig carries them
+        (cloth_compute.rs:142-179, 1599-1700): vec4 positions and the packed
+        Jacobi adjacency of pack_neighbor_gpu (cloth_jacobi.rs:95-111)."""
+        f = self.fields()
+        n = f["num_particles"]
+
+        def vec4(a):
+            out = np.zeros((n, 4), np.float32)
+            out[:, :3] = a
+            return out
+
+        packed = np.stack([f["neighbor_other"].astype(np.float32), f["neighbor_rest_len"],
+                           f["neighbor_compliance"], f["neighbor_constraint_id"].astype(np.float32)], axis=1)
+        return {"initial_pos": vec4(f["positions"]), "rest_pos": vec4(f["rest_positions"]),
+                "initial_normals": vec4(f["normals"]) if len(f["normals"]) == n else None,
+                "inv_mass": f["inv_mass"], "constraint_i": f["constraint_i"], "constraint_j": f["constraint_j"],
+                "constraint_rest_len": f["constraint_rest"], "constraint_compliance": f["constraint_comp"],
+                "constraint_batch_offsets": f["batch_offsets"], "tri_indices": f["indices"],
+                "neighbor_offsets": f["neighbor_offsets"], "neighbor_packed": packed, "ground": f["ground"],
+                "cloth_offsets": f["cloth_offsets"]}
 
     def field(self, name: str) -> np.ndarray:
         fid, dt, width = MESH_FIELDS[name]
@@ -393,6 +482,10 @@ class ClothComputePlugin:
         _check(lib().xpbd_import_render_buffers_fd(self._h, int(pos_fd), int(pos_size), int(pos_offset), int(nrm_fd),
                                                    int(nrm_size), int(nrm_offset)),
                "xpbd_import_render_buffers_fd", self._h)
+
+    def raise_device_error(self):
+        """Test hook: set the device error word, as an expired barrier wait does."""
+        _check(lib().xpbd_debug_raise_device_error(self._h), "xpbd_debug_raise_device_error", self._h)
 
     def residual(self) -> float:
         r = C.c_double()

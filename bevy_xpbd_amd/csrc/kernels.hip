@@ -1430,11 +1430,18 @@ __device__ __forceinline__ uint32_t gs_load(uint32_t* p) {
 __device__ __forceinline__ uint32_t gs_add(uint32_t* p, uint32_t v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Once any wait has expired (the error word is set, and it stays set until
+// the host reads it), every later wait of every block returns at once, so a
+// launch whose grid was not co-resident drains after one expiry instead of
+// spending the full bound at each of its remaining barriers.
 __device__ __forceinline__ void gs_wait(uint32_t* p, uint32_t v, uint32_t* err) {
+    if (gs_load(err)) return;
     uint32_t spins = 0;
     while (gs_load(p) < v) {
         __builtin_amdgcn_s_sleep(1);
-        if (++spins > (1u << 22)) {   // seconds at most: report instead of hanging
+        ++spins;
+        if ((spins & 1023u) == 0 && gs_load(err)) return;
+        if (spins > (1u << 22)) {   // seconds at most: report instead of hanging
             atomicExch(err, 1u);
             return;
         }

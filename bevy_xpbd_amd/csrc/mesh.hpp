@@ -37,10 +37,16 @@ struct Mesh {
 Mesh* grid_cloth_hanging(uint32_t quad_cols, uint32_t quad_rows, float cell_size);
 Mesh* parse_welded_obj(const std::string& text);
 Mesh* sheet_batch(const xpbd_sheet_desc* descs, uint32_t count);
+// ClothSimConfig's topology arrays (cloth_compute.rs:142-179), validated;
+// nullptr with the reason in `why` when an array is inconsistent.
+Mesh* mesh_from_arrays(const xpbd_mesh_desc& d, std::string& why);
 
 // Greedy edge colouring + colour-sorted order + CSR cursor scatter
 // (mesh_prep.rs:153-221, 584-635).  `constraints` is consumed.
 void assemble_constraints(Mesh& m, size_t n, std::vector<Constraint>&& constraints);
+// The CSR of m's constraint rows by cursor scatter in row order
+// (mesh_prep.rs:588-634).
+void scatter_csr(Mesh& m, size_t n);
 
 void derive_collision_grid(const float* rest_vec4, uint32_t n, float thickness, xpbd_coll_grid* g);
 

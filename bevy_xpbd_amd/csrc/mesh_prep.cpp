@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <unordered_map>
@@ -125,7 +126,13 @@ void assemble_constraints(Mesh& m, size_t n, std::vector<Constraint>&& cons) {
         }
     }
     std::vector<Constraint>().swap(cons);
-    // CSR by cursor scatter in colour-sorted order (mesh_prep.rs:588-634).
+    scatter_csr(m, n);
+}
+
+// CSR by cursor scatter in row order (mesh_prep.rs:588-634): row k is
+// appended to the slice of its first endpoint, then of its second.
+void scatter_csr(Mesh& m, size_t n) {
+    const size_t e = m.constraint_i.size();
     m.neighbor_offsets.assign(n + 1, 0);
     for (size_t k = 0; k < e; ++k) {
         m.neighbor_offsets[m.constraint_i[k] + 1]++;
@@ -544,6 +551,164 @@ Mesh* sheet_batch(const xpbd_sheet_desc* descs, uint32_t count) {
     }
     assemble_constraints(*m, m->inv_mass.size(), std::move(cons));
     return m;
+}
+
+// ClothSimConfig's topology arrays (cloth_compute.rs:142-179), as
+// to_sim_config copies them out of a ClothMeshData (cloth_compute.rs:
+// 1599-1700).  Rows keep the caller's order when batch offsets come with them
+// (each batch checked vertex-disjoint); without, they are coloured and sorted
+// as finalize_cloth_mesh does.  A given adjacency (pack_neighbor_gpu's vec4
+// entries, cloth_jacobi.rs:95-111) is checked row by row and kept in its
+// order; without one, the cursor scatter of mesh_prep.rs:588-635 builds it.
+Mesh* mesh_from_arrays(const xpbd_mesh_desc& d, std::string& why) {
+    const size_t n = d.num_particles, e = d.num_distance_constraints, t = d.num_tris;
+    auto bad = [&](const std::string& w) -> Mesh* {
+        why = w;
+        return nullptr;
+    };
+    auto at = [](size_t k) { return std::to_string(k); };
+    if (n == 0) return bad("num_particles is 0");
+    if (n > (size_t(1) << 29)) return bad("num_particles above 2^29");
+    if (e > (size_t(1) << 30)) return bad("num_distance_constraints above 2^30");
+    if (t > (size_t(1) << 29)) return bad("num_tris above 2^29");
+    if (!d.initial_pos || !d.inv_mass) return bad("initial_pos and inv_mass are required");
+    if (e && (!d.constraint_i || !d.constraint_j || !d.constraint_rest_len || !d.constraint_compliance))
+        return bad("constraint_i/j/rest_len/compliance are required when num_distance_constraints > 0");
+    if (t && !d.tri_indices) return bad("tri_indices is required when num_tris > 0");
+    if (!d.neighbor_offsets != !d.neighbor_packed) return bad("neighbor_offsets and neighbor_packed come together");
+    if (d.neighbor_offsets && !d.constraint_batch_offsets)
+        return bad("a given adjacency names constraint rows: it needs the constraint_batch_offsets of those rows");
+
+    auto m = std::make_unique<Mesh>();
+    m->positions.resize(3 * n);
+    m->rest_positions.resize(3 * n);
+    m->normals.resize(3 * n);
+    m->inv_mass.resize(n);
+    const float* rest = d.rest_pos ? d.rest_pos : d.initial_pos;
+    for (size_t i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            const float p = d.initial_pos[4 * i + a], r = rest[4 * i + a];
+            if (!std::isfinite(p) || !std::isfinite(r)) return bad("position of particle " + at(i) + " is not finite");
+            m->positions[3 * i + a] = p;
+            m->rest_positions[3 * i + a] = r;
+            m->normals[3 * i + a] = d.initial_normals ? d.initial_normals[4 * i + a] : (a == 1 ? 1.f : 0.f);
+        }
+        const float w = d.inv_mass[i];
+        if (!std::isfinite(w) || w < 0.f) return bad("inv_mass of particle " + at(i) + " is negative or not finite");
+        m->inv_mass[i] = w;
+    }
+    if (t) {
+        m->indices.assign(d.tri_indices, d.tri_indices + 3 * t);
+        for (size_t k = 0; k < 3 * t; ++k)
+            if (m->indices[k] >= n) return bad("tri_indices[" + at(k) + "] out of range");
+    }
+    m->ground.assign(n, 0);
+    if (d.ground)
+        for (size_t i = 0; i < n; ++i) m->ground[i] = d.ground[i] ? 1u : 0u;
+    m->cloth_offsets = {0, uint32_t(n)};
+    std::vector<uint32_t> cloth_of;
+    if (d.cloth_offsets && d.num_cloths > 1) {
+        const uint32_t* co = d.cloth_offsets;
+        if (co[0] != 0 || co[d.num_cloths] != n) return bad("cloth_offsets must start at 0 and end at N");
+        cloth_of.resize(n);
+        for (uint32_t c = 0; c < d.num_cloths; ++c) {
+            if (co[c] >= co[c + 1]) return bad("cloth " + at(c) + " is empty or cloth_offsets decrease");
+            for (uint32_t i = co[c]; i < co[c + 1]; ++i) cloth_of[i] = c;
+        }
+        m->cloth_offsets.assign(co, co + d.num_cloths + 1);
+    }
+
+    std::vector<Constraint> cons(e);
+    for (size_t k = 0; k < e; ++k) {
+        const Constraint c{d.constraint_i[k], d.constraint_j[k], d.constraint_rest_len[k], d.constraint_compliance[k]};
+        if (c.i >= n || c.j >= n) return bad("constraint row " + at(k) + " has an endpoint out of range");
+        if (c.i == c.j) return bad("constraint row " + at(k) + " joins a particle to itself");
+        if (!cloth_of.empty() && cloth_of[c.i] != cloth_of[c.j])
+            return bad("constraint row " + at(k) + " joins two cloths");
+        if (!std::isfinite(c.rest) || c.rest < 0.f) return bad("rest length of row " + at(k) + " is negative or not finite");
+        if (!std::isfinite(c.comp) || c.comp < 0.f) return bad("compliance of row " + at(k) + " is negative or not finite");
+        cons[k] = c;
+    }
+    if (!d.constraint_batch_offsets) {
+        assemble_constraints(*m, n, std::move(cons));
+        return m.release();
+    }
+
+    // Rows in the caller's order; every batch vertex-disjoint.
+    const size_t nb = d.constraint_batch_count;
+    const uint32_t* bo = d.constraint_batch_offsets;
+    if (e == 0 && nb == 0) {
+        m->batch_offsets = {0, 0};   // partition_constraints_for_gs_batches on no rows
+    } else {
+        if (nb == 0) return bad("constraint_batch_count is 0 with rows present");
+        if (bo[0] != 0 || bo[nb] != e) return bad("constraint_batch_offsets must start at 0 and end at E");
+        std::vector<uint32_t> stamp(n, 0);
+        for (size_t b = 0; b < nb; ++b) {
+            if (bo[b] > bo[b + 1]) return bad("constraint_batch_offsets decrease at batch " + at(b));
+            for (uint32_t k = bo[b]; k < bo[b + 1]; ++k) {
+                const uint32_t i = cons[k].i, j = cons[k].j, s = uint32_t(b + 1);
+                if (stamp[i] == s || stamp[j] == s)
+                    return bad("batch " + at(b) + " is not vertex-disjoint (row " + at(k) + ")");
+                stamp[i] = stamp[j] = s;
+            }
+        }
+        m->batch_offsets.assign(bo, bo + nb + 1);
+    }
+    m->constraint_i.resize(e);
+    m->constraint_j.resize(e);
+    m->constraint_rest.resize(e);
+    m->constraint_comp.resize(e);
+    for (size_t k = 0; k < e; ++k) {
+        m->constraint_i[k] = cons[k].i;
+        m->constraint_j[k] = cons[k].j;
+        m->constraint_rest[k] = cons[k].rest;
+        m->constraint_comp[k] = cons[k].comp;
+    }
+
+    if (!d.neighbor_offsets) {   // cursor scatter over the rows (mesh_prep.rs:588-635)
+        scatter_csr(*m, n);
+        return m.release();
+    }
+    // The caller's adjacency: every row exactly once at each endpoint.
+    const uint32_t* no = d.neighbor_offsets;
+    if (no[0] != 0 || no[n] != 2 * e) return bad("neighbor_offsets must start at 0 and end at 2E");
+    for (size_t i = 0; i < n; ++i)
+        if (no[i] > no[i + 1]) return bad("neighbor_offsets decrease at particle " + at(i));
+    if (e >= (size_t(1) << 24) || n >= (size_t(1) << 24))
+        return bad("neighbor_packed holds indices as f32: N and E must stay below 2^24");
+    m->neighbor_offsets.assign(no, no + n + 1);
+    m->neighbor_other.resize(2 * e);
+    m->neighbor_rest_len.resize(2 * e);
+    m->neighbor_compliance.resize(2 * e);
+    m->neighbor_constraint_id.resize(2 * e);
+    std::vector<uint8_t> seen(e, 0);   // bit 0: seen from endpoint i, bit 1: from j
+    auto as_index = [](float v, size_t lim, uint32_t& out) {
+        if (!(v >= 0.f) || v != std::floor(v) || v >= float(lim)) return false;
+        out = uint32_t(v);
+        return true;
+    };
+    for (size_t i = 0; i < n; ++i)
+        for (uint32_t k = no[i]; k < no[i + 1]; ++k) {
+            const float* q = d.neighbor_packed + 4 * size_t(k);
+            uint32_t other = 0, row = 0;
+            if (!as_index(q[0], n, other) || !as_index(q[3], e, row))
+                return bad("neighbor_packed entry " + at(k) + " has a bad particle or row index");
+            const Constraint& c = cons[row];
+            uint8_t side;
+            if (c.i == i && c.j == other) side = 1;
+            else if (c.j == i && c.i == other) side = 2;
+            else return bad("neighbor_packed entry " + at(k) + " names row " + at(row) + ", which does not join " +
+                            at(i) + " and " + at(other));
+            if (seen[row] & side) return bad("row " + at(row) + " appears twice in the slice of particle " + at(i));
+            seen[row] |= side;
+            if (std::memcmp(&q[1], &c.rest, 4) != 0 || std::memcmp(&q[2], &c.comp, 4) != 0)
+                return bad("neighbor_packed entry " + at(k) + " differs from row " + at(row) + " in rest or compliance");
+            m->neighbor_other[k] = other;
+            m->neighbor_rest_len[k] = q[1];
+            m->neighbor_compliance[k] = q[2];
+            m->neighbor_constraint_id[k] = row;
+        }
+    return m.release();   // 2E entries, each row seen at most once per side: all seen
 }
 
 // derive_collision_grid (mesh_prep.rs:16-85).

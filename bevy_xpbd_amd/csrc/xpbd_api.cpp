@@ -1125,6 +1125,31 @@ XPBD_API int xpbd_mesh_parse_welded_obj(const char* text, size_t len, xpbd_mesh*
 
 XPBD_API int xpbd_mesh_sheet(const xpbd_sheet_desc* d, xpbd_mesh** out) { return xpbd_mesh_batch(d, 1, out); }
 
+// Reason for the last failed xpbd_mesh_from_arrays on this thread.
+static thread_local std::string g_mesh_error;
+
+XPBD_API const char* xpbd_mesh_last_error(void) { return g_mesh_error.c_str(); }
+
+XPBD_API int xpbd_mesh_from_arrays(const xpbd_mesh_desc* d, xpbd_mesh** out) {
+    g_mesh_error.clear();
+    if (!d || !out) {
+        g_mesh_error = "null argument";
+        return XPBD_E_INVALID;
+    }
+    try {
+        std::string why;
+        Mesh* m = mesh_from_arrays(*d, why);
+        if (!m) {
+            g_mesh_error = why;
+            return XPBD_E_INVALID;
+        }
+        return wrap_mesh(m, out);
+    } catch (const std::bad_alloc&) {
+        g_mesh_error = "out of host memory";
+        return XPBD_E_NOMEM;
+    }
+}
+
 XPBD_API int xpbd_mesh_batch(const xpbd_sheet_desc* d, uint32_t count, xpbd_mesh** out) {
     if (!out || !d || count == 0) return XPBD_E_INVALID;
     uint64_t total = 0;
@@ -1954,6 +1979,14 @@ XPBD_API int xpbd_get_layout(xpbd_cloth* h, uint32_t* out4) {
     out4[1] = h->tile_cols;
     out4[2] = h->tile_rows;
     out4[3] = fast_division_active(h) ? 1u : 0u;
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_debug_raise_device_error(xpbd_cloth* h) {
+    if (!h) return XPBD_E_INVALID;
+    const uint32_t one = 1;
+    HIPC(h, hipMemcpyAsync(h->err, &one, sizeof(one), hipMemcpyHostToDevice, h->stream));
+    HIPC(h, hipStreamSynchronize(h->stream));
     return XPBD_OK;
 }
 

@@ -103,6 +103,62 @@ XPBD_API int xpbd_mesh_sheet(const xpbd_sheet_desc* desc, xpbd_mesh** out);
 /* Many independent cloths in one mesh (block-diagonal constraint graph). */
 XPBD_API int xpbd_mesh_batch(const xpbd_sheet_desc* descs, uint32_t count, xpbd_mesh** out);
 
+/* A caller's own topology: the arrays of ClothSimConfig (cloth_compute.rs:
+ * 142-179) as to_sim_config copies them out of a ClothMeshData
+ * (cloth_compute.rs:1599-1700; mesh_prep.rs:103-131).  Host arrays; the
+ * library copies them and validates every one.
+ *
+ * Constraint rows: with constraint_batch_offsets, the rows are taken in the
+ * order given, and each batch must be vertex-disjoint (the coloured
+ * Gauss-Seidel sweep relies on it; the reference's colouring guarantees it,
+ * mesh_prep.rs:153-221).  With constraint_batch_offsets NULL, the rows are
+ * coloured and colour-sorted here exactly as finalize_cloth_mesh does
+ * (greedy colouring, mesh_prep.rs:153-221), so any foreign constraint list
+ * can be given.
+ *
+ * Adjacency: neighbor_offsets + neighbor_packed are the Jacobi build's
+ * fields (pack_neighbor_gpu, cloth_jacobi.rs:95-111: vec4 per directed entry
+ * = (other particle, rest length, compliance, constraint row), indices as
+ * exact f32 integers).  Given, they are checked against the rows (every row
+ * once at each endpoint, equal rest and compliance) and kept in their order,
+ * which is the Jacobi gather's summation order (xpbd_cpu.rs:187-226); they
+ * then need constraint_batch_offsets.  NULL, they are built by cursor
+ * scatter over the rows (mesh_prep.rs:588-635).
+ *
+ * A mesh from arrays carries no grid columns: the grid-tile Jacobi layout
+ * and the learned warm start (both grid-only) are not available for it.
+ * Errors: XPBD_E_INVALID, with the reason in xpbd_mesh_last_error(). */
+typedef struct xpbd_mesh_desc {
+    uint32_t num_particles;                   /* N, 1 .. 2^29                          */
+    uint32_t num_tris;                        /* T                                     */
+    uint32_t num_distance_constraints;        /* E                                     */
+    uint32_t constraint_batch_count;          /* B (with constraint_batch_offsets)     */
+    const float* initial_pos;                 /* vec4 x N (w ignored), finite; required */
+    const float* rest_pos;                    /* vec4 x N, finite; NULL = initial_pos  */
+    const float* initial_normals;             /* vec4 x N; NULL = +Y                   */
+    const float* inv_mass;                    /* f32 x N, finite, >= 0 (0 = pinned); required */
+    const uint32_t* constraint_i;             /* u32 x E, < N                          */
+    const uint32_t* constraint_j;             /* u32 x E, < N, != constraint_i         */
+    const float* constraint_rest_len;         /* f32 x E, finite, >= 0                 */
+    const float* constraint_compliance;       /* f32 x E, finite, >= 0                 */
+    const uint32_t* constraint_batch_offsets; /* u32 x (B+1) or NULL                   */
+    const uint32_t* tri_indices;              /* u32 x 3T, < N (render normals)        */
+    const uint32_t* neighbor_offsets;         /* u32 x (N+1) or NULL                   */
+    const float* neighbor_packed;             /* vec4 x 2E or NULL                     */
+    const uint32_t* ground;                   /* u32 x N or NULL: nonzero = ground
+                                                 plane y = 0 for that particle (no
+                                                 reference counterpart)               */
+    uint32_t num_cloths;                      /* C; 0 or 1 = one cloth                 */
+    const uint32_t* cloth_offsets;            /* u32 x (C+1) first particle of each
+                                                 cloth (no reference counterpart: many
+                                                 cloths in one handle collide only
+                                                 within themselves); rows may not join
+                                                 two cloths; NULL = one cloth          */
+} xpbd_mesh_desc;
+XPBD_API int xpbd_mesh_from_arrays(const xpbd_mesh_desc* desc, xpbd_mesh** out);
+/* Why the last xpbd_mesh_* builder call on this thread failed ("" if none). */
+XPBD_API const char* xpbd_mesh_last_error(void);
+
 XPBD_API uint64_t xpbd_mesh_count(const xpbd_mesh* mesh, int field);
 XPBD_API int xpbd_mesh_read(const xpbd_mesh* mesh, int field, void* dst, uint64_t dst_bytes);
 XPBD_API void xpbd_mesh_destroy(xpbd_mesh* mesh);
@@ -268,6 +324,12 @@ XPBD_API int xpbd_profile_frames(xpbd_cloth* h, uint32_t frames, int which, floa
  * inv_cell (floats), dims[3], key base (u32). */
 XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t* grid_out,
                                        uint32_t* sorted_keys, uint32_t* sorted_ids);
+
+/* Test hook (no reference counterpart): sets the handle's device error word,
+ * as an expired grid-barrier wait does.  Every later barrier wait of a
+ * persistent launch (coloured Gauss-Seidel, persistent Jacobi) then returns at
+ * once, and the next xpbd_sync reports XPBD_E_STATE and clears the word. */
+XPBD_API int xpbd_debug_raise_device_error(xpbd_cloth* h);
 
 /* Layout the handle chose at create (no reference counterpart; for tests and
  * tools): out4 = {adjacency: 0 plain 12-byte entries, 1 packed 4-byte entries,
