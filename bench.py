@@ -51,6 +51,71 @@ def algorithmic_bytes_per_substep(n: int, e: int, iters: int, collision: bool, t
     return b
 
 
+class TraceRange:
+    """A named ROCTX range around a region (rocprofv3 --marker-trace records
+    it; outside a profiler the calls are no-ops).  bench.py brackets its
+    timed frames with one, so --trace-summary can keep exactly their kernels."""
+
+    def __init__(self, name: str):
+        self.name = name.encode()
+        self.lib = None
+        try:
+            import ctypes
+            self.lib = ctypes.CDLL("librocprofiler-sdk-roctx.so")
+            self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+        except OSError:
+            self.lib = None
+
+    def __enter__(self):
+        if self.lib is not None:
+            self.lib.roctxRangePushA(self.name)
+        return self
+
+    def __exit__(self, *exc):
+        if self.lib is not None:
+            self.lib.roctxRangePop()
+
+
+def trace_summary(trace_dir: str, range_name: str = "timed frames", frames: int | None = None) -> str:
+    """Per-kernel durations from a rocprofv3 CSV trace (--kernel-trace
+    --marker-trace -f csv), restricted to the dispatches that ran inside the
+    ROCTX range `range_name`: a markdown table."""
+    import csv
+    import glob
+    kt = glob.glob(os.path.join(trace_dir, "**", "*kernel_trace.csv"), recursive=True)
+    mt = glob.glob(os.path.join(trace_dir, "**", "*marker_api_trace.csv"), recursive=True)
+    if not kt or not mt:
+        raise SystemExit(f"bench.py: no kernel/marker trace CSV under {trace_dir}")
+    lo = hi = None
+    with open(mt[0]) as fh:
+        for row in csv.DictReader(fh):
+            text = " ".join(str(v) for v in row.values())
+            if range_name in text:
+                lo, hi = int(row["Start_Timestamp"]), int(row["End_Timestamp"])
+    if lo is None:
+        raise SystemExit(f"bench.py: range {range_name!r} not in {mt[0]}")
+    stats = {}
+    with open(kt[0]) as fh:
+        for row in csv.DictReader(fh):
+            a, b = int(row["Start_Timestamp"]), int(row["End_Timestamp"])
+            if a >= lo and b <= hi:
+                k = row["Kernel_Name"]
+                c, t = stats.get(k, (0, 0))
+                stats[k] = (c + 1, t + (b - a))
+    total = sum(t for _, t in stats.values())
+    lines = [f"Window: ROCTX range '{range_name}', {(hi - lo) / 1e3:.1f} us wall"
+             + (f", {frames} frames" if frames else "") + f"; kernel time {total / 1e3:.1f} us.", "",
+             "| kernel | calls | total us | average us | % |" + (" per frame us |" if frames else ""),
+             "|---|---:|---:|---:|---:|" + ("---:|" if frames else "")]
+    for k, (c, t) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        name = k if len(k) <= 90 else k[:90]
+        row = f"| `{name}` | {c} | {t / 1e3:.1f} | {t / c / 1e3:.2f} | {100.0 * t / total:.2f} |"
+        if frames:
+            row += f" {t / frames / 1e3:.1f} |"
+        lines.append(row)
+    return "\n".join(lines)
+
+
 def cpu_baseline(mesh_fields, u, cfg, budget_s: float, threads: int):
     """Oracle (C++ restatement of the reference CPU solver) on this host, a
     bounded number of substeps of the same workload."""
@@ -72,6 +137,17 @@ def cpu_baseline(mesh_fields, u, cfg, budget_s: float, threads: int):
         if el >= budget_s or done >= 64:
             break
     return done / el, done, el
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def learned_warm_start():
@@ -148,7 +224,13 @@ def main():
     ap.add_argument("--no-warm", action="store_true", help="skip the learned warm start's train + evaluation")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-kernel timing launches (for rocprofv3 counter passes)")
+    ap.add_argument("--trace-summary", metavar="DIR",
+                    help="print the per-kernel table of the timed frames from a rocprofv3 CSV trace "
+                         "(--kernel-trace --marker-trace -f csv) of an earlier bench.py run, and exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary, frames=args.steps))
+        return
     if args.gpus != 1:
         print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is design-only)", file=sys.stderr)
@@ -179,10 +261,11 @@ def main():
     sim.step_frames(args.warmup)
     sim.sync()
     log(f"bench: timing {args.steps} frames")
-    t0 = time.perf_counter()
-    sim.step_frames(args.steps)
-    sim.sync()
-    el = time.perf_counter() - t0
+    with TraceRange("timed frames"):
+        t0 = time.perf_counter()
+        sim.step_frames(args.steps)
+        sim.sync()
+        el = time.perf_counter() - t0
     substeps = args.steps * S
     value = substeps / el
     ms_per_step = 1000.0 * el / args.steps
@@ -229,12 +312,18 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        log(f"bench: CPU baseline (oracle, {threads} threads, ~{args.cpu_budget:.0f}s)")
-        rate, done, cel = cpu_baseline(mesh.fields(), u, cfg, args.cpu_budget, threads)
+        host_cpus = os.cpu_count() or 1
+        threads = args.cpu_threads or min(host_cpus, 16)
+        fields = mesh.fields()
+        log(f"bench: CPU baseline (oracle, {threads} threads, ~{args.cpu_budget:.0f}s; then 1 thread)")
+        rate, done, cel = cpu_baseline(fields, u, cfg, args.cpu_budget, threads)
+        rate1, done1, cel1 = cpu_baseline(fields, u, cfg, args.cpu_budget, 1)
         cpu = {"value": round(rate, 4), "unit": "substeps/s", "cores": threads, "kind": "port",
                "sample": f"{done} substeps of the same workload ({args.workload}: 8x10 Jacobi + hash-grid "
-                         f"self-collision) from the initial state, {cel:.1f}s"}
+                         f"self-collision) from the initial state, {cel:.1f}s",
+               "host_cpus": host_cpus, "host_cpu_model": cpu_model(),
+               "single_thread": {"value": round(rate1, 4), "unit": "substeps/s", "cores": 1,
+                                 "sample": f"{done1} substeps, same start, {cel1:.1f}s"}}
 
     config = {"workload": workload, "particles": n, "constraints": e, "substeps_per_step": S, "iterations": I,
               "collision": "every substep, contact distance 0.8 x spacing",
