@@ -157,7 +157,7 @@ def lib():
             "xpbd_mesh_sheet": (i32, [C.POINTER(SheetDesc), pp]),
             "xpbd_mesh_batch": (i32, [C.POINTER(SheetDesc), u32, pp]),
             "xpbd_mesh_from_arrays": (i32, [C.POINTER(MeshDesc), pp]),
-            "xpbd_mesh_last_error": (C.c_char_p, []),
+            "xpbd_thread_last_error": (C.c_char_p, []),
             "xpbd_mesh_count": (u64, [vp, i32]),
             "xpbd_mesh_read": (i32, [vp, i32, vp, u64]),
             "xpbd_mesh_destroy": (None, [vp]),
@@ -180,6 +180,7 @@ def lib():
             "xpbd_bind_render_buffers": (i32, [vp, vp, vp]),
             "xpbd_import_render_buffers_fd": (i32, [vp, i32, u64, u64, i32, u64, u64]),
             "xpbd_residual": (i32, [vp, C.POINTER(C.c_double)]),
+            "xpbd_residual_noalloc": (i32, [vp, u32, vp, vp, vp, u32, vp, vp]),
             "xpbd_time_kernel": (i32, [vp, i32, u32, C.POINTER(C.c_float)]),
             "xpbd_profile_frames": (i32, [vp, u32, i32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
             "xpbd_debug_collision_keys": (i32, [vp, f32, vp, vp, vp]),
@@ -306,7 +307,7 @@ class ClothMeshData:
         h = C.c_void_p()
         code = lib().xpbd_mesh_from_arrays(C.byref(d), C.byref(h))
         if code != XPBD_OK:
-            raise XpbdError(code, "xpbd_mesh_from_arrays", lib().xpbd_mesh_last_error().decode())
+            raise XpbdError(code, "xpbd_mesh_from_arrays", lib().xpbd_thread_last_error().decode())
         return cls(h)
 
     def sim_config_arrays(self) -> dict:

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "kernels.hpp"
 
@@ -1695,6 +1696,59 @@ __global__ __launch_bounds__(256) void k_residual(const float4* X, const uint2* 
 }
 
 // ---------------------------------------------------------------------------
+// Allocation-free relative residual: the sums of k_residual plus the sum of
+// rest^2, over caller-owned device arrays, with the per-block partials in a
+// fixed file-scope device array instead of handle memory.  Each block owns
+// two slots (C^2 at [b], rest^2 at [blocks + b]) and rewrites them on every
+// call, so nothing has to start at zero; the finishing block sums them in
+// block order.  A constraint index >= n is not loaded: its block's C^2
+// partial becomes NaN, and so does the result.
+// ---------------------------------------------------------------------------
+constexpr uint32_t NOALLOC_RESIDUAL_BLOCKS = 2048;
+static __device__ __align__(256) double scratch[2 * NOALLOC_RESIDUAL_BLOCKS];
+
+__global__ __launch_bounds__(256) void k_residual_parts(const float4* X, uint32_t n, const uint32_t* ci,
+                                                        const uint32_t* cj, const float* rest, uint32_t e,
+                                                        double* part) {
+    double s = 0.0, r = 0.0;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < e; k += gridDim.x * blockDim.x) {
+        const uint32_t i = ci[k], j = cj[k];
+        const double rk = double(rest[k]);
+        r += rk * rk;
+        if (i >= n || j >= n) {
+            s = __builtin_nan("");
+            continue;
+        }
+        const float4 a = X[i], b = X[j];
+        const double dx = double(a.x) - double(b.x), dy = double(a.y) - double(b.y), dz = double(a.z) - double(b.z);
+        const double c = sqrt((dx * dx + dy * dy) + dz * dz) - rk;
+        s += c * c;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off, 64);
+        r += __shfl_xor(r, off, 64);
+    }
+    __shared__ double red[8];
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = s;
+        red[4 + (threadIdx.x >> 6)] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+        part[gridDim.x + blockIdx.x] = (red[4] + red[5]) + (red[6] + red[7]);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_residual_finish(const double* part, uint32_t blocks, double* out) {
+    if (threadIdx.x != 0) return;
+    double c2 = 0.0, r2 = 0.0;
+    for (uint32_t b = 0; b < blocks; ++b) c2 += part[b];
+    for (uint32_t b = 0; b < blocks; ++b) r2 += part[blocks + b];
+    out[0] = c2 != c2 ? c2 : (r2 > 0.0 ? sqrt(c2) / sqrt(r2) : 0.0);
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
@@ -1999,6 +2053,38 @@ void launch_publish(const float4* X, uint32_t n, const uint32_t* tris, const uin
                     float4* dpos, float4* dnrm, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_publish, dim3(cdiv(n, 256)), dim3(256), 0, s, X, n, tris, vt_off, vt_tri, dpos, dnrm);
+}
+
+uint32_t residual_noalloc_max_blocks() { return NOALLOC_RESIDUAL_BLOCKS; }
+
+namespace {
+// Device address of `scratch` on each device, looked up on the first call
+// made with that device current (each device holds its own copy).
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[64] = {};
+}  // namespace
+
+hipError_t launch_residual_noalloc(const float4* X, uint32_t n, const uint32_t* ci, const uint32_t* cj,
+                                   const float* rest, uint32_t e, hipStream_t s, double* out) {
+    int dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    double* part = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void* p = nullptr;
+            err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (err != hipSuccess) return err;
+            g_scratch_ptr[dev] = static_cast<double*>(p);
+        }
+        part = g_scratch_ptr[dev];
+    }
+    const uint32_t blocks = e ? std::min(cdiv(e, 256 * 4), NOALLOC_RESIDUAL_BLOCKS) : 0u;
+    if (blocks) hipLaunchKernelGGL(k_residual_parts, dim3(blocks), dim3(256), 0, s, X, n, ci, cj, rest, e, part);
+    hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(64), 0, s, part, blocks, out);
+    return hipGetLastError();
 }
 
 uint32_t residual_max_blocks() { return 2048u; }

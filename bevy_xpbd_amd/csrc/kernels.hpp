@@ -221,6 +221,13 @@ void launch_publish(const float4* X, uint32_t n, const uint32_t* tris, const uin
 // sum of C^2 over the constraints (ij, rest) in double: one partial per
 // block into part[0, blocks), blocks <= residual_max_blocks(); returns blocks
 uint32_t residual_max_blocks();
+// The same relative residual over caller-owned device arrays (separate
+// endpoint arrays, as ClothSimConfig holds them), allocating nothing: block
+// partials live in a fixed device array of 2 x residual_noalloc_max_blocks()
+// doubles; out = device double.  Calls that overlap on one device race.
+uint32_t residual_noalloc_max_blocks();
+hipError_t launch_residual_noalloc(const float4* X, uint32_t n, const uint32_t* ci, const uint32_t* cj,
+                                   const float* rest, uint32_t e, hipStream_t s, double* out);
 uint32_t launch_residual(const float4* X, const uint2* ij, const float* rest, uint32_t e, double* part,
                          hipStream_t s);
 uint32_t launch_aabb_of(const float4* P, uint32_t n, float* part, hipStream_t s);

@@ -1087,6 +1087,11 @@ int autotune(xpbd_cloth* h) {
 
 }  // namespace
 
+// Reason for the last failed call without a handle on this thread: a
+// static string, or the text kept in g_thread_text.
+static thread_local const char* g_thread_error = "";
+static thread_local std::string g_thread_text;
+
 extern "C" {
 
 XPBD_API int xpbd_abi_version(void) { return XPBD_ABI_VERSION; }
@@ -1125,27 +1130,26 @@ XPBD_API int xpbd_mesh_parse_welded_obj(const char* text, size_t len, xpbd_mesh*
 
 XPBD_API int xpbd_mesh_sheet(const xpbd_sheet_desc* d, xpbd_mesh** out) { return xpbd_mesh_batch(d, 1, out); }
 
-// Reason for the last failed xpbd_mesh_from_arrays on this thread.
-static thread_local std::string g_mesh_error;
 
-XPBD_API const char* xpbd_mesh_last_error(void) { return g_mesh_error.c_str(); }
+XPBD_API const char* xpbd_thread_last_error(void) { return g_thread_error; }
 
 XPBD_API int xpbd_mesh_from_arrays(const xpbd_mesh_desc* d, xpbd_mesh** out) {
-    g_mesh_error.clear();
+    g_thread_error = "";
     if (!d || !out) {
-        g_mesh_error = "null argument";
+        g_thread_error = "null argument";
         return XPBD_E_INVALID;
     }
     try {
         std::string why;
         Mesh* m = mesh_from_arrays(*d, why);
         if (!m) {
-            g_mesh_error = why;
+            g_thread_text = why;
+            g_thread_error = g_thread_text.c_str();
             return XPBD_E_INVALID;
         }
         return wrap_mesh(m, out);
     } catch (const std::bad_alloc&) {
-        g_mesh_error = "out of host memory";
+        g_thread_error = "out of host memory";
         return XPBD_E_NOMEM;
     }
 }
@@ -1471,6 +1475,27 @@ XPBD_API int xpbd_residual(xpbd_cloth* h, double* out) {
     double c2 = 0.0;
     for (double v : part) c2 += v;
     *out = h->rest_sq_sum > 0.0 ? std::sqrt(c2) / std::sqrt(h->rest_sq_sum) : 0.0;
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_residual_noalloc(const void* d_pos, uint32_t n, const void* d_ci, const void* d_cj,
+                                   const void* d_rest, uint32_t e, void* stream, void* d_out) {
+    g_thread_error = "";
+    if (!d_out || (e && (!d_pos || !d_ci || !d_cj || !d_rest))) {
+        g_thread_error = "null device pointer";
+        return XPBD_E_INVALID;
+    }
+    if (e > XPBD_RESIDUAL_NOALLOC_MAX_CONSTRAINTS) {
+        g_thread_error = "num_constraints above XPBD_RESIDUAL_NOALLOC_MAX_CONSTRAINTS (2^31)";
+        return XPBD_E_INVALID;
+    }
+    const hipError_t err = launch_residual_noalloc(static_cast<const float4*>(d_pos), n, static_cast<const uint32_t*>(d_ci),
+                                                   static_cast<const uint32_t*>(d_cj), static_cast<const float*>(d_rest),
+                                                   e, static_cast<hipStream_t>(stream), static_cast<double*>(d_out));
+    if (err != hipSuccess) {
+        g_thread_error = hipGetErrorString(err);
+        return XPBD_E_HIP;
+    }
     return XPBD_OK;
 }
 

@@ -127,7 +127,7 @@ XPBD_API int xpbd_mesh_batch(const xpbd_sheet_desc* descs, uint32_t count, xpbd_
  *
  * A mesh from arrays carries no grid columns: the grid-tile Jacobi layout
  * and the learned warm start (both grid-only) are not available for it.
- * Errors: XPBD_E_INVALID, with the reason in xpbd_mesh_last_error(). */
+ * Errors: XPBD_E_INVALID, with the reason in xpbd_thread_last_error(). */
 typedef struct xpbd_mesh_desc {
     uint32_t num_particles;                   /* N, 1 .. 2^29                          */
     uint32_t num_tris;                        /* T                                     */
@@ -156,8 +156,9 @@ typedef struct xpbd_mesh_desc {
                                                  two cloths; NULL = one cloth          */
 } xpbd_mesh_desc;
 XPBD_API int xpbd_mesh_from_arrays(const xpbd_mesh_desc* desc, xpbd_mesh** out);
-/* Why the last xpbd_mesh_* builder call on this thread failed ("" if none). */
-XPBD_API const char* xpbd_mesh_last_error(void);
+/* Why the last failed call without a handle on this thread
+ * (xpbd_mesh_from_arrays, xpbd_residual_noalloc) failed; "" if it succeeded. */
+XPBD_API const char* xpbd_thread_last_error(void);
 
 XPBD_API uint64_t xpbd_mesh_count(const xpbd_mesh* mesh, int field);
 XPBD_API int xpbd_mesh_read(const xpbd_mesh* mesh, int field, void* dst, uint64_t dst_bytes);
@@ -300,6 +301,25 @@ XPBD_API int xpbd_import_render_buffers_fd(xpbd_cloth* h, int pos_fd, uint64_t p
  * state (no reference counterpart; the metric the learned warm start is
  * measured and gated by).  Synchronous. */
 XPBD_API int xpbd_residual(xpbd_cloth* h, double* out);
+
+/* No counterpart in the reference.  xpbd_residual's relative residual,
+ * sqrt(sum C^2) / sqrt(sum rest^2) in f64, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc): it takes caller-owned device arrays -- positions
+ * vec4 x num_particles, constraint_i / constraint_j u32 and rest lengths f32
+ * x num_constraints, as ClothSimConfig holds them (cloth_compute.rs:142-179)
+ * -- and writes the result to d_out (one device double), enqueued on
+ * hip_stream (NULL: the default stream) of the current device, without
+ * synchronising.  It allocates nothing: its block partials live in a fixed
+ * 32 KiB device array that each device holds for the life of the process, so
+ * calls that overlap on one device (other streams, other threads) race and
+ * must be ordered by the caller.  num_constraints above
+ * XPBD_RESIDUAL_NOALLOC_MAX_CONSTRAINTS is refused with XPBD_E_INVALID; an
+ * endpoint index >= num_particles is not read and makes the result NaN. */
+#define XPBD_RESIDUAL_NOALLOC_MAX_CONSTRAINTS 0x80000000u
+XPBD_API int xpbd_residual_noalloc(const void* d_pos_vec4, uint32_t num_particles, const void* d_constraint_i,
+                                   const void* d_constraint_j, const void* d_constraint_rest_len,
+                                   uint32_t num_constraints, void* hip_stream, void* d_out);
 
 /* Measurement: average duration [ms] of one launch of kernel `which`
  * (XPBD_KERNEL_*) on the handle's stream, over `reps` launches bracketed by

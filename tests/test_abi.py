@@ -41,6 +41,16 @@ def test_invalid_arguments_are_rejected(X):
         X.ClothMeshData.grid_cloth_hanging(0, 0, 0.1)
 
 
+def test_residual_noalloc_refuses_oversize_before_any_device_call(X):
+    """Argument checks come first: no HIP call is made for these."""
+    L = X.lib()
+    dummy = C.c_void_p(16)
+    assert L.xpbd_residual_noalloc(dummy, 4, dummy, dummy, dummy, 0x80000001, None, dummy) == 1
+    assert b"XPBD_RESIDUAL_NOALLOC_MAX_CONSTRAINTS" in L.xpbd_thread_last_error()
+    assert L.xpbd_residual_noalloc(dummy, 4, None, dummy, dummy, 3, None, dummy) == 1
+    assert L.xpbd_residual_noalloc(dummy, 4, dummy, dummy, dummy, 3, None, None) == 1
+
+
 def test_mesh_read_rejects_short_buffer(X):
     m = X.ClothMeshData.grid_cloth_hanging(3, 3, 0.1)
     buf = np.zeros(2, np.float32)
