@@ -76,41 +76,50 @@ class TraceRange:
             self.lib.roctxRangePop()
 
 
-def trace_summary(trace_dir: str, range_name: str = "timed frames", frames: int | None = None) -> str:
-    """Per-kernel durations from a rocprofv3 CSV trace (--kernel-trace
-    --marker-trace -f csv), restricted to the dispatches that ran inside the
-    ROCTX range `range_name`: a markdown table."""
+def trace_summary(trace_dir: str, range_name: str = "timed frames", frames: int | None = None,
+                  by_grid: bool = False) -> str:
+    """Per-kernel durations from a rocprofv3 CSV trace (--kernel-trace -f csv,
+    with --marker-trace for a bench.py run), restricted to the dispatches that
+    ran inside the ROCTX range `range_name` when the trace holds it (else all
+    dispatches); with by_grid, one row per (kernel, grid size): a markdown
+    table."""
     import csv
     import glob
     kt = glob.glob(os.path.join(trace_dir, "**", "*kernel_trace.csv"), recursive=True)
     mt = glob.glob(os.path.join(trace_dir, "**", "*marker_api_trace.csv"), recursive=True)
-    if not kt or not mt:
-        raise SystemExit(f"bench.py: no kernel/marker trace CSV under {trace_dir}")
+    if not kt:
+        raise SystemExit(f"bench.py: no kernel trace CSV under {trace_dir}")
     lo = hi = None
-    with open(mt[0]) as fh:
-        for row in csv.DictReader(fh):
-            text = " ".join(str(v) for v in row.values())
-            if range_name in text:
-                lo, hi = int(row["Start_Timestamp"]), int(row["End_Timestamp"])
-    if lo is None:
-        raise SystemExit(f"bench.py: range {range_name!r} not in {mt[0]}")
+    if mt:
+        with open(mt[0]) as fh:
+            for row in csv.DictReader(fh):
+                text = " ".join(str(v) for v in row.values())
+                if range_name in text:
+                    lo, hi = int(row["Start_Timestamp"]), int(row["End_Timestamp"])
     stats = {}
     with open(kt[0]) as fh:
         for row in csv.DictReader(fh):
             a, b = int(row["Start_Timestamp"]), int(row["End_Timestamp"])
-            if a >= lo and b <= hi:
+            if lo is None or (a >= lo and b <= hi):
                 k = row["Kernel_Name"]
+                if by_grid:
+                    k = (k, int(row.get("Grid_Size") or row.get("Grid_Size_X") or 0))
                 c, t = stats.get(k, (0, 0))
                 stats[k] = (c + 1, t + (b - a))
     total = sum(t for _, t in stats.values())
-    lines = [f"Window: ROCTX range '{range_name}', {(hi - lo) / 1e3:.1f} us wall"
-             + (f", {frames} frames" if frames else "") + f"; kernel time {total / 1e3:.1f} us.", "",
-             "| kernel | calls | total us | average us | % |" + (" per frame us |" if frames else ""),
-             "|---|---:|---:|---:|---:|" + ("---:|" if frames else "")]
+    head = (f"Window: ROCTX range '{range_name}', {(hi - lo) / 1e3:.1f} us wall" if lo is not None
+            else "Window: the whole trace") + (f", {frames} frames" if frames and lo is not None else "")
+    lines = [head + f"; kernel time {total / 1e3:.1f} us.", ""]
+    per_frame = bool(frames) and lo is not None
+    cols = "| kernel |" + (" grid threads |" if by_grid else "") + " calls | total us | average us | % |"
+    lines += [cols + (" per frame us |" if per_frame else ""),
+              "|---|" + ("---:|" if by_grid else "") + "---:|---:|---:|---:|" + ("---:|" if per_frame else "")]
     for k, (c, t) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
-        name = k if len(k) <= 90 else k[:90]
-        row = f"| `{name}` | {c} | {t / 1e3:.1f} | {t / c / 1e3:.2f} | {100.0 * t / total:.2f} |"
-        if frames:
+        name, grid = (k if by_grid else (k, None))
+        name = name if len(name) <= 90 else name[:90]
+        row = f"| `{name}` |" + (f" {grid} |" if by_grid else "")
+        row += f" {c} | {t / 1e3:.1f} | {t / c / 1e3:.2f} | {100.0 * t / total:.2f} |"
+        if per_frame:
             row += f" {t / frames / 1e3:.1f} |"
         lines.append(row)
     return "\n".join(lines)
@@ -190,6 +199,39 @@ def learned_warm_start():
         return {"error": repr(exc)}
 
 
+def config2_rates(X, frames: int = 40, warmup: int = 5):
+    """The 256x256 sheet of the second configuration (horizontal at 1 m,
+    centre pinned, seeded rotation, self-collision every substep, contact
+    distance 0.8 x spacing), 4 substeps x 8 iterations, both solvers (Jacobi
+    omega 0.32, coloured Gauss-Seidel omega 1.0), frames replayed from the
+    hipGraph: substeps/s, the GS substep's launch time and its colours."""
+    import numpy as np
+    h = 1.0 / 255.0
+    angle = float(np.random.default_rng(2).uniform(0, 2 * np.pi))
+    out = {"workload": "256x256 horizontal sheet, centre pin, seeded rotation, self-collision every substep, "
+                       "4 substeps x 8 iterations (step = 1 frame)", "angle_rad": round(angle, 6)}
+    for name, solver in (("jacobi", X.SOLVER_JACOBI), ("gauss_seidel", X.SOLVER_GAUSS_SEIDEL)):
+        mesh = X.ClothMeshData.sheet(cols=256, rows=256, spacing=h, layout=X.LAYOUT_HORIZONTAL_XZ, height=1.0,
+                                     angle=angle, pins=X.PIN_CENTER)
+        cfg = X.ClothSimConfig.default(solve_substeps=4, solve_inner_iterations=8, solver=solver, use_graph=1)
+        u = X.ClothSimUniforms.default(solver, 4, thickness=2.0 * (0.4 * h))
+        sim = X.ClothComputePlugin(mesh, cfg, u)
+        sim.step_frames(warmup)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step_frames(frames)
+        sim.sync()
+        rate = 4 * frames / (time.perf_counter() - t0)
+        rec = {"substeps_per_s": round(rate, 1)}
+        if solver == X.SOLVER_GAUSS_SEIDEL:
+            ms, cnt = sim.profile_frames(5, X.KERNEL_GS_SUBSTEP)
+            rec.update({"gs_substep_launch_us": round(ms * 1e3, 2), "colours": len(mesh.field("batch_offsets")) - 1})
+        sim.close()
+        out[name] = rec
+        log(f"bench: 256x256 sheet, {name}: {rate:.1f} substeps/s")
+    return out
+
+
 def build_workload(X, name: str):
     """Returns (mesh, thickness, description, config extras)."""
     if name == "sheet1024":
@@ -221,15 +263,17 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-side", action="store_true", help="skip the 256x256 Jacobi / Gauss-Seidel side record")
     ap.add_argument("--no-warm", action="store_true", help="skip the learned warm start's train + evaluation")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-kernel timing launches (for rocprofv3 counter passes)")
     ap.add_argument("--trace-summary", metavar="DIR",
                     help="print the per-kernel table of the timed frames from a rocprofv3 CSV trace "
                          "(--kernel-trace --marker-trace -f csv) of an earlier bench.py run, and exit")
+    ap.add_argument("--by-grid", action="store_true", help="--trace-summary: one row per kernel and grid size")
     args = ap.parse_args()
     if args.trace_summary:
-        print(trace_summary(args.trace_summary, frames=args.steps))
+        print(trace_summary(args.trace_summary, frames=args.steps, by_grid=args.by_grid))
         return
     if args.gpus != 1:
         print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
@@ -306,6 +350,10 @@ def main():
     log(f"bench: dominant kernel {rkernel} {r_us:.1f} us = {achieved:.0f} GB/s algorithmic; "
         f"collision sweep {sweep_ms * 1e3:.1f} us; radix pass {radix_ms * 1e3:.1f} us")
 
+    side = None
+    if args.workload == "sheet1024" and not args.no_side:
+        side = config2_rates(X)
+
     warm = None
     if not args.no_warm:
         warm = learned_warm_start()
@@ -336,6 +384,8 @@ def main():
               "copy_kernel_GBps": round(32 * n / (copy_ms * 1e-3) / 1e9, 1),
               "persistent_predict_plus_iterations_us": round(pers_ms * 1e3, 2)}
     config.update(extra)
+    if side is not None:
+        config["sheet256_both_solvers"] = side
     if warm is not None:
         config["learned_warm_start"] = warm
     out = {

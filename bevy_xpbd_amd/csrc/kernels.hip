@@ -711,9 +711,12 @@ __device__ __forceinline__ void cell_of(float4 p, const CollGrid& g, uint32_t c[
 // counts over tiles (per digit), and a stable scatter.  A tile is 2048 keys;
 // each of its 4 wavefronts owns a contiguous quarter (512 keys, 8 rounds of
 // 64), so ranks in input order are (wavefront, round, lane) and need no block
-// barrier per round.  Lanes with equal digits are found by matching the 8
-// digit bits with ballots; one leader per distinct digit updates the
-// wavefront's own counters (distinct addresses, no atomics).
+// barrier per round.  The count kernels (k_coll_keys' pass-0 counts and
+// k_radix_hist) add each key to its wavefront's LDS histogram with an LDS
+// atomic: exact integer counts, independent of the order of the adds.  The
+// scatter ranks lanes stably: lanes with equal digits are found by matching
+// the 8 digit bits with ballots, and one leader per distinct digit updates
+// the wavefront's own counters (distinct addresses, no atomics).
 constexpr uint32_t RADIX_BLOCK = 256;
 constexpr uint32_t RADIX_WAVES = RADIX_BLOCK / 64;
 constexpr uint32_t RADIX_ITEMS = 8;
@@ -1329,29 +1332,20 @@ __global__ void k_copy4_if(const float4* a, float4* b, uint32_t n, const uint32_
 // of X over the constraints (ij, rest; ordered so that neighbouring threads
 // load neighbouring vertices): C in f32 per constraint (hardware
 // sqrt, about 1 ulp: a gate statistic, not the reported residual), summed in
-// f32 per thread and in double across threads.  Block sums meet in
-// WG_SLOTS accumulators (block b in slot b mod WG_SLOTS, each slot on its
-// own 256-byte line, so that same-address atomics form short parallel
-// chains); the last block of a slot passes the slot total on to the final
-// accumulator, and the last slot's last block compares the total with the
-// threshold -- a NaN residual falls back -- sets flag[0] (1: fall back),
-// counts the substep (flag[1]) and the fallback (flag[2]), and clears the
-// grid-barrier words of the fallback launch.  Every accumulator and counter
-// is reset by the block that reads it last, so the next launch finds zeros.
-constexpr uint32_t WG_SLOTS = 32;
-constexpr uint32_t WG_LINE = 64;   // words per slot: (sum as double, count)
-
-__device__ __forceinline__ bool wg_arrive(uint32_t* slot, double v, uint32_t arrivals, double* total) {
-    double* sum = reinterpret_cast<double*>(slot);
-    __hip_atomic_fetch_add(sum, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // the add has been performed before this block is counted
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    if (__hip_atomic_fetch_add(slot + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 != arrivals) return false;
-    *total = __hip_atomic_load(sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(sum, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(slot + 2, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
-}
+// f32 per thread and in double across threads.  Every block stores its sum
+// in its own slot, then arrives on one of WG_SLOTS counters (block b on
+// counter b mod WG_SLOTS, each on its own 256-byte line, so that same-address
+// atomics form short parallel chains); the last block of a counter arrives on
+// the top counter, and the last of those adds the block slots in block order
+// (a fixed-shape reduction: the total and the decision do not depend on
+// arrival order), compares the total with the threshold -- a NaN residual
+// falls back -- sets flag[0] (1: fall back), counts the substep (flag[1]) and
+// the fallback (flag[2]), and clears the grid-barrier words of the fallback
+// launch.  Each counter is reset by the block that reads it last, so the next
+// launch finds zeros.
+constexpr uint32_t WG_LINE = 64;           // words per counter line
+constexpr uint32_t WG_SLOTS = 32;          // arrival counters (line 0 is the top counter)
+constexpr uint32_t WG_MAX_BLOCKS = 2048;   // block sums (doubles) after the counter lines
 
 __global__ __launch_bounds__(256) void k_warm_gate(const float4* X, const uint2* ij, const float* rest, uint32_t e,
                                                   double rest_sq, float threshold, uint32_t* acc, uint32_t* flag,
@@ -1370,24 +1364,50 @@ __global__ __launch_bounds__(256) void k_warm_gate(const float4* X, const uint2*
     __shared__ uint32_t last;
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sd;
     __syncthreads();
+    double* part = reinterpret_cast<double*>(acc + WG_LINE * (WG_SLOTS + 1));
     if (threadIdx.x == 0) {
         const uint32_t slot = blockIdx.x % WG_SLOTS;
         const uint32_t in_slot = (gridDim.x - slot + WG_SLOTS - 1) / WG_SLOTS;
         const uint32_t slots = min(gridDim.x, WG_SLOTS);
-        double tot = 0.0;
+        uint32_t* counter = acc + WG_LINE * (1 + slot);
         last = 0u;
-        if (wg_arrive(acc + WG_LINE * slot, (red[0] + red[1]) + (red[2] + red[3]), in_slot, &tot) &&
-            wg_arrive(acc + WG_LINE * WG_SLOTS, tot, slots, &tot)) {
-            const double rel = rest_sq > 0.0 ? sqrt(tot) / sqrt(rest_sq) : 0.0;
-            const uint32_t f = rel <= double(threshold) ? 0u : 1u;
-            flag[0] = f;
-            flag[1] = flag[1] + 1u;
-            flag[2] = flag[2] + f;
-            last = 1u;
+        __hip_atomic_store(part + blockIdx.x, (red[0] + red[1]) + (red[2] + red[3]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        // the block sum is written before this block is counted
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (__hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == in_slot) {
+            __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_fetch_add(acc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == slots) {
+                __hip_atomic_store(acc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                last = 1u;
+            }
         }
     }
     __syncthreads();
     if (!last) return;
+    // all of this thread's slots in flight at once, then added in slot order
+    constexpr uint32_t PER_THREAD = WG_MAX_BLOCKS / 256;
+    double v[PER_THREAD];
+#pragma unroll
+    for (uint32_t r = 0; r < PER_THREAD; ++r) {
+        const uint32_t b = threadIdx.x + 256 * r;
+        v[r] = b < gridDim.x ? __hip_atomic_load(part + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+    }
+    double t = 0.0;
+#pragma unroll
+    for (uint32_t r = 0; r < PER_THREAD; ++r) t += v[r];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double tot = (red[0] + red[1]) + (red[2] + red[3]);
+        const double rel = rest_sq > 0.0 ? sqrt(tot) / sqrt(rest_sq) : 0.0;
+        const uint32_t f = rel <= double(threshold) ? 0u : 1u;
+        flag[0] = f;
+        flag[1] = flag[1] + 1u;
+        flag[2] = flag[2] + f;
+    }
     for (uint32_t w = threadIdx.x; w < GRID_SYNC_WORDS; w += blockDim.x) barrier[w] = 0u;
 }
 
@@ -1510,6 +1530,10 @@ __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
     const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
     GridSync gsync;
     gsync.init(a.barrier, a.error);
+    if (a.probe) {   // uniform: the co-residency check of this very kernel's grid
+        gsync.sync();
+        return;
+    }
     for (uint32_t i = gt; i < p.n; i += nthreads) a.W[i] = predict(a.X[i], a.V[i], i, p);
     gsync.sync();
     for (uint32_t it = 0; it < a.iterations; ++it) {
@@ -1585,6 +1609,10 @@ __global__ __launch_bounds__(256) void k_jacobi_persistent(JacobiArgs a, Persist
     const DevParams& p = a.p;
     GridSync gsync;
     gsync.init(q.barrier, q.error);
+    if (q.probe) {   // uniform: the co-residency check of this very kernel's grid
+        gsync.sync();
+        return;
+    }
     const uint32_t nthreads = gridDim.x * blockDim.x;
     // predict (whole wavefronts per 64 particles: the slice tame bytes of A)
     const bool track = FASTDIV && q.tameA && q.tameB;
@@ -1908,11 +1936,11 @@ void launch_copy4_if(const float4* a, float4* b, uint32_t n, const uint32_t* gat
     hipLaunchKernelGGL(k_copy4_if, dim3(cdiv(n, 256)), dim3(256), 0, s, a, b, n, gate);
 }
 
-uint32_t warm_gate_words() { return WG_LINE * (WG_SLOTS + 1); }
+uint32_t warm_gate_words() { return WG_LINE * (WG_SLOTS + 1) + 2 * WG_MAX_BLOCKS; }
 
 void launch_warm_gate(const float4* X, const uint2* ij, const float* rest, uint32_t e, double rest_sq,
                       float threshold, uint32_t* acc, uint32_t* flag, uint32_t* barrier, hipStream_t s) {
-    const uint32_t blocks = std::max(1u, std::min(cdiv(e, 256 * 4), 2048u));
+    const uint32_t blocks = std::max(1u, std::min(cdiv(e, 256 * 4), WG_MAX_BLOCKS));
     hipLaunchKernelGGL(k_warm_gate, dim3(blocks), dim3(256), 0, s, X, ij, rest, e, rest_sq, threshold, acc, flag,
                        barrier);
 }

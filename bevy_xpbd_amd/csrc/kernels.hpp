@@ -105,6 +105,7 @@ struct PersistArgs {
     uint32_t* barrier;
     uint32_t* error;
     const uint32_t* gate;   // non-null: the launch does nothing unless *gate != 0
+    uint32_t probe;         // 1: only the barrier set-up and one barrier (co-residency check)
     // per-slice tame bytes of A and B (both non-null and fast division on:
     // the short exact operations on slices whose neighbourhood is tame)
     uint8_t* tameA;
@@ -159,6 +160,7 @@ struct GsArgs {
     uint32_t* barrier;
     uint32_t* error;
     uint32_t final_write;
+    uint32_t probe;      // 1: only the barrier set-up and one barrier (co-residency check)
     DevParams p;
 };
 
@@ -205,8 +207,9 @@ void launch_predict_gated(const float4* X, const float4* V, float4* P, DevParams
 void launch_copy4_if(const float4* a, float4* b, uint32_t n, const uint32_t* gate, hipStream_t s);
 // warm-start gate: flag[0] = relative residual of X (f32 per constraint) >
 // threshold or NaN; flag[1] += 1; flag[2] += flag[0]; clears barrier
-// (GRID_SYNC_WORDS).  acc: warm_gate_words() words (256-byte aligned), zero
-// before the first launch; each launch leaves them zero
+// (GRID_SYNC_WORDS).  acc: warm_gate_words() words (256-byte aligned):
+// arrival counters, zero before the first launch and left zero by each, and
+// one partial-sum slot per block, summed in block order (deterministic)
 uint32_t warm_gate_words();
 void launch_warm_gate(const float4* X, const uint2* ij, const float* rest, uint32_t e, double rest_sq,
                       float threshold, uint32_t* acc, uint32_t* flag, uint32_t* barrier, hipStream_t s);

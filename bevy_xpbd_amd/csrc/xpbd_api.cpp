@@ -1085,6 +1085,69 @@ int autotune(xpbd_cloth* h) {
     return check_device_flag(h);
 }
 
+// Co-residency of the persistent grids, checked on this device.  A grid
+// barrier needs every block of the launch resident at once; the starting grid
+// is the occupancy figure capped at 7 of 8 wavefront slots per SIMD
+// (coresident_per_cu).  Each grid the handle will launch is run once in probe
+// mode -- the kernel itself, so the same registers and LDS, doing its barrier
+// set-up and one barrier -- and where the bounded wait expires (a block did
+// not become resident in time) the grid shrinks by one block per CU and the
+// probe repeats.  Work of other processes on the same GPU can still take
+// slots at run time; the bounded wait stays the guard there.
+int verify_coresident(xpbd_cloth* h, bool gs, bool persistent) {
+    TRY(check_device_flag(h));
+    int dev = 0, cus = 0;
+    HIPC(h, hipGetDevice(&dev));
+    HIPC(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const DevParams p = dev_params(h);
+    hipStream_t s = h->stream;
+    auto check = [&](auto launch, uint32_t& blocks, const char* what) -> int {
+        for (int tries = 0; tries < 8 && blocks > 0; ++tries) {
+            launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
+            launch(blocks);
+            HIPC(h, hipGetLastError());
+            uint32_t flag = 0;
+            HIPC(h, hipMemcpyAsync(&flag, h->err, sizeof(flag), hipMemcpyDeviceToHost, s));
+            HIPC(h, hipStreamSynchronize(s));
+            if (!flag) return XPBD_OK;
+            const uint32_t zero = 0;
+            HIPC(h, hipMemcpy(h->err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+            blocks = blocks > uint32_t(cus) ? blocks - uint32_t(cus) : blocks / 2;
+        }
+        return fail(h, XPBD_E_STATE, std::string("no co-resident grid found for ") + what);
+    };
+    if (gs && h->e) {
+        TRY(check(
+            [&](uint32_t b) {
+                GsArgs g{};
+                g.barrier = h->barrier;
+                g.error = h->err;
+                g.probe = 1;
+                g.p = p;
+                launch_gs(g, b, h->gs_block, s);
+            },
+            h->gs_blocks, "the coloured Gauss-Seidel kernel"));
+    }
+    if (persistent) {
+        TRY(check(
+            [&](uint32_t b) {
+                JacobiArgs a = jacobi_args(h, p);
+                PersistArgs q{};
+                q.barrier = h->barrier;
+                q.error = h->err;
+                q.probe = 1;
+                launch_jacobi_persistent(a, q, b, s);
+            },
+            h->pj_blocks, "the persistent Jacobi kernel"));
+    }
+    return XPBD_OK;
+}
+
+bool uses_persistent_jacobi(const xpbd_cloth* h) {
+    return h->cfg.solver == XPBD_SOLVER_JACOBI &&
+           (h->cfg.persistent_jacobi || (h->warm && std::isfinite(h->warm_threshold)));
+}
+
 }  // namespace
 
 // Reason for the last failed call without a handle on this thread: a
@@ -1307,6 +1370,8 @@ XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, cons
         r = autotune(h);
         if (r != XPBD_OK) return r;
     }
+    r = verify_coresident(h, h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL, uses_persistent_jacobi(h));
+    if (r != XPBD_OK) return r;
     if (hipStreamSynchronize(h->stream) != hipSuccess) return XPBD_E_HIP;
     *out = guard.release();
     return XPBD_OK;
@@ -1340,8 +1405,8 @@ XPBD_API int xpbd_set_config(xpbd_cloth* h, const xpbd_sim_config* cfg) {
     h->cfg = *cfg;
     apply_tuning(h);
     drop_graph(h);
-    if (retune) return autotune(h);
-    return XPBD_OK;
+    if (retune) TRY(autotune(h));
+    return verify_coresident(h, h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL, uses_persistent_jacobi(h));
 }
 
 XPBD_API int xpbd_step_frames(xpbd_cloth* h, uint32_t frames) {
@@ -1933,7 +1998,7 @@ XPBD_API int xpbd_set_warm_start(xpbd_cloth* h, xpbd_mlp* m, uint32_t warm_itera
     h->warm_iters = warm_iterations;
     h->warm_threshold = residual_threshold;
     drop_graph(h);
-    return XPBD_OK;
+    return m && std::isfinite(residual_threshold) ? verify_coresident(h, false, true) : XPBD_OK;
 }
 
 XPBD_API int xpbd_warm_stats(xpbd_cloth* h, uint32_t* substeps, uint32_t* fallbacks, uint32_t reset) {

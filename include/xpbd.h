@@ -9,6 +9,12 @@
  * Threading: calls on one xpbd_cloth are externally synchronised (the
  * reference runs its render-world system on one thread).  Different handles
  * are independent.  All device work of a handle is ordered on its stream.
+ * One exception: the persistent launches (coloured Gauss-Seidel, persistent
+ * Jacobi, the learned warm start's gated fallback) need every block of their
+ * grid resident on the GPU at once, so while one runs, no other stream may
+ * run work on the same GPU: order handles that step on one GPU with
+ * xpbd_sync.  Otherwise the persistent launch's bounded wait can expire, and
+ * the next synchronising call reports XPBD_E_STATE.
  */
 #ifndef XPBD_H
 #define XPBD_H

@@ -51,6 +51,54 @@ def scaled_log_compliance(c):
 
 
 def grid_topology(f):
+    """nbr8 (N x 8 int32, -1 absent) and vaux (N x 4 float32) from mesh
+    fields; array form of grid_topology_loop (same results, checked against
+    it on small meshes), for meshes of a million vertices."""
+    offs = np.asarray(f["cloth_offsets"], np.int64)
+    colsv = np.asarray(f["cloth_cols"], np.int64)
+    noff = np.asarray(f["neighbor_offsets"], np.int64)
+    no = np.asarray(f["neighbor_other"], np.int64)
+    nr, nc = np.asarray(f["neighbor_rest_len"], np.float32), np.asarray(f["neighbor_compliance"], np.float32)
+    n = len(f["inv_mass"])
+    nbr = np.full((n, 8), -1, np.int32)
+    aux = np.zeros((n, 4), np.float32)
+    cloth = np.repeat(np.arange(len(colsv)), np.diff(offs)[:len(colsv)])
+    idx = np.arange(len(cloth))
+    off, cols = offs[cloth], colsv[cloth]
+    rows = (offs[cloth + 1] - off) // cols
+    col, row = (idx - off) % cols, (idx - off) // cols
+    for k in range(8):
+        x, y = col + DX[k], row + DY[k]
+        ok = (x >= 0) & (x < cols) & (y >= 0) & (y < rows)
+        nbr[idx[ok], k] = (off + y * cols + x)[ok]
+    # CSR entries of the grid vertices: owner, partner offsets in its cloth
+    owner = np.repeat(np.arange(n), np.diff(noff))
+    keep = owner < len(cloth)
+    q = np.nonzero(keep)[0]
+    i, j = owner[q], no[q]
+    same = (j >= off[i]) & (j < offs[cloth[i] + 1])
+    q, i, j = q[same], i[same], j[same]
+    dc = np.abs((j - off[i]) % cols[i] - col[i])
+    dr = np.abs((j - off[i]) // cols[i] - row[i])
+    cls = {"stretch": dc + dr == 1, "shear": (dc == 1) & (dr == 1),
+           "bend": ((dc == 2) & (dr == 0)) | ((dc == 0) & (dr == 2))}
+    comp = np.full((len(cloth), 3), np.float32(0))
+    hs = np.zeros(len(cloth), np.float32)
+    for a, name in enumerate(("stretch", "shear", "bend")):
+        m = cls[name]
+        first_owner, first = np.unique(i[m], return_index=True)   # entries are in CSR order
+        comp[first_owner, a] = nc[q[m][first]]
+        if name == "stretch":
+            hs[first_owner] = nr[q[m][first]]
+    lc = np.log10(np.maximum(comp, np.float32(1e-10)).astype(np.float64)).astype(np.float32)
+    aux[:len(cloth), :3] = ((lc + np.float32(5.5)) / np.float32(2.5)).astype(np.float32)
+    inv = np.zeros_like(hs)
+    inv[hs > 0] = (np.float32(1.0) / hs[hs > 0]).astype(np.float32)
+    aux[:len(cloth), 3] = inv
+    return nbr, aux
+
+
+def grid_topology_loop(f):
     """nbr8 (N x 8 int32, -1 absent) and vaux (N x 4 float32) from mesh fields."""
     offs, colsv = f["cloth_offsets"], f["cloth_cols"]
     no, nr, nc, noff = f["neighbor_other"], f["neighbor_rest_len"], f["neighbor_compliance"], f["neighbor_offsets"]

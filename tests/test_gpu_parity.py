@@ -289,6 +289,7 @@ def test_graph_replay_equals_eager(X, persistent):
     b, _, _ = make(X, X.ClothMeshData.sheet(**kw), S=4, I=8, use_graph=0, thickness=th)
     for k in range(4):
         a.step_frames(5)
+        a.sync()   # a persistent launch runs alone on the GPU (xpbd.h threading)
         b.step_frames(5)
         pa, va = a.read_state()
         pb, vb = b.read_state()

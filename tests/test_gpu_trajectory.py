@@ -114,30 +114,53 @@ def test_config2_ten_frames_error_against_f64(O, X, solver):
     sim.close()
 
 
-def test_config4_batch_one_frame_error_against_f64(O, X):
-    """The 512-cloth uneven batch (seed 4), one frame of 8 substeps x 10
-    Jacobi iterations with per-cloth self-collision, from the hipGraph:
-    bit-exact against the f32 oracle; the f64 error is reported."""
+@pytest.fixture(scope="module")
+def batch4(O, X):
+    """The 512-cloth uneven batch (seed 4) and the f32 oracle's state after one
+    frame of 8 substeps x 10 Jacobi iterations with per-cloth self-collision."""
     from bevy_xpbd_amd.workloads import batch_descs
-    note("case config4 batch one frame")
     O.set_threads(16)
     descs, _ = batch_descs(count=512, seed=4, spacing=0.01)
     mesh = X.ClothMeshData.batch(descs)
-    cfg = X.ClothSimConfig.default(solve_substeps=8, solve_inner_iterations=10, use_graph=1)
-    u = X.ClothSimUniforms.default(X.SOLVER_JACOBI, 8, thickness=2 * (0.4 * 0.01))
-    sim = X.ClothComputePlugin(mesh, cfg, u)
     f = mesh.fields()
-    s = O.Solver(f)
+    cfg = X.ClothSimConfig.default(solve_substeps=8, solve_inner_iterations=10)
+    u = X.ClothSimUniforms.default(X.SOLVER_JACOBI, 8, thickness=2 * (0.4 * 0.01))
     p = oracle_params(O, X, u, cfg, collision=True)
+    note("  oracle f32 (512-cloth batch, one frame)")
+    x32, v32 = O.Solver(f).frames(f["positions"].copy(), np.zeros_like(f["positions"]), p, 8, 1, 1)
+    return mesh, f, u, p, x32, v32
+
+
+def test_config4_batch_one_frame_error_against_f64(O, X, batch4):
+    """The 512-cloth uneven batch (seed 4), one frame of 8 substeps x 10
+    Jacobi iterations with per-cloth self-collision, from the hipGraph:
+    bit-exact against the f32 oracle; the f64 error is reported."""
+    note("case config4 batch one frame")
+    mesh, f, u, p, x32, v32 = batch4
+    cfg = X.ClothSimConfig.default(solve_substeps=8, solve_inner_iterations=10, use_graph=1)
+    sim = X.ClothComputePlugin(mesh, cfg, u)
     sim.step_frames(1)
-    gp, _ = sim.read_state()
-    note("  oracle f32")
-    x32, _ = s.frames(f["positions"].copy(), np.zeros_like(f["positions"]), p, 8, 1, 1)
-    assert bits_equal(gp, x32)
+    gp, gv = sim.read_state()
+    assert bits_equal(gp, x32) and bits_equal(gv, v32)
     note("  oracle f64")
-    x64, _ = s.frames(f["positions"].astype(np.float64), np.zeros(f["positions"].shape), p, 8, 1, 1,
-                      dtype=np.float64)
+    x64, _ = O.Solver(f).frames(f["positions"].astype(np.float64), np.zeros(f["positions"].shape), p, 8, 1, 1,
+                                dtype=np.float64)
     err = max_abs(gp, x64)
     note(f"  config4 frame 1: max |GPU f32 - oracle f64| = {err:.3e} m")
     assert err < 1e-4
+    sim.close()
+
+
+def test_config4_batch_persistent_one_launch_per_substep(O, X, batch4):
+    """The same frame with predict + all 10 iterations of each substep in one
+    persistent launch (grid barriers between iterations; its grid checked
+    co-resident at create), replayed from the hipGraph: bit-exact against
+    the f32 oracle, as the per-iteration launches are."""
+    note("case config4 batch persistent")
+    mesh, f, u, p, x32, v32 = batch4
+    cfg = X.ClothSimConfig.default(solve_substeps=8, solve_inner_iterations=10, use_graph=1, persistent_jacobi=1)
+    sim = X.ClothComputePlugin(mesh, cfg, u)
+    sim.step_frames(1)
+    gp, gv = sim.read_state()
+    assert bits_equal(gp, x32) and bits_equal(gv, v32)
     sim.close()

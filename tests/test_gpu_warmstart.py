@@ -200,7 +200,9 @@ def test_threshold_zero_always_falls_back_to_plain(X, use_graph, coll):
     P, _ = mlp.params()
     mlp.set_params(P, out_scale=10.0)
     a.set_warm_start(mlp, warm_iterations=2, residual_threshold=0.0)
+    # the fallback is a persistent launch: it runs alone (xpbd.h threading)
     a.step_frames(2)
+    a.sync()
     b.step_frames(2)
     pa, va = a.read_state()
     pb, vb = b.read_state()
@@ -230,4 +232,36 @@ def test_gate_residual_matches_reported_residual(X):
         a.step_substeps(1, False)
         assert a.warm_stats() == (1, want), scale
         a.close()
+    mlp.close()
+
+
+def test_gated_warm_start_is_reproducible(X):
+    """The gate sums its block partials in block order, so two runs of the
+    same gated warm start give the same states and the same fallback counts,
+    bit for bit (the threshold sits inside the range of the warm residuals,
+    where an arrival-order sum could flip a decision)."""
+    note("case warm gate reproducible")
+    mesh = small_batch(X)
+    mlp = X.WarmStartMLP(seed=2)
+    P, _ = mlp.params()
+    mlp.set_params(P, out_scale=40.0)
+    probe, _, _ = make(X, mesh, I=10)
+    probe.set_warm_start(mlp, warm_iterations=2, residual_threshold=float("inf"))
+    rs = []
+    for _ in range(8):
+        probe.step_substeps(1, False)
+        rs.append(probe.residual())
+    probe.close()
+    threshold = float(np.median(rs))
+    runs = []
+    for _ in range(2):
+        a, _, _ = make(X, mesh, I=10, use_graph=1)
+        a.set_warm_start(mlp, warm_iterations=2, residual_threshold=threshold)
+        a.step_frames(2)
+        p, v = a.read_state()
+        runs.append((p, v, a.warm_stats()))
+        a.close()
+    note(f"  fallbacks {runs[0][2]} and {runs[1][2]} at threshold {threshold:.3e}")
+    assert bits_equal(runs[0][0], runs[1][0]) and bits_equal(runs[0][1], runs[1][1])
+    assert runs[0][2] == runs[1][2]
     mlp.close()

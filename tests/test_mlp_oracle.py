@@ -96,3 +96,17 @@ def test_grid_topology_and_features_closed_forms(X):
     moved = rest + np.float32([0.5, 0.25, -0.125])
     feats = M.features(moved, np.zeros_like(rest), rest, nbr, aux, 1.0 / 480.0)
     assert np.all(feats[:, :24] == 0)
+
+
+def test_grid_topology_array_form_equals_loop():
+    """The array form used at 256x256 scale gives the loop's neighbours and
+    feature constants, bit for bit, on batches with uneven cloths."""
+    import bevy_xpbd_amd as X
+    descs = [X.SheetDesc.make(cols=32, rows=32, spacing=1 / 31, pins=1, jitter=1e-3, seed=1, stretch_compliance=0.0),
+             X.SheetDesc.make(cols=40, rows=24, spacing=0.02, layout=1, height=1.0, angle=0.7, pins=3, jitter=1e-3,
+                              seed=3, shear_compliance=1e-4, bend_compliance=1e-2),
+             X.SheetDesc.make(cols=3, rows=7, spacing=0.02, stretch_compliance=1e-7),
+             X.SheetDesc.make(cols=2, rows=2, spacing=0.05)]
+    f = X.ClothMeshData.batch(descs).fields()
+    a, b = M.grid_topology(f), M.grid_topology_loop(f)
+    assert a[0].tobytes() == b[0].tobytes() and a[1].tobytes() == b[1].tobytes()
