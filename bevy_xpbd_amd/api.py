@@ -86,7 +86,8 @@ class MeshDesc(C.Structure):
         ("initial_normals", C.c_void_p), ("inv_mass", C.c_void_p), ("constraint_i", C.c_void_p),
         ("constraint_j", C.c_void_p), ("constraint_rest_len", C.c_void_p), ("constraint_compliance", C.c_void_p),
         ("constraint_batch_offsets", C.c_void_p), ("tri_indices", C.c_void_p), ("neighbor_offsets", C.c_void_p),
-        ("neighbor_packed", C.c_void_p), ("ground", C.c_void_p), ("num_cloths", C.c_uint32),
+        ("neighbor_packed", C.c_void_p), ("ground", C.c_void_p), ("device_prepare", C.c_uint32),
+        ("num_cloths", C.c_uint32),
         ("cloth_offsets", C.c_void_p),
     ]
 
@@ -263,14 +264,16 @@ class ClothMeshData:
     def from_arrays(cls, initial_pos, inv_mass, constraint_i=(), constraint_j=(), constraint_rest_len=(),
                     constraint_compliance=(), rest_pos=None, initial_normals=None, tri_indices=None,
                     constraint_batch_offsets=None, neighbor_offsets=None, neighbor_packed=None,
-                    ground=None, cloth_offsets=None) -> "ClothMeshData":
+                    ground=None, cloth_offsets=None, device_prepare=False) -> "ClothMeshData":
         """A caller's topology in ClothSimConfig's arrays (cloth_compute.rs:142-179):
         positions as (N, 3) or (N, 4) rows, constraint rows, optional colour
         batch offsets (None: coloured here as finalize_cloth_mesh does) and
         optional Jacobi adjacency (neighbor_offsets + neighbor_packed, the
         vec4 entries of pack_neighbor_gpu; None: built by cursor scatter);
         cloth_offsets splits the particles into cloths that collide only
-        within themselves (None: one cloth)."""
+        within themselves (None: one cloth); device_prepare runs the
+        colouring, colour sort and adjacency scatter on the GPU (same
+        results)."""
         def vec4(a):
             if a is None:
                 return None
@@ -302,6 +305,7 @@ class ClothMeshData:
         d.constraint_batch_count = 0 if bo is None else max(len(bo) - 1, 0)
         co = keep["cloth_offsets"]
         d.num_cloths = 0 if co is None else max(len(co) - 1, 0)
+        d.device_prepare = 1 if device_prepare else 0
         for k, v in keep.items():
             setattr(d, k, None if v is None or v.size == 0 else v.ctypes.data)
         h = C.c_void_p()

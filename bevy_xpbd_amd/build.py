@@ -29,6 +29,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [
     ("kernels.hip", "hip"),
     ("mlp.hip", "hip"),
+    ("topology.hip", "hip"),
     ("xpbd_api.cpp", "hip"),
     ("mesh_prep.cpp", "c++"),
 ]

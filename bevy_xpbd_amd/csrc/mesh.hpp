@@ -38,8 +38,14 @@ Mesh* grid_cloth_hanging(uint32_t quad_cols, uint32_t quad_rows, float cell_size
 Mesh* parse_welded_obj(const std::string& text);
 Mesh* sheet_batch(const xpbd_sheet_desc* descs, uint32_t count);
 // ClothSimConfig's topology arrays (cloth_compute.rs:142-179), validated;
-// nullptr with the reason in `why` when an array is inconsistent.
-Mesh* mesh_from_arrays(const xpbd_mesh_desc& d, std::string& why);
+// nullptr with the reason in `why` and an XPBD_E_* status in `code` when an
+// array is inconsistent or the device preparation fails.
+Mesh* mesh_from_arrays(const xpbd_mesh_desc& d, std::string& why, int& code);
+// The colouring, colour sort and CSR of assemble_constraints computed on the
+// GPU (topology.hip), same results; `rows` is consumed.  Returns XPBD_OK or an
+// XPBD_E_* status with the reason in `why`.
+int topology_on_device(size_t n, const std::vector<uint32_t>& cloth_offsets, std::vector<Constraint>& rows, Mesh& m,
+                       std::string& why);
 
 // Greedy edge colouring + colour-sorted order + CSR cursor scatter
 // (mesh_prep.rs:153-221, 584-635).  `constraints` is consumed.

@@ -560,8 +560,9 @@ Mesh* sheet_batch(const xpbd_sheet_desc* descs, uint32_t count) {
 // as finalize_cloth_mesh does.  A given adjacency (pack_neighbor_gpu's vec4
 // entries, cloth_jacobi.rs:95-111) is checked row by row and kept in its
 // order; without one, the cursor scatter of mesh_prep.rs:588-635 builds it.
-Mesh* mesh_from_arrays(const xpbd_mesh_desc& d, std::string& why) {
+Mesh* mesh_from_arrays(const xpbd_mesh_desc& d, std::string& why, int& code) {
     const size_t n = d.num_particles, e = d.num_distance_constraints, t = d.num_tris;
+    code = XPBD_E_INVALID;
     auto bad = [&](const std::string& w) -> Mesh* {
         why = w;
         return nullptr;
@@ -578,6 +579,8 @@ Mesh* mesh_from_arrays(const xpbd_mesh_desc& d, std::string& why) {
     if (!d.neighbor_offsets != !d.neighbor_packed) return bad("neighbor_offsets and neighbor_packed come together");
     if (d.neighbor_offsets && !d.constraint_batch_offsets)
         return bad("a given adjacency names constraint rows: it needs the constraint_batch_offsets of those rows");
+    if (d.device_prepare && (d.constraint_batch_offsets || d.neighbor_offsets))
+        return bad("device_prepare computes the colouring and the adjacency: give neither");
 
     auto m = std::make_unique<Mesh>();
     m->positions.resize(3 * n);
@@ -630,7 +633,15 @@ Mesh* mesh_from_arrays(const xpbd_mesh_desc& d, std::string& why) {
         cons[k] = c;
     }
     if (!d.constraint_batch_offsets) {
-        assemble_constraints(*m, n, std::move(cons));
+        if (d.device_prepare) {
+            const int r = topology_on_device(n, m->cloth_offsets, cons, *m, why);
+            if (r != XPBD_OK) {
+                code = r;
+                return nullptr;
+            }
+        } else {
+            assemble_constraints(*m, n, std::move(cons));
+        }
         return m.release();
     }
 

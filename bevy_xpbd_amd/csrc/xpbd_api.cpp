@@ -1204,11 +1204,12 @@ XPBD_API int xpbd_mesh_from_arrays(const xpbd_mesh_desc* d, xpbd_mesh** out) {
     }
     try {
         std::string why;
-        Mesh* m = mesh_from_arrays(*d, why);
+        int code = XPBD_E_INVALID;
+        Mesh* m = mesh_from_arrays(*d, why, code);
         if (!m) {
             g_thread_text = why;
             g_thread_error = g_thread_text.c_str();
-            return XPBD_E_INVALID;
+            return code;
         }
         return wrap_mesh(m, out);
     } catch (const std::bad_alloc&) {

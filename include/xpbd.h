@@ -154,6 +154,14 @@ typedef struct xpbd_mesh_desc {
     const uint32_t* ground;                   /* u32 x N or NULL: nonzero = ground
                                                  plane y = 0 for that particle (no
                                                  reference counterpart)               */
+    uint32_t device_prepare;                  /* 1: the colouring, colour sort and
+                                                 adjacency scatter run on the GPU
+                                                 (constraint_batch_offsets and the
+                                                 adjacency NULL; the same results as
+                                                 the host; at most 64 colours, else
+                                                 XPBD_E_UNSUPPORTED; no device:
+                                                 XPBD_E_NO_DEVICE).  Rows of different
+                                                 cloths are coloured in parallel.     */
     uint32_t num_cloths;                      /* C; 0 or 1 = one cloth                 */
     const uint32_t* cloth_offsets;            /* u32 x (C+1) first particle of each
                                                  cloth (no reference counterpart: many

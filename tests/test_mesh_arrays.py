@@ -133,3 +133,24 @@ def test_inconsistent_arrays_are_refused(X):
     t = base["tri_indices"].copy()
     t[0] = 99
     _expect_invalid(X, "tri_indices", **edit(tri_indices=t))
+
+
+def test_device_prepare_without_a_gpu_reports_no_device(X):
+    import ctypes as C
+    hip = None
+    try:
+        hip = C.CDLL("libamdhip64.so")
+        cnt = C.c_int(0)
+        present = hip.hipGetDeviceCount(C.byref(cnt)) == 0 and cnt.value > 0
+    except OSError:
+        present = False
+    if present:
+        pytest.skip("a GPU is present")
+    with pytest.raises(X.XpbdError) as e:
+        X.ClothMeshData.from_arrays([[0, 0, 0], [1, 0, 0]], [1.0, 1.0], [0], [1], [1.0], [0.0], device_prepare=True)
+    assert e.value.code == X.XPBD_E_NO_DEVICE
+
+
+def test_device_prepare_refuses_given_colouring(X):
+    base = X.ClothMeshData.sheet(cols=6, rows=5, spacing=0.1, pins=2).sim_config_arrays()
+    _expect_invalid(X, "give neither", **dict(base, device_prepare=True))
