@@ -167,7 +167,9 @@ def learned_warm_start():
     10 frames behind the residual gate (threshold 1e-4).  Then at scale: a
     network chosen by a gated rollout on validation sheets
     (warmstart.train_selected), on 512 held-out stiff sheets in one batch,
-    gated at the plain 10-iteration residual (warmstart.evaluate_batch)."""
+    gated at the plain 10-iteration residual (warmstart.evaluate_batch).
+    Last, on the stated workload (8 x 256x256 sheets in one batch): the time to
+    residual against plain Jacobi at 2..40 iterations (warmstart.residual_curve)."""
     try:
         from bevy_xpbd_amd import warmstart as W
         t0 = time.perf_counter()
@@ -187,8 +189,20 @@ def learned_warm_start():
         mlp, selection = W.train_selected(log=log)
         batch = W.evaluate_batch(mlp, count=512, warm_iterations=(2, 3, 4))
         batch.update({"train_steps": 10000, "selection": selection})
+        # the stated workload: 256x256 sheets in the batch layout, time to
+        # residual against plain Jacobi at 2..40 iterations
+        curve = W.residual_curve(mlp, log=log)
         mlp.close()
         res["batch_512_gated"] = batch
+        res["sheets256_time_to_residual"] = curve
+        best = max(curve["warm"].items(), key=lambda kv: kv[1]["speedup_at_equal_residual"])
+        res["headline"] = {"workload": curve["workload"], "warm_iterations": int(best[0]),
+                           "speedup_at_equal_residual": best[1]["speedup_at_equal_residual"],
+                           "equivalent_plain_iterations": best[1]["equivalent_plain_iterations"],
+                           "gated_speedup_vs_plain_at_gate": best[1]["gated"]["speedup_vs_that_plain"],
+                           "gated_fallbacks": [best[1]["gated"]["fallbacks"], best[1]["gated"]["warm_substeps"]],
+                           "gate_threshold": best[1]["gated"]["threshold"],
+                           "gate_plain_iterations": best[1]["gated"]["plain_iterations_of_threshold"]}
         for k, v in batch["warm"].items():
             log(f"bench: warm start, 512 sheets: warm+{k} gated at the plain I=10 residual: {v['fallbacks']}/"
                 f"{v['substeps']} fallbacks, {v['speedup_vs_plain']}x plain I=10")

@@ -281,6 +281,93 @@ def evaluate_batch(mlp, count: int = 512, iterations: int = 10, warm_iterations=
     return out
 
 
+def _rollout(mesh, state, frames, iterations=10, mlp=None, warm_iterations=2, threshold=float("inf")):
+    """`frames` frames (hipGraph) from `state`: substeps/s (frames timed one by
+    one, residual read outside the timing), the relative residual after each
+    frame, and the warm start's (substeps, fallbacks)."""
+    sim = make_sim(mesh, iterations=iterations, use_graph=1)
+    if mlp is not None:
+        sim.set_warm_start(mlp, warm_iterations=warm_iterations, residual_threshold=threshold)
+    sim.write_state(*state)
+    sim.step_frames(1)   # capture the graph, then start again from the state
+    sim.sync()
+    sim.write_state(*state)
+    if mlp is not None:
+        sim.warm_stats(reset=True)
+    res, t = [], 0.0
+    for _ in range(frames):
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step_frames(1)
+        sim.sync()
+        t += time.perf_counter() - t0
+        res.append(sim.residual())
+    stats = sim.warm_stats() if mlp is not None else (0, 0)
+    sim.close()
+    return {"substeps_per_s": round(frames * 8 / t, 1), "final_residual": res[-1], "max_residual": max(res),
+            "residuals": res, "warm_substeps": stats[0], "fallbacks": stats[1]}
+
+
+def residual_curve(mlp, count: int = 8, seed: int = 101, frames: int = 10, start_frames: int = 2,
+                   plain_iterations=(2, 4, 6, 8, 10, 15, 20, 30, 40), warm_iterations=(2, 3, 4), log=None):
+    """Time to residual on the learned warm start's stated workload: `count`
+    256x256 sheets in one batch (eval_descs: hanging and horizontal, log-uniform
+    compliances).  From one plain-solver state, `frames` frames of plain Jacobi
+    at each iteration count give the curve (iterations -> residual after the
+    last frame, substeps/s); warm + k (ungated) ends at some residual r_k, and
+    the plain iteration count I* ending at r_k (log-linear interpolation of the
+    curve) gives the speedup at equal residual, rate(warm + k) / rate(plain
+    at I*).  Warm + k gated at the largest per-frame residual of plain at the
+    iteration count just above I* shows what the gate keeps."""
+    mesh = A.ClothMeshData.batch(eval_descs(seed=seed, count=count))
+    sim = make_sim(mesh, iterations=10)
+    sim.step_frames(start_frames)
+    state = sim.read_state()
+    sim.close()
+    plain = {}
+    for it in plain_iterations:
+        plain[it] = _rollout(mesh, state, frames, iterations=it)
+        if log:
+            log(f"warmstart: 256x256 x {count}: plain I={it}: {plain[it]['substeps_per_s']} substeps/s, "
+                f"residual {plain[it]['final_residual']:.4e}")
+    pts = sorted((it, r["final_residual"]) for it, r in plain.items())
+
+    def rate_at(i_star):
+        lo = max([p for p in pts if p[0] <= i_star], default=pts[0])
+        hi = min([p for p in pts if p[0] >= i_star], default=pts[-1])
+        if lo[0] == hi[0]:
+            return plain[lo[0]]["substeps_per_s"]
+        # substep time is affine in the iteration count: interpolate it
+        t_lo, t_hi = 1.0 / plain[lo[0]]["substeps_per_s"], 1.0 / plain[hi[0]]["substeps_per_s"]
+        return 1.0 / (t_lo + (i_star - lo[0]) / (hi[0] - lo[0]) * (t_hi - t_lo))
+
+    warm = {}
+    for k in warm_iterations:
+        w = _rollout(mesh, state, frames, mlp=mlp, warm_iterations=k)
+        i_star = equivalent_iterations(dict(pts), w["final_residual"])
+        above = min([it for it, _ in pts if it >= i_star], default=pts[-1][0])
+        gate = plain[above]["max_residual"]
+        g = _rollout(mesh, state, frames, mlp=mlp, warm_iterations=k, threshold=gate)
+        rec = {"ungated": {k2: v for k2, v in w.items() if k2 != "residuals"},
+               "equivalent_plain_iterations": round(i_star, 1),
+               "plain_substeps_per_s_at_equal_residual": round(rate_at(i_star), 1),
+               "speedup_at_equal_residual": round(w["substeps_per_s"] / rate_at(i_star), 3),
+               "gated": {"threshold": gate, "plain_iterations_of_threshold": above,
+                         **{k2: v for k2, v in g.items() if k2 != "residuals"},
+                         "speedup_vs_that_plain": round(g["substeps_per_s"] / plain[above]["substeps_per_s"], 3)}}
+        warm[str(k)] = rec
+        if log:
+            log(f"warmstart: 256x256 x {count}: warm+{k}: residual {w['final_residual']:.4e} = plain I~{i_star:.1f}, "
+                f"{rec['speedup_at_equal_residual']}x at equal residual; gated at plain I={above}'s max "
+                f"({gate:.3e}): {g['fallbacks']}/{g['warm_substeps']} fallbacks, "
+                f"{rec['gated']['speedup_vs_that_plain']}x")
+    return {"workload": f"{count} x 256x256 sheets in one batch (eval_descs seed {seed}), {frames} frames of "
+                        f"8 substeps from a shared plain-solver state", "particles": mesh.num_particles,
+            "plain": {str(it): {k2: v for k2, v in r.items() if k2 not in ("residuals", "warm_substeps", "fallbacks")}
+                      for it, r in plain.items()},
+            "warm": warm}
+
+
 def validation_fallbacks(mlp, count: int = 64, seed: int = 91, iterations: int = 10, warm_iterations: int = 2,
                          frames: int = 6, start_frames: int = 2):
     """Fallbacks of the gated warm start over a short rollout of `count`
