@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+CALIBRATION_BYTES = 1 << 30
 PROFILE_FRAMES = 10
 
 
@@ -123,6 +124,66 @@ def trace_summary(trace_dir: str, range_name: str = "timed frames", frames: int 
             row += f" {t / frames / 1e3:.1f} |"
         lines.append(row)
     return "\n".join(lines)
+
+
+def counter_summary(fetch_dir: str, write_dir: str, kernels=("k_jacobi_tiled<0u", "k_jacobi<0u")):
+    """Bytes per dispatch from two rocprofv3 counter passes (--pmc FETCH_SIZE,
+    then --pmc WRITE_SIZE, -f csv) of `bench.py --kernels-only`.  Both counters
+    are KiB per dispatch; each is scaled by what it reads for the 1 GiB
+    device-to-device calibration copy (the largest runtime copy dispatch),
+    which reads and writes CALIBRATION_BYTES.  Returns (markdown table, the
+    JSON record for the plain Jacobi iteration: the first kernel whose name
+    contains one of `kernels`)."""
+    import csv
+    import glob
+
+    def load(d, counter):
+        f = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not f:
+            raise SystemExit(f"bench.py: no counter CSV under {d}")
+        per = {}
+        with open(f[0]) as fh:
+            for row in csv.DictReader(fh):
+                if row["Counter_Name"] == counter:
+                    per.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+        return per
+
+    fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    copy = [k for k in fetch if "copyBuffer" in k]
+    if not copy:
+        raise SystemExit("bench.py: no calibration copy in the counter CSV")
+    kib = CALIBRATION_BYTES / 1024.0
+    fs = kib / max(max(fetch[k]) for k in copy)
+    ws = kib / max(max(write[k]) for k in copy)
+    rows = []
+    for k in fetch:
+        if k not in write or k.startswith("__amd_rocclr"):
+            continue
+        fk, wk = sum(fetch[k]) / len(fetch[k]), sum(write[k]) / len(write[k])
+        rows.append((k, len(fetch[k]), fk, wk, fk * fs * 1024, wk * ws * 1024))
+    rows.sort(key=lambda r: -(r[4] + r[5]))
+    lines = [f"Calibration: one device-to-device copy of {CALIBRATION_BYTES} B (larger than the 256 MB "
+             "Infinity Cache), read and written once.", "",
+             f"- FETCH_SIZE reads {kib / fs:.1f} KiB for it -> scale {fs:.5f}",
+             f"- WRITE_SIZE reads {kib / ws:.1f} KiB for it -> scale {ws:.5f}", "",
+             "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | scaled read MB | scaled write MB "
+             "| traffic MB |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for k, c, fk, wk, rb, wb in rows:
+        name = k if len(k) <= 60 else k[:60]
+        lines.append(f"| `{name}` | {c} | {fk:.1f} | {wk:.1f} | {rb / 1e6:.1f} | {wb / 1e6:.1f} "
+                     f"| {(rb + wb) / 1e6:.1f} |")
+    rec = None
+    for k, c, fk, wk, rb, wb in rows:
+        if any(x in k for x in kernels):
+            sweep = [r for r in rows if "k_coll_sweep(" in r[0]]
+            rec = {"kernel": k, "fetch_kib_raw": fk, "write_kib_raw": wk, "fetch_scale": fs,
+                   "write_scale": ws, "calibration_bytes": CALIBRATION_BYTES,
+                   "bytes_per_launch": int(round(rb + wb)), "read_bytes": int(round(rb)),
+                   "write_bytes": int(round(wb))}
+            if sweep:
+                rec["collision_sweep_bytes_per_launch"] = int(round(sweep[0][4] + sweep[0][5]))
+            break
+    return "\n".join(lines), rec
 
 
 def cpu_baseline(mesh_fields, u, cfg, budget_s: float, threads: int):
@@ -285,9 +346,17 @@ def main():
                     help="print the per-kernel table of the timed frames from a rocprofv3 CSV trace "
                          "(--kernel-trace --marker-trace -f csv) of an earlier bench.py run, and exit")
     ap.add_argument("--by-grid", action="store_true", help="--trace-summary: one row per kernel and grid size")
+    ap.add_argument("--counter-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="print the bytes-per-dispatch table from FETCH_SIZE and WRITE_SIZE counter passes "
+                         "(-f csv) of `bench.py --kernels-only`, then the Jacobi iteration's JSON record, and exit")
     args = ap.parse_args()
     if args.trace_summary:
         print(trace_summary(args.trace_summary, frames=args.steps, by_grid=args.by_grid))
+        return
+    if args.counter_summary:
+        table, rec = counter_summary(*args.counter_summary)
+        print(table)
+        print(json.dumps(dict(rec or {}, workload=args.workload), indent=1))
         return
     if args.gpus != 1:
         print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
@@ -314,6 +383,22 @@ def main():
         log(f"bench: kernels only: jacobi {jac_ms * 1e3:.1f} us, sweep {sweep_ms * 1e3:.1f} us, "
             f"copy of {32 * n} B {copy_ms * 1e3:.1f} us")
         sim.close()
+        # counter calibration beyond the 256 MB Infinity Cache: one 1 GiB
+        # device-to-device copy (the runtime's copy kernel reads and writes
+        # CALIBRATION_BYTES each)
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        hip.hipFree.argtypes = [ctypes.c_void_p]
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(src), CALIBRATION_BYTES) == 0 and \
+                hip.hipMalloc(ctypes.byref(dst), CALIBRATION_BYTES) == 0:
+            hip.hipMemcpy(dst, src, CALIBRATION_BYTES, 3)   # hipMemcpyDeviceToDevice
+            hip.hipDeviceSynchronize()
+            log(f"bench: calibration copy of {CALIBRATION_BYTES} B done")
+        hip.hipFree(src)
+        hip.hipFree(dst)
         return
     log(f"bench: device state uploaded; warmup {args.warmup} frames")
     sim.step_frames(args.warmup)
