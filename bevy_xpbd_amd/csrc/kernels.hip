@@ -594,6 +594,11 @@ __global__ void k_aabb_of(const float4* P, uint32_t n, float* part) {
 // two cells share a key and different cloths never meet.  Mirrors
 // oracle_cell_grid / oracle_cell_keys / oracle_cell_keys_batch.
 // ---------------------------------------------------------------------------
+// One cloth's grid has at most 2^27 cells, so its keys sort in 3 passes of
+// 9-bit digits (the cell doubles until the grid fits; larger cells only cost
+// sweep time: the pair set is the same).
+constexpr uint64_t ONE_CLOTH_MAX_CELLS = 1ull << 27;
+
 __device__ void grid_from_bounds(float lo[3], float hi[3], float cell_size, uint64_t max_cells, uint32_t key_base,
                                  CollGrid* g) {
     for (int a = 0; a < 3; ++a)
@@ -659,7 +664,7 @@ __global__ __launch_bounds__(1024) void k_coll_grid(const float* part, uint32_t 
             lo[a] = fminf(lo[a], red[w][a]);
             hi[a] = fmaxf(hi[a], red[w][3 + a]);
         }
-    grid_from_bounds(lo, hi, cell_size, 0xFFFFFFFEull, 0u, g);
+    grid_from_bounds(lo, hi, cell_size, ONE_CLOTH_MAX_CELLS, 0u, g);
 }
 
 // Many cloths: one block per cloth reduces its particle range.
@@ -706,28 +711,35 @@ __device__ __forceinline__ void cell_of(float4 p, const CollGrid& g, uint32_t c[
     }
 }
 
-// ---- stable LSD radix sort of (key, id), 8-bit digits, 4 passes ----------
+// ---- stable LSD radix sort of (key, id), DB-bit digits ----------------------
 // A pass is three kernels: per-tile digit counts, an exclusive scan of the
-// counts over tiles (per digit), and a stable scatter.  A tile is 2048 keys;
-// each of its 4 wavefronts owns a contiguous quarter (512 keys, 8 rounds of
+// counts over tiles (per digit), and a stable scatter.  A tile is 4096 keys;
+// each of its 4 wavefronts owns a contiguous quarter (1024 keys, 16 rounds of
 // 64), so ranks in input order are (wavefront, round, lane) and need no block
-// barrier per round.  The count kernels (k_coll_keys' pass-0 counts and
+// barrier per round.  (2048-key tiles left 9-bit digit runs of about 4 keys
+// in the scatter's output: poorly coalesced stores.)  The count kernels (k_coll_keys' pass-0 counts and
 // k_radix_hist) add each key to its wavefront's LDS histogram with an LDS
 // atomic: exact integer counts, independent of the order of the adds.  The
 // scatter ranks lanes stably: lanes with equal digits are found by matching
-// the 8 digit bits with ballots, and one leader per distinct digit updates
+// the DB digit bits with ballots, and one leader per distinct digit updates
 // the wavefront's own counters (distinct addresses, no atomics).
+// Digit widths: 8 (4 passes over 32-bit keys; batches and topology sorts) or
+// 9 (3 passes over the 27-bit keys of one cloth's grid: a pass is three
+// launches of mostly fixed latency, so one pass fewer outweighs the larger
+// per-tile count table).  Thread t of a block owns digits [t*DPT, t*DPT+DPT).
 constexpr uint32_t RADIX_BLOCK = 256;
 constexpr uint32_t RADIX_WAVES = RADIX_BLOCK / 64;
-constexpr uint32_t RADIX_ITEMS = 8;
+constexpr uint32_t RADIX_ITEMS = 16;
 constexpr uint32_t RADIX_TILE = RADIX_BLOCK * RADIX_ITEMS;
 constexpr uint32_t RADIX_WAVE_SPAN = 64 * RADIX_ITEMS;   // keys owned by one wavefront
+constexpr uint32_t RADIX_MAX_DIGITS = 512;
 
 // Lanes (among `valid` ones) whose digit equals this lane's.
+template <uint32_t DB>
 __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (uint32_t b = 0; b < 8; ++b) {
+    for (uint32_t b = 0; b < DB; ++b) {
         const bool bit = (d >> b) & 1u;
         const uint64_t bal = __ballot(bit);
         peers &= bit ? bal : ~bal;
@@ -735,12 +747,36 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
     return peers;
 }
 
+// Block-wide digit counts of the per-wavefront LDS histograms -> hist[d][tile].
+template <uint32_t DB>
+__device__ __forceinline__ void store_tile_counts(uint32_t (*h)[1u << DB], uint32_t* hist, uint32_t ntiles) {
+    constexpr uint32_t DPT = (1u << DB) / RADIX_BLOCK;
+#pragma unroll
+    for (uint32_t k = 0; k < DPT; ++k) {
+        const uint32_t d = threadIdx.x * DPT + k;
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < RADIX_WAVES; ++w) c += h[w][d];
+        hist[size_t(d) * ntiles + blockIdx.x] = c;
+    }
+}
+
+template <uint32_t DB>
+__device__ __forceinline__ void clear_wave_hists(uint32_t (*h)[1u << DB]) {
+    constexpr uint32_t DPT = (1u << DB) / RADIX_BLOCK;
+#pragma unroll
+    for (uint32_t w = 0; w < RADIX_WAVES; ++w)
+#pragma unroll
+        for (uint32_t k = 0; k < DPT; ++k) h[w][threadIdx.x * DPT + k] = 0;
+}
+
+template <uint32_t DB>
 __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const uint32_t* keys, uint32_t n, uint32_t shift,
                                                           uint32_t* hist, uint32_t ntiles) {
-    __shared__ uint32_t h[RADIX_WAVES][256];
+    constexpr uint32_t MASK = (1u << DB) - 1u;
+    __shared__ uint32_t h[RADIX_WAVES][1u << DB];
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-#pragma unroll
-    for (uint32_t w = 0; w < RADIX_WAVES; ++w) h[w][t] = 0;
+    clear_wave_hists<DB>(h);
     __syncthreads();
     const uint32_t base = blockIdx.x * RADIX_TILE + wave * RADIX_WAVE_SPAN + lane;
     uint32_t kk[RADIX_ITEMS];
@@ -752,55 +788,59 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const uint32_t* keys
 #pragma unroll
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
         const bool valid = base + r * 64 < n;
-        const uint32_t d = (kk[r] >> shift) & 255u;
+        const uint32_t d = (kk[r] >> shift) & MASK;
         if (valid) atomicAdd(&h[wave][d], 1u);   // LDS atomics: cheaper here than ranking the wave
     }
     __syncthreads();
-    uint32_t c = 0;
-#pragma unroll
-    for (uint32_t w = 0; w < RADIX_WAVES; ++w) c += h[w][t];
-    hist[t * ntiles + blockIdx.x] = c;
+    store_tile_counts<DB>(h, hist, ntiles);
 }
 
 // Cell keys of the current positions, laid out as the sort's tiles (one
-// block per 2048 keys), with the sort's first per-tile digit counts (pass 0,
-// the low 8 bits) taken on the way, so that pass needs no histogram launch.
+// block per 4096 keys), with the sort's first per-tile digit counts (pass 0,
+// the low DB bits) taken on the way, so that pass needs no histogram launch.
 // Also empties the group-table slots the previous collision pass filled
 // (slot_rec holds, per sorted index, the slot its group was inserted at, or
 // EMPTY_KEY): the table is empty again before this pass inserts.
+template <uint32_t DB>
 __global__ __launch_bounds__(RADIX_BLOCK) void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grids,
                                                          const uint32_t* cloth_of, uint32_t* keys, uint32_t* ids,
                                                          uint2* table, const uint32_t* slot_rec, uint32_t* hist,
                                                          uint32_t ntiles) {
-    __shared__ uint32_t h[RADIX_WAVES][256];
+    constexpr uint32_t MASK = (1u << DB) - 1u;
+    __shared__ uint32_t h[RADIX_WAVES][1u << DB];
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-#pragma unroll
-    for (uint32_t w = 0; w < RADIX_WAVES; ++w) h[w][t] = 0;
+    clear_wave_hists<DB>(h);
     __syncthreads();
     const uint32_t base = blockIdx.x * RADIX_TILE + wave * RADIX_WAVE_SPAN + lane;
+    // every load of the thread's rounds is issued before the first store (the
+    // table stores could otherwise alias them and serialise the rounds)
+    const uint32_t last = n - 1;
+    float4 f[RADIX_ITEMS];
+    uint32_t sl[RADIX_ITEMS], co[RADIX_ITEMS];
+#pragma unroll
+    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+        const uint32_t i = min(base + r * 64, last);
+        f[r] = F[i];
+        sl[r] = base + r * 64 < n ? slot_rec[i] : EMPTY_KEY;
+        co[r] = cloth_of ? cloth_of[i] : 0u;
+    }
+    const CollGrid g0 = grids[0];
 #pragma unroll
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
         const uint32_t i = base + r * 64;
-        const bool valid = i < n;
-        uint32_t key = 0;
-        if (valid) {
-            const uint32_t sl = slot_rec[i];
-            if (sl != EMPTY_KEY) table[sl] = make_uint2(EMPTY_KEY, EMPTY_KEY);
-            const CollGrid& g = grids[cloth_of ? cloth_of[i] : 0u];
+        if (sl[r] != EMPTY_KEY) table[sl[r]] = make_uint2(EMPTY_KEY, EMPTY_KEY);
+        if (i < n) {
+            const CollGrid g = cloth_of ? grids[co[r]] : g0;
             uint32_t c[3];
-            cell_of(F[i], g, c);
-            key = g.key_base + (c[0] + g.dims[0] * (c[1] + g.dims[1] * c[2]));
+            cell_of(f[r], g, c);
+            const uint32_t key = g.key_base + (c[0] + g.dims[0] * (c[1] + g.dims[1] * c[2]));
             keys[i] = key;
             ids[i] = i;
+            atomicAdd(&h[wave][key & MASK], 1u);   // LDS atomics: cheaper here than ranking the wave
         }
-        const uint32_t d = key & 255u;
-        if (valid) atomicAdd(&h[wave][d], 1u);   // LDS atomics: cheaper here than ranking the wave
     }
     __syncthreads();
-    uint32_t cnt = 0;
-#pragma unroll
-    for (uint32_t w = 0; w < RADIX_WAVES; ++w) cnt += h[w][t];
-    hist[t * ntiles + blockIdx.x] = cnt;
+    store_tile_counts<DB>(h, hist, ntiles);
 }
 
 // Exclusive scan of 256 values held one per thread (block of 256).
@@ -844,20 +884,30 @@ __global__ __launch_bounds__(256) void k_radix_scan_tiles(uint32_t* hist, uint32
 // Stable scatter: the tile is ranked in input order (see above), placed in
 // LDS in digit order, and written out in that order, so each digit's run of
 // the tile leaves as contiguous stores.
+template <uint32_t DB>
 __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* kin, const uint32_t* iin,
                                                              uint32_t* kout, uint32_t* iout, uint32_t n,
                                                              uint32_t shift, const uint32_t* hist,
                                                              uint32_t ntiles, const uint32_t* dtot) {
-    __shared__ uint32_t wcnt[RADIX_WAVES][256];   // per-wavefront digit counts, then its start in the tile
-    __shared__ int32_t gbase[256];                 // global position of local sorted position 0 of digit d
+    constexpr uint32_t DIGITS = 1u << DB, MASK = DIGITS - 1u, DPT = DIGITS / RADIX_BLOCK;
+    __shared__ uint32_t wcnt[RADIX_WAVES][DIGITS];   // per-wavefront digit counts, then its start in the tile
+    __shared__ int32_t gbase[DIGITS];                 // global position of local sorted position 0 of digit d
     __shared__ uint32_t skey[RADIX_TILE];
     __shared__ uint32_t sid[RADIX_TILE];
     __shared__ uint32_t lds[4];
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t tile0 = blockIdx.x * RADIX_TILE;
     const uint32_t count = min(RADIX_TILE, n - tile0);
+    // this thread's digits' tile offsets and totals, loaded first: their
+    // latency then hides behind the key loads and the ranking
+    uint32_t hv[DPT], dv[DPT];
 #pragma unroll
-    for (uint32_t w = 0; w < RADIX_WAVES; ++w) wcnt[w][t] = 0;
+    for (uint32_t k = 0; k < DPT; ++k) {
+        const uint32_t d = t * DPT + k;
+        hv[k] = hist[size_t(d) * ntiles + blockIdx.x];
+        dv[k] = dtot[d];
+    }
+    clear_wave_hists<DB>(wcnt);
     __syncthreads();
     const uint32_t base = wave * RADIX_WAVE_SPAN + lane;   // tile-local index of round 0
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -871,35 +921,45 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
 #pragma unroll
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
         const bool valid = base + r * 64 < count;
-        const uint32_t d = (kk[r] >> shift) & 255u;
-        const uint64_t peers = digit_peers(d, valid);
+        const uint32_t d = (kk[r] >> shift) & MASK;
+        const uint64_t peers = digit_peers<DB>(d, valid);
         const uint32_t before = wcnt[wave][d];   // read by every lane before the leader's update
         rk[r] = before + uint32_t(__popcll(peers & lt));
         if (valid && (peers & lt) == 0) wcnt[wave][d] = before + uint32_t(__popcll(peers));
     }
     __syncthreads();
-    // digit t: tile-local start, then each wavefront's start within it
-    uint32_t c[RADIX_WAVES], tot_t = 0;
+    // this thread's digits: tile-local starts, then each wavefront's start
+    // within them
+    uint32_t c[DPT][RADIX_WAVES], tot_t = 0, dsum = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < RADIX_WAVES; ++w) {
-        c[w] = wcnt[w][t];
-        tot_t += c[w];
+    for (uint32_t k = 0; k < DPT; ++k) {
+        const uint32_t d = t * DPT + k;
+#pragma unroll
+        for (uint32_t w = 0; w < RADIX_WAVES; ++w) {
+            c[k][w] = wcnt[w][d];
+            tot_t += c[k][w];
+        }
+        dsum += dv[k];
     }
     uint32_t tot;
-    const uint32_t lstart = block_exclusive_scan_256(tot_t, lds, tot);
-    const uint32_t digit_base = block_exclusive_scan_256(dtot[t], lds, tot);
-    gbase[t] = int32_t(digit_base + hist[t * ntiles + blockIdx.x]) - int32_t(lstart);
-    uint32_t run = lstart;
+    uint32_t lstart = block_exclusive_scan_256(tot_t, lds, tot);
+    uint32_t digit_base = block_exclusive_scan_256(dsum, lds, tot);
 #pragma unroll
-    for (uint32_t w = 0; w < RADIX_WAVES; ++w) {
-        wcnt[w][t] = run;
-        run += c[w];
+    for (uint32_t k = 0; k < DPT; ++k) {
+        const uint32_t d = t * DPT + k;
+        gbase[d] = int32_t(digit_base + hv[k]) - int32_t(lstart);
+        digit_base += dv[k];
+#pragma unroll
+        for (uint32_t w = 0; w < RADIX_WAVES; ++w) {
+            wcnt[w][d] = lstart;
+            lstart += c[k][w];
+        }
     }
     __syncthreads();
 #pragma unroll
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
         if (base + r * 64 < count) {
-            const uint32_t pos = wcnt[wave][(kk[r] >> shift) & 255u] + rk[r];
+            const uint32_t pos = wcnt[wave][(kk[r] >> shift) & MASK] + rk[r];
             skey[pos] = kk[r];
             sid[pos] = ii[r];
         }
@@ -910,7 +970,7 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
         const uint32_t q = r * RADIX_BLOCK + t;
         if (q < count) {
             const uint32_t key = skey[q];
-            const uint32_t dst = uint32_t(gbase[(key >> shift) & 255u] + int32_t(q));
+            const uint32_t dst = uint32_t(gbase[(key >> shift) & MASK] + int32_t(q));
             kout[dst] = key;
             iout[dst] = sid[q];
         }
@@ -1985,32 +2045,52 @@ void launch_coll_grids(const float4* P, const uint2* span, uint32_t nspans, cons
 uint32_t cloth_span_max() { return CB_SPAN; }
 
 void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
-                      uint32_t* ids, uint2* table, const uint32_t* slot_rec, uint32_t* hist, hipStream_t s) {
+                      uint32_t* ids, uint2* table, const uint32_t* slot_rec, uint32_t* hist, uint32_t digit_bits,
+                      hipStream_t s) {
     if (n == 0) return;
     const uint32_t ntiles = std::max(1u, cdiv(n, RADIX_TILE));
-    hipLaunchKernelGGL(k_coll_keys, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, F, n, grids, cloth_of, keys, ids, table,
-                       slot_rec, hist, ntiles);
+    if (digit_bits == 9)
+        hipLaunchKernelGGL(k_coll_keys<9>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, F, n, grids, cloth_of, keys, ids,
+                           table, slot_rec, hist, ntiles);
+    else
+        hipLaunchKernelGGL(k_coll_keys<8>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, F, n, grids, cloth_of, keys, ids,
+                           table, slot_rec, hist, ntiles);
 }
 
-uint32_t radix_hist_words(uint32_t n) { return 256u * std::max(1u, cdiv(n, RADIX_TILE)) + 256u; }
+uint32_t radix_hist_words(uint32_t n) {
+    return RADIX_MAX_DIGITS * std::max(1u, cdiv(n, RADIX_TILE)) + RADIX_MAX_DIGITS;
+}
 
 void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
-                       uint32_t shift, uint32_t* hist, hipStream_t s, bool hist_done) {
+                       uint32_t shift, uint32_t* hist, hipStream_t s, bool hist_done, uint32_t digit_bits) {
     const uint32_t ntiles = std::max(1u, cdiv(n, RADIX_TILE));
-    uint32_t* dtot = hist + 256u * ntiles;
-    if (!hist_done) hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, n, shift, hist, ntiles);
-    hipLaunchKernelGGL(k_radix_scan_tiles, dim3(256), dim3(256), 0, s, hist, ntiles, dtot);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, iin, kout, iout, n, shift, hist,
-                       ntiles, dtot);
+    const uint32_t digits = 1u << digit_bits;
+    uint32_t* dtot = hist + size_t(digits) * ntiles;
+    if (digit_bits == 9) {
+        if (!hist_done)
+            hipLaunchKernelGGL(k_radix_hist<9>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, n, shift, hist, ntiles);
+        hipLaunchKernelGGL(k_radix_scan_tiles, dim3(digits), dim3(256), 0, s, hist, ntiles, dtot);
+        hipLaunchKernelGGL(k_radix_scatter<9>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, iin, kout, iout, n, shift,
+                           hist, ntiles, dtot);
+    } else {
+        if (!hist_done)
+            hipLaunchKernelGGL(k_radix_hist<8>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, n, shift, hist, ntiles);
+        hipLaunchKernelGGL(k_radix_scan_tiles, dim3(digits), dim3(256), 0, s, hist, ntiles, dtot);
+        hipLaunchKernelGGL(k_radix_scatter<8>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, iin, kout, iout, n, shift,
+                           hist, ntiles, dtot);
+    }
 }
 
-void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,
-                       uint32_t* hist, hipStream_t s, bool first_hist_done) {
-    if (n == 0) return;
-    launch_radix_pass(keys, ids, keys_alt, ids_alt, n, 0, hist, s, first_hist_done);
-    launch_radix_pass(keys_alt, ids_alt, keys, ids, n, 8, hist, s);
-    launch_radix_pass(keys, ids, keys_alt, ids_alt, n, 16, hist, s);
-    launch_radix_pass(keys_alt, ids_alt, keys, ids, n, 24, hist, s);
+bool launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,
+                       uint32_t* hist, hipStream_t s, bool first_hist_done, uint32_t digit_bits) {
+    const uint32_t passes = digit_bits == 9 ? 3u : 4u;
+    if (n == 0) return false;
+    for (uint32_t p = 0; p < passes; ++p) {
+        const bool odd = p & 1u;
+        launch_radix_pass(odd ? keys_alt : keys, odd ? ids_alt : ids, odd ? keys : keys_alt, odd ? ids : ids_alt, n,
+                          p * digit_bits, hist, s, p == 0 && first_hist_done, digit_bits);
+    }
+    return passes & 1u;
 }
 
 void launch_coll_gather(const float4* F, const uint32_t* sids, const uint32_t* skeys, uint32_t n, float4* sP,

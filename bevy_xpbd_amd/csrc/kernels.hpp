@@ -182,13 +182,19 @@ uint32_t cloth_span_max();   // particles per piece at most
 // cell keys (and ids = identity) plus the sort's pass-0 tile counts in hist;
 // also empties the group-table slots the previous pass recorded in slot_rec
 void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
-                      uint32_t* ids, uint2* table, const uint32_t* slot_rec, uint32_t* hist, hipStream_t s);
-// first_hist_done: hist already holds pass 0's tile counts (launch_coll_keys)
-void launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,
-                       uint32_t* hist, hipStream_t s, bool first_hist_done = false);
+                      uint32_t* ids, uint2* table, const uint32_t* slot_rec, uint32_t* hist, uint32_t digit_bits,
+                      hipStream_t s);
+// Stable sort of (keys, ids) by key: 4 passes of 8-bit digits (32-bit keys) or
+// 3 passes of 9-bit digits (keys below 2^27: one cloth's grid).  Returns true
+// when the result is in (keys_alt, ids_alt) (an odd pass count), false when it
+// is in (keys, ids).  first_hist_done: hist already holds pass 0's tile counts
+// (launch_coll_keys with the same digit_bits).
+bool launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,
+                       uint32_t* hist, hipStream_t s, bool first_hist_done, uint32_t digit_bits);
 uint32_t radix_hist_words(uint32_t n);
 void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
-                       uint32_t shift, uint32_t* hist, hipStream_t s, bool hist_done = false);
+                       uint32_t shift, uint32_t* hist, hipStream_t s, bool hist_done = false,
+                       uint32_t digit_bits = 8);
 // the sorted copy for the sweep (sP = position, key bits), group-table
 // inserts (slots recorded in slot_rec, n words; the table must be empty on
 // entry), overflow count clear

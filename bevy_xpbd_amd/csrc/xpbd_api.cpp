@@ -103,6 +103,7 @@ struct xpbd_cloth {
     uint32_t ncspans = 0;
     CollGrid* grids = nullptr;     // one per cloth
     uint32_t nclothes = 1, local_bits = 32;
+    uint32_t coll_digit_bits = 8;  // the collision sort's digits: 9 (3 passes) for one cloth, else 8
     uint32_t* cloth_of = nullptr;  // particle -> cloth (nullptr for one cloth)
     uint32_t* cloth_off = nullptr; // C + 1 vertex offsets
     uint32_t *tris = nullptr, *vt_off = nullptr, *vt_tri = nullptr;
@@ -284,10 +285,11 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
                        float4* vout) {
     hipStream_t s = h->stream;
     enqueue_grids(h, F, nparts, coll_cell(h), s);
-    launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist, s);
-    launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true);
-    launch_coll_gather(F, h->ids, h->keys, h->n, h->sP, h->ovf, h->table, h->table_slots - 1,
-                       h->slot_rec, s);
+    launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist,
+                     h->coll_digit_bits, s);
+    const bool alt = launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true, h->coll_digit_bits);
+    launch_coll_gather(F, alt ? h->ids2 : h->ids, alt ? h->keys2 : h->keys, h->n, h->sP, h->ovf, h->table,
+                       h->table_slots - 1, h->slot_rec, s);
     CollArgs c{};
     c.F = F;
     c.sP = h->sP;
@@ -298,8 +300,8 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     c.ground = h->has_ground ? h->ground : nullptr;
     c.grids = h->grids;
     c.cloth_of = h->cloth_of;
-    c.skeys = h->keys;
-    c.sids = h->ids;
+    c.skeys = alt ? h->keys2 : h->keys;
+    c.sids = alt ? h->ids2 : h->ids;
     c.table = h->table;
     c.table_mask = h->table_slots - 1;
     c.p = p;
@@ -833,6 +835,7 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
         uint32_t bits = 0;
         while ((1u << bits) < h->nclothes) ++bits;
         h->local_bits = h->nclothes <= 1 ? 32u : 32u - bits;
+        h->coll_digit_bits = h->nclothes <= 1 ? 9u : 8u;
         TRY(dalloc(h, &h->grids, h->nclothes));
         if (h->nclothes > 1) {
             std::vector<uint32_t> cof(n);
@@ -1584,13 +1587,15 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         a.tame_in = tame_slot(h, h->A);
         a.tame_out = tame_slot(h, h->B);
     }
+    bool alt = false;   // the sorted (key, id) pairs are in keys2 / ids2
     auto prep_collision = [&]() {
         const uint32_t np = h->nclothes == 1 ? launch_aabb_of(h->A, h->n, h->aabb, s) : 0u;
         enqueue_grids(h, h->A, np, coll_cell(h), s);
-        launch_coll_keys(h->A, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist, s);
-        launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true);
-        launch_coll_gather(h->A, h->ids, h->keys, h->n, h->sP, h->ovf, h->table, h->table_slots - 1,
-                           h->slot_rec, s);
+        launch_coll_keys(h->A, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist,
+                         h->coll_digit_bits, s);
+        alt = launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true, h->coll_digit_bits);
+        launch_coll_gather(h->A, alt ? h->ids2 : h->ids, alt ? h->keys2 : h->keys, h->n, h->sP, h->ovf, h->table,
+                           h->table_slots - 1, h->slot_rec, s);
     };
     CollArgs c{};
     if (which == XPBD_KERNEL_COLLISION_SWEEP || which == XPBD_KERNEL_RADIX_PASS) {
@@ -1604,8 +1609,8 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         c.ground = h->has_ground ? h->ground : nullptr;
         c.grids = h->grids;
         c.cloth_of = h->cloth_of;
-        c.skeys = h->keys;
-        c.sids = h->ids;
+        c.skeys = alt ? h->keys2 : h->keys;
+        c.sids = alt ? h->ids2 : h->ids;
         c.table = h->table;
         c.table_mask = h->table_slots - 1;
         c.p = p;
@@ -1646,7 +1651,7 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
                 launch_coll_sweep(c, s);
                 break;
             case XPBD_KERNEL_RADIX_PASS:
-                launch_radix_pass(h->keys, h->ids, h->keys2, h->ids2, h->n, 0, h->hist, s);
+                launch_radix_pass(h->keys, h->ids, h->keys2, h->ids2, h->n, 0, h->hist, s, false, h->coll_digit_bits);
                 break;
             case XPBD_KERNEL_GS_SUBSTEP:
                 launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
@@ -2087,13 +2092,14 @@ XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t*
     hipStream_t s = h->stream;
     const uint32_t np = h->nclothes == 1 ? launch_aabb_of(h->X, h->n, h->aabb, s) : 0u;
     enqueue_grids(h, h->X, np, cell_size, s);
-    launch_coll_keys(h->X, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist, s);
-    launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true);
+    launch_coll_keys(h->X, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist,
+                     h->coll_digit_bits, s);
+    const bool alt = launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true, h->coll_digit_bits);
     HIPC(h, hipGetLastError());
     HIPC(h, hipStreamSynchronize(s));
     HIPC(h, hipMemcpy(grid_out, h->grids, size_t(h->nclothes) * sizeof(CollGrid), hipMemcpyDeviceToHost));
-    HIPC(h, hipMemcpy(sorted_keys, h->keys, size_t(h->n) * 4, hipMemcpyDeviceToHost));
-    HIPC(h, hipMemcpy(sorted_ids, h->ids, size_t(h->n) * 4, hipMemcpyDeviceToHost));
+    HIPC(h, hipMemcpy(sorted_keys, alt ? h->keys2 : h->keys, size_t(h->n) * 4, hipMemcpyDeviceToHost));
+    HIPC(h, hipMemcpy(sorted_ids, alt ? h->ids2 : h->ids, size_t(h->n) * 4, hipMemcpyDeviceToHost));
     return XPBD_OK;
 }
 

@@ -1356,10 +1356,10 @@ static void cell_grid_capped(const float* pos4, size_t b, size_t e, float cell_s
 // collision_flat_packed, cloth_sim_jacobi.wgsl:243-253, clamps into a capped
 // rest-pose grid instead).  Per substep: axis-aligned bounds of the current
 // positions; cell = cell_size * 65/64; the cell doubles until every axis has
-// fewer than 16383 cells and the cell count fits the key (2^32 - 2 cells for
-// one cloth).
+// fewer than 16383 cells and the cell count fits the key (2^27 cells for
+// one cloth, so its keys sort in 3 passes of 9-bit digits).
 void oracle_cell_grid(const float* pos4, uint32_t n, float cell_size, om_grid* g) {
-    cell_grid_capped(pos4, 0, n, cell_size, 0xFFFFFFFEull, g);
+    cell_grid_capped(pos4, 0, n, cell_size, 1ull << 27, g);
 }
 
 void oracle_cell_keys(const float* pos4, uint32_t n, const om_grid* g, uint32_t* keys) {

@@ -247,6 +247,32 @@ def test_collision_keys_and_sort_bit_exact(O, X):
     sim.close()
 
 
+def test_collision_sort_wide_keys_many_tiles(O, X):
+    """One cloth's keys sort in 3 passes of 9-bit digits: 90,000 particles
+    (44 sort tiles) scattered through a cube whose grid nearly fills the
+    2^27-cell key space, so every digit of every pass varies; and the same
+    particles packed into a few cells (long equal-key runs)."""
+    note("case wide keys")
+    mesh = X.ClothMeshData.sheet(cols=300, rows=300, spacing=0.01, pins=0, jitter=0.0, seed=1)
+    sim, _, _ = make(X, mesh)
+    pos, _ = sim.read_state()
+    rng = np.random.default_rng(12)
+    for cell, spread in ((0.00215, 1.0), (0.1, 0.15)):
+        pos[:, :3] = rng.uniform(0.0, spread, (len(pos), 3)).astype(np.float32)
+        sim.write_state(pos)
+        g, keys, ids = sim.debug_collision_keys(cell)
+        og = O.cell_grid(pos, cell)
+        assert list(og.dims) == list(g["dims"])
+        cells = int(np.prod(np.asarray(g["dims"], np.int64)))
+        assert cells <= 2**27
+        ok = O.cell_keys(pos, og)
+        if spread == 1.0:
+            assert cells > 2**26 and int(ok.max()) >= 2**26   # the top digit is in use
+        sk, si = O.sort_pairs(ok)
+        assert (keys == sk).all() and (ids == si).all()
+    sim.close()
+
+
 @pytest.mark.parametrize("bound", [False, True])
 def test_publish_layout_and_normals(O, X, bound):
     """finalize_normals / render_positions layout (cloth_sim_jacobi.wgsl:527-547):

@@ -155,7 +155,7 @@ def test_cell_keys_are_dense_and_neighbourhood_complete(O):
     g = O.cell_grid(pos, cell)
     keys = O.cell_keys(pos, g)
     gx, gy, gz = g.dims
-    assert gx * gy * gz < 2**32 and max(g.dims) < 16383
+    assert gx * gy * gz <= 2**27 and max(g.dims) < 16383
     cx, cy, cz = keys % gx, (keys // gx) % gy, keys // (gx * gy)
     d = np.linalg.norm(pos[:, None, :3] - pos[None, :, :3], axis=2)
     ii, jj = np.nonzero(d <= cell)
@@ -165,6 +165,19 @@ def test_cell_keys_are_dense_and_neighbourhood_complete(O):
     # dense key: decodes back to the cell of the position
     q = np.floor((pos[:, :3] - np.array(g.lo[:], np.float32)) * np.float32(g.inv_cell))
     assert (cx == np.clip(q[:, 0], 0, gx - 1)).all()
+
+
+def test_cell_grid_doubles_until_the_one_cloth_key_fits_27_bits(O):
+    """Bounds 1000 x 1000 x 200 with cell_size 1: the first cell, 1 * 65/64,
+    gives 985 x 985 x 197 = 191,136,325 cells > 2^27 (every axis is under
+    16383, so only the cell count forces a doubling); at 2 * 65/64 the grid is
+    493 x 493 x 99 = 24,062,331 cells, which fits."""
+    pos = np.zeros((2, 4), np.float32)
+    pos[1, :3] = [1000.0, 1000.0, 200.0]
+    g = O.cell_grid(pos, 1.0)
+    assert g.cell == np.float32(2.03125)
+    assert list(g.dims) == [493, 493, 99]
+    assert 985 * 985 * 197 > 2**27 >= 493 * 493 * 99
 
 
 def test_vertex_normals_flat_sheet_is_plus_z(O):
