@@ -589,7 +589,8 @@ __global__ void k_aabb_of(const float4* P, uint32_t n, float* part) {
 // cell side is cell_size * 65/64 (a guard so f32 rounding in the key can never
 // put two particles within cell_size two cells apart), doubled until each axis
 // has < 16383 cells and the cloth's cell count fits its key range
-// (2^(32-b) - 2 cells, b = bits numbering the cloths; b = 0 for one cloth).
+// (2^27 cells for one cloth; 2^(32-b) - 2 cells in a batch, b = bits
+// numbering the cloths).
 // Key = (cloth << (32-b)) + ix + gx * (iy + gy * iz): dense per cloth, so no
 // two cells share a key and different cloths never meet.  Mirrors
 // oracle_cell_grid / oracle_cell_keys / oracle_cell_keys_batch.
@@ -1126,9 +1127,24 @@ __device__ __forceinline__ float clamp_apply(float v) {
     return v;
 }
 
+// Sorted chunk swept by block b.  Blocks reach the XCDs round-robin (block b
+// on XCD b % 8), so with chunk = b every XCD's L2 fetches nearly every row of
+// the sorted array that a neighbourhood touches.  With seg > 0, XCD x sweeps
+// segments x, x + 8, x + 16, ... of `seg` consecutive chunks: neighbouring
+// rows stay in one L2, and contact clusters still spread over the XCDs.
+// A bijection on [0, nb): blocks past the last whole round of 8 segments keep
+// chunk = b.
+__device__ __forceinline__ uint32_t sweep_chunk(uint32_t b, uint32_t nb, uint32_t seg) {
+    if (seg == 0) return b;
+    const uint32_t whole = nb / (8u * seg) * (8u * seg);
+    if (b >= whole) return b;
+    const uint32_t x = b & 7u, l = b >> 3;
+    return ((l / seg) * 8u + x) * seg + l % seg;
+}
+
 __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     const uint32_t n = a.p.n;
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = sweep_chunk(blockIdx.x, gridDim.x, a.xcd_seg) * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const float4 fi = a.sP[s];    // xyz, w = cell key bits
     const uint32_t i = a.sids[s];

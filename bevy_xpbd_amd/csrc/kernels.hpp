@@ -138,7 +138,13 @@ struct CollArgs {
     // particles with more than LMAX candidates, resolved by a second kernel:
     // [0] count (cleared by launch_coll_gather), [2..] sorted indices; n + 2 words
     uint32_t* ovf;
+    uint32_t xcd_seg;   // sweep blocks per XCD segment (0: block b sweeps chunk b)
 };
+
+// Sweep segment length in blocks of 256 particles.  On the 1024^2 sheet 16
+// cut the sweep's fetched bytes from 223 to 133 MB per launch at the same
+// time (4..64 all within 1% of each other).
+constexpr uint32_t SWEEP_XCD_SEG = 16;
 
 // Words of the grid-barrier state of a persistent launch (zeroed before each
 // launch with launch_zero_u32).

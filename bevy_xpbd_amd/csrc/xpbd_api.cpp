@@ -306,6 +306,7 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     c.table_mask = h->table_slots - 1;
     c.p = p;
     c.ovf = h->ovf;
+    c.xcd_seg = SWEEP_XCD_SEG;
     ProbeScope ps(h, XPBD_KERNEL_COLLISION_SWEEP);
     launch_coll_sweep(c, s);
 }
@@ -1615,6 +1616,7 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         c.table_mask = h->table_slots - 1;
         c.p = p;
         c.ovf = h->ovf;
+    c.xcd_seg = SWEEP_XCD_SEG;
     }
     GsArgs g{};
     if (which == XPBD_KERNEL_GS_SUBSTEP) {

@@ -4,6 +4,9 @@
   bit-exact against the f32 oracle's frozen trajectory (tests/golden), and
   within the stated trajectory tolerance of the f64 oracle (1e-4 m; the f32
   solver itself is 1.8e-5 m away after 120 frames).
+- The bench's 1024x1024 sheet in the bench's configuration, through its 5
+  warmup and 60 timed frames: bit-exact against the f32 oracle's frozen
+  trajectory at frames 5, 25 and 65.
 - 256x256 horizontal sheet (config 2), both solvers, 10 frames; the
   512-cloth batch (config 4), one frame: bit-exact against the f32 oracle.
 - The error against the f64 oracle is printed for each config run here.
@@ -74,6 +77,41 @@ def test_config3_one_frame_error_against_f64(O, X):
     x64, _ = s.frames(f["positions"].astype(np.float64), np.zeros(f["positions"].shape), p, 8, 1, 1,
                       dtype=np.float64)
     note(f"  config3 frame 1: max |GPU f32 - oracle f64| = {max_abs(gp, x64):.3e} m")
+    sim.close()
+
+
+def test_bench_sheet_65_frames_match_the_frozen_oracle(X):
+    """The bench's own workload and configuration (bench.build_workload
+    "sheet1024": 1024x1024 self-colliding sheet, Jacobi 8x10, collision every
+    substep, hipGraph replay, autotuned launch shapes), stepped through the
+    bench's 5 warmup and 60 timed frames: positions and velocities bit-exact
+    against the f32 oracle's frozen trajectory (tests/golden/make_fixtures.py
+    --sheet1024) at frames 5, 25 and 65, and the residual the bench reports
+    equal to the oracle's."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    import bench
+    want = json.load(open(os.path.join(HERE, "golden", "oracle_sheet1024.json")))
+    note("case bench sheet 65 frames")
+    mesh, thickness, _, _ = bench.build_workload(X, "sheet1024")
+    assert np.float32(thickness) == np.float32(want["thickness"])
+    cfg = X.ClothSimConfig.default(solve_substeps=8, solve_inner_iterations=10, use_graph=1, autotune=1)
+    u = X.ClothSimUniforms.default(X.SOLVER_JACOBI, 8, thickness=float(thickness))
+    sim = X.ClothComputePlugin(mesh, cfg, u)
+    done = 0
+    for frames in (5, 25, 65):
+        sim.step_frames(frames - done)
+        done = frames
+        gp, gv = sim.read_state()
+        w = want["frames"][str(frames)]
+        dp = hashlib.sha256(np.ascontiguousarray(gp[:, :3], np.float32).tobytes()).hexdigest()
+        dv = hashlib.sha256(np.ascontiguousarray(gv[:, :3], np.float32).tobytes()).hexdigest()
+        note(f"  frame {frames}: positions {'equal' if dp == w['f32_sha256'] else 'DIFFER'}, "
+             f"velocities {'equal' if dv == w['f32_vel_sha256'] else 'DIFFER'}")
+        assert dp == w["f32_sha256"] and dv == w["f32_vel_sha256"], f"frame {frames}"
+        # same positions, f64 sums in a different order: the residual test's
+        # tolerance (test_gpu_parity.test_residual_matches_oracle)
+        r = sim.residual()
+        assert abs(r - w["residual"]) <= 1e-9 * abs(w["residual"]), (frames, r, w["residual"])
     sim.close()
 
 
