@@ -637,37 +637,6 @@ __device__ void grid_from_bounds(float lo[3], float hi[3], float cell_size, uint
     g->key_base = key_base;
 }
 
-// One cloth: reduce the per-block bounds partials of the producing kernel.
-__global__ __launch_bounds__(1024) void k_coll_grid(const float* part, uint32_t nparts, CollGrid* g,
-                                                     float cell_size) {
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x)
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = fminf(lo[a], part[size_t(k) * 8 + a]);
-            hi[a] = fmaxf(hi[a], part[size_t(k) * 8 + 3 + a]);
-        }
-    __shared__ float red[16][6];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
-            hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
-        }
-    if ((threadIdx.x & 63) == 0)
-        for (int a = 0; a < 3; ++a) {
-            red[threadIdx.x >> 6][a] = lo[a];
-            red[threadIdx.x >> 6][3 + a] = hi[a];
-        }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (uint32_t w = 1; w < (blockDim.x >> 6); ++w)
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = fminf(lo[a], red[w][a]);
-            hi[a] = fmaxf(hi[a], red[w][3 + a]);
-        }
-    grid_from_bounds(lo, hi, cell_size, ONE_CLOTH_MAX_CELLS, 0u, g);
-}
-
 // Many cloths: one block per cloth reduces its particle range.
 // Per-cloth bounds of a batch in balanced pieces: block b reduces particles
 // [span[b].x, span[b].y) of one cloth (at most CB_SPAN of them) into partial
@@ -802,16 +771,22 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const uint32_t* keys
 // Also empties the group-table slots the previous collision pass filled
 // (slot_rec holds, per sorted index, the slot its group was inserted at, or
 // EMPTY_KEY): the table is empty again before this pass inserts.
+// One cloth (part != nullptr): every block derives the grid itself from the
+// nparts bounds partials of the producing kernel (min / max are exact and
+// order-free, so all blocks agree), and block 0 stores it for the sweep;
+// this replaces a one-block grid launch.  Batches read grids[] (k_coll_grids).
 template <uint32_t DB>
 __global__ __launch_bounds__(RADIX_BLOCK) void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grids,
                                                          const uint32_t* cloth_of, uint32_t* keys, uint32_t* ids,
                                                          uint2* table, const uint32_t* slot_rec, uint32_t* hist,
-                                                         uint32_t ntiles) {
+                                                         uint32_t ntiles, const float* part, uint32_t nparts,
+                                                         float cell_size, CollGrid* grid_out) {
     constexpr uint32_t MASK = (1u << DB) - 1u;
     __shared__ uint32_t h[RADIX_WAVES][1u << DB];
+    __shared__ float red[RADIX_WAVES][6];
+    __shared__ CollGrid sgrid;
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     clear_wave_hists<DB>(h);
-    __syncthreads();
     const uint32_t base = blockIdx.x * RADIX_TILE + wave * RADIX_WAVE_SPAN + lane;
     // every load of the thread's rounds is issued before the first store (the
     // table stores could otherwise alias them and serialise the rounds)
@@ -825,7 +800,43 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_coll_keys(const float4* F, uint
         sl[r] = base + r * 64 < n ? slot_rec[i] : EMPTY_KEY;
         co[r] = cloth_of ? cloth_of[i] : 0u;
     }
-    const CollGrid g0 = grids[0];
+    if (part) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        const float4* p4 = reinterpret_cast<const float4*>(part);   // (lo xyz, hi x), (hi yz, -, -) per partial
+        for (uint32_t k = t; k < nparts; k += RADIX_BLOCK) {
+            const float4 a = p4[2 * k], b = p4[2 * k + 1];
+            lo[0] = fminf(lo[0], a.x);
+            lo[1] = fminf(lo[1], a.y);
+            lo[2] = fminf(lo[2], a.z);
+            hi[0] = fmaxf(hi[0], a.w);
+            hi[1] = fmaxf(hi[1], b.x);
+            hi[2] = fmaxf(hi[2], b.y);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+                hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+            }
+        if (lane == 0)
+            for (int a = 0; a < 3; ++a) {
+                red[wave][a] = lo[a];
+                red[wave][3 + a] = hi[a];
+            }
+        __syncthreads();
+        if (t == 0) {
+            for (uint32_t w = 1; w < RADIX_WAVES; ++w)
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = fminf(lo[a], red[w][a]);
+                    hi[a] = fmaxf(hi[a], red[w][3 + a]);
+                }
+            grid_from_bounds(lo, hi, cell_size, ONE_CLOTH_MAX_CELLS, 0u, &sgrid);
+            if (blockIdx.x == 0) *grid_out = sgrid;
+        }
+    }
+    __syncthreads();   // the cleared histograms (and the grid) are published
+    const CollGrid g0 = part ? sgrid : grids[0];
 #pragma unroll
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
         const uint32_t i = base + r * 64;
@@ -2046,10 +2057,6 @@ uint32_t launch_aabb_of(const float4* P, uint32_t n, float* part, hipStream_t s)
     return blocks;
 }
 
-void launch_coll_grid(const float* part, uint32_t nparts, CollGrid* grid, float cell_size, hipStream_t s) {
-    hipLaunchKernelGGL(k_coll_grid, dim3(1), dim3(1024), 0, s, part, nparts, grid, cell_size);
-}
-
 void launch_coll_grids(const float4* P, const uint2* span, uint32_t nspans, const uint32_t* pfirst,
                        uint32_t nclothes, float* part, float cell_size, uint32_t local_bits, CollGrid* grids,
                        hipStream_t s) {
@@ -2060,17 +2067,17 @@ void launch_coll_grids(const float4* P, const uint2* span, uint32_t nspans, cons
 
 uint32_t cloth_span_max() { return CB_SPAN; }
 
-void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
+void launch_coll_keys(const float4* F, uint32_t n, CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
                       uint32_t* ids, uint2* table, const uint32_t* slot_rec, uint32_t* hist, uint32_t digit_bits,
-                      hipStream_t s) {
+                      const float* part, uint32_t nparts, float cell_size, hipStream_t s) {
     if (n == 0) return;
     const uint32_t ntiles = std::max(1u, cdiv(n, RADIX_TILE));
     if (digit_bits == 9)
         hipLaunchKernelGGL(k_coll_keys<9>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, F, n, grids, cloth_of, keys, ids,
-                           table, slot_rec, hist, ntiles);
+                           table, slot_rec, hist, ntiles, part, nparts, cell_size, grids);
     else
         hipLaunchKernelGGL(k_coll_keys<8>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, F, n, grids, cloth_of, keys, ids,
-                           table, slot_rec, hist, ntiles);
+                           table, slot_rec, hist, ntiles, part, nparts, cell_size, grids);
 }
 
 uint32_t radix_hist_words(uint32_t n) {

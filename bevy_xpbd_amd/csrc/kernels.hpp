@@ -177,7 +177,6 @@ void launch_predict(const float4* X, const float4* V, float4* P, DevParams p, hi
                     uint8_t* tame_out = nullptr);
 void launch_finalize(const float4* F, float4* X, float4* V, const uint8_t* ground, DevParams p,
                      hipStream_t s);
-void launch_coll_grid(const float* part, uint32_t nparts, CollGrid* grid, float cell_size, hipStream_t s);
 // per-cloth bounds of P: piece b reduces particles [span[b].x, span[b].y)
 // of one cloth into part[8b..]; cloth c owns pieces [pfirst[c], pfirst[c+1]);
 // then the grids
@@ -186,10 +185,12 @@ void launch_coll_grids(const float4* P, const uint2* span, uint32_t nspans, cons
                        hipStream_t s);
 uint32_t cloth_span_max();   // particles per piece at most
 // cell keys (and ids = identity) plus the sort's pass-0 tile counts in hist;
-// also empties the group-table slots the previous pass recorded in slot_rec
-void launch_coll_keys(const float4* F, uint32_t n, const CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
+// also empties the group-table slots the previous pass recorded in slot_rec.
+// part != nullptr (one cloth): the grid is derived from the nparts bounds
+// partials and stored to grids[0]; otherwise grids[] is read (batches).
+void launch_coll_keys(const float4* F, uint32_t n, CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
                       uint32_t* ids, uint2* table, const uint32_t* slot_rec, uint32_t* hist, uint32_t digit_bits,
-                      hipStream_t s);
+                      const float* part, uint32_t nparts, float cell_size, hipStream_t s);
 // Stable sort of (keys, ids) by key: 4 passes of 8-bit digits (32-bit keys) or
 // 3 passes of 9-bit digits (keys below 2^27: one cloth's grid).  Returns true
 // when the result is in (keys_alt, ids_alt) (an odd pass count), false when it

@@ -258,12 +258,16 @@ float coll_cell(const xpbd_cloth* h) {
 // Collision chain on iterate F: grid, keys, sort, table, sweep (which also
 // applies the ground plane and the velocity update).  Replaces the
 // reference's clear_atomics .. collide_apply sequence (cloth_jacobi.rs:774-868).
-// Cell grids of the current iterate: one cloth reduces the bounds partials
-// its last iteration wrote; many cloths get one bounds block each.
-void enqueue_grids(xpbd_cloth* h, const float4* F, uint32_t nparts, float cell, hipStream_t s) {
-    if (h->nclothes == 1) launch_coll_grid(h->aabb, nparts, h->grids, cell, s);
-    else launch_coll_grids(F, h->cspan, h->ncspans, h->cpfirst, h->nclothes, h->aabb, cell, h->local_bits, h->grids,
-                           s);
+// Cell grids and keys of the current iterate: one cloth's key kernel derives
+// the grid from the nparts bounds partials its last iteration wrote; many
+// cloths get their grids from bounds pieces first.
+void enqueue_keys(xpbd_cloth* h, const float4* F, uint32_t nparts, float cell, hipStream_t s) {
+    const bool one = h->nclothes == 1;
+    if (!one)
+        launch_coll_grids(F, h->cspan, h->ncspans, h->cpfirst, h->nclothes, h->aabb, cell, h->local_bits, h->grids,
+                          s);
+    launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist,
+                     h->coll_digit_bits, one ? h->aabb : nullptr, nparts, cell, s);
 }
 
 // Instrumented frames: bracket a launch of kind `which` with a pair of events.
@@ -284,9 +288,7 @@ struct ProbeScope {
 void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint32_t nparts, float4* xout,
                        float4* vout) {
     hipStream_t s = h->stream;
-    enqueue_grids(h, F, nparts, coll_cell(h), s);
-    launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist,
-                     h->coll_digit_bits, s);
+    enqueue_keys(h, F, nparts, coll_cell(h), s);
     const bool alt = launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true, h->coll_digit_bits);
     launch_coll_gather(F, alt ? h->ids2 : h->ids, alt ? h->keys2 : h->keys, h->n, h->sP, h->ovf, h->table,
                        h->table_slots - 1, h->slot_rec, s);
@@ -1591,9 +1593,7 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
     bool alt = false;   // the sorted (key, id) pairs are in keys2 / ids2
     auto prep_collision = [&]() {
         const uint32_t np = h->nclothes == 1 ? launch_aabb_of(h->A, h->n, h->aabb, s) : 0u;
-        enqueue_grids(h, h->A, np, coll_cell(h), s);
-        launch_coll_keys(h->A, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist,
-                         h->coll_digit_bits, s);
+        enqueue_keys(h, h->A, np, coll_cell(h), s);
         alt = launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true, h->coll_digit_bits);
         launch_coll_gather(h->A, alt ? h->ids2 : h->ids, alt ? h->keys2 : h->keys, h->n, h->sP, h->ovf, h->table,
                            h->table_slots - 1, h->slot_rec, s);
@@ -2093,9 +2093,7 @@ XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t*
     if (!h || !grid_out || !sorted_keys || !sorted_ids || !(cell_size > 0.f)) return XPBD_E_INVALID;
     hipStream_t s = h->stream;
     const uint32_t np = h->nclothes == 1 ? launch_aabb_of(h->X, h->n, h->aabb, s) : 0u;
-    enqueue_grids(h, h->X, np, cell_size, s);
-    launch_coll_keys(h->X, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist,
-                     h->coll_digit_bits, s);
+    enqueue_keys(h, h->X, np, cell_size, s);
     const bool alt = launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true, h->coll_digit_bits);
     HIPC(h, hipGetLastError());
     HIPC(h, hipStreamSynchronize(s));
