@@ -278,17 +278,22 @@ def config2_rates(X, frames: int = 40, warmup: int = 5):
     """The 256x256 sheet of the second configuration (horizontal at 1 m,
     centre pinned, seeded rotation, self-collision every substep, contact
     distance 0.8 x spacing), 4 substeps x 8 iterations, both solvers (Jacobi
-    omega 0.32, coloured Gauss-Seidel omega 1.0), frames replayed from the
-    hipGraph: substeps/s, the GS substep's launch time and its colours."""
+    omega 0.32, coloured Gauss-Seidel omega 1.0; GS in its autotuned launch
+    form and in each form: one persistent launch per substep, one launch per
+    colour), frames replayed from the hipGraph: substeps/s, the GS substep's
+    time and its colours."""
     import numpy as np
     h = 1.0 / 255.0
     angle = float(np.random.default_rng(2).uniform(0, 2 * np.pi))
     out = {"workload": "256x256 horizontal sheet, centre pin, seeded rotation, self-collision every substep, "
                        "4 substeps x 8 iterations (step = 1 frame)", "angle_rad": round(angle, 6)}
-    for name, solver in (("jacobi", X.SOLVER_JACOBI), ("gauss_seidel", X.SOLVER_GAUSS_SEIDEL)):
+    for name, solver, form in (("jacobi", X.SOLVER_JACOBI, 0), ("gauss_seidel", X.SOLVER_GAUSS_SEIDEL, 0),
+                               ("gauss_seidel_persistent", X.SOLVER_GAUSS_SEIDEL, 1),
+                               ("gauss_seidel_per_colour", X.SOLVER_GAUSS_SEIDEL, 2)):
         mesh = X.ClothMeshData.sheet(cols=256, rows=256, spacing=h, layout=X.LAYOUT_HORIZONTAL_XZ, height=1.0,
                                      angle=angle, pins=X.PIN_CENTER)
-        cfg = X.ClothSimConfig.default(solve_substeps=4, solve_inner_iterations=8, solver=solver, use_graph=1)
+        cfg = X.ClothSimConfig.default(solve_substeps=4, solve_inner_iterations=8, solver=solver, use_graph=1,
+                                       gs_launch=form)
         u = X.ClothSimUniforms.default(solver, 4, thickness=2.0 * (0.4 * h))
         sim = X.ClothComputePlugin(mesh, cfg, u)
         sim.step_frames(warmup)
@@ -304,6 +309,30 @@ def config2_rates(X, frames: int = 40, warmup: int = 5):
         sim.close()
         out[name] = rec
         log(f"bench: 256x256 sheet, {name}: {rate:.1f} substeps/s")
+    return out
+
+
+def gs_forms_1024(X, frames: int = 5, warmup: int = 2):
+    """Coloured Gauss-Seidel on the 1024x1024 bench sheet (8 substeps x 10
+    iterations, omega 1.0, collision every substep), frames replayed from the
+    hipGraph, in its autotuned launch form and in each form: substeps/s."""
+    out = {"workload": "1024x1024 bench sheet, coloured Gauss-Seidel 8 substeps x 10 iterations, "
+                       "self-collision every substep"}
+    for name, form in (("autotuned", 0), ("persistent", 1), ("per_colour", 2)):
+        mesh, thickness, _, _ = build_workload(X, "sheet1024")
+        cfg = X.ClothSimConfig.default(solve_substeps=8, solve_inner_iterations=10, solver=X.SOLVER_GAUSS_SEIDEL,
+                                       use_graph=1, gs_launch=form)
+        u = X.ClothSimUniforms.default(X.SOLVER_GAUSS_SEIDEL, 8, thickness=float(thickness))
+        sim = X.ClothComputePlugin(mesh, cfg, u)
+        sim.step_frames(warmup)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step_frames(frames)
+        sim.sync()
+        rate = 8 * frames / (time.perf_counter() - t0)
+        sim.close()
+        out[name] = {"substeps_per_s": round(rate, 1)}
+        log(f"bench: 1024x1024 sheet, Gauss-Seidel {name}: {rate:.1f} substeps/s")
     return out
 
 
@@ -449,9 +478,10 @@ def main():
     log(f"bench: dominant kernel {rkernel} {r_us:.1f} us = {achieved:.0f} GB/s algorithmic; "
         f"collision sweep {sweep_ms * 1e3:.1f} us; radix pass {radix_ms * 1e3:.1f} us")
 
-    side = None
+    side = gs1024 = None
     if args.workload == "sheet1024" and not args.no_side:
         side = config2_rates(X)
+        gs1024 = gs_forms_1024(X)
 
     warm = None
     if not args.no_warm:
@@ -485,6 +515,8 @@ def main():
     config.update(extra)
     if side is not None:
         config["sheet256_both_solvers"] = side
+    if gs1024 is not None:
+        config["sheet1024_gauss_seidel"] = gs1024
     if warm is not None:
         config["learned_warm_start"] = warm
     out = {

@@ -101,6 +101,7 @@ class ClothSimConfig(C.Structure):
         ("jacobi_block", C.c_uint32), ("fused_predict", C.c_uint32), ("plain_adjacency", C.c_uint32),
         ("fast_division", C.c_uint32), ("xcd_remap", C.c_uint32), ("persistent_jacobi", C.c_uint32),
         ("grid_tiles", C.c_uint32),
+        ("gs_launch", C.c_uint32),
     ]
 
     @classmethod

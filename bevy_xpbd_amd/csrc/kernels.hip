@@ -1611,6 +1611,57 @@ struct GridSync {
     }
 };
 
+// One constraint of coloured Gauss-Seidel (xpbd_cpu.rs:312-391): project
+// constraint e of the colour-sorted arrays on the work iterate W in place.
+// Constraints of one colour share no vertex, so any order within a colour,
+// and any split of a colour over launches or blocks, gives the same bits.
+__device__ __forceinline__ void gs_project(const GsArgs& a, uint32_t e) {
+    const DevParams& p = a.p;
+    const uint2 ij = a.ij[e];
+    const float4 pi = a.W[ij.x], pj = a.W[ij.y];
+    const float wi = pi.w, wj = pj.w;
+    if (wj <= 0.f && wi <= 0.f) return;
+    const float2 rc = a.rc[e];
+    float gx = pi.x - pj.x, gy = pi.y - pj.y, gz = pi.z - pj.z;
+    const float len = len3(gx, gy, gz);
+    if (len < 1e-8f) return;
+    gx = gx / len;
+    gy = gy / len;
+    gz = gz / len;
+    const float c = len - rc.x;
+    const float alpha_t = rc.y * p.inv_dt_sq;
+    const float wsum = wi + wj + alpha_t;
+    if (wsum < 1e-8f) return;
+    const float dlam = (-c - alpha_t * 0.0f) / wsum;
+    const float ox = p.omega * gx, oy = p.omega * gy, oz = p.omega * gz;
+    float ix = (ox * wi) * dlam, iy = (oy * wi) * dlam, iz = (oz * wi) * dlam;
+    float jx = -((ox * wj) * dlam), jy = -((oy * wj) * dlam), jz = -((oz * wj) * dlam);
+    float ml = len3(ix, iy, iz);
+    if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
+        const float s = JACOBI_CORRECTION_CAP / ml;
+        ix = ix * s;
+        iy = iy * s;
+        iz = iz * s;
+    }
+    ml = len3(jx, jy, jz);
+    if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
+        const float s = JACOBI_CORRECTION_CAP / ml;
+        jx = jx * s;
+        jy = jy * s;
+        jz = jz * s;
+    }
+    if (wi > 0.f) a.W[ij.x] = make_float4(pi.x + ix, pi.y + iy, pi.z + iz, wi);
+    if (wj > 0.f) a.W[ij.y] = make_float4(pj.x + jx, pj.y + jy, pj.z + jz, wj);
+}
+
+// One colour of one GS iteration as its own launch: constraints [e0, e1) of
+// the colour-sorted arrays, one per thread.  The launch boundary orders the
+// colours, in place of a grid barrier.
+__global__ __launch_bounds__(256) void k_gs_colour(GsArgs a, uint32_t e0, uint32_t e1) {
+    const uint32_t e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < e1) gs_project(a, e);
+}
+
 __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
     const DevParams& p = a.p;
     const uint32_t nthreads = gridDim.x * blockDim.x;
@@ -1626,43 +1677,7 @@ __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
     for (uint32_t it = 0; it < a.iterations; ++it) {
         for (uint32_t b = 0; b < a.num_batches; ++b) {
             const uint32_t e1 = a.batch_offsets[b + 1];
-            for (uint32_t e = a.batch_offsets[b] + gt; e < e1; e += nthreads) {
-                const uint2 ij = a.ij[e];
-                const float4 pi = a.W[ij.x], pj = a.W[ij.y];
-                const float wi = pi.w, wj = pj.w;
-                if (wj <= 0.f && wi <= 0.f) continue;
-                const float2 rc = a.rc[e];
-                float gx = pi.x - pj.x, gy = pi.y - pj.y, gz = pi.z - pj.z;
-                const float len = len3(gx, gy, gz);
-                if (len < 1e-8f) continue;
-                gx = gx / len;
-                gy = gy / len;
-                gz = gz / len;
-                const float c = len - rc.x;
-                const float alpha_t = rc.y * p.inv_dt_sq;
-                const float wsum = wi + wj + alpha_t;
-                if (wsum < 1e-8f) continue;
-                const float dlam = (-c - alpha_t * 0.0f) / wsum;
-                const float ox = p.omega * gx, oy = p.omega * gy, oz = p.omega * gz;
-                float ix = (ox * wi) * dlam, iy = (oy * wi) * dlam, iz = (oz * wi) * dlam;
-                float jx = -((ox * wj) * dlam), jy = -((oy * wj) * dlam), jz = -((oz * wj) * dlam);
-                float ml = len3(ix, iy, iz);
-                if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
-                    const float s = JACOBI_CORRECTION_CAP / ml;
-                    ix = ix * s;
-                    iy = iy * s;
-                    iz = iz * s;
-                }
-                ml = len3(jx, jy, jz);
-                if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
-                    const float s = JACOBI_CORRECTION_CAP / ml;
-                    jx = jx * s;
-                    jy = jy * s;
-                    jz = jz * s;
-                }
-                if (wi > 0.f) a.W[ij.x] = make_float4(pi.x + ix, pi.y + iy, pi.z + iz, wi);
-                if (wj > 0.f) a.W[ij.y] = make_float4(pj.x + jx, pj.y + jy, pj.z + jz, wj);
-            }
+            for (uint32_t e = a.batch_offsets[b] + gt; e < e1; e += nthreads) gs_project(a, e);
             gsync.sync();
         }
     }
@@ -2166,6 +2181,14 @@ void launch_jacobi_persistent(const JacobiArgs& a, const PersistArgs& q, uint32_
         if (a.fastdiv) hipLaunchKernelGGL((k_jacobi_persistent<false, true>), dim3(blocks), dim3(256), 0, s, a, q);
         else hipLaunchKernelGGL((k_jacobi_persistent<false, false>), dim3(blocks), dim3(256), 0, s, a, q);
     }
+}
+
+void launch_gs_colours(const GsArgs& a, const std::vector<uint32_t>& off, uint32_t iterations, hipStream_t s) {
+    for (uint32_t it = 0; it < iterations; ++it)
+        for (size_t b = 0; b + 1 < off.size(); ++b) {
+            const uint32_t e0 = off[b], e1 = off[b + 1];
+            if (e1 > e0) hipLaunchKernelGGL(k_gs_colour, dim3(cdiv(e1 - e0, 256)), dim3(256), 0, s, a, e0, e1);
+        }
 }
 
 uint32_t gs_max_blocks(uint32_t block) {

@@ -209,6 +209,11 @@ void launch_coll_gather(const float4* F, const uint32_t* sids, const uint32_t* s
                         uint32_t* ovf, uint2* table, uint32_t mask, uint32_t* slot_rec, hipStream_t s);
 void launch_coll_sweep(const CollArgs& a, hipStream_t s);
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
+// The constraint iterations of a GS substep as one launch per colour and
+// iteration (iterations x colours launches; off = the colour offsets on the
+// host): the launch boundaries order the colours.  W must hold the predicted
+// positions; a.W, a.ij, a.rc and a.p are read.
+void launch_gs_colours(const GsArgs& a, const std::vector<uint32_t>& off, uint32_t iterations, hipStream_t s);
 uint32_t gs_max_blocks(uint32_t block);
 void launch_copy4(const float4* a, float4* b, uint32_t n, hipStream_t s);
 // per-slice tame bytes of P (1: every coordinate 0 or of magnitude in [2^-50, 2^20])

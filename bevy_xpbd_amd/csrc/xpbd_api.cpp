@@ -122,6 +122,8 @@ struct xpbd_cloth {
 
     uint32_t jacobi_block = 256, remap = 1, fused_predict = 0;
     uint32_t gs_block = 256, gs_blocks = 0;
+    uint32_t gs_colour_tuned = 0;   // the autotuner found one launch per colour faster
+    uint32_t gs_colour = 0;         // GS substeps run one launch per colour (else persistent)
     uint32_t pj_blocks = 0;        // persistent Jacobi grid (co-resident blocks)
 
     // fast correctly rounded division / sqrt in the Jacobi gather: the static
@@ -313,6 +315,17 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     launch_coll_sweep(c, s);
 }
 
+// Coloured GS as one launch per colour and iteration: predict X,V -> W, the
+// colours, then either the state update (no collision) or the bounds
+// partials of W for the collision pass (nparts is set).  The same steps as
+// one persistent k_gs_substep, with launch boundaries for its grid barriers.
+void enqueue_gs_colours(xpbd_cloth* h, const GsArgs& g, bool collision, uint32_t& nparts, hipStream_t s) {
+    launch_predict(g.X, g.V, g.W, g.p, s);
+    if (g.num_batches) launch_gs_colours(g, h->batch_offsets, g.iterations, s);
+    if (collision) nparts = launch_aabb_of(g.W, h->n, h->aabb, s);
+    else launch_finalize(g.W, g.xout, g.vout, g.ground, g.p, s);
+}
+
 // One substep (cloth_jacobi.rs:702-883 / cloth_compute.rs:1313-1488).  The
 // state is always (X, V) at substep boundaries; A/B hold the iterates.
 void enqueue_substep(xpbd_cloth* h, bool collision) {
@@ -322,7 +335,7 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
     const float4* F = nullptr;
     uint32_t nparts = 0;
     if (h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL) {
-        launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
+        if (!h->gs_colour) launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
         GsArgs g{};
         g.X = h->X;
         g.V = h->V;
@@ -342,10 +355,11 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         g.p = p;
         {
             ProbeScope ps(h, XPBD_KERNEL_GS_SUBSTEP);
-            launch_gs(g, h->gs_blocks, h->gs_block, s);
+            if (h->gs_colour) enqueue_gs_colours(h, g, collision, nparts, s);
+            else launch_gs(g, h->gs_blocks, h->gs_block, s);
         }
         F = h->A;
-        nparts = h->gs_blocks;
+        if (!h->gs_colour) nparts = h->gs_blocks;
     } else if (h->warm) {
         // learned warm start: predict -> (features, MLP) x0 = xp + MLP -> warm
         // iterations.  With a finite threshold, the relative residual of the
@@ -680,8 +694,10 @@ int validate_mesh(const Mesh& m, std::string& why) {
     if (m.neighbor_offsets[n] != m.neighbor_other.size()) return why = "neighbor_offsets end", XPBD_E_INVALID;
     for (uint32_t v : m.indices)
         if (v >= n) return why = "triangle index out of range", XPBD_E_INVALID;
-    if (m.batch_offsets.empty() || m.batch_offsets.back() != m.num_constraints())
+    if (m.batch_offsets.empty() || m.batch_offsets.front() != 0 || m.batch_offsets.back() != m.num_constraints())
         return why = "batch offsets", XPBD_E_INVALID;
+    for (size_t b = 0; b + 1 < m.batch_offsets.size(); ++b)
+        if (m.batch_offsets[b] > m.batch_offsets[b + 1]) return why = "batch offsets not monotone", XPBD_E_INVALID;
     return XPBD_OK;
 }
 
@@ -768,10 +784,32 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
         rc[k] = make_float2(m.constraint_rest[k], m.constraint_comp[k]);
         h->rest_sq_sum += double(m.constraint_rest[k]) * double(m.constraint_rest[k]);
     }
-    TRY(dalloc(h, &h->gs_ij, e));
-    TRY(dalloc(h, &h->gs_rc, e));
-    TRY(upload(h, h->gs_ij, ij.data(), e));
-    TRY(upload(h, h->gs_rc, rc.data(), e));
+    {
+        // Coloured GS reads the rows colour by colour.  Within a colour the
+        // constraints share no vertex, so their order is free: sorted by
+        // (lower, higher) endpoint, neighbouring threads touch neighbouring
+        // particles instead of the colouring's degree-first order.
+        std::vector<uint32_t> ord(e);
+        for (size_t k = 0; k < e; ++k) ord[k] = uint32_t(k);
+        auto key = [&](uint32_t k) {
+            const uint32_t a = ij[k].x, b = ij[k].y;
+            return (uint64_t(std::min(a, b)) << 32) | std::max(a, b);
+        };
+        const std::vector<uint32_t>& off = m.batch_offsets;
+        for (size_t c = 0; c + 1 < off.size(); ++c)
+            std::sort(ord.begin() + off[c], ord.begin() + off[c + 1],
+                      [&](uint32_t a, uint32_t b) { return key(a) < key(b); });
+        std::vector<uint2> gij(e);
+        std::vector<float2> grc(e);
+        for (size_t k = 0; k < e; ++k) {
+            gij[k] = ij[ord[k]];
+            grc[k] = rc[ord[k]];
+        }
+        TRY(dalloc(h, &h->gs_ij, e));
+        TRY(dalloc(h, &h->gs_rc, e));
+        TRY(upload(h, h->gs_ij, gij.data(), e));
+        TRY(upload(h, h->gs_rc, grc.data(), e));
+    }
     {
         std::vector<uint32_t> ord(e);
         for (size_t k = 0; k < e; ++k) ord[k] = uint32_t(k);
@@ -918,6 +956,7 @@ void apply_tuning(xpbd_cloth* h) {
     // persistent Jacobi: every block must be resident at once (grid barrier)
     const uint32_t groups = std::max(1u, (h->nslices + 3) / 4);
     h->pj_blocks = std::max(1u, std::min(persistent_max_blocks(h->nbpk != nullptr, h->cfg.fast_division != 0), groups));
+    h->gs_colour = h->cfg.gs_launch == 2 ? 1u : h->cfg.gs_launch == 1 ? 0u : h->gs_colour_tuned;
 }
 
 // Grid neighbours and feature constants of the warm-start network.  Needs a
@@ -1085,6 +1124,59 @@ int autotune(xpbd_cloth* h) {
         }
         h->gs_blocks = gbest_blocks;
     }
+    // Coloured GS: one persistent launch (grid barriers between colours)
+    // against one launch per colour (launch boundaries instead), each timed
+    // as a replayed graph of one substep's GS part, the form frames replay.
+    // Barriers cost more than graph kernel boundaries on large grids, while
+    // small colours leave launches mostly latency.
+    h->gs_colour_tuned = 0;
+    if (h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL && h->e) {
+        GsArgs g{};
+        g.X = h->X;
+        g.V = h->V;
+        g.W = h->A;
+        g.xout = h->B;
+        g.vout = h->V2;
+        g.ij = h->gs_ij;
+        g.rc = h->gs_rc;
+        g.batch_offsets = h->gs_off;
+        g.num_batches = h->num_batches;
+        g.iterations = std::max(h->cfg.solve_inner_iterations, 1u);
+        g.barrier = h->barrier;
+        g.error = h->err;
+        g.aabb = h->aabb;
+        g.final_write = 1;
+        g.p = p;
+        auto time_graph = [&](auto body) -> float {
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            float ms = 1e30f;
+            if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return ms;
+            body();
+            if (hipStreamEndCapture(h->stream, &graph) != hipSuccess || !graph) return ms;
+            if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+                (void)hipGraphLaunch(exec, h->stream);   // warm-up
+                (void)hipEventRecord(e0, h->stream);
+                for (uint32_t r = 0; r < 3; ++r) (void)hipGraphLaunch(exec, h->stream);
+                (void)hipEventRecord(e1, h->stream);
+                (void)hipEventSynchronize(e1);
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                (void)hipGraphExecDestroy(exec);
+            }
+            (void)hipGraphDestroy(graph);
+            return ms;
+        };
+        const float tp = time_graph([&]() {
+            launch_zero_u32(h->barrier, GRID_SYNC_WORDS, h->stream);
+            launch_gs(g, h->gs_blocks, h->gs_block, h->stream);
+        });
+        const float tc = time_graph([&]() {
+            uint32_t np = 0;
+            enqueue_gs_colours(h, g, false, np, h->stream);
+        });
+        h->gs_colour_tuned = tc < tp ? 1u : 0u;
+    }
+    apply_tuning(h);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     HIPC(h, hipGetLastError());
@@ -1324,6 +1416,7 @@ XPBD_API void xpbd_sim_config_default(xpbd_sim_config* out) {
     c.fused_predict = 0;
     c.fast_division = 1;
     c.grid_tiles = 1;
+    c.gs_launch = 0;
     *out = c;
 }
 
@@ -1656,8 +1749,13 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
                 launch_radix_pass(h->keys, h->ids, h->keys2, h->ids2, h->n, 0, h->hist, s, false, h->coll_digit_bits);
                 break;
             case XPBD_KERNEL_GS_SUBSTEP:
-                launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
-                launch_gs(g, h->gs_blocks, h->gs_block, s);
+                if (h->gs_colour) {
+                    uint32_t np = 0;
+                    enqueue_gs_colours(h, g, false, np, s);
+                } else {
+                    launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
+                    launch_gs(g, h->gs_blocks, h->gs_block, s);
+                }
                 break;
             case XPBD_KERNEL_COPY: launch_copy4(h->X, h->B, h->n, s); break;
             case XPBD_KERNEL_JACOBI_PERSISTENT: {

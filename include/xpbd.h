@@ -204,7 +204,8 @@ typedef struct xpbd_sim_config {
     uint32_t use_graph;                   /* 1: replay frames from a hipGraph */
     uint32_t autotune;                    /* 1 (default): time launch shapes at create
                                              (Jacobi block size and order, coloured GS
-                                             grid size); same results either way */
+                                             grid size and launch form); same results
+                                             either way */
     uint32_t jacobi_block;                /* Jacobi workgroup size 64..512; 0 = 256 or autotuned */
     uint32_t fused_predict;               /* predict inside the first iteration:
                                              0 (default) where grid tiles run,
@@ -229,6 +230,11 @@ typedef struct xpbd_sim_config {
                                              Jacobi gather on 64x8 tiles staged in LDS
                                              with a 2-particle halo (read at create;
                                              same results) */
+    uint32_t gs_launch;                   /* coloured GS launch form: 0 (default) =
+                                             autotuned (else persistent), 1 = one
+                                             persistent launch per substep with grid
+                                             barriers between colours, 2 = one launch
+                                             per colour and iteration (same results) */
 } xpbd_sim_config;
 
 /* ClothSimUniforms (cloth_compute.rs:182-202). */

@@ -48,9 +48,9 @@ def overlapped_state(f):
     return pos
 
 
-def make(X, mesh, solver=0, S=8, I=6, persistent=0, **uk):
+def make(X, mesh, solver=0, S=8, I=6, persistent=0, gs_launch=0, **uk):
     cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=I, solver=solver, use_graph=0,
-                                   persistent_jacobi=persistent)
+                                   persistent_jacobi=persistent, gs_launch=gs_launch)
     u = X.ClothSimUniforms.default(solver, S, **uk)
     return X.ClothComputePlugin(mesh, cfg, u), cfg, u
 
@@ -82,13 +82,14 @@ def test_batch_collision_keys_and_sort_bit_exact(O, X):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["jacobi", "jacobi_persistent", "gs"])
+@pytest.mark.parametrize("mode", ["jacobi", "jacobi_persistent", "gs", "gs_colours"])
 def test_batch_substeps_with_collision_bit_exact(O, X, mode):
     note(f"case batch substeps mode={mode}")
-    solver = 1 if mode == "gs" else 0
+    solver = 1 if mode.startswith("gs") else 0
     mesh = X.ClothMeshData.batch([X.SheetDesc.make(**d) for d in descs()])
     f = mesh.fields()
     sim, cfg, u = make(X, mesh, solver=solver, persistent=1 if mode == "jacobi_persistent" else 0,
+                       gs_launch=2 if mode == "gs_colours" else 1 if mode == "gs" else 0,
                        thickness=0.012, coll_scale=1.0, jacobi_omega=0.32 if solver == 0 else 1.0)
     pos = overlapped_state(f)
     sim.write_state(pos)

@@ -21,25 +21,33 @@ def note(msg):
     sys.stdout.flush()
 
 
+# (solver, gs_launch): Jacobi; coloured GS as one persistent launch per
+# substep; coloured GS as one launch per colour and iteration
+SOLVER_FORMS = [(0, 0), (1, 1), (1, 2)]
+SOLVER_FORM_IDS = ["jacobi", "gs_persistent", "gs_colours"]
+
+
 def make(X, mesh, solver=0, S=8, I=10, coll_stride=1, use_graph=0, fused_predict=0, jacobi_block=0,
-         plain_adjacency=0, fast_division=0, persistent_jacobi=0, grid_tiles=1, autotune=0, **uk):
+         plain_adjacency=0, fast_division=0, persistent_jacobi=0, grid_tiles=1, autotune=0, gs_launch=0, **uk):
     cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=I, solver=solver,
                                    collision_every_n_substeps=coll_stride, use_graph=use_graph,
                                    fused_predict=fused_predict, jacobi_block=jacobi_block,
                                    plain_adjacency=plain_adjacency, fast_division=fast_division,
-                                   persistent_jacobi=persistent_jacobi, grid_tiles=grid_tiles, autotune=autotune)
+                                   persistent_jacobi=persistent_jacobi, grid_tiles=grid_tiles, autotune=autotune,
+                                   gs_launch=gs_launch)
     u = X.ClothSimUniforms.default(solver, S, **uk)
     return X.ClothComputePlugin(mesh, cfg, u), cfg, u
 
 
 def run_pair(O, X, mesh_kw=None, mesh=None, solver=0, S=8, I=10, substeps=8, collision=False, fused_predict=0,
              jacobi_block=0, plain_adjacency=0, fast_division=0, persistent_jacobi=0, grid_tiles=1, layout=None,
-             autotune=0, **uk):
+             autotune=0, gs_launch=0, **uk):
     mesh = mesh or X.ClothMeshData.sheet(**mesh_kw)
     f = mesh.fields()
     sim, cfg, u = make(X, mesh, solver, S, I, fused_predict=fused_predict, jacobi_block=jacobi_block,
                        plain_adjacency=plain_adjacency, fast_division=fast_division,
-                       persistent_jacobi=persistent_jacobi, grid_tiles=grid_tiles, autotune=autotune, **uk)
+                       persistent_jacobi=persistent_jacobi, grid_tiles=grid_tiles, autotune=autotune,
+                       gs_launch=gs_launch, **uk)
     if layout is not None:
         assert sim.layout()["adjacency"] == layout, sim.layout()
     s = O.Solver(f)
@@ -58,11 +66,11 @@ def run_pair(O, X, mesh_kw=None, mesh=None, solver=0, S=8, I=10, substeps=8, col
     return pos, vel
 
 
-@pytest.mark.parametrize("solver", [0, 1])
-def test_config1_sheet_bit_exact_per_substep(O, X, solver):
-    note(f"case config1 32x32 solver={solver}")
+@pytest.mark.parametrize("solver,gs_launch", SOLVER_FORMS, ids=SOLVER_FORM_IDS)
+def test_config1_sheet_bit_exact_per_substep(O, X, solver, gs_launch):
+    note(f"case config1 32x32 solver={solver} gs_launch={gs_launch}")
     run_pair(O, X, dict(cols=32, rows=32, spacing=1 / 31, pins=1, jitter=1e-3, seed=1), solver=solver,
-             S=8, I=10, substeps=16)
+             S=8, I=10, substeps=16, gs_launch=gs_launch)
 
 
 @pytest.mark.parametrize("plain", [0, 1])
@@ -158,25 +166,25 @@ def test_collision_dense_cluster_overflow_path(O, X):
     assert ((cand > 8) & (cand <= 32)).any() and (cand > 32).any(), (cand.min(), cand.max())
 
 
-@pytest.mark.parametrize("solver", [0, 1])
-def test_reference_triangle_one_substep(O, X, solver):
+@pytest.mark.parametrize("solver,gs_launch", SOLVER_FORMS, ids=SOLVER_FORM_IDS)
+def test_reference_triangle_one_substep(O, X, solver, gs_launch):
     """gpu_cpu_one_substep_single_triangle_positions_close
     (gpu_cpu_parity_jacobi.rs:817-924; gpu_cpu_parity.rs:821-907): one substep
     with collision on, gravity (-1,-2,0.5), SUBSTEPS=36, INNER_ITERS=22."""
     note(f"case triangle solver={solver}")
     mesh = X.ClothMeshData.parse_welded_obj("\nv 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0 1\nf 1/1 2/2 3/3\n")
     pos, _ = run_pair(O, X, mesh=mesh, solver=solver, S=36, I=22, substeps=1, collision=True,
-                      gravity=(-1.0, -2.0, 0.5), grab_stiffness=0.0)
+                      gravity=(-1.0, -2.0, 0.5), grab_stiffness=0.0, gs_launch=gs_launch)
     assert np.isfinite(pos).all()
 
 
-@pytest.mark.parametrize("solver", [0, 1])
-def test_reference_grid_18x18_one_substep(O, X, solver):
+@pytest.mark.parametrize("solver,gs_launch", SOLVER_FORMS, ids=SOLVER_FORM_IDS)
+def test_reference_grid_18x18_one_substep(O, X, solver, gs_launch):
     """gpu_cpu_grid_cloth_substep_positions_close (gpu_cpu_parity_jacobi.rs:926-1027,
     tolerance 3e-2; gpu_cpu_parity.rs:909-989, 2.5e-2) — here bit-exact."""
     note(f"case grid18 solver={solver}")
     mesh = X.ClothMeshData.grid_cloth_hanging(18, 18, 0.042)
-    run_pair(O, X, mesh=mesh, solver=solver, S=36, I=22, substeps=3, collision=True)
+    run_pair(O, X, mesh=mesh, solver=solver, S=36, I=22, substeps=3, collision=True, gs_launch=gs_launch)
 
 
 def test_reference_grab_and_frames(O, X):
@@ -376,15 +384,15 @@ def test_residual_matches_oracle(O, X):
     sim.close()
 
 
-@pytest.mark.parametrize("solver", [0, 1])
-def test_config2_256_collision_substeps_bit_exact(O, X, solver):
+@pytest.mark.parametrize("solver,gs_launch", SOLVER_FORMS, ids=SOLVER_FORM_IDS)
+def test_config2_256_collision_substeps_bit_exact(O, X, solver, gs_launch):
     """256x256 horizontal sheet, centre pin, seeded rotation, self-collision."""
-    note(f"case 256 solver={solver}")
+    note(f"case 256 solver={solver} gs_launch={gs_launch}")
     h = 1 / 255
     rng = np.random.default_rng(2)
     kw = dict(cols=256, rows=256, spacing=h, layout=1, height=1.0, angle=float(rng.uniform(0, 2 * np.pi)), pins=3)
     run_pair(O, X, kw, solver=solver, S=4, I=8, substeps=3, collision=True, thickness=2 * (0.4 * h),
-             jacobi_omega=0.32 if solver == 0 else 1.0)
+             jacobi_omega=0.32 if solver == 0 else 1.0, gs_launch=gs_launch)
 
 
 @pytest.mark.parametrize("solver", [0, 1])

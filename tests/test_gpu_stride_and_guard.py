@@ -37,14 +37,15 @@ def _squeezed(X):
 
 
 @pytest.mark.parametrize("graph", [0, 1])
-@pytest.mark.parametrize("solver", [0, 1])
+@pytest.mark.parametrize("solver,gs_launch", [(0, 0), (1, 1), (1, 2)], ids=["jacobi", "gs_persistent", "gs_colours"])
 @pytest.mark.parametrize("stride", [4, 8])
-def test_collision_stride_frames_bit_exact(O, X, stride, solver, graph):
-    note(f"case stride={stride} solver={solver} graph={graph}")
+def test_collision_stride_frames_bit_exact(O, X, stride, solver, gs_launch, graph):
+    note(f"case stride={stride} solver={solver} gs_launch={gs_launch} graph={graph}")
     mesh, f = _squeezed(X)
     S = 8
     cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=6, solver=solver,
-                                   collision_every_n_substeps=stride, use_graph=graph, autotune=0)
+                                   collision_every_n_substeps=stride, use_graph=graph, autotune=0,
+                                   gs_launch=gs_launch)
     u = X.ClothSimUniforms.default(solver, S, thickness=0.016)
     sim = X.ClothComputePlugin(mesh, cfg, u)
     sim.write_state(f["positions"])
