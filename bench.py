@@ -326,10 +326,11 @@ def gs_forms_1024(X, frames: int = 5, warmup: int = 2):
         sim = X.ClothComputePlugin(mesh, cfg, u)
         sim.step_frames(warmup)
         sim.sync()
-        t0 = time.perf_counter()
-        sim.step_frames(frames)
-        sim.sync()
-        rate = 8 * frames / (time.perf_counter() - t0)
+        with TraceRange(f"gauss-seidel {name} frames"):
+            t0 = time.perf_counter()
+            sim.step_frames(frames)
+            sim.sync()
+            rate = 8 * frames / (time.perf_counter() - t0)
         sim.close()
         out[name] = {"substeps_per_s": round(rate, 1)}
         log(f"bench: 1024x1024 sheet, Gauss-Seidel {name}: {rate:.1f} substeps/s")
@@ -375,12 +376,15 @@ def main():
                     help="print the per-kernel table of the timed frames from a rocprofv3 CSV trace "
                          "(--kernel-trace --marker-trace -f csv) of an earlier bench.py run, and exit")
     ap.add_argument("--by-grid", action="store_true", help="--trace-summary: one row per kernel and grid size")
+    ap.add_argument("--trace-range", default="timed frames",
+                    help="--trace-summary: the ROCTX range to keep (default: the timed Jacobi frames; "
+                         "'gauss-seidel per_colour frames' etc. for the 1024x1024 Gauss-Seidel records)")
     ap.add_argument("--counter-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the bytes-per-dispatch table from FETCH_SIZE and WRITE_SIZE counter passes "
                          "(-f csv) of `bench.py --kernels-only`, then the Jacobi iteration's JSON record, and exit")
     args = ap.parse_args()
     if args.trace_summary:
-        print(trace_summary(args.trace_summary, frames=args.steps, by_grid=args.by_grid))
+        print(trace_summary(args.trace_summary, range_name=args.trace_range, frames=args.steps, by_grid=args.by_grid))
         return
     if args.counter_summary:
         table, rec = counter_summary(*args.counter_summary)
