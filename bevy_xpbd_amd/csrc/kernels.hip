@@ -684,10 +684,11 @@ __device__ __forceinline__ void cell_of(float4 p, const CollGrid& g, uint32_t c[
 // ---- stable LSD radix sort of (key, id), DB-bit digits ----------------------
 // A pass is three kernels: per-tile digit counts, an exclusive scan of the
 // counts over tiles (per digit), and a stable scatter.  A tile is 4096 keys;
-// each of its 4 wavefronts owns a contiguous quarter (1024 keys, 16 rounds of
+// each of its 8 wavefronts owns a contiguous eighth (512 keys, 8 rounds of
 // 64), so ranks in input order are (wavefront, round, lane) and need no block
 // barrier per round.  (2048-key tiles left 9-bit digit runs of about 4 keys
-// in the scatter's output: poorly coalesced stores.)  The count kernels (k_coll_keys' pass-0 counts and
+// in the scatter's output: poorly coalesced stores; 256-thread blocks left
+// one wavefront per SIMD on a 1024^2 sheet.)  The count kernels (k_coll_keys' pass-0 counts and
 // k_radix_hist) add each key to its wavefront's LDS histogram with an LDS
 // atomic: exact integer counts, independent of the order of the adds.  The
 // scatter ranks lanes stably: lanes with equal digits are found by matching
@@ -696,10 +697,12 @@ __device__ __forceinline__ void cell_of(float4 p, const CollGrid& g, uint32_t c[
 // Digit widths: 8 (4 passes over 32-bit keys; batches and topology sorts) or
 // 9 (3 passes over the 27-bit keys of one cloth's grid: a pass is three
 // launches of mostly fixed latency, so one pass fewer outweighs the larger
-// per-tile count table).  Thread t of a block owns digits [t*DPT, t*DPT+DPT).
-constexpr uint32_t RADIX_BLOCK = 256;
+// per-tile count table).  Thread t of a block owns digits [t*DPT, t*DPT+DPT)
+// (DPT = 1 and threads past the last digit own none when a block has more
+// threads than digits).
+constexpr uint32_t RADIX_BLOCK = 512;
 constexpr uint32_t RADIX_WAVES = RADIX_BLOCK / 64;
-constexpr uint32_t RADIX_ITEMS = 16;
+constexpr uint32_t RADIX_ITEMS = 8;
 constexpr uint32_t RADIX_TILE = RADIX_BLOCK * RADIX_ITEMS;
 constexpr uint32_t RADIX_WAVE_SPAN = 64 * RADIX_ITEMS;   // keys owned by one wavefront
 constexpr uint32_t RADIX_MAX_DIGITS = 512;
@@ -717,13 +720,17 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
     return peers;
 }
 
+// Digits owned by one thread of a radix block.
+constexpr uint32_t radix_dpt(uint32_t digits) { return digits >= RADIX_BLOCK ? digits / RADIX_BLOCK : 1u; }
+
 // Block-wide digit counts of the per-wavefront LDS histograms -> hist[d][tile].
 template <uint32_t DB>
 __device__ __forceinline__ void store_tile_counts(uint32_t (*h)[1u << DB], uint32_t* hist, uint32_t ntiles) {
-    constexpr uint32_t DPT = (1u << DB) / RADIX_BLOCK;
+    constexpr uint32_t DIGITS = 1u << DB, DPT = radix_dpt(DIGITS);
 #pragma unroll
     for (uint32_t k = 0; k < DPT; ++k) {
         const uint32_t d = threadIdx.x * DPT + k;
+        if (d >= DIGITS) break;
         uint32_t c = 0;
 #pragma unroll
         for (uint32_t w = 0; w < RADIX_WAVES; ++w) c += h[w][d];
@@ -733,11 +740,12 @@ __device__ __forceinline__ void store_tile_counts(uint32_t (*h)[1u << DB], uint3
 
 template <uint32_t DB>
 __device__ __forceinline__ void clear_wave_hists(uint32_t (*h)[1u << DB]) {
-    constexpr uint32_t DPT = (1u << DB) / RADIX_BLOCK;
+    constexpr uint32_t DIGITS = 1u << DB, DPT = radix_dpt(DIGITS);
 #pragma unroll
     for (uint32_t w = 0; w < RADIX_WAVES; ++w)
 #pragma unroll
-        for (uint32_t k = 0; k < DPT; ++k) h[w][threadIdx.x * DPT + k] = 0;
+        for (uint32_t k = 0; k < DPT; ++k)
+            if (threadIdx.x * DPT + k < DIGITS) h[w][threadIdx.x * DPT + k] = 0;
 }
 
 template <uint32_t DB>
@@ -855,8 +863,10 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_coll_keys(const float4* F, uint
     store_tile_counts<DB>(h, hist, ntiles);
 }
 
-// Exclusive scan of 256 values held one per thread (block of 256).
-__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* lds, uint32_t& total) {
+// Exclusive scan of one value per thread over a block of WAVES wavefronts;
+// `total` gets the block's sum.  lds holds WAVES words.
+template <uint32_t WAVES>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t incl = v;
 #pragma unroll
@@ -867,8 +877,12 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_
     if (lane == 63) lds[wave] = incl;
     __syncthreads();
     uint32_t before = 0;
-    for (uint32_t w = 0; w < wave; ++w) before += lds[w];
-    total = lds[0] + lds[1] + lds[2] + lds[3];
+    total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < WAVES; ++w) {
+        before += w < wave ? lds[w] : 0u;
+        total += lds[w];
+    }
     __syncthreads();
     return before + incl - v;
 }
@@ -884,7 +898,7 @@ __global__ __launch_bounds__(256) void k_radix_scan_tiles(uint32_t* hist, uint32
     uint32_t sum = 0;
     for (uint32_t k = b; k < e; ++k) sum += row[k];
     uint32_t tot;
-    uint32_t run = block_exclusive_scan_256(sum, lds, tot);
+    uint32_t run = block_exclusive_scan<4>(sum, lds, tot);
     for (uint32_t k = b; k < e; ++k) {
         const uint32_t v = row[k];
         row[k] = run;
@@ -901,12 +915,12 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
                                                              uint32_t* kout, uint32_t* iout, uint32_t n,
                                                              uint32_t shift, const uint32_t* hist,
                                                              uint32_t ntiles, const uint32_t* dtot) {
-    constexpr uint32_t DIGITS = 1u << DB, MASK = DIGITS - 1u, DPT = DIGITS / RADIX_BLOCK;
+    constexpr uint32_t DIGITS = 1u << DB, MASK = DIGITS - 1u, DPT = radix_dpt(DIGITS);
     __shared__ uint32_t wcnt[RADIX_WAVES][DIGITS];   // per-wavefront digit counts, then its start in the tile
     __shared__ int32_t gbase[DIGITS];                 // global position of local sorted position 0 of digit d
     __shared__ uint32_t skey[RADIX_TILE];
     __shared__ uint32_t sid[RADIX_TILE];
-    __shared__ uint32_t lds[4];
+    __shared__ uint32_t lds[RADIX_WAVES];
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t tile0 = blockIdx.x * RADIX_TILE;
     const uint32_t count = min(RADIX_TILE, n - tile0);
@@ -916,8 +930,8 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
 #pragma unroll
     for (uint32_t k = 0; k < DPT; ++k) {
         const uint32_t d = t * DPT + k;
-        hv[k] = hist[size_t(d) * ntiles + blockIdx.x];
-        dv[k] = dtot[d];
+        hv[k] = d < DIGITS ? hist[size_t(d) * ntiles + blockIdx.x] : 0u;
+        dv[k] = d < DIGITS ? dtot[d] : 0u;
     }
     clear_wave_hists<DB>(wcnt);
     __syncthreads();
@@ -948,17 +962,18 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
         const uint32_t d = t * DPT + k;
 #pragma unroll
         for (uint32_t w = 0; w < RADIX_WAVES; ++w) {
-            c[k][w] = wcnt[w][d];
+            c[k][w] = d < DIGITS ? wcnt[w][d] : 0u;
             tot_t += c[k][w];
         }
         dsum += dv[k];
     }
     uint32_t tot;
-    uint32_t lstart = block_exclusive_scan_256(tot_t, lds, tot);
-    uint32_t digit_base = block_exclusive_scan_256(dsum, lds, tot);
+    uint32_t lstart = block_exclusive_scan<RADIX_WAVES>(tot_t, lds, tot);
+    uint32_t digit_base = block_exclusive_scan<RADIX_WAVES>(dsum, lds, tot);
 #pragma unroll
     for (uint32_t k = 0; k < DPT; ++k) {
         const uint32_t d = t * DPT + k;
+        if (d >= DIGITS) break;
         gbase[d] = int32_t(digit_base + hv[k]) - int32_t(lstart);
         digit_base += dv[k];
 #pragma unroll
