@@ -1425,7 +1425,7 @@ XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, cons
                          void* stream, xpbd_cloth** out) {
     if (!mesh || !cfg || !u || !out) return XPBD_E_INVALID;
     *out = nullptr;
-    if (cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL) return XPBD_E_INVALID;
+    if (cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL || cfg->gs_launch > 2) return XPBD_E_INVALID;
     if (cfg->jacobi_block && (cfg->jacobi_block % 64 || cfg->jacobi_block > 512)) return XPBD_E_INVALID;
     std::string why;
     if (validate_mesh(mesh->m, why) != XPBD_OK) return XPBD_E_INVALID;
@@ -1499,7 +1499,7 @@ XPBD_API int xpbd_set_uniforms(xpbd_cloth* h, const xpbd_sim_uniforms* u) {
 }
 
 XPBD_API int xpbd_set_config(xpbd_cloth* h, const xpbd_sim_config* cfg) {
-    if (!h || !cfg || cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL) return XPBD_E_INVALID;
+    if (!h || !cfg || cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL || cfg->gs_launch > 2) return XPBD_E_INVALID;
     if (cfg->jacobi_block && (cfg->jacobi_block % 64 || cfg->jacobi_block > 512)) return XPBD_E_INVALID;
     const bool retune = cfg->autotune && !h->cfg.autotune;
     h->cfg = *cfg;

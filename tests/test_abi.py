@@ -39,6 +39,13 @@ def test_invalid_arguments_are_rejected(X):
     assert L.xpbd_create(None, None, None, None, C.byref(h)) == 1
     with pytest.raises(X.XpbdError):
         X.ClothMeshData.grid_cloth_hanging(0, 0, 0.1)
+    # config ranges are checked before any device call
+    mesh = X.ClothMeshData.grid_cloth_hanging(4, 4, 0.1)
+    u = X.ClothSimUniforms.default(X.SOLVER_JACOBI, 8)
+    for bad in (dict(solver=2), dict(gs_launch=3), dict(jacobi_block=96)):
+        cfg = X.ClothSimConfig.default(**bad)
+        sim = C.c_void_p()
+        assert L.xpbd_create(mesh._h, C.byref(cfg), C.byref(u), None, C.byref(sim)) == X.XPBD_E_INVALID, bad
 
 
 def test_residual_noalloc_refuses_oversize_before_any_device_call(X):
