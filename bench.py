@@ -454,7 +454,7 @@ def main():
     sweep_ms, _ = sim.profile_frames(PROFILE_FRAMES, X.KERNEL_COLLISION_SWEEP)
     pers_live_ms = None
     if args.persistent:
-        pers_live_ms, _ = sim.profile_frames(PROFILE_FRAMES, X.KERNEL_JACOBI_PERSISTENT)
+        pers_live_ms, pers_launches = sim.profile_frames(PROFILE_FRAMES, X.KERNEL_JACOBI_PERSISTENT)
     jac_rep_ms = sim.time_kernel(X.KERNEL_JACOBI_ITER, 50)
     radix_ms = sim.time_kernel(X.KERNEL_RADIX_PASS, 20)
     copy_ms = sim.time_kernel(X.KERNEL_COPY, 20)
@@ -479,6 +479,8 @@ def main():
     else:   # persistent: the dominant launch is the whole predict + I iterations
         achieved = (I * b_iter) / (pers_live_ms * 1e-3) / 1e9
         rkernel, r_us, r_bytes = "k_jacobi_persistent (predict + I iterations)", pers_live_ms * 1e3, I * b_iter
+        jac_launches = pers_launches
+        traffic = None   # the counter summaries measure the per-iteration kernel, not this launch
     log(f"bench: dominant kernel {rkernel} {r_us:.1f} us = {achieved:.0f} GB/s algorithmic; "
         f"collision sweep {sweep_ms * 1e3:.1f} us; radix pass {radix_ms * 1e3:.1f} us")
 
