@@ -153,8 +153,11 @@ def counter_summary(fetch_dir: str, write_dir: str, kernels=("k_jacobi_tiled<0u"
     if not copy:
         raise SystemExit("bench.py: no calibration copy in the counter CSV")
     kib = CALIBRATION_BYTES / 1024.0
+    wmax = max(max(write.get(k, [0.0])) for k in copy)
+    if wmax < 0.5 * kib:   # only the small state uploads: the 1 GiB copy did not run
+        raise SystemExit(f"bench.py: no {CALIBRATION_BYTES}-byte calibration copy in the counter CSV")
     fs = kib / max(max(fetch[k]) for k in copy)
-    ws = kib / max(max(write[k]) for k in copy)
+    ws = kib / wmax
     rows = []
     for k in fetch:
         if k not in write or k.startswith("__amd_rocclr"):

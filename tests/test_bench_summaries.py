@@ -5,6 +5,8 @@ import csv
 import os
 import sys
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 import bench  # noqa: E402
@@ -55,3 +57,14 @@ def test_counter_summary_scales_by_the_calibration_copy(tmp_path):
     assert rec["bytes_per_launch"] == rec["read_bytes"] + rec["write_bytes"]
     assert rec["collision_sweep_bytes_per_launch"] == (50 * 2 + 10) * 1024
     assert "__amd_rocclr" not in table.split("|---|")[1]
+
+
+def test_counter_summary_refuses_a_run_without_the_calibration_copy(tmp_path):
+    hdr = ["Kernel_Name", "Counter_Name", "Counter_Value"]
+    fd, wd = str(tmp_path / "f"), str(tmp_path / "w")
+    _write(os.path.join(fd, "k_counter_collection.csv"), hdr,
+           [["__amd_rocclr_copyBuffer", "FETCH_SIZE", 3.0], ["xpbd::k_coll_sweep(a)", "FETCH_SIZE", 50.0]])
+    _write(os.path.join(wd, "k_counter_collection.csv"), hdr,
+           [["__amd_rocclr_copyBuffer", "WRITE_SIZE", 2.0], ["xpbd::k_coll_sweep(a)", "WRITE_SIZE", 10.0]])
+    with pytest.raises(SystemExit):
+        bench.counter_summary(fd, wd)
