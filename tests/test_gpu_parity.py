@@ -332,6 +332,33 @@ def test_graph_replay_equals_eager(X, persistent):
     b.close()
 
 
+def test_gs_launch_form_switched_at_run_time(X):
+    """set_config between frames moves coloured GS from one persistent
+    launch per substep to one launch per colour and back (the frame graph is
+    rebuilt each time); the trajectory equals a run that never switches, and
+    the GS substep is timed in each form."""
+    note("case gs launch switch")
+    kw = dict(cols=48, rows=40, spacing=1 / 47, layout=1, height=1.0, angle=0.7, pins=3)
+    th = 2 * 0.4 / 47
+    a, cfg, _ = make(X, X.ClothMeshData.sheet(**kw), solver=1, S=4, I=6, use_graph=1, thickness=th, gs_launch=1,
+                     jacobi_omega=1.0)
+    b, _, _ = make(X, X.ClothMeshData.sheet(**kw), solver=1, S=4, I=6, use_graph=1, thickness=th, gs_launch=1,
+                   jacobi_omega=1.0)
+    for k, form in enumerate((2, 1, 2)):
+        cfg.gs_launch = form
+        a.set_config(cfg)
+        a.step_frames(3)
+        a.sync()
+        b.step_frames(3)
+        b.sync()
+        pa, va = a.read_state()
+        pb, vb = b.read_state()
+        assert bits_equal(pa, pb) and bits_equal(va, vb), f"after switch {k} to gs_launch={form}"
+        assert a.time_kernel(X.KERNEL_GS_SUBSTEP, 3) > 0.0
+    a.close()
+    b.close()
+
+
 def test_profile_frames_counts_launches_and_steps_like_frames(X):
     """The instrumented frames advance the state exactly as step_frames does
     and time one event pair per launch of the chosen kind."""
