@@ -234,7 +234,12 @@ __device__ __forceinline__ void jacobi_entry(float4 pi, float4 pj, float4 pr, fl
         gx = div_rn_nz(gx, len, yl);
         gy = div_rn_nz(gy, len, yl);
         gz = div_rn_nz(gz, len, yl);
-        dlam = div_rn(num, wsum, rcp_rn(wsum));
+        // -C = -0 (len == rest) gives +0 here where '/' gives -0.  dlam only
+        // enters ax + (g * w) * dlam, with finite g * w on fast slices, so
+        // the term is a zero either way; the accumulators start at +0 and a
+        // round-to-nearest sum is -0 only when both addends are, so they are
+        // never -0 and adding either zero leaves them unchanged.
+        dlam = div_rn_nz(num, wsum, rcp_rn(wsum));
     } else {
         gx = gx / len;
         gy = gy / len;
@@ -451,9 +456,12 @@ __device__ __forceinline__ float4 tile_pair(const JacobiArgs& a, const float4* l
     else return stage_pair(a.sell.pairs[w >> 20], a.p);
 }
 
+// pre: the slice's first JT_JU entry words (those below width), loaded by
+// the caller; base = the lane's first entry, width = the slice's width.
 template <uint32_t MODE, bool FAST, bool LDSP>
 __device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, const float4* lpairs, const float4* tl, uint32_t slice, uint32_t lane,
-                                                        uint32_t lidx, bool row_ok, bool& valid) {
+                                                        uint32_t lidx, bool row_ok, bool& valid, const uint32_t* pre,
+                                                        uint32_t base, uint32_t width) {
     const uint32_t i = slice * 64 + lane;
     const DevParams& p = a.p;
     valid = row_ok && slice < p.nslices && i < p.n;
@@ -465,14 +473,12 @@ __device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, con
     const float wi = pi.w;
     if (wi > 0.f) {
         float ax = 0.f, ay = 0.f, az = 0.f;
-        const uint32_t base = a.sell.soff[slice] + lane;
-        const uint32_t width = a.sell.swid[slice];
         uint32_t k0 = 0;
         if (width >= JT_JU) {
             // the next chunk's entry words load while this chunk computes
             uint32_t wn[JT_JU];
 #pragma unroll
-            for (uint32_t u = 0; u < JT_JU; ++u) wn[u] = __builtin_nontemporal_load(a.sell.tpacked + base + u * 64);
+            for (uint32_t u = 0; u < JT_JU; ++u) wn[u] = pre[u];
             for (; k0 + JT_JU <= width; k0 += JT_JU) {
                 uint32_t wc[JT_JU];
 #pragma unroll
@@ -519,6 +525,15 @@ __global__ __launch_bounds__(64 * JT_ROWS) __attribute__((amdgpu_waves_per_eu(8)
     const uint32_t cols = a.sell.tcols, rows = a.sell.trows, ntx = cols / 64;
     const uint32_t b = a.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t tx = b % ntx, ty = b / ntx;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t row = ty * JT_ROWS + wave;
+    const bool row_ok = row < rows;
+    const uint32_t slice = row_ok ? row * ntx + tx : a.p.nslices;
+    uint32_t base = 0, width = 0;
+    if (slice < a.p.nslices) {
+        base = a.sell.soff[slice] + lane;
+        width = a.sell.swid[slice];
+    }
     const int32_t r0 = int32_t(ty * JT_ROWS) - 2, c0 = int32_t(tx * 64) - 2;
     bool tame = true;
     for (uint32_t q = threadIdx.x; q < JT_H * JT_W; q += blockDim.x) {
@@ -530,24 +545,27 @@ __global__ __launch_bounds__(64 * JT_ROWS) __attribute__((amdgpu_waves_per_eu(8)
             tame = tame && tame3(v);
         }
     }
+    // the first chunk of entry words is requested before the barrier, so its
+    // latency overlaps the other wavefronts' staging (-0.6 us per iteration
+    // on the 1024^2 sheet against loading it after the barrier)
+    uint32_t pre[JT_JU];
+#pragma unroll
+    for (uint32_t u = 0; u < JT_JU; ++u)
+        pre[u] = u < width ? __builtin_nontemporal_load(a.sell.tpacked + base + u * 64) : 0u;
     const bool lds = stage_pairs<true, JT_PAIRS_MAX>(a, lpairs);
     // The block's entries read only staged positions, so the fast operations
     // need only these to be tame (predicted ones, in JM_PREDICT): one
     // block-wide AND, which is also the barrier that publishes the tile.
     const bool fast = FASTDIV && __syncthreads_and(tame ? 1 : 0) != 0;
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t row = ty * JT_ROWS + wave;
-    const bool row_ok = row < rows;
-    const uint32_t slice = row_ok ? row * ntx + tx : a.p.nslices;
     const uint32_t lidx = (wave + 2) * JT_W + lane + 2;
     bool valid;
     float4 out;
     if (lds)
-        out = fast ? jacobi_particle_tiled<MODE, true, true>(a, lpairs, tl, slice, lane, lidx, row_ok, valid)
-                   : jacobi_particle_tiled<MODE, false, true>(a, lpairs, tl, slice, lane, lidx, row_ok, valid);
+        out = fast ? jacobi_particle_tiled<MODE, true, true>(a, lpairs, tl, slice, lane, lidx, row_ok, valid, pre, base, width)
+                   : jacobi_particle_tiled<MODE, false, true>(a, lpairs, tl, slice, lane, lidx, row_ok, valid, pre, base, width);
     else
-        out = fast ? jacobi_particle_tiled<MODE, true, false>(a, lpairs, tl, slice, lane, lidx, row_ok, valid)
-                   : jacobi_particle_tiled<MODE, false, false>(a, lpairs, tl, slice, lane, lidx, row_ok, valid);
+        out = fast ? jacobi_particle_tiled<MODE, true, false>(a, lpairs, tl, slice, lane, lidx, row_ok, valid, pre, base, width)
+                   : jacobi_particle_tiled<MODE, false, false>(a, lpairs, tl, slice, lane, lidx, row_ok, valid, pre, base, width);
     if (!(MODE & JM_FINAL) && a.tame_out)
         write_slice_tame(a.tame_out, slice, slice < a.p.nslices, !valid || tame3(out));
     if (MODE & JM_AABB) {
@@ -786,7 +804,7 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const uint32_t* keys
 template <uint32_t DB>
 __global__ __launch_bounds__(RADIX_BLOCK) void k_coll_keys(const float4* F, uint32_t n, const CollGrid* grids,
                                                          const uint32_t* cloth_of, uint32_t* keys, uint32_t* ids,
-                                                         uint2* table, const uint32_t* slot_rec, uint32_t* hist,
+                                                         uint4* table, const uint32_t* slot_rec, uint32_t* hist,
                                                          uint32_t ntiles, const float* part, uint32_t nparts,
                                                          float cell_size, CollGrid* grid_out) {
     constexpr uint32_t MASK = (1u << DB) - 1u;
@@ -848,7 +866,7 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_coll_keys(const float4* F, uint
 #pragma unroll
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
         const uint32_t i = base + r * 64;
-        if (sl[r] != EMPTY_KEY) table[sl[r]] = make_uint2(EMPTY_KEY, EMPTY_KEY);
+        if (sl[r] != EMPTY_KEY) table[sl[r]] = make_uint4(EMPTY_KEY, EMPTY_KEY, 0u, 0u);
         if (i < n) {
             const CollGrid g = cloth_of ? grids[co[r]] : g0;
             uint32_t c[3];
@@ -1004,9 +1022,15 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
     }
 }
 
-// Hash table: group of 8 consecutive x-cells (key >> 3) -> first sorted
-// index of the group.  A 3-cell row [kmin, kmax] spans one or two groups, so
-// one probe (rarely two) finds where its particles start; one insert per
+// Hash table: group of 8 consecutive x-cells (key >> 3) -> (group, first
+// sorted index of the group, cell offsets).  The offsets are 8 bytes: byte
+// k - 1 (k = 1..8) counts the group's particles in its cells below k, so the
+// particles of cell k sit at [start + off[k], start + off[k + 1]) with
+// off[0] = 0 and off[8] the group's count.  A group of more than 255
+// particles stores no offsets (off[8] = 0; a stored group is never empty):
+// rows through it are scanned from the group start with key tests.  A 3-cell
+// row [kmin, kmax] spans one or two groups, so one probe (two where it
+// crosses a group boundary) gives its exact sorted range; one insert per
 // occupied group keeps the atomic traffic ~8x below one per cell.  The table
 // has >= 4 slots per particle (>= 32 per occupied group on a dense sheet):
 // a wave waits for its longest probe chain, and on a 1024x1024 sheet halving
@@ -1028,17 +1052,26 @@ __device__ __forceinline__ uint32_t slot_of(uint32_t g, uint32_t mask) {
 
 // Lookup whose first probe `e` (the entry at slot_of(g)) was loaded by the
 // caller: most lookups end there; a collision continues at the next slot.
-__device__ __forceinline__ uint32_t table_resolve(const uint2* table, uint32_t mask, uint32_t g, uint2 e) {
-    if (e.x == g) return e.y;
-    if (e.x == EMPTY_KEY) return EMPTY_KEY;
+// Returns the group's entry, or one with .y = EMPTY_KEY if it is absent.
+__device__ __forceinline__ uint4 table_resolve(const uint4* table, uint32_t mask, uint32_t g, uint4 e) {
+    const uint4 none = make_uint4(EMPTY_KEY, EMPTY_KEY, 0u, 0u);
+    if (e.x == g) return e;
+    if (e.x == EMPTY_KEY) return none;
     uint32_t h = (slot_of(g, mask) + 1) & mask;
     for (uint32_t probe = 1; probe <= mask; ++probe) {
-        const uint2 f = table[h];
-        if (f.x == g) return f.y;
-        if (f.x == EMPTY_KEY) return EMPTY_KEY;
+        const uint4 f = table[h];
+        if (f.x == g) return f;
+        if (f.x == EMPTY_KEY) return none;
         h = (h + 1) & mask;
     }
-    return EMPTY_KEY;
+    return none;
+}
+
+// off[k] of a table entry (k = 0..8; see above).
+__device__ __forceinline__ uint32_t group_off(const uint4& e, uint32_t k) {
+    if (k == 0) return 0u;
+    const uint32_t w = k <= 4 ? e.z : e.w;
+    return (w >> (8u * ((k - 1u) & 3u))) & 0xFFu;
 }
 
 // Pair kernel of self_collision_resolve (xpbd_cpu.rs:84-121) for the pair
@@ -1091,52 +1124,81 @@ __device__ __forceinline__ bool collision_contrib(uint32_t i, uint32_t j, float4
 // the contributions run in one short convergent loop (frame 65 of the
 // 1024^2 sheet: 146 -> 88 us).
 constexpr int LMAX = 8;   // candidates inside the contact distance kept per particle
-// candidates loaded per window of a row scan: 5 measured best on the 1024^2
-// sheet and the 512-cloth batch (4..12 tried; wider windows cost occupancy,
-// narrower ones more window steps)
-constexpr uint32_t SW = 5;
+// candidates loaded per window of a row scan.  With the table's cell offsets
+// a row starts at its first particle and most rows hold 1-3: on the 1024^2
+// sheet 4 measured best (sweep 42.9 us; 3, 5, 6: 44.5, 44.8, 47.9 us).
+constexpr uint32_t SW = 4;
 
+// A neighbour row's sorted range.  The scan starts at lo, tests each key
+// against [kmin, kmax] and stops past kmax or at hi.  With the table's
+// offsets lo is the row's first particle, and hi its end when the row lies
+// in one group (n otherwise: the scan's key test ends it); a group without
+// offsets gives its start.  lo = EMPTY_KEY: no particles.
 struct RowRange {
-    uint32_t start, kmin, kmax;
+    uint32_t lo, hi, kmin, kmax;
 };
 
-__device__ __forceinline__ void sweep_rows(const CollArgs& a, const CollGrid& g, const uint32_t c[3],
-                                           RowRange rows[9]) {
-    uint32_t g0[9], g1[9], st[9];
+// The 3 rows of the neighbourhood at z offset dz - 1 (rows dy = 0, 1, 2):
+// their first groups are probed together, then the second groups of rows
+// whose first group is empty and that cross into the next group.
+__device__ __forceinline__ void sweep_rows3(const CollArgs& a, const CollGrid& g, const uint32_t c[3], int dz,
+                                            RowRange rows[3]) {
+    uint32_t g0[3], g1[3];
+    const int nz = int(c[2]) + dz - 1;
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        rows[r].start = EMPTY_KEY;
+    for (int r = 0; r < 3; ++r) {
+        rows[r].lo = EMPTY_KEY;
+        rows[r].hi = 0u;
         rows[r].kmin = 1u;
         rows[r].kmax = 0u;
         g0[r] = g1[r] = EMPTY_KEY;
-        const int ny = int(c[1]) + (r % 3) - 1, nz = int(c[2]) + (r / 3) - 1;
+        const int ny = int(c[1]) + r - 1;
         if (ny < 0 || nz < 0 || ny >= int(g.dims[1]) || nz >= int(g.dims[2])) continue;
         const uint32_t kb = g.key_base + g.dims[0] * (uint32_t(ny) + g.dims[1] * uint32_t(nz));
         rows[r].kmin = kb + (c[0] > 0 ? c[0] - 1 : 0);
         rows[r].kmax = kb + (c[0] + 1 < g.dims[0] ? c[0] + 1 : c[0]);
         g0[r] = rows[r].kmin >> GROUP_SHIFT;
-        g1[r] = rows[r].kmax >> GROUP_SHIFT;
+        const uint32_t gk = rows[r].kmax >> GROUP_SHIFT;
+        g1[r] = gk != g0[r] ? gk : EMPTY_KEY;
     }
-    // first probes of all rows issued together (independent loads), then
-    // resolved; the same for the rows whose first group is empty
-    // (unconditional loads, slot 0 for absent rows, so no branch splits them)
-    uint2 e[9];
+    // unconditional loads (slot 0 for absent rows), so no branch splits them
+    uint4 e[3];
 #pragma unroll
-    for (int r = 0; r < 9; ++r) e[r] = a.table[g0[r] != EMPTY_KEY ? slot_of(g0[r], a.table_mask) : 0u];
+    for (int r = 0; r < 3; ++r) e[r] = a.table[g0[r] != EMPTY_KEY ? slot_of(g0[r], a.table_mask) : 0u];
+    const uint32_t n = a.p.n;
+    bool second[3];
 #pragma unroll
-    for (int r = 0; r < 9; ++r)
-        st[r] = g0[r] != EMPTY_KEY ? table_resolve(a.table, a.table_mask, g0[r], e[r]) : EMPTY_KEY;
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        const bool second = st[r] == EMPTY_KEY && g1[r] != g0[r] && g1[r] != EMPTY_KEY;
-        g1[r] = second ? g1[r] : EMPTY_KEY;
-        e[r] = a.table[second ? slot_of(g1[r], a.table_mask) : 0u];
+    for (int r = 0; r < 3; ++r) {
+        second[r] = false;
+        if (g0[r] == EMPTY_KEY) continue;
+        const uint4 f = table_resolve(a.table, a.table_mask, g0[r], e[r]);
+        if (f.y == EMPTY_KEY) {
+            second[r] = g1[r] != EMPTY_KEY;
+            continue;
+        }
+        const uint32_t cnt = group_off(f, 8);
+        if (cnt == 0u) {   // no offsets
+            rows[r].lo = f.y;
+            rows[r].hi = n;
+        } else {
+            rows[r].lo = f.y + group_off(f, rows[r].kmin & 7u);
+            rows[r].hi = g1[r] == EMPTY_KEY ? f.y + group_off(f, (rows[r].kmax & 7u) + 1u) : n;
+        }
     }
 #pragma unroll
-    for (int r = 0; r < 9; ++r)
-        if (g1[r] != EMPTY_KEY) st[r] = table_resolve(a.table, a.table_mask, g1[r], e[r]);
+    for (int r = 0; r < 3; ++r) e[r] = a.table[second[r] ? slot_of(g1[r], a.table_mask) : 0u];
 #pragma unroll
-    for (int r = 0; r < 9; ++r) rows[r].start = st[r];
+    for (int r = 0; r < 3; ++r) {
+        if (!second[r]) continue;
+        const uint4 f = table_resolve(a.table, a.table_mask, g1[r], e[r]);
+        if (f.y == EMPTY_KEY) continue;
+        const uint32_t cnt = group_off(f, 8);
+        rows[r].lo = f.y;
+        rows[r].hi = cnt == 0u ? n : f.y + group_off(f, (rows[r].kmax & 7u) + 1u);
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        if (rows[r].lo != EMPTY_KEY && rows[r].lo >= rows[r].hi) rows[r].lo = EMPTY_KEY;
 }
 
 // Squared distance of two particles; symmetric bit for bit (the components
@@ -1184,8 +1246,6 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     const float th2 = a.p.thickness * a.p.thickness;
     uint32_t c[3];
     cell_of(fi, g, c);
-    RowRange rows[9];
-    sweep_rows(a, g, c, rows);
 
     // the scan only collects the sorted indices of candidates inside the
     // contact distance (cheap, convergent); their partner ids are loaded
@@ -1197,29 +1257,33 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     __shared__ uint32_t cand[LMAX + 1][256];
     const uint32_t t = threadIdx.x;
     uint32_t cnt = 0;
+    for (int dz = 0; dz < 3; ++dz) {
+    RowRange rows[3];
+    sweep_rows3(a, g, c, dz, rows);
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        if (rows[r].start == EMPTY_KEY) continue;
-        const uint32_t kmin = rows[r].kmin, kmax = rows[r].kmax;
-        uint32_t idx = rows[r].start;
+    for (int r = 0; r < 3; ++r) {
+        if (rows[r].lo == EMPTY_KEY) continue;
+        const uint32_t kmin = rows[r].kmin, kmax = rows[r].kmax, hi = rows[r].hi;
+        uint32_t idx = rows[r].lo;
         for (;;) {
             float4 ff[SW];
 #pragma unroll
-            for (uint32_t u = 0; u < SW; ++u) ff[u] = a.sP[min(idx + u, n - 1)];
+            for (uint32_t u = 0; u < SW; ++u) ff[u] = a.sP[min(idx + u, hi - 1)];
             uint32_t last = 0;
 #pragma unroll
             for (uint32_t u = 0; u < SW; ++u) {
                 const uint32_t q = idx + u;
-                const uint32_t key = q < n ? __float_as_uint(ff[u].w) : EMPTY_KEY;
+                const uint32_t key = q < hi ? __float_as_uint(ff[u].w) : EMPTY_KEY;
                 const float d2 = dist2_of(fi, ff[u]);
                 const bool hit = key >= kmin && key <= kmax && q != s && !(d2 > th2 || d2 < 1e-18f);
                 cand[hit ? min(cnt, uint32_t(LMAX)) : uint32_t(LMAX)][t] = q;
                 cnt += hit ? 1u : 0u;
                 last = key;
             }
-            if (last > kmax || idx + SW >= n) break;   // the row ends inside this window
+            if (last > kmax || idx + SW >= hi) break;   // the row ends inside this window
             idx += SW;
         }
+    }
     }
     uint32_t lq[LMAX];
 #pragma unroll
@@ -1296,12 +1360,13 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
         uint32_t c[3];
         cell_of(fi, g, c);
         RowRange rows[9];
-        sweep_rows(a, g, c, rows);
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz) sweep_rows3(a, g, c, dz, rows + 3 * dz);
         int cnt = 0;
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
-            if (rows[r].start == EMPTY_KEY) continue;
-            for (uint32_t idx = rows[r].start; idx < n; ++idx) {
+            if (rows[r].lo == EMPTY_KEY) continue;
+            for (uint32_t idx = rows[r].lo; idx < rows[r].hi; ++idx) {
                 const float4 fj = a.sP[idx];
                 const uint32_t key = __float_as_uint(fj.w);
                 if (key > rows[r].kmax) break;
@@ -1341,8 +1406,8 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
                 float bx = 0.f, by = 0.f, bz = 0.f;
 #pragma unroll
                 for (int r = 0; r < 9; ++r) {
-                    if (rows[r].start == EMPTY_KEY) continue;
-                    for (uint32_t idx = rows[r].start; idx < n; ++idx) {
+                    if (rows[r].lo == EMPTY_KEY) continue;
+                    for (uint32_t idx = rows[r].lo; idx < rows[r].hi; ++idx) {
                         const float4 fj = a.sP[idx];
                         const uint32_t key = __float_as_uint(fj.w);
                         if (key > rows[r].kmax) break;
@@ -1374,28 +1439,81 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
     }
 }
 
+// Cell offsets of the group starting at sorted index s (see the table
+// notes): the group's keys are read 8 at a time until one leaves the group;
+// a group of more than 255 particles gets none (both words 0).  Keys below
+// index lend come from the block's staged copy lk (index - l0), later ones
+// from skeys.
+constexpr uint32_t GATHER_BLOCK = 256;
+constexpr uint32_t GATHER_STAGED = 2 * GATHER_BLOCK;   // the block's keys and the next block's
+
+__device__ __forceinline__ uint2 group_offsets(const uint32_t* lk, uint32_t l0, uint32_t lend, const uint32_t* skeys,
+                                               uint32_t n, uint32_t s, uint32_t g) {
+    uint32_t lo = 0u, hi = 0u, cnt = 0u;
+    for (uint32_t b = s; b < n; b += 8) {
+        uint32_t kk[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            const uint32_t q = b + u;
+            kk[u] = q < lend ? lk[q - l0] : (q < n ? skeys[q] : EMPTY_KEY);
+        }
+        bool more = true;
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            const bool in = more && (kk[u] >> GROUP_SHIFT) == g && kk[u] != EMPTY_KEY;
+            more = in;
+            if (in) {
+                // byte k - 1 (k = 1..8) counts the cells below k: add 1 to
+                // bytes k - 1 >= c of the particle's cell c
+                const uint32_t c = kk[u] & 7u;
+                lo += c < 4u ? (0x01010101u << (8u * c)) : 0u;
+                hi += c < 4u ? 0x01010101u : (0x01010101u << (8u * (c - 4u)));
+                ++cnt;
+            }
+        }
+        if (!more || cnt > 255u) break;
+    }
+    return cnt > 255u ? make_uint2(0u, 0u) : make_uint2(lo, hi);
+}
+
 // The sorted copy for the sweep: sP = (position, key bits), so one 16-byte
 // load gives a candidate's key and position (rest positions and masses are
 // read by particle id, for contacts only).  The first sorted index of each
-// group is inserted into the group table (slot recorded in slot_rec,
-// EMPTY_KEY elsewhere, for k_coll_keys to empty next pass).  Also clears the
-// sweep's overflow count.
-__global__ void k_coll_gather(const float4* F, const uint32_t* sids, const uint32_t* skeys, uint32_t n, float4* sP,
-                              uint32_t* ovf, uint2* table, uint32_t mask, uint32_t* slot_rec) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+// group is inserted into the group table with its cell offsets (slot
+// recorded in slot_rec, EMPTY_KEY elsewhere, for k_coll_keys to empty next
+// pass); the block stages its keys and the next block's in LDS, so the
+// offsets of groups shorter than that read no global memory.  Also clears
+// the sweep's overflow count.
+__global__ __launch_bounds__(GATHER_BLOCK) void k_coll_gather(const float4* F, const uint32_t* sids,
+                                                            const uint32_t* skeys, uint32_t n, float4* sP,
+                                                            uint32_t* ovf, uint4* table, uint32_t mask,
+                                                            uint32_t* slot_rec) {
+    __shared__ uint32_t lk[GATHER_STAGED];
+    const uint32_t l0 = blockIdx.x * GATHER_BLOCK;
+    const uint32_t s = l0 + threadIdx.x;
     if (ovf && s == 0) ovf[0] = 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < GATHER_STAGED / GATHER_BLOCK; ++u) {
+        const uint32_t q = l0 + u * GATHER_BLOCK + threadIdx.x;
+        lk[u * GATHER_BLOCK + threadIdx.x] = q < n ? skeys[q] : EMPTY_KEY;
+    }
+    __syncthreads();
     if (s >= n) return;
     const uint32_t i = sids[s];
-    const uint32_t key = skeys[s];
+    const uint32_t key = lk[threadIdx.x];
     const float4 f = F[i];
     sP[s] = make_float4(f.x, f.y, f.z, __uint_as_float(key));
     uint32_t rec = EMPTY_KEY;
     const uint32_t g = key >> GROUP_SHIFT;
-    if (s == 0 || (skeys[s - 1] >> GROUP_SHIFT) != g) {
+    const uint32_t prev = threadIdx.x > 0 ? lk[threadIdx.x - 1] : (s > 0 ? skeys[s - 1] : EMPTY_KEY);
+    if (s == 0 || (prev >> GROUP_SHIFT) != g) {
+        const uint2 off = group_offsets(lk, l0, min(l0 + GATHER_STAGED, n), skeys, n, s, g);
         uint32_t h = slot_of(g, mask);
         for (uint32_t probe = 0; probe <= mask; ++probe) {
             if (atomicCAS(&table[h].x, EMPTY_KEY, g) == EMPTY_KEY) {
                 table[h].y = s;
+                table[h].z = off.x;
+                table[h].w = off.y;
                 rec = h;
                 break;
             }
@@ -2098,7 +2216,7 @@ void launch_coll_grids(const float4* P, const uint2* span, uint32_t nspans, cons
 uint32_t cloth_span_max() { return CB_SPAN; }
 
 void launch_coll_keys(const float4* F, uint32_t n, CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
-                      uint32_t* ids, uint2* table, const uint32_t* slot_rec, uint32_t* hist, uint32_t digit_bits,
+                      uint32_t* ids, uint4* table, const uint32_t* slot_rec, uint32_t* hist, uint32_t digit_bits,
                       const float* part, uint32_t nparts, float cell_size, hipStream_t s) {
     if (n == 0) return;
     const uint32_t ntiles = std::max(1u, cdiv(n, RADIX_TILE));
@@ -2147,10 +2265,10 @@ bool launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32
 }
 
 void launch_coll_gather(const float4* F, const uint32_t* sids, const uint32_t* skeys, uint32_t n, float4* sP,
-                        uint32_t* ovf, uint2* table, uint32_t mask, uint32_t* slot_rec, hipStream_t s) {
+                        uint32_t* ovf, uint4* table, uint32_t mask, uint32_t* slot_rec, hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_coll_gather, dim3(cdiv(n, 256)), dim3(256), 0, s, F, sids, skeys, n, sP, ovf, table, mask,
-                       slot_rec);
+    hipLaunchKernelGGL(k_coll_gather, dim3(cdiv(n, GATHER_BLOCK)), dim3(GATHER_BLOCK), 0, s, F, sids, skeys, n, sP,
+                       ovf, table, mask, slot_rec);
 }
 
 void launch_coll_sweep(const CollArgs& a, hipStream_t s) {

@@ -132,7 +132,7 @@ struct CollArgs {
     const uint32_t* cloth_of;   // particle -> cloth (nullptr: one cloth)
     const uint32_t* skeys;
     const uint32_t* sids;
-    const uint2* table;
+    const uint4* table;
     uint32_t table_mask;
     DevParams p;
     // particles with more than LMAX candidates, resolved by a second kernel:
@@ -189,7 +189,7 @@ uint32_t cloth_span_max();   // particles per piece at most
 // part != nullptr (one cloth): the grid is derived from the nparts bounds
 // partials and stored to grids[0]; otherwise grids[] is read (batches).
 void launch_coll_keys(const float4* F, uint32_t n, CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
-                      uint32_t* ids, uint2* table, const uint32_t* slot_rec, uint32_t* hist, uint32_t digit_bits,
+                      uint32_t* ids, uint4* table, const uint32_t* slot_rec, uint32_t* hist, uint32_t digit_bits,
                       const float* part, uint32_t nparts, float cell_size, hipStream_t s);
 // Stable sort of (keys, ids) by key: 4 passes of 8-bit digits (32-bit keys) or
 // 3 passes of 9-bit digits (keys below 2^27: one cloth's grid).  Returns true
@@ -206,7 +206,7 @@ void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout,
 // inserts (slots recorded in slot_rec, n words; the table must be empty on
 // entry), overflow count clear
 void launch_coll_gather(const float4* F, const uint32_t* sids, const uint32_t* skeys, uint32_t n, float4* sP,
-                        uint32_t* ovf, uint2* table, uint32_t mask, uint32_t* slot_rec, hipStream_t s);
+                        uint32_t* ovf, uint4* table, uint32_t mask, uint32_t* slot_rec, hipStream_t s);
 void launch_coll_sweep(const CollArgs& a, hipStream_t s);
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
 // The constraint iterations of a GS substep as one launch per colour and

@@ -93,7 +93,7 @@ struct xpbd_cloth {
     float* near_rest = nullptr;
     uint32_t* gs_off = nullptr;
     uint32_t *keys = nullptr, *ids = nullptr, *keys2 = nullptr, *ids2 = nullptr, *hist = nullptr;
-    uint2* table = nullptr;
+    uint4* table = nullptr;
     uint32_t table_slots = 0;
     uint32_t* ovf = nullptr;       // collision overflow list (count, done, n sorted indices)
     float* aabb = nullptr;         // bounds partials, 8 floats per producing block / cloth
@@ -843,7 +843,7 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     // still leaves load <= 1/4 up to the particle limit checked at creation.
     h->table_slots = next_pow2(uint32_t(std::min<size_t>(std::max<size_t>(4 * size_t(n), 64), 0x80000000u)));
     TRY(dalloc(h, &h->table, h->table_slots));
-    HIPC(h, hipMemset(h->table, 0xFF, size_t(h->table_slots) * sizeof(uint2)));   // every slot empty
+    HIPC(h, hipMemset(h->table, 0xFF, size_t(h->table_slots) * sizeof(uint4)));   // every slot empty
     TRY(dalloc(h, &h->slot_rec, n));
     HIPC(h, hipMemset(h->slot_rec, 0xFF, size_t(n) * sizeof(uint32_t)));   // no slot recorded
     TRY(dalloc(h, &h->ovf, size_t(n) + 2));
