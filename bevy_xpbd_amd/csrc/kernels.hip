@@ -927,12 +927,15 @@ __global__ __launch_bounds__(256) void k_radix_scan_tiles(uint32_t* hist, uint32
 
 // Stable scatter: the tile is ranked in input order (see above), placed in
 // LDS in digit order, and written out in that order, so each digit's run of
-// the tile leaves as contiguous stores.
-template <uint32_t DB>
+// the tile leaves as contiguous stores.  GATHER (the last pass of a
+// collision sort): also the sweep's sorted copy sP[dst] = (F[id] xyz, key
+// bits), read while the ids are at hand.
+template <uint32_t DB, bool GATHER>
 __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* kin, const uint32_t* iin,
                                                              uint32_t* kout, uint32_t* iout, uint32_t n,
                                                              uint32_t shift, const uint32_t* hist,
-                                                             uint32_t ntiles, const uint32_t* dtot) {
+                                                             uint32_t ntiles, const uint32_t* dtot,
+                                                             const float4* F, float4* sP) {
     constexpr uint32_t DIGITS = 1u << DB, MASK = DIGITS - 1u, DPT = radix_dpt(DIGITS);
     __shared__ uint32_t wcnt[RADIX_WAVES][DIGITS];   // per-wavefront digit counts, then its start in the tile
     __shared__ int32_t gbase[DIGITS];                 // global position of local sorted position 0 of digit d
@@ -1010,6 +1013,26 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
         }
     }
     __syncthreads();
+    if (GATHER) {
+        float4 f[RADIX_ITEMS];
+#pragma unroll
+        for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+            const uint32_t q = r * RADIX_BLOCK + t;
+            f[r] = F[sid[q < count ? q : 0u]];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+            const uint32_t q = r * RADIX_BLOCK + t;
+            if (q < count) {
+                const uint32_t key = skey[q];
+                const uint32_t dst = uint32_t(gbase[(key >> shift) & MASK] + int32_t(q));
+                kout[dst] = key;
+                iout[dst] = sid[q];
+                sP[dst] = make_float4(f[r].x, f[r].y, f[r].z, __uint_as_float(key));
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
         const uint32_t q = r * RADIX_BLOCK + t;
@@ -1338,7 +1361,7 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
 // LDS (one list per thread); more than that falls back to repeated selection
 // of the next-smallest partner.  Either way the sum runs in ascending partner
 // id, the brute-force loop's order.  The list's count is cleared by
-// k_coll_gather, which runs before every sweep.
+// k_coll_table, which runs before every sweep.
 constexpr int OVF_MAX = 32;
 constexpr uint32_t OVF_BLOCK = 64;
 
@@ -1476,18 +1499,13 @@ __device__ __forceinline__ uint2 group_offsets(const uint32_t* lk, uint32_t l0, 
     return cnt > 255u ? make_uint2(0u, 0u) : make_uint2(lo, hi);
 }
 
-// The sorted copy for the sweep: sP = (position, key bits), so one 16-byte
-// load gives a candidate's key and position (rest positions and masses are
-// read by particle id, for contacts only).  The first sorted index of each
-// group is inserted into the group table with its cell offsets (slot
-// recorded in slot_rec, EMPTY_KEY elsewhere, for k_coll_keys to empty next
-// pass); the block stages its keys and the next block's in LDS, so the
-// offsets of groups shorter than that read no global memory.  Also clears
-// the sweep's overflow count.
-__global__ __launch_bounds__(GATHER_BLOCK) void k_coll_gather(const float4* F, const uint32_t* sids,
-                                                            const uint32_t* skeys, uint32_t n, float4* sP,
-                                                            uint32_t* ovf, uint4* table, uint32_t mask,
-                                                            uint32_t* slot_rec) {
+// Group-table inserts over the sorted keys: the first sorted index of each
+// group is inserted with its cell offsets (slot recorded in slot_rec,
+// EMPTY_KEY elsewhere, for k_coll_keys to empty next pass); the block stages
+// its keys and the next block's in LDS, so the offsets of groups shorter
+// than that read no global memory.  Also clears the sweep's overflow count.
+__global__ __launch_bounds__(GATHER_BLOCK) void k_coll_table(const uint32_t* skeys, uint32_t n, uint32_t* ovf,
+                                                           uint4* table, uint32_t mask, uint32_t* slot_rec) {
     __shared__ uint32_t lk[GATHER_STAGED];
     const uint32_t l0 = blockIdx.x * GATHER_BLOCK;
     const uint32_t s = l0 + threadIdx.x;
@@ -1499,10 +1517,7 @@ __global__ __launch_bounds__(GATHER_BLOCK) void k_coll_gather(const float4* F, c
     }
     __syncthreads();
     if (s >= n) return;
-    const uint32_t i = sids[s];
     const uint32_t key = lk[threadIdx.x];
-    const float4 f = F[i];
-    sP[s] = make_float4(f.x, f.y, f.z, __uint_as_float(key));
     uint32_t rec = EMPTY_KEY;
     const uint32_t g = key >> GROUP_SHIFT;
     const uint32_t prev = threadIdx.x > 0 ? lk[threadIdx.x - 1] : (s > 0 ? skeys[s - 1] : EMPTY_KEY);
@@ -2233,42 +2248,52 @@ uint32_t radix_hist_words(uint32_t n) {
 }
 
 void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
-                       uint32_t shift, uint32_t* hist, hipStream_t s, bool hist_done, uint32_t digit_bits) {
+                       uint32_t shift, uint32_t* hist, hipStream_t s, bool hist_done, uint32_t digit_bits,
+                       const float4* F, float4* sP) {
     const uint32_t ntiles = std::max(1u, cdiv(n, RADIX_TILE));
     const uint32_t digits = 1u << digit_bits;
     uint32_t* dtot = hist + size_t(digits) * ntiles;
+    const dim3 g(ntiles), b(RADIX_BLOCK);
     if (digit_bits == 9) {
-        if (!hist_done)
-            hipLaunchKernelGGL(k_radix_hist<9>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, n, shift, hist, ntiles);
+        if (!hist_done) hipLaunchKernelGGL(k_radix_hist<9>, g, b, 0, s, kin, n, shift, hist, ntiles);
         hipLaunchKernelGGL(k_radix_scan_tiles, dim3(digits), dim3(256), 0, s, hist, ntiles, dtot);
-        hipLaunchKernelGGL(k_radix_scatter<9>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, iin, kout, iout, n, shift,
-                           hist, ntiles, dtot);
+        if (sP)
+            hipLaunchKernelGGL((k_radix_scatter<9, true>), g, b, 0, s, kin, iin, kout, iout, n, shift, hist, ntiles,
+                               dtot, F, sP);
+        else
+            hipLaunchKernelGGL((k_radix_scatter<9, false>), g, b, 0, s, kin, iin, kout, iout, n, shift, hist, ntiles,
+                               dtot, F, sP);
     } else {
-        if (!hist_done)
-            hipLaunchKernelGGL(k_radix_hist<8>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, n, shift, hist, ntiles);
+        if (!hist_done) hipLaunchKernelGGL(k_radix_hist<8>, g, b, 0, s, kin, n, shift, hist, ntiles);
         hipLaunchKernelGGL(k_radix_scan_tiles, dim3(digits), dim3(256), 0, s, hist, ntiles, dtot);
-        hipLaunchKernelGGL(k_radix_scatter<8>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, kin, iin, kout, iout, n, shift,
-                           hist, ntiles, dtot);
+        if (sP)
+            hipLaunchKernelGGL((k_radix_scatter<8, true>), g, b, 0, s, kin, iin, kout, iout, n, shift, hist, ntiles,
+                               dtot, F, sP);
+        else
+            hipLaunchKernelGGL((k_radix_scatter<8, false>), g, b, 0, s, kin, iin, kout, iout, n, shift, hist, ntiles,
+                               dtot, F, sP);
     }
 }
 
 bool launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,
-                       uint32_t* hist, hipStream_t s, bool first_hist_done, uint32_t digit_bits) {
+                       uint32_t* hist, hipStream_t s, bool first_hist_done, uint32_t digit_bits, const float4* F,
+                       float4* sP) {
     const uint32_t passes = digit_bits == 9 ? 3u : 4u;
     if (n == 0) return false;
     for (uint32_t p = 0; p < passes; ++p) {
-        const bool odd = p & 1u;
+        const bool odd = p & 1u, last = p + 1 == passes;
         launch_radix_pass(odd ? keys_alt : keys, odd ? ids_alt : ids, odd ? keys : keys_alt, odd ? ids : ids_alt, n,
-                          p * digit_bits, hist, s, p == 0 && first_hist_done, digit_bits);
+                          p * digit_bits, hist, s, p == 0 && first_hist_done, digit_bits, last ? F : nullptr,
+                          last ? sP : nullptr);
     }
     return passes & 1u;
 }
 
-void launch_coll_gather(const float4* F, const uint32_t* sids, const uint32_t* skeys, uint32_t n, float4* sP,
-                        uint32_t* ovf, uint4* table, uint32_t mask, uint32_t* slot_rec, hipStream_t s) {
+void launch_coll_table(const uint32_t* skeys, uint32_t n, uint32_t* ovf, uint4* table, uint32_t mask,
+                       uint32_t* slot_rec, hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_coll_gather, dim3(cdiv(n, GATHER_BLOCK)), dim3(GATHER_BLOCK), 0, s, F, sids, skeys, n, sP,
-                       ovf, table, mask, slot_rec);
+    hipLaunchKernelGGL(k_coll_table, dim3(cdiv(n, GATHER_BLOCK)), dim3(GATHER_BLOCK), 0, s, skeys, n, ovf, table, mask,
+                       slot_rec);
 }
 
 void launch_coll_sweep(const CollArgs& a, hipStream_t s) {

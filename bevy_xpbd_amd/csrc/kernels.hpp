@@ -136,7 +136,7 @@ struct CollArgs {
     uint32_t table_mask;
     DevParams p;
     // particles with more than LMAX candidates, resolved by a second kernel:
-    // [0] count (cleared by launch_coll_gather), [2..] sorted indices; n + 2 words
+    // [0] count (cleared by launch_coll_table), [2..] sorted indices; n + 2 words
     uint32_t* ovf;
     uint32_t xcd_seg;   // sweep blocks per XCD segment (0: block b sweeps chunk b)
 };
@@ -196,17 +196,19 @@ void launch_coll_keys(const float4* F, uint32_t n, CollGrid* grids, const uint32
 // when the result is in (keys_alt, ids_alt) (an odd pass count), false when it
 // is in (keys, ids).  first_hist_done: hist already holds pass 0's tile counts
 // (launch_coll_keys with the same digit_bits).
+// F, sP (a collision sort): the last pass also writes the sweep's sorted
+// copy sP[s] = (F[id] xyz, key bits).
 bool launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32_t* ids_alt, uint32_t n,
-                       uint32_t* hist, hipStream_t s, bool first_hist_done, uint32_t digit_bits);
+                       uint32_t* hist, hipStream_t s, bool first_hist_done, uint32_t digit_bits,
+                       const float4* F = nullptr, float4* sP = nullptr);
 uint32_t radix_hist_words(uint32_t n);
 void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout, uint32_t* iout, uint32_t n,
                        uint32_t shift, uint32_t* hist, hipStream_t s, bool hist_done = false,
-                       uint32_t digit_bits = 8);
-// the sorted copy for the sweep (sP = position, key bits), group-table
-// inserts (slots recorded in slot_rec, n words; the table must be empty on
-// entry), overflow count clear
-void launch_coll_gather(const float4* F, const uint32_t* sids, const uint32_t* skeys, uint32_t n, float4* sP,
-                        uint32_t* ovf, uint4* table, uint32_t mask, uint32_t* slot_rec, hipStream_t s);
+                       uint32_t digit_bits = 8, const float4* F = nullptr, float4* sP = nullptr);
+// group-table inserts over the sorted keys (slots recorded in slot_rec, n
+// words; the table must be empty on entry), overflow count clear
+void launch_coll_table(const uint32_t* skeys, uint32_t n, uint32_t* ovf, uint4* table, uint32_t mask,
+                       uint32_t* slot_rec, hipStream_t s);
 void launch_coll_sweep(const CollArgs& a, hipStream_t s);
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
 // The constraint iterations of a GS substep as one launch per colour and

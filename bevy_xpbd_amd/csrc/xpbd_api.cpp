@@ -291,9 +291,9 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
                        float4* vout) {
     hipStream_t s = h->stream;
     enqueue_keys(h, F, nparts, coll_cell(h), s);
-    const bool alt = launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true, h->coll_digit_bits);
-    launch_coll_gather(F, alt ? h->ids2 : h->ids, alt ? h->keys2 : h->keys, h->n, h->sP, h->ovf, h->table,
-                       h->table_slots - 1, h->slot_rec, s);
+    const bool alt = launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true, h->coll_digit_bits,
+                                       F, h->sP);
+    launch_coll_table(alt ? h->keys2 : h->keys, h->n, h->ovf, h->table, h->table_slots - 1, h->slot_rec, s);
     CollArgs c{};
     c.F = F;
     c.sP = h->sP;
@@ -1687,9 +1687,9 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
     auto prep_collision = [&]() {
         const uint32_t np = h->nclothes == 1 ? launch_aabb_of(h->A, h->n, h->aabb, s) : 0u;
         enqueue_keys(h, h->A, np, coll_cell(h), s);
-        alt = launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true, h->coll_digit_bits);
-        launch_coll_gather(h->A, alt ? h->ids2 : h->ids, alt ? h->keys2 : h->keys, h->n, h->sP, h->ovf, h->table,
-                           h->table_slots - 1, h->slot_rec, s);
+        alt = launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true, h->coll_digit_bits, h->A,
+                                h->sP);
+        launch_coll_table(alt ? h->keys2 : h->keys, h->n, h->ovf, h->table, h->table_slots - 1, h->slot_rec, s);
     };
     CollArgs c{};
     if (which == XPBD_KERNEL_COLLISION_SWEEP || which == XPBD_KERNEL_RADIX_PASS) {
