@@ -794,9 +794,11 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_hist(const uint32_t* keys
 // Cell keys of the current positions, laid out as the sort's tiles (one
 // block per 4096 keys), with the sort's first per-tile digit counts (pass 0,
 // the low DB bits) taken on the way, so that pass needs no histogram launch.
-// Also empties the group-table slots the previous collision pass filled
-// (slot_rec holds, per sorted index, the slot its group was inserted at, or
-// EMPTY_KEY): the table is empty again before this pass inserts.
+// Hashed group table (slot_rec != nullptr): also empties the slots the
+// previous collision pass filled (slot_rec holds, per sorted index, the slot
+// its group was inserted at, or EMPTY_KEY), so the table is empty again
+// before this pass inserts.  Direct group table (epoch != nullptr): advances
+// the pass tag instead; entries of earlier passes no longer match it.
 // One cloth (part != nullptr): every block derives the grid itself from the
 // nparts bounds partials of the producing kernel (min / max are exact and
 // order-free, so all blocks agree), and block 0 stores it for the sweep;
@@ -806,7 +808,7 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_coll_keys(const float4* F, uint
                                                          const uint32_t* cloth_of, uint32_t* keys, uint32_t* ids,
                                                          uint4* table, const uint32_t* slot_rec, uint32_t* hist,
                                                          uint32_t ntiles, const float* part, uint32_t nparts,
-                                                         float cell_size, CollGrid* grid_out) {
+                                                         float cell_size, CollGrid* grid_out, uint32_t* epoch) {
     constexpr uint32_t MASK = (1u << DB) - 1u;
     __shared__ uint32_t h[RADIX_WAVES][1u << DB];
     __shared__ float red[RADIX_WAVES][6];
@@ -823,9 +825,10 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_coll_keys(const float4* F, uint
     for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
         const uint32_t i = min(base + r * 64, last);
         f[r] = F[i];
-        sl[r] = base + r * 64 < n ? slot_rec[i] : EMPTY_KEY;
+        sl[r] = slot_rec && base + r * 64 < n ? slot_rec[i] : EMPTY_KEY;
         co[r] = cloth_of ? cloth_of[i] : 0u;
     }
+    if (epoch && blockIdx.x == 0 && t == 0) epoch[0] = epoch[0] + 1u == 0u ? 1u : epoch[0] + 1u;   // 0: never written
     if (part) {
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         const float4* p4 = reinterpret_cast<const float4*>(part);   // (lo xyz, hi x), (hi yz, -, -) per partial
@@ -1060,6 +1063,7 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_radix_scatter(const uint32_t* k
 // the slots (or the group) lengthened those chains enough to cost 1.5-2.5x in
 // the sweep; the skipped cells before kmin cost less than the extra probes.
 constexpr uint32_t GROUP_SHIFT = 3;
+static_assert((ONE_CLOTH_MAX_CELLS >> GROUP_SHIFT) <= ONE_CLOTH_GROUPS, "direct group table too small");
 // Slot of a group key: the low 3 bits stay (8 consecutive groups = 64 cells
 // share one 64-byte line, so threads walking the sorted order probe the same
 // lines); the block index g >> 3 is placed by Fibonacci hashing, taking the
@@ -1165,7 +1169,7 @@ struct RowRange {
 // their first groups are probed together, then the second groups of rows
 // whose first group is empty and that cross into the next group.
 __device__ __forceinline__ void sweep_rows3(const CollArgs& a, const CollGrid& g, const uint32_t c[3], int dz,
-                                            RowRange rows[3]) {
+                                            uint32_t ep, RowRange rows[3]) {
     uint32_t g0[3], g1[3];
     const int nz = int(c[2]) + dz - 1;
 #pragma unroll
@@ -1184,17 +1188,21 @@ __device__ __forceinline__ void sweep_rows3(const CollArgs& a, const CollGrid& g
         const uint32_t gk = rows[r].kmax >> GROUP_SHIFT;
         g1[r] = gk != g0[r] ? gk : EMPTY_KEY;
     }
-    // unconditional loads (slot 0 for absent rows), so no branch splits them
+    // unconditional loads (slot 0 for absent rows), so no branch splits them;
+    // a direct table's entry is the group's own (current iff its tag is ep)
+    const bool direct = a.epoch != nullptr;
+    const uint4 none = make_uint4(EMPTY_KEY, EMPTY_KEY, 0u, 0u);
     uint4 e[3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) e[r] = a.table[g0[r] != EMPTY_KEY ? slot_of(g0[r], a.table_mask) : 0u];
+    for (int r = 0; r < 3; ++r)
+        e[r] = a.table[g0[r] == EMPTY_KEY ? 0u : direct ? g0[r] : slot_of(g0[r], a.table_mask)];
     const uint32_t n = a.p.n;
     bool second[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         second[r] = false;
         if (g0[r] == EMPTY_KEY) continue;
-        const uint4 f = table_resolve(a.table, a.table_mask, g0[r], e[r]);
+        const uint4 f = direct ? (e[r].x == ep ? e[r] : none) : table_resolve(a.table, a.table_mask, g0[r], e[r]);
         if (f.y == EMPTY_KEY) {
             second[r] = g1[r] != EMPTY_KEY;
             continue;
@@ -1209,11 +1217,11 @@ __device__ __forceinline__ void sweep_rows3(const CollArgs& a, const CollGrid& g
         }
     }
 #pragma unroll
-    for (int r = 0; r < 3; ++r) e[r] = a.table[second[r] ? slot_of(g1[r], a.table_mask) : 0u];
+    for (int r = 0; r < 3; ++r) e[r] = a.table[!second[r] ? 0u : direct ? g1[r] : slot_of(g1[r], a.table_mask)];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         if (!second[r]) continue;
-        const uint4 f = table_resolve(a.table, a.table_mask, g1[r], e[r]);
+        const uint4 f = direct ? (e[r].x == ep ? e[r] : none) : table_resolve(a.table, a.table_mask, g1[r], e[r]);
         if (f.y == EMPTY_KEY) continue;
         const uint32_t cnt = group_off(f, 8);
         rows[r].lo = f.y;
@@ -1266,6 +1274,7 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
         return;
     }
     const CollGrid g = a.grids[a.cloth_of ? a.cloth_of[i] : 0u];
+    const uint32_t ep = a.epoch ? a.epoch[0] : 0u;
     const float th2 = a.p.thickness * a.p.thickness;
     uint32_t c[3];
     cell_of(fi, g, c);
@@ -1282,7 +1291,7 @@ __global__ __launch_bounds__(256) void k_coll_sweep(CollArgs a) {
     uint32_t cnt = 0;
     for (int dz = 0; dz < 3; ++dz) {
     RowRange rows[3];
-    sweep_rows3(a, g, c, dz, rows);
+    sweep_rows3(a, g, c, dz, ep, rows);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         if (rows[r].lo == EMPTY_KEY) continue;
@@ -1380,11 +1389,12 @@ __global__ __launch_bounds__(OVF_BLOCK) void k_coll_sweep_overflow(CollArgs a) {
         const float4 ri = a.rest4[i];
         const float wi = pv.w;
         const CollGrid g = a.grids[a.cloth_of ? a.cloth_of[i] : 0u];
+        const uint32_t ep = a.epoch ? a.epoch[0] : 0u;
         uint32_t c[3];
         cell_of(fi, g, c);
         RowRange rows[9];
 #pragma unroll
-        for (int dz = 0; dz < 3; ++dz) sweep_rows3(a, g, c, dz, rows + 3 * dz);
+        for (int dz = 0; dz < 3; ++dz) sweep_rows3(a, g, c, dz, ep, rows + 3 * dz);
         int cnt = 0;
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
@@ -1499,13 +1509,17 @@ __device__ __forceinline__ uint2 group_offsets(const uint32_t* lk, uint32_t l0, 
     return cnt > 255u ? make_uint2(0u, 0u) : make_uint2(lo, hi);
 }
 
-// Group-table inserts over the sorted keys: the first sorted index of each
-// group is inserted with its cell offsets (slot recorded in slot_rec,
-// EMPTY_KEY elsewhere, for k_coll_keys to empty next pass); the block stages
-// its keys and the next block's in LDS, so the offsets of groups shorter
-// than that read no global memory.  Also clears the sweep's overflow count.
+// Group-table entries over the sorted keys: the first sorted index of each
+// group, with its cell offsets.  Direct table (one cloth, epoch != nullptr):
+// entry g = (pass tag, start, offsets), one plain store per group.  Hashed
+// table (batches): inserted by CAS probing, the slot recorded in slot_rec
+// (EMPTY_KEY elsewhere) for k_coll_keys to empty next pass.  The block
+// stages its keys and the next block's in LDS, so the offsets of groups
+// shorter than that read no global memory.  Also clears the sweep's
+// overflow count.
 __global__ __launch_bounds__(GATHER_BLOCK) void k_coll_table(const uint32_t* skeys, uint32_t n, uint32_t* ovf,
-                                                           uint4* table, uint32_t mask, uint32_t* slot_rec) {
+                                                           uint4* table, uint32_t mask, uint32_t* slot_rec,
+                                                           const uint32_t* epoch) {
     __shared__ uint32_t lk[GATHER_STAGED];
     const uint32_t l0 = blockIdx.x * GATHER_BLOCK;
     const uint32_t s = l0 + threadIdx.x;
@@ -1521,7 +1535,15 @@ __global__ __launch_bounds__(GATHER_BLOCK) void k_coll_table(const uint32_t* ske
     uint32_t rec = EMPTY_KEY;
     const uint32_t g = key >> GROUP_SHIFT;
     const uint32_t prev = threadIdx.x > 0 ? lk[threadIdx.x - 1] : (s > 0 ? skeys[s - 1] : EMPTY_KEY);
-    if (s == 0 || (prev >> GROUP_SHIFT) != g) {
+    const bool first = s == 0 || (prev >> GROUP_SHIFT) != g;
+    if (epoch) {
+        if (first) {
+            const uint2 off = group_offsets(lk, l0, min(l0 + GATHER_STAGED, n), skeys, n, s, g);
+            table[g] = make_uint4(epoch[0], s, off.x, off.y);
+        }
+        return;
+    }
+    if (first) {
         const uint2 off = group_offsets(lk, l0, min(l0 + GATHER_STAGED, n), skeys, n, s, g);
         uint32_t h = slot_of(g, mask);
         for (uint32_t probe = 0; probe <= mask; ++probe) {
@@ -2232,15 +2254,15 @@ uint32_t cloth_span_max() { return CB_SPAN; }
 
 void launch_coll_keys(const float4* F, uint32_t n, CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
                       uint32_t* ids, uint4* table, const uint32_t* slot_rec, uint32_t* hist, uint32_t digit_bits,
-                      const float* part, uint32_t nparts, float cell_size, hipStream_t s) {
+                      const float* part, uint32_t nparts, float cell_size, uint32_t* epoch, hipStream_t s) {
     if (n == 0) return;
     const uint32_t ntiles = std::max(1u, cdiv(n, RADIX_TILE));
     if (digit_bits == 9)
         hipLaunchKernelGGL(k_coll_keys<9>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, F, n, grids, cloth_of, keys, ids,
-                           table, slot_rec, hist, ntiles, part, nparts, cell_size, grids);
+                           table, slot_rec, hist, ntiles, part, nparts, cell_size, grids, epoch);
     else
         hipLaunchKernelGGL(k_coll_keys<8>, dim3(ntiles), dim3(RADIX_BLOCK), 0, s, F, n, grids, cloth_of, keys, ids,
-                           table, slot_rec, hist, ntiles, part, nparts, cell_size, grids);
+                           table, slot_rec, hist, ntiles, part, nparts, cell_size, grids, epoch);
 }
 
 uint32_t radix_hist_words(uint32_t n) {
@@ -2290,10 +2312,10 @@ bool launch_radix_sort(uint32_t* keys, uint32_t* ids, uint32_t* keys_alt, uint32
 }
 
 void launch_coll_table(const uint32_t* skeys, uint32_t n, uint32_t* ovf, uint4* table, uint32_t mask,
-                       uint32_t* slot_rec, hipStream_t s) {
+                       uint32_t* slot_rec, const uint32_t* epoch, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_coll_table, dim3(cdiv(n, GATHER_BLOCK)), dim3(GATHER_BLOCK), 0, s, skeys, n, ovf, table, mask,
-                       slot_rec);
+                       slot_rec, epoch);
 }
 
 void launch_coll_sweep(const CollArgs& a, hipStream_t s) {

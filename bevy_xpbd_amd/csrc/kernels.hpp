@@ -134,12 +134,19 @@ struct CollArgs {
     const uint32_t* sids;
     const uint4* table;
     uint32_t table_mask;
+    // direct group table (one cloth): the current pass tag; nullptr: the
+    // hashed table (batches)
+    const uint32_t* epoch;
     DevParams p;
     // particles with more than LMAX candidates, resolved by a second kernel:
     // [0] count (cleared by launch_coll_table), [2..] sorted indices; n + 2 words
     uint32_t* ovf;
     uint32_t xcd_seg;   // sweep blocks per XCD segment (0: block b sweeps chunk b)
 };
+
+// Groups of the direct group table of one cloth: its keys are below 2^27
+// (the grid's cell cap), so its 8-cell groups are below 2^24.
+constexpr uint32_t ONE_CLOTH_GROUPS = 1u << 24;
 
 // Sweep segment length in blocks of 256 particles.  On the 1024^2 sheet 16
 // cut the sweep's fetched bytes from 223 to 133 MB per launch at the same
@@ -190,7 +197,7 @@ uint32_t cloth_span_max();   // particles per piece at most
 // partials and stored to grids[0]; otherwise grids[] is read (batches).
 void launch_coll_keys(const float4* F, uint32_t n, CollGrid* grids, const uint32_t* cloth_of, uint32_t* keys,
                       uint32_t* ids, uint4* table, const uint32_t* slot_rec, uint32_t* hist, uint32_t digit_bits,
-                      const float* part, uint32_t nparts, float cell_size, hipStream_t s);
+                      const float* part, uint32_t nparts, float cell_size, uint32_t* epoch, hipStream_t s);
 // Stable sort of (keys, ids) by key: 4 passes of 8-bit digits (32-bit keys) or
 // 3 passes of 9-bit digits (keys below 2^27: one cloth's grid).  Returns true
 // when the result is in (keys_alt, ids_alt) (an odd pass count), false when it
@@ -208,7 +215,7 @@ void launch_radix_pass(const uint32_t* kin, const uint32_t* iin, uint32_t* kout,
 // group-table inserts over the sorted keys (slots recorded in slot_rec, n
 // words; the table must be empty on entry), overflow count clear
 void launch_coll_table(const uint32_t* skeys, uint32_t n, uint32_t* ovf, uint4* table, uint32_t mask,
-                       uint32_t* slot_rec, hipStream_t s);
+                       uint32_t* slot_rec, const uint32_t* epoch, hipStream_t s);
 void launch_coll_sweep(const CollArgs& a, hipStream_t s);
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
 // The constraint iterations of a GS substep as one launch per colour and

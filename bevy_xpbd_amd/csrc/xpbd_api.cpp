@@ -95,6 +95,7 @@ struct xpbd_cloth {
     uint32_t *keys = nullptr, *ids = nullptr, *keys2 = nullptr, *ids2 = nullptr, *hist = nullptr;
     uint4* table = nullptr;
     uint32_t table_slots = 0;
+    uint32_t* epoch = nullptr;     // one cloth: the direct group table's pass tag (nullptr: hashed table)
     uint32_t* ovf = nullptr;       // collision overflow list (count, done, n sorted indices)
     float* aabb = nullptr;         // bounds partials, 8 floats per producing block / cloth
     uint32_t aabb_slots = 0;
@@ -269,7 +270,7 @@ void enqueue_keys(xpbd_cloth* h, const float4* F, uint32_t nparts, float cell, h
         launch_coll_grids(F, h->cspan, h->ncspans, h->cpfirst, h->nclothes, h->aabb, cell, h->local_bits, h->grids,
                           s);
     launch_coll_keys(F, h->n, h->grids, h->cloth_of, h->keys, h->ids, h->table, h->slot_rec, h->hist,
-                     h->coll_digit_bits, one ? h->aabb : nullptr, nparts, cell, s);
+                     h->coll_digit_bits, one ? h->aabb : nullptr, nparts, cell, h->epoch, s);
 }
 
 // Instrumented frames: bracket a launch of kind `which` with a pair of events.
@@ -293,7 +294,7 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     enqueue_keys(h, F, nparts, coll_cell(h), s);
     const bool alt = launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true, h->coll_digit_bits,
                                        F, h->sP);
-    launch_coll_table(alt ? h->keys2 : h->keys, h->n, h->ovf, h->table, h->table_slots - 1, h->slot_rec, s);
+    launch_coll_table(alt ? h->keys2 : h->keys, h->n, h->ovf, h->table, h->table_slots - 1, h->slot_rec, h->epoch, s);
     CollArgs c{};
     c.F = F;
     c.sP = h->sP;
@@ -308,6 +309,7 @@ void enqueue_collision(xpbd_cloth* h, const DevParams& p, const float4* F, uint3
     c.sids = alt ? h->ids2 : h->ids;
     c.table = h->table;
     c.table_mask = h->table_slots - 1;
+    c.epoch = h->epoch;
     c.p = p;
     c.ovf = h->ovf;
     c.xcd_seg = SWEEP_XCD_SEG;
@@ -836,16 +838,31 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
     TRY(dalloc(h, &h->keys2, n));
     TRY(dalloc(h, &h->ids2, n));
     TRY(dalloc(h, &h->hist, radix_hist_words(uint32_t(n))));
-    // one entry per occupied 8-cell group: at most n groups (a stretched
-    // sheet puts nearly every particle in its own group), so load <= 1/4;
-    // >= 4 slots per particle keeps the sweep's probe chains short (the
-    // hash table notes in kernels.hip).  next_pow2 caps at 2^31 slots, which
-    // still leaves load <= 1/4 up to the particle limit checked at creation.
-    h->table_slots = next_pow2(uint32_t(std::min<size_t>(std::max<size_t>(4 * size_t(n), 64), 0x80000000u)));
-    TRY(dalloc(h, &h->table, h->table_slots));
-    HIPC(h, hipMemset(h->table, 0xFF, size_t(h->table_slots) * sizeof(uint4)));   // every slot empty
-    TRY(dalloc(h, &h->slot_rec, n));
-    HIPC(h, hipMemset(h->slot_rec, 0xFF, size_t(n) * sizeof(uint32_t)));   // no slot recorded
+    // the cloth count as settled below (offsets that do not cover [0, n) in
+    // two or more pieces make one cloth)
+    const bool one_cloth = !(m.cloth_offsets.size() >= 3 && m.cloth_offsets.front() == 0 && m.cloth_offsets.back() == n);
+    if (one_cloth) {
+        // direct group table: entry g for 8-cell group g (keys < 2^27), valid
+        // when its tag is the pass's; tag 0 is never current, so zeros are
+        // empty entries and nothing is cleared between passes (256 MB)
+        h->table_slots = ONE_CLOTH_GROUPS;
+        TRY(dalloc(h, &h->table, h->table_slots));
+        HIPC(h, hipMemset(h->table, 0, size_t(h->table_slots) * sizeof(uint4)));
+        TRY(dalloc(h, &h->epoch, 1));
+        HIPC(h, hipMemset(h->epoch, 0, sizeof(uint32_t)));
+    } else {
+        // hashed: one entry per occupied 8-cell group: at most n groups (a
+        // stretched sheet puts nearly every particle in its own group), so
+        // load <= 1/4; >= 4 slots per particle keeps the sweep's probe chains
+        // short (the hash table notes in kernels.hip).  next_pow2 caps at
+        // 2^31 slots, which still leaves load <= 1/4 up to the particle limit
+        // checked at creation.
+        h->table_slots = next_pow2(uint32_t(std::min<size_t>(std::max<size_t>(4 * size_t(n), 64), 0x80000000u)));
+        TRY(dalloc(h, &h->table, h->table_slots));
+        HIPC(h, hipMemset(h->table, 0xFF, size_t(h->table_slots) * sizeof(uint4)));   // every slot empty
+        TRY(dalloc(h, &h->slot_rec, n));
+        HIPC(h, hipMemset(h->slot_rec, 0xFF, size_t(n) * sizeof(uint32_t)));   // no slot recorded
+    }
     TRY(dalloc(h, &h->ovf, size_t(n) + 2));
     HIPC(h, hipMemset(h->ovf, 0, (size_t(n) + 2) * sizeof(uint32_t)));
     h->aabb_slots = std::max<uint32_t>({h->nslices, (h->n + 255) / 256, 4096u, uint32_t(m.cloth_offsets.size()), 1u});
@@ -1689,7 +1706,8 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         enqueue_keys(h, h->A, np, coll_cell(h), s);
         alt = launch_radix_sort(h->keys, h->ids, h->keys2, h->ids2, h->n, h->hist, s, true, h->coll_digit_bits, h->A,
                                 h->sP);
-        launch_coll_table(alt ? h->keys2 : h->keys, h->n, h->ovf, h->table, h->table_slots - 1, h->slot_rec, s);
+        launch_coll_table(alt ? h->keys2 : h->keys, h->n, h->ovf, h->table, h->table_slots - 1, h->slot_rec, h->epoch,
+                          s);
     };
     CollArgs c{};
     if (which == XPBD_KERNEL_COLLISION_SWEEP || which == XPBD_KERNEL_RADIX_PASS) {
@@ -1707,6 +1725,7 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         c.sids = alt ? h->ids2 : h->ids;
         c.table = h->table;
         c.table_mask = h->table_slots - 1;
+        c.epoch = h->epoch;
         c.p = p;
         c.ovf = h->ovf;
     c.xcd_seg = SWEEP_XCD_SEG;
