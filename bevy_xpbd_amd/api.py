@@ -533,12 +533,13 @@ class ClothComputePlugin:
         return sub.value, fb.value
 
     def layout(self) -> dict:
-        """Adjacency layout chosen at create: "plain", "packed" or "grid_tiles"
-        (with the tile grid's columns and rows), and whether the fast
+        """Adjacency layout in use: "plain", "packed", "grid_tiles" or
+        "shared_tiles" (grid tiles with each constraint evaluated once per
+        tile), with the tile grid's columns and rows, and whether the fast
         correctly rounded division is active."""
         out = (C.c_uint32 * 4)()
         _check(lib().xpbd_get_layout(self._h, out), "xpbd_get_layout", self._h)
-        return {"adjacency": ("plain", "packed", "grid_tiles")[out[0]], "tile_cols": int(out[1]),
+        return {"adjacency": ("plain", "packed", "grid_tiles", "shared_tiles")[out[0]], "tile_cols": int(out[1]),
                 "tile_rows": int(out[2]), "fast_division": bool(out[3])}
 
     def debug_collision_keys(self, cell_size: float, num_cloths: int = 1):

@@ -555,8 +555,10 @@ __global__ __launch_bounds__(64 * JT_ROWS) __attribute__((amdgpu_waves_per_eu(8)
     const bool lds = stage_pairs<true, JT_PAIRS_MAX>(a, lpairs);
     // The block's entries read only staged positions, so the fast operations
     // need only these to be tame (predicted ones, in JM_PREDICT): one
-    // block-wide AND, which is also the barrier that publishes the tile.
-    const bool fast = FASTDIV && __syncthreads_and(tame ? 1 : 0) != 0;
+    // block-wide AND, which is also the barrier that publishes the tile (so
+    // it runs whatever FASTDIV is).
+    const int all_tame = __syncthreads_and(tame ? 1 : 0);
+    const bool fast = FASTDIV && all_tame != 0;
     const uint32_t lidx = (wave + 2) * JT_W + lane + 2;
     bool valid;
     float4 out;
@@ -566,6 +568,287 @@ __global__ __launch_bounds__(64 * JT_ROWS) __attribute__((amdgpu_waves_per_eu(8)
     else
         out = fast ? jacobi_particle_tiled<MODE, true, false>(a, lpairs, tl, slice, lane, lidx, row_ok, valid, pre, base, width)
                    : jacobi_particle_tiled<MODE, false, false>(a, lpairs, tl, slice, lane, lidx, row_ok, valid, pre, base, width);
+    if (!(MODE & JM_FINAL) && a.tame_out)
+        write_slice_tame(a.tame_out, slice, slice < a.p.nslices, !valid || tame3(out));
+    if (MODE & JM_AABB) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        aabb_fold(valid, out.x, out.y, out.z, lo, hi);
+        block_aabb_store(lo, hi, a.aabb, b);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Constraint-shared grid tiles: the tiled iteration with each constraint that
+// touches a tile evaluated once per tile (the layout is described with
+// SharedTiles).  Up to the last product, an entry's arithmetic is the same
+// for both endpoints, bit for bit: the difference vector and the normal
+// only change sign (a - b = -(b - a) and (-g) / len = -(g / len) exactly in
+// round-to-nearest), and |g|, C, w_i + w_j + alpha~ and dlam are equal.
+// So phase 1 stores (n, dlam) per constraint as its anchor a sees it, and
+// phase 2 adds, in each particle's CSR order, (n * w) * dlam for the anchor
+// and (n * (-w)) * dlam = ((-n) * w) * dlam for the other endpoint: the
+// terms, and the sums, of k_jacobi_tiled.  A skipped constraint stores
+// zeros; its term is a zero, and the accumulators (starting at +0, never -0
+// in round-to-nearest) do not change when a zero is added.
+// ---------------------------------------------------------------------------
+// (n, dlam) of the constraint (a, b) as jacobi_entry computes it for a
+// (zeros where jacobi_entry adds nothing).
+template <bool FAST>
+__device__ __forceinline__ float4 constraint_result(float4 pa, float4 pb, float4 pr) {
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    float gx = pa.x - pb.x, gy = pa.y - pb.y, gz = pa.z - pb.z;
+    const float dot = (gx * gx + gy * gy) + gz * gz;
+    const float len = FAST ? sqrt_rn(dot) : sqrtf(dot);
+    if (len < 1e-8f) return zero;
+    const float c = len - pr.x;
+    const float wsum = pa.w + pb.w + pr.y;
+    if (!FAST && wsum < 1e-8f) return zero;
+    const float num = FAST ? -c : -c - pr.z;
+    float dlam;
+    if (FAST) {
+        const float yl = rcp_rn(len);
+        gx = div_rn_nz(gx, len, yl);
+        gy = div_rn_nz(gy, len, yl);
+        gz = div_rn_nz(gz, len, yl);
+        dlam = div_rn_nz(num, wsum, rcp_rn(wsum));   // see jacobi_entry for the zero's sign
+    } else {
+        gx = gx / len;
+        gy = gy / len;
+        gz = gz / len;
+        dlam = num / wsum;
+    }
+    return make_float4(gx, gy, gz, dlam);
+}
+
+// constraint_result without branches (the same operations; skipped results
+// are selected away), so several constraints can be in flight at once.
+template <bool FAST>
+__device__ __forceinline__ float4 constraint_result_nb(float4 pa, float4 pb, float4 pr, bool none) {
+    float gx = pa.x - pb.x, gy = pa.y - pb.y, gz = pa.z - pb.z;
+    const float dot = (gx * gx + gy * gy) + gz * gz;
+    const float len = FAST ? sqrt_rn(dot) : sqrtf(dot);
+    const float c = len - pr.x;
+    const float wsum = pa.w + pb.w + pr.y;
+    const bool skip = none || len < 1e-8f || (!FAST && wsum < 1e-8f);
+    const float num = FAST ? -c : -c - pr.z;
+    float dlam;
+    if (FAST) {
+        const float yl = rcp_rn(len);
+        gx = div_rn_nz(gx, len, yl);
+        gy = div_rn_nz(gy, len, yl);
+        gz = div_rn_nz(gz, len, yl);
+        dlam = div_rn_nz(num, wsum, rcp_rn(wsum));
+    } else {
+        gx = gx / len;
+        gy = gy / len;
+        gz = gz / len;
+        dlam = num / wsum;
+    }
+    return skip ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(gx, gy, gz, dlam);
+}
+
+// Phase 1 of a block: every job's constraints into the result slots.  A
+// wavefront's jobs are wave, wave + JT_ROWS, ...; the first JS_JOBS_PF of
+// them are decoded, and their pair indices loaded, before the tile barrier
+// (shared_prefetch), so those loads overlap the staging.
+constexpr uint32_t JS_JOBS_PF = 8;
+
+struct SharedJob {
+    uint32_t item, cl, pid;   // pid 0xFFFF: nothing to compute (the slot is stored as zeros)
+};
+
+__device__ __forceinline__ SharedJob shared_job(const JacobiArgs& a, uint32_t j, int32_t r0, int32_t c0) {
+    const uint32_t lane = threadIdx.x & 63;
+    SharedJob J;
+    if (j < a.sh.njobs_main) {
+        J.item = a.sh.jobs[j];
+        J.cl = lane + 2;
+    } else {
+        J.item = a.sh.jobs[a.sh.njobs_main + (j - a.sh.njobs_main) * 64 + lane];
+        J.cl = (J.item >> 12) & 0xFFu;
+    }
+    J.pid = 0xFFFFu;
+    if (J.item == 0xFFFFFFFFu) return J;
+    const int32_t rows = int32_t(a.sell.trows), cols = int32_t(a.sell.tcols);
+    const int32_t dy = int32_t(J.item & 3u), dx = int32_t((J.item >> 2) & 7u) - 2;
+    const uint32_t t = (J.item >> 5) & 7u, lr = (J.item >> 8) & 15u;
+    const int32_t R = r0 + int32_t(lr), C = c0 + int32_t(J.cl);   // anchor (grid row, column)
+    if (R >= 0 && R + dy < rows && C >= 0 && C < cols && C + dx >= 0 && C + dx < cols)
+        J.pid = a.sh.jpair[size_t(t) * a.p.n + uint32_t(R) * uint32_t(cols) + uint32_t(C)];
+    return J;
+}
+
+template <bool FAST>
+__device__ __forceinline__ void shared_compute(const SharedJob& J, const float4* tl, const float4* lpairs,
+                                               float4* res) {
+    if (J.item == 0xFFFFFFFFu) return;
+    const int32_t dy = int32_t(J.item & 3u), dx = int32_t((J.item >> 2) & 7u) - 2;
+    const uint32_t lr = (J.item >> 8) & 15u, row = J.item >> 20;
+    float4 r4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (J.pid != 0xFFFFu) {
+        const float4 pa = tl[lr * JT_W + J.cl];
+        const float4 pb = tl[(lr + uint32_t(dy)) * JT_W + uint32_t(int32_t(J.cl) + dx)];
+        r4 = constraint_result<FAST>(pa, pb, lpairs[J.pid]);
+    }
+    res[row * JT_W + J.cl] = r4;
+}
+
+// Two prefetched jobs at once, branch-free until the stores: a job without
+// a constraint reads tile slot 0 and pair 0 (both staged) and stores zeros.
+template <bool FAST>
+__device__ __forceinline__ void shared_compute2(const SharedJob& J0, const SharedJob& J1, const float4* tl,
+                                                const float4* lpairs, float4* res) {
+    float4 pa[2], pb[2], pr[2];
+    bool none[2];
+    const SharedJob* J[2] = {&J0, &J1};
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const uint32_t item = J[u]->item;
+        none[u] = item == 0xFFFFFFFFu || J[u]->pid == 0xFFFFu;
+        const int32_t dy = int32_t(item & 3u), dx = int32_t((item >> 2) & 7u) - 2;
+        const uint32_t lr = (item >> 8) & 15u;
+        const uint32_t ia = none[u] ? 0u : lr * JT_W + J[u]->cl;
+        const uint32_t ib = none[u] ? 0u : (lr + uint32_t(dy)) * JT_W + uint32_t(int32_t(J[u]->cl) + dx);
+        pa[u] = tl[ia];
+        pb[u] = tl[ib];
+        pr[u] = lpairs[none[u] ? 0u : J[u]->pid];
+    }
+    float4 r[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) r[u] = constraint_result_nb<FAST>(pa[u], pb[u], pr[u], none[u]);
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+        if (J[u]->item != 0xFFFFFFFFu) res[(J[u]->item >> 20) * JT_W + J[u]->cl] = r[u];
+}
+
+template <bool FAST>
+__device__ __forceinline__ void shared_phase1(const JacobiArgs& a, const float4* tl, const float4* lpairs, float4* res,
+                                              int32_t r0, int32_t c0, const SharedJob (&pf)[JS_JOBS_PF]) {
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t njobs = a.sh.njobs_main + a.sh.njobs_edge;
+#pragma unroll
+    for (uint32_t k = 0; k < JS_JOBS_PF; k += 2) shared_compute2<FAST>(pf[k], pf[k + 1], tl, lpairs, res);   // unused: item none
+    for (uint32_t j = wave + JS_JOBS_PF * JT_ROWS; j < njobs; j += JT_ROWS)
+        shared_compute<FAST>(shared_job(a, j, r0, c0), tl, lpairs, res);
+}
+
+// pre: the lane's first JS_WORDS_PF entry words (those below ceil(width / 2)).
+constexpr uint32_t JS_WORDS_PF = 8;
+
+template <uint32_t MODE>
+__device__ __forceinline__ float4 shared_particle(const JacobiArgs& a, const float4* tl, const float4* res,
+                                                  uint32_t slice, uint32_t lane, uint32_t lidx, bool row_ok,
+                                                  bool& valid, const uint32_t (&pre)[JS_WORDS_PF], uint32_t width) {
+    const uint32_t i = slice * 64 + lane;
+    const DevParams& p = a.p;
+    valid = row_ok && slice < p.nslices && i < p.n;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 out = pi;
+    if (!valid) return out;
+    pi = tl[lidx];
+    out = pi;
+    const float wi = pi.w;
+    if (wi > 0.f) {
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        const float4* rb = res + lane;
+        const uint32_t* ew = a.sh.entries + a.sh.eoff[slice] + lane;
+        const uint32_t wsign = __float_as_uint(wi);
+        auto add = [&](uint32_t code) {
+            const float4 v = rb[code & 0x7FFFu];
+            const float sw = __uint_as_float(wsign ^ ((code & 0x8000u) << 16));
+            ax = ax + (v.x * sw) * v.w;
+            ay = ay + (v.y * sw) * v.w;
+            az = az + (v.z * sw) * v.w;
+        };
+        // the prefetched words (static indices keep them in registers), then
+        // any further words from memory
+#pragma unroll
+        for (uint32_t q = 0; q < JS_WORDS_PF; ++q) {
+            if (2 * q < width) add(pre[q] & 0xFFFFu);
+            if (2 * q + 1 < width) add(pre[q] >> 16);
+        }
+        for (uint32_t k = 2 * JS_WORDS_PF; k < width; k += 2) {
+            const uint32_t w = __builtin_nontemporal_load(ew + (k >> 1) * 64);
+            add(w & 0xFFFFu);
+            if (k + 1 < width) add(w >> 16);
+        }
+        float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
+        const float ml = len3(dx, dy, dz);
+        if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
+            const float sc = JACOBI_CORRECTION_CAP / ml;
+            dx = dx * sc;
+            dy = dy * sc;
+            dz = dz * sc;
+        }
+        out = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, wi);
+    }
+    if (MODE & JM_FINAL) {
+        if (pi.w > 0.f) finish_particle(out, a.dst[i], i, a.ground, p, a.dst, a.vout);
+    } else {
+        a.dst[i] = out;
+    }
+    return out;
+}
+
+template <uint32_t MODE, bool FASTDIV>
+// LDS (position tile, result slots) allows 2 blocks per CU: 4 wavefronts per
+// SIMD, so the compiler may use up to 128 VGPRs.
+__global__ __launch_bounds__(64 * JT_ROWS) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_jacobi_shared(JacobiArgs a) {
+    if (a.gate && *a.gate == 0u) return;   // uniform over the grid
+    __shared__ float4 tl[JT_H * JT_W];
+    extern __shared__ float4 js_dyn[];     // result slots, zero slot, then the staged pairs
+    float4* res = js_dyn;
+    float4* lpairs = js_dyn + a.sh.nslots + 1;
+    const uint32_t cols = a.sell.tcols, rows = a.sell.trows, ntx = cols / 64;
+    const uint32_t b = a.remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t tx = b % ntx, ty = b / ntx;
+    const int32_t r0 = int32_t(ty * JT_ROWS) - 2, c0 = int32_t(tx * 64) - 2;
+    bool tame = true;
+    for (uint32_t q = threadIdx.x; q < JT_H * JT_W; q += blockDim.x) {
+        const int32_t r = r0 + int32_t(q / JT_W), c = c0 + int32_t(q % JT_W);
+        if (r >= 0 && r < int32_t(rows) && c >= 0 && c < int32_t(cols)) {
+            const uint32_t id = uint32_t(r) * cols + uint32_t(c);
+            const float4 v = (MODE & JM_PREDICT) ? predict(a.src[id], a.vel[id], id, a.p) : a.src[id];
+            tl[q] = v;
+            tame = tame && tame3(v);
+        }
+    }
+    for (uint32_t q = threadIdx.x; q < a.num_pairs; q += blockDim.x) to_lds(lpairs[q], a.sell.pairs[q], a.p);
+    if (threadIdx.x == 0) res[a.sh.nslots] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // the wavefront's first jobs (pair indices) and the lane's first entry
+    // words are requested before the barrier: their latency overlaps the
+    // staging
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t njobs = a.sh.njobs_main + a.sh.njobs_edge;
+    SharedJob pf[JS_JOBS_PF];
+#pragma unroll
+    for (uint32_t k = 0; k < JS_JOBS_PF; ++k) {
+        pf[k].item = 0xFFFFFFFFu;
+        pf[k].cl = 0u;
+        pf[k].pid = 0xFFFFu;
+        if (wave + k * JT_ROWS < njobs) pf[k] = shared_job(a, wave + k * JT_ROWS, r0, c0);
+    }
+    const uint32_t row = ty * JT_ROWS + wave;
+    const bool row_ok = row < rows;
+    const uint32_t slice = row_ok ? row * ntx + tx : a.p.nslices;
+    const uint32_t width = slice < a.p.nslices ? a.sell.swid[slice] : 0u;
+    uint32_t pre[JS_WORDS_PF];
+    {
+        const uint32_t* ew = a.sh.entries + (slice < a.p.nslices ? a.sh.eoff[slice] : 0u) + lane;
+#pragma unroll
+        for (uint32_t k = 0; k < JS_WORDS_PF; ++k) pre[k] = 2 * k < width ? __builtin_nontemporal_load(ew + k * 64) : 0u;
+    }
+    // one block-wide AND, run whatever FASTDIV is: the tile and pairs are
+    // published, and the fast operations apply where every staged position
+    // is tame
+    const int all_tame = __syncthreads_and(tame ? 1 : 0);
+    const bool fast = FASTDIV && all_tame != 0;
+    if (fast) shared_phase1<true>(a, tl, lpairs, res, r0, c0, pf);
+    else shared_phase1<false>(a, tl, lpairs, res, r0, c0, pf);
+    __syncthreads();
+    const uint32_t lidx = (wave + 2) * JT_W + lane + 2;
+    bool valid;
+    const float4 out = shared_particle<MODE>(a, tl, res, slice, lane, lidx, row_ok, valid, pre, width);
     if (!(MODE & JM_FINAL) && a.tame_out)
         write_slice_tame(a.tame_out, slice, slice < a.p.nslices, !valid || tame3(out));
     if (MODE & JM_AABB) {
@@ -2087,7 +2370,34 @@ static void launch_jacobi_tiled_t(uint32_t mode, const JacobiArgs& a, uint32_t b
     }
 }
 
+template <bool FASTDIV>
+static void launch_jacobi_shared_t(uint32_t mode, const JacobiArgs& a, uint32_t blocks, hipStream_t s) {
+    const dim3 g(blocks), b(64 * JT_ROWS);
+    const size_t lds = (size_t(a.sh.nslots) + 1 + std::max(a.num_pairs, 1u)) * sizeof(float4);
+    switch (mode) {
+        case 0: hipLaunchKernelGGL((k_jacobi_shared<0, FASTDIV>), g, b, lds, s, a); break;
+        case JM_PREDICT: hipLaunchKernelGGL((k_jacobi_shared<JM_PREDICT, FASTDIV>), g, b, lds, s, a); break;
+        case JM_FINAL: hipLaunchKernelGGL((k_jacobi_shared<JM_FINAL, FASTDIV>), g, b, lds, s, a); break;
+        case JM_AABB: hipLaunchKernelGGL((k_jacobi_shared<JM_AABB, FASTDIV>), g, b, lds, s, a); break;
+        case JM_PREDICT | JM_AABB:
+            hipLaunchKernelGGL((k_jacobi_shared<JM_PREDICT | JM_AABB, FASTDIV>), g, b, lds, s, a);
+            break;
+        default: break;
+    }
+}
+
+size_t jacobi_shared_lds_bytes(uint32_t nslots, uint32_t num_pairs) {
+    return (size_t(nslots) + 1 + std::max(num_pairs, 1u)) * sizeof(float4) + sizeof(float4) * JT_H * JT_W;
+}
+
 uint32_t launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s) {
+    if (a.sh.jobs && a.sell.tcols >= 64 && a.sell.tcols % 64 == 0) {   // constraint-shared grid tiles
+        const uint32_t blocks = (a.sell.tcols / 64) * cdiv(a.sell.trows, JT_ROWS);
+        if (blocks == 0) return 0;
+        if (a.fastdiv) launch_jacobi_shared_t<true>(mode, a, blocks, s);
+        else launch_jacobi_shared_t<false>(mode, a, blocks, s);
+        return blocks;
+    }
     if (a.sell.tpacked && a.sell.tcols >= 64 && a.sell.tcols % 64 == 0) {   // grid tiles: one 64 x 8 tile per block
         const uint32_t blocks = (a.sell.tcols / 64) * cdiv(a.sell.trows, JT_ROWS);
         if (blocks == 0) return 0;

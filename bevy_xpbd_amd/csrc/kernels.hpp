@@ -46,6 +46,36 @@ struct Sell {
     uint32_t tcols, trows;
 };
 
+// Constraint-shared grid tiles (k_jacobi_shared): the tiles of the grid-tiled
+// kernel, with each constraint touching a tile evaluated once per tile.
+// Directions: the forward offsets (dy > 0, or dy == 0 and dx > 0) of the
+// mesh's constraints, at most JS_DIRS; the constraint of particle a in
+// direction t joins a and a + (dy_t, dx_t), and a is its anchor.
+//   jpair[t * n + a]: its pair index (0xFFFF: none).
+//   Result slots in LDS: slot row r = (direction t, anchor tile row lr) for
+//   lr in [2 - dy_t, 2 + JT_ROWS) (tile rows as in the position tile), 68
+//   columns each (tile columns), then one zero slot at index nslots.
+//   jobs[0 .. njobs_main): one slot row each, anchors at the tile's 64
+//   columns; then njobs_edge wavefronts of per-lane items for anchors left or
+//   right of the tile whose partner is inside (0xFFFFFFFF: none).  An item
+//   packs dy | (dx + 2) << 2 | t << 5 | lr << 8 | cl << 12 | row << 20.
+//   entries: per particle, its CSR entries in order as 16-bit codes (slot -
+//   lane) | sign << 15 -- the slot of the entry's constraint relative to the
+//   lane's column, sign 1 when the particle is the constraint's second
+//   endpoint; padding codes point at the zero slot.  Two codes per word,
+//   slice s at words [eoff[s], eoff[s] + ceil(width / 2) * 64).
+struct SharedTiles {
+    const uint16_t* jpair;
+    const uint32_t* jobs;
+    const uint32_t* entries;
+    const uint32_t* eoff;
+    uint32_t njobs_main, njobs_edge, nslots;   // nslots = slot rows x 68
+};
+constexpr uint32_t JS_DIRS = 8;
+constexpr uint32_t JS_PAIRS_MAX = 1024;
+// LDS of one k_jacobi_shared block (position tile, result slots, pairs).
+size_t jacobi_shared_lds_bytes(uint32_t nslots, uint32_t num_pairs);
+
 // LDS tile of the grid-tiled Jacobi: JT_ROWS rows of 64 particles (one
 // wavefront each) and a 2-particle halo on every side
 constexpr uint32_t JT_ROWS = 8;
@@ -77,6 +107,7 @@ struct JacobiArgs {
     const uint8_t* ground;
     float* aabb;         // JM_AABB: one (lo xyz, hi xyz) partial per block, stride 8
     Sell sell;
+    SharedTiles sh;      // sh.jobs == nullptr: no constraint-shared path
     DevParams p;
     uint32_t remap;      // 1: XCD-aware block order
     uint32_t fastdiv;    // 1: shared-reciprocal correctly rounded divisions

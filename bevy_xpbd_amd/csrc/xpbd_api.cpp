@@ -82,6 +82,11 @@ struct xpbd_cloth {
     uint32_t* nbpk = nullptr;       // packed adjacency (nullptr: plain layout in use)
     uint32_t* nbtile = nullptr;     // packed adjacency with LDS-tile offsets (nullptr: no grid tiles)
     uint32_t tile_cols = 0, tile_rows = 0;
+    // constraint-shared grid tiles (layout built at create when it applies;
+    // used when js_on)
+    SharedTiles js{};
+    bool js_ok = false, js_tuned = false;
+    uint32_t js_on = 0;
     float2* pairs = nullptr;
     uint32_t num_pairs = 0;
     uint8_t* ground = nullptr;
@@ -244,6 +249,7 @@ uint8_t* tame_slot(xpbd_cloth* h, const float4* buf) {
 JacobiArgs jacobi_args(const xpbd_cloth* h, const DevParams& p) {
     JacobiArgs a{};
     a.sell = Sell{h->soff, h->swid, h->nbj, h->nbr, h->nbc, h->nbpk, h->pairs, h->nbtile, h->tile_cols, h->tile_rows};
+    if (h->js_on) a.sh = h->js;
     a.ground = h->has_ground ? h->ground : nullptr;
     a.aabb = h->aabb;
     a.p = p;
@@ -681,6 +687,143 @@ bool tile_sell(const Mesh& m, const std::vector<uint32_t>& soff, const std::vect
     return true;
 }
 
+// Constraint-shared grid tiles (the SharedTiles layout in kernels.hpp), from
+// the packed entries of a grid cloth: every entry (i, j) of particle i is
+// the constraint anchored at i in direction j - i (forward) or at j in
+// direction i - j.  False (no shared path) when the mesh needs more than
+// JS_DIRS directions, two constraints of one anchor and direction differ in
+// their pair, the pairs exceed JS_PAIRS_MAX, or a code does not fit 15 bits.
+struct SharedLayout {
+    std::vector<uint16_t> jpair;
+    std::vector<uint32_t> jobs, entries, eoff;
+    uint32_t njobs_main = 0, njobs_edge = 0, nslots = 0;
+};
+
+bool share_sell(const Mesh& m, const std::vector<uint32_t>& soff, const std::vector<uint32_t>& swid,
+                const std::vector<uint32_t>& packed, uint32_t num_pairs, uint32_t cols, uint32_t rows,
+                SharedLayout& L) {
+    const size_t n = m.num_particles();
+    if (num_pairs > JS_PAIRS_MAX || size_t(cols) * rows != n || n == 0) return false;
+    const int64_t C = cols;
+    auto forward = [](int64_t dy, int64_t dx) { return dy > 0 || (dy == 0 && dx > 0); };
+    // directions, sorted by (dy, dx)
+    std::vector<std::pair<int, int>> dirs;
+    for (size_t i = 0; i < n; ++i) {
+        const size_t s = i / 64, l = i % 64;
+        const uint32_t d = m.neighbor_offsets[i + 1] - m.neighbor_offsets[i];
+        for (uint32_t k = 0; k < d; ++k) {
+            const uint32_t w = packed[soff[s] + size_t(k) * 64 + l];
+            const int64_t j = int64_t(i) + int64_t(int16_t(uint16_t(w & 0xFFFFu)));
+            int64_t dy = j / C - int64_t(i) / C, dx = j % C - int64_t(i) % C;
+            if (!forward(dy, dx)) {
+                dy = -dy;
+                dx = -dx;
+            }
+            if (dx < -2 || dx > 2 || dy > 2) return false;
+            const std::pair<int, int> dd{int(dy), int(dx)};
+            if (std::find(dirs.begin(), dirs.end(), dd) == dirs.end()) {
+                if (dirs.size() == JS_DIRS) return false;
+                dirs.push_back(dd);
+            }
+        }
+    }
+    std::sort(dirs.begin(), dirs.end());
+    const size_t nd = dirs.size();
+    auto dir_of = [&](int64_t dy, int64_t dx) {
+        return uint32_t(std::find(dirs.begin(), dirs.end(), std::pair<int, int>(int(dy), int(dx))) - dirs.begin());
+    };
+    // slot rows: direction t, tile rows lr in [2 - dy_t, 2 + JT_ROWS)
+    std::vector<uint32_t> row_base(nd);
+    uint32_t nrows = 0;
+    for (size_t t = 0; t < nd; ++t) {
+        row_base[t] = nrows;
+        nrows += JT_ROWS + uint32_t(dirs[t].first);
+    }
+    auto slot_row = [&](uint32_t t, uint32_t lr) { return row_base[t] + lr - (2u - uint32_t(dirs[t].first)); };
+    L.nslots = nrows * JT_W;
+    // pair of every (direction, anchor)
+    L.jpair.assign(nd * n, 0xFFFFu);
+    for (size_t i = 0; i < n; ++i) {
+        const size_t s = i / 64, l = i % 64;
+        const uint32_t d = m.neighbor_offsets[i + 1] - m.neighbor_offsets[i];
+        for (uint32_t k = 0; k < d; ++k) {
+            const uint32_t w = packed[soff[s] + size_t(k) * 64 + l];
+            const int64_t j = int64_t(i) + int64_t(int16_t(uint16_t(w & 0xFFFFu)));
+            int64_t dy = j / C - int64_t(i) / C, dx = j % C - int64_t(i) % C;
+            int64_t anchor = int64_t(i);
+            if (!forward(dy, dx)) {
+                dy = -dy;
+                dx = -dx;
+                anchor = j;
+            }
+            uint16_t& q = L.jpair[size_t(dir_of(dy, dx)) * n + size_t(anchor)];
+            const uint16_t pair = uint16_t(w >> 16);
+            if (q != 0xFFFFu && q != pair) return false;
+            q = pair;
+        }
+    }
+    // jobs: every slot row over the tile's 64 columns, then the anchors left
+    // (dx > 0) or right (dx < 0) of the tile whose partner is inside
+    auto item = [&](uint32_t t, uint32_t lr, uint32_t cl) {
+        return uint32_t(dirs[t].first) | (uint32_t(dirs[t].second + 2) << 2) | (t << 5) | (lr << 8) | (cl << 12) |
+               (slot_row(t, lr) << 20);
+    };
+    L.jobs.clear();
+    for (uint32_t t = 0; t < nd; ++t)
+        for (uint32_t lr = 2 - uint32_t(dirs[t].first); lr < 2 + JT_ROWS; ++lr) L.jobs.push_back(item(t, lr, 0));
+    L.njobs_main = uint32_t(L.jobs.size());
+    std::vector<uint32_t> edge;
+    for (uint32_t t = 0; t < nd; ++t) {
+        const int dy = dirs[t].first, dx = dirs[t].second;
+        if (dx == 0) continue;
+        for (uint32_t lr = 2 - uint32_t(dy); lr < 2 + JT_ROWS - uint32_t(dy); ++lr) {
+            if (dx > 0)
+                for (int cl = 2 - dx; cl < 2; ++cl) edge.push_back(item(t, lr, uint32_t(cl)));
+            else
+                for (int cl = 66; cl < 66 - dx; ++cl) edge.push_back(item(t, lr, uint32_t(cl)));
+        }
+    }
+    L.njobs_edge = uint32_t((edge.size() + 63) / 64);
+    edge.resize(size_t(L.njobs_edge) * 64, 0xFFFFFFFFu);
+    L.jobs.insert(L.jobs.end(), edge.begin(), edge.end());
+    // entry codes (slot - lane) | sign << 15, two per word
+    const size_t ns = swid.size();
+    L.eoff.assign(ns + 1, 0u);
+    for (size_t s = 0; s < ns; ++s) L.eoff[s + 1] = L.eoff[s] + (swid[s] + 1) / 2 * 64;
+    L.entries.assign(L.eoff[ns], 0u);
+    for (size_t i = 0; i < n; ++i) {
+        const size_t s = i / 64, l = i % 64;
+        const int64_t row = int64_t(i) / C, col = int64_t(i) % C;
+        const uint32_t lr_i = uint32_t(row % JT_ROWS) + 2, cl_i = uint32_t(col % 64) + 2;
+        const uint32_t d = m.neighbor_offsets[i + 1] - m.neighbor_offsets[i];
+        for (uint32_t k = 0; k < swid[s]; ++k) {
+            uint32_t code;
+            if (k < d) {
+                const uint32_t w = packed[soff[s] + size_t(k) * 64 + l];
+                const int64_t j = int64_t(i) + int64_t(int16_t(uint16_t(w & 0xFFFFu)));
+                int64_t dy = j / C - row, dx = j % C - col;
+                uint32_t lr = lr_i, cl = cl_i, sign = 0;
+                if (!forward(dy, dx)) {   // anchored at j
+                    lr = uint32_t(int64_t(lr_i) + dy);
+                    cl = uint32_t(int64_t(cl_i) + dx);
+                    dy = -dy;
+                    dx = -dx;
+                    sign = 1;
+                }
+                const uint32_t slot = slot_row(dir_of(dy, dx), lr) * JT_W + cl;
+                if (slot - uint32_t(l) >= 0x8000u) return false;
+                code = (slot - uint32_t(l)) | (sign << 15);
+            } else {   // padding: the zero slot
+                if (L.nslots - uint32_t(l) >= 0x8000u) return false;
+                code = L.nslots - uint32_t(l);
+            }
+            uint32_t& word = L.entries[L.eoff[s] + size_t(k / 2) * 64 + l];
+            word |= (k & 1u) ? (code << 16) : code;
+        }
+    }
+    return true;
+}
+
 int validate_mesh(const Mesh& m, std::string& why) {
     const size_t n = m.num_particles();
     if (m.positions.size() != 3 * n || m.rest_positions.size() != 3 * n) return why = "position arrays", XPBD_E_INVALID;
@@ -766,6 +909,24 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
                 TRY(upload(h, h->nbtile, tk.data(), tk.size()));
                 h->tile_cols = tc;
                 h->tile_rows = tr;
+                SharedLayout L;
+                int lds_max = 0;
+                (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device);
+                if (h->cfg.grid_tiles != 2 && share_sell(m, soff, swid, pk, h->num_pairs, tc, tr, L) &&
+                    jacobi_shared_lds_bytes(L.nslots, h->num_pairs) <= size_t(lds_max)) {
+                    uint16_t* jp = nullptr;
+                    uint32_t *jobs = nullptr, *ent = nullptr, *eoff = nullptr;
+                    TRY(dalloc(h, &jp, L.jpair.size()));
+                    TRY(dalloc(h, &jobs, L.jobs.size()));
+                    TRY(dalloc(h, &ent, L.entries.size()));
+                    TRY(dalloc(h, &eoff, L.eoff.size()));
+                    TRY(upload(h, jp, L.jpair.data(), L.jpair.size()));
+                    TRY(upload(h, jobs, L.jobs.data(), L.jobs.size()));
+                    TRY(upload(h, ent, L.entries.data(), L.entries.size()));
+                    TRY(upload(h, eoff, L.eoff.data(), L.eoff.size()));
+                    h->js = SharedTiles{jp, jobs, ent, eoff, L.njobs_main, L.njobs_edge, L.nslots};
+                    h->js_ok = true;
+                }
             }
         }
     }
@@ -974,6 +1135,9 @@ void apply_tuning(xpbd_cloth* h) {
     const uint32_t groups = std::max(1u, (h->nslices + 3) / 4);
     h->pj_blocks = std::max(1u, std::min(persistent_max_blocks(h->nbpk != nullptr, h->cfg.fast_division != 0), groups));
     h->gs_colour = h->cfg.gs_launch == 2 ? 1u : h->cfg.gs_launch == 1 ? 0u : h->gs_colour_tuned;
+    // grid tiles: 1 = the form the autotuner found faster, 3 = constraint-
+    // shared, 2 = per entry
+    h->js_on = h->js_ok && (h->cfg.grid_tiles == 3 || (h->cfg.grid_tiles == 1 && h->js_tuned)) ? 1u : 0u;
 }
 
 // Grid neighbours and feature constants of the warm-start network.  Needs a
@@ -1098,6 +1262,37 @@ int autotune(xpbd_cloth* h) {
         }
     h->jacobi_block = best_block;
     h->remap = best_remap;
+    // grid tiles: per-entry or constraint-shared, whichever runs faster here
+    h->js_tuned = false;
+    if (h->js_ok && h->cfg.grid_tiles == 1) {
+        float t_form[2] = {0.f, 0.f};
+        for (uint32_t form = 0; form < 2; ++form) {
+            JacobiArgs a = jacobi_args(h, p);
+            a.sh = form ? h->js : SharedTiles{};
+            a.src = h->A;
+            a.dst = h->B;
+            if (a.fastdiv) {
+                launch_tame_scan(h->A, h->n, tame_slot(h, h->A), h->stream);
+                a.tame_in = tame_slot(h, h->A);
+                a.tame_out = tame_slot(h, h->B);
+            }
+            launch_jacobi(0, a, h->jacobi_block, h->stream);   // warm-up
+            (void)hipEventRecord(e0, h->stream);
+            for (uint32_t r = 0; r < reps; ++r) {
+                a.src = a.dst;
+                a.dst = (a.dst == h->A) ? h->B : h->A;
+                if (a.fastdiv) {
+                    a.tame_in = tame_slot(h, a.src);
+                    a.tame_out = tame_slot(h, a.dst);
+                }
+                launch_jacobi(0, a, h->jacobi_block, h->stream);
+            }
+            (void)hipEventRecord(e1, h->stream);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&t_form[form], e0, e1);
+        }
+        h->js_tuned = t_form[1] < t_form[0];
+    }
     // Coloured GS: the grid size of the persistent substep.  Every block takes
     // part in each of the I x colours grid barriers, whose cost grows with the
     // grid, while fewer blocks leave more constraints per thread per colour;
@@ -2190,7 +2385,8 @@ XPBD_API int xpbd_selftest_sqrt(int exp_lo, int exp_hi, uint32_t* out2) {
 
 XPBD_API int xpbd_get_layout(xpbd_cloth* h, uint32_t* out4) {
     if (!h || !out4) return XPBD_E_INVALID;
-    out4[0] = h->nbtile ? XPBD_ADJ_GRID_TILES : (h->nbpk ? XPBD_ADJ_PACKED : XPBD_ADJ_PLAIN);
+    out4[0] = h->js_on ? XPBD_ADJ_SHARED_TILES
+                       : h->nbtile ? XPBD_ADJ_GRID_TILES : (h->nbpk ? XPBD_ADJ_PACKED : XPBD_ADJ_PLAIN);
     out4[1] = h->tile_cols;
     out4[2] = h->tile_rows;
     out4[3] = fast_division_active(h) ? 1u : 0u;

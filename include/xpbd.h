@@ -228,8 +228,13 @@ typedef struct xpbd_sim_config {
     uint32_t grid_tiles;                  /* 1 (default): a single grid cloth whose
                                              column count is a multiple of 64 runs the
                                              Jacobi gather on 64x8 tiles staged in LDS
-                                             with a 2-particle halo (read at create;
-                                             same results) */
+                                             with a 2-particle halo, per entry or with
+                                             each constraint evaluated once per tile,
+                                             whichever the autotuner measures faster
+                                             (per entry without autotuning); 2: per
+                                             entry; 3: once per tile; 0: no tiles.
+                                             Tiles are chosen at create; all forms
+                                             give the same results */
     uint32_t gs_launch;                   /* coloured GS launch form: 0 (default) =
                                              autotuned (else persistent), 1 = one
                                              persistent launch per substep with grid
@@ -371,13 +376,15 @@ XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t*
  * once, and the next xpbd_sync reports XPBD_E_STATE and clears the word. */
 XPBD_API int xpbd_debug_raise_device_error(xpbd_cloth* h);
 
-/* Layout the handle chose at create (no reference counterpart; for tests and
- * tools): out4 = {adjacency: 0 plain 12-byte entries, 1 packed 4-byte entries,
- * 2 packed entries on LDS grid tiles; grid-tile columns; grid-tile rows;
- * 1 if the fast correctly rounded division is active}. */
+/* Layout the handle uses (no reference counterpart; for tests and tools):
+ * out4 = {adjacency: 0 plain 12-byte entries, 1 packed 4-byte entries,
+ * 2 packed entries on LDS grid tiles, 3 grid tiles with each constraint
+ * evaluated once per tile; grid-tile columns; grid-tile rows; 1 if the fast
+ * correctly rounded division is active}. */
 #define XPBD_ADJ_PLAIN 0
 #define XPBD_ADJ_PACKED 1
 #define XPBD_ADJ_GRID_TILES 2
+#define XPBD_ADJ_SHARED_TILES 3
 XPBD_API int xpbd_get_layout(xpbd_cloth* h, uint32_t* out4);
 
 /* Device self-test of the shared-reciprocal division used by the Jacobi
