@@ -83,17 +83,37 @@ __device__ __forceinline__ void aabb_fold(bool valid, float x, float y, float z,
         }
 }
 
+// v from lane (per DPP_CTRL) of the same wavefront; a lane whose source does
+// not exist, or whose row ROW_MASK leaves out, gets its own v.
+template <int DPP_CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_or_self(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), DPP_CTRL, ROW_MASK, 0xF, false));
+}
+
+// Wavefront bounds into lane 63 (every lane must be active): row_shr 1, 2,
+// 4, 8 leave each row's extremum in its lane 15, row_bcast 15 and 31 carry
+// them up to lane 63.  Each step is one DPP min / max (no LDS crossbar);
+// min / max are exact, so the order does not matter.
+__device__ __forceinline__ void wave_bounds_to_lane63(float lo[3], float hi[3]) {
+#define XPBD_DPP_STEP(CTRL, ROWS)                                     \
+    _Pragma("unroll") for (int a = 0; a < 3; ++a) {                   \
+        lo[a] = fminf(lo[a], dpp_or_self<CTRL, ROWS>(lo[a]));         \
+        hi[a] = fmaxf(hi[a], dpp_or_self<CTRL, ROWS>(hi[a]));         \
+    }
+    XPBD_DPP_STEP(0x111, 0xF)   // row_shr:1
+    XPBD_DPP_STEP(0x112, 0xF)   // row_shr:2
+    XPBD_DPP_STEP(0x114, 0xF)   // row_shr:4
+    XPBD_DPP_STEP(0x118, 0xF)   // row_shr:8
+    XPBD_DPP_STEP(0x142, 0xA)   // row_bcast:15 into rows 1 and 3
+    XPBD_DPP_STEP(0x143, 0xC)   // row_bcast:31 into rows 2 and 3
+#undef XPBD_DPP_STEP
+}
+
 __device__ __forceinline__ void block_aabb_store(float lo[3], float hi[3], float* part, uint32_t slot) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
-            hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
-        }
+    wave_bounds_to_lane63(lo, hi);
     __shared__ float red[16][6];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0)
+    if (lane == 63)
         for (int a = 0; a < 3; ++a) {
             red[wave][a] = lo[a];
             red[wave][3 + a] = hi[a];
@@ -1124,20 +1144,18 @@ __global__ __launch_bounds__(RADIX_BLOCK) void k_coll_keys(const float4* F, uint
             hi[1] = fmaxf(hi[1], b.x);
             hi[2] = fmaxf(hi[2], b.y);
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
-                hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
-            }
-        if (lane == 0)
+        wave_bounds_to_lane63(lo, hi);
+        if (lane == 63)
             for (int a = 0; a < 3; ++a) {
                 red[wave][a] = lo[a];
                 red[wave][3 + a] = hi[a];
             }
         __syncthreads();
         if (t == 0) {
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = red[0][a];
+                hi[a] = red[0][3 + a];
+            }
             for (uint32_t w = 1; w < RADIX_WAVES; ++w)
                 for (int a = 0; a < 3; ++a) {
                     lo[a] = fminf(lo[a], red[w][a]);
