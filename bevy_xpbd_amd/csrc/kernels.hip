@@ -611,18 +611,18 @@ __global__ __launch_bounds__(64 * JT_ROWS) __attribute__((amdgpu_waves_per_eu(8)
 // zeros; its term is a zero, and the accumulators (starting at +0, never -0
 // in round-to-nearest) do not change when a zero is added.
 // ---------------------------------------------------------------------------
-// (n, dlam) of the constraint (a, b) as jacobi_entry computes it for a
-// (zeros where jacobi_entry adds nothing).
+// (n, dlam) of the constraint (a, b) as jacobi_entry computes it for a, or
+// zeros where jacobi_entry adds nothing (and for `none`).  Branch-free (the
+// skipped results are selected away), so several constraints can be in
+// flight at once.
 template <bool FAST>
-__device__ __forceinline__ float4 constraint_result(float4 pa, float4 pb, float4 pr) {
-    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+__device__ __forceinline__ float4 constraint_result(float4 pa, float4 pb, float4 pr, bool none) {
     float gx = pa.x - pb.x, gy = pa.y - pb.y, gz = pa.z - pb.z;
     const float dot = (gx * gx + gy * gy) + gz * gz;
     const float len = FAST ? sqrt_rn(dot) : sqrtf(dot);
-    if (len < 1e-8f) return zero;
     const float c = len - pr.x;
     const float wsum = pa.w + pb.w + pr.y;
-    if (!FAST && wsum < 1e-8f) return zero;
+    const bool skip = none || len < 1e-8f || (!FAST && wsum < 1e-8f);
     const float num = FAST ? -c : -c - pr.z;
     float dlam;
     if (FAST) {
@@ -637,40 +637,13 @@ __device__ __forceinline__ float4 constraint_result(float4 pa, float4 pb, float4
         gz = gz / len;
         dlam = num / wsum;
     }
-    return make_float4(gx, gy, gz, dlam);
-}
-
-// constraint_result without branches (the same operations; skipped results
-// are selected away), so several constraints can be in flight at once.
-template <bool FAST>
-__device__ __forceinline__ float4 constraint_result_nb(float4 pa, float4 pb, float4 pr, bool none) {
-    float gx = pa.x - pb.x, gy = pa.y - pb.y, gz = pa.z - pb.z;
-    const float dot = (gx * gx + gy * gy) + gz * gz;
-    const float len = FAST ? sqrt_rn(dot) : sqrtf(dot);
-    const float c = len - pr.x;
-    const float wsum = pa.w + pb.w + pr.y;
-    const bool skip = none || len < 1e-8f || (!FAST && wsum < 1e-8f);
-    const float num = FAST ? -c : -c - pr.z;
-    float dlam;
-    if (FAST) {
-        const float yl = rcp_rn(len);
-        gx = div_rn_nz(gx, len, yl);
-        gy = div_rn_nz(gy, len, yl);
-        gz = div_rn_nz(gz, len, yl);
-        dlam = div_rn_nz(num, wsum, rcp_rn(wsum));
-    } else {
-        gx = gx / len;
-        gy = gy / len;
-        gz = gz / len;
-        dlam = num / wsum;
-    }
     return skip ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(gx, gy, gz, dlam);
 }
 
 // Phase 1 of a block: every job's constraints into the result slots.  A
 // wavefront's jobs are wave, wave + JT_ROWS, ...; the first JS_JOBS_PF of
 // them are decoded, and their pair indices loaded, before the tile barrier
-// (shared_prefetch), so those loads overlap the staging.
+// (in k_jacobi_shared), so those loads overlap the staging.
 constexpr uint32_t JS_JOBS_PF = 8;
 
 struct SharedJob {
@@ -698,25 +671,10 @@ __device__ __forceinline__ SharedJob shared_job(const JacobiArgs& a, uint32_t j,
     return J;
 }
 
-template <bool FAST>
-__device__ __forceinline__ void shared_compute(const SharedJob& J, const float4* tl, const float4* lpairs,
-                                               float4* res) {
-    if (J.item == 0xFFFFFFFFu) return;
-    const int32_t dy = int32_t(J.item & 3u), dx = int32_t((J.item >> 2) & 7u) - 2;
-    const uint32_t lr = (J.item >> 8) & 15u, row = J.item >> 20;
-    float4 r4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (J.pid != 0xFFFFu) {
-        const float4 pa = tl[lr * JT_W + J.cl];
-        const float4 pb = tl[(lr + uint32_t(dy)) * JT_W + uint32_t(int32_t(J.cl) + dx)];
-        r4 = constraint_result<FAST>(pa, pb, lpairs[J.pid]);
-    }
-    res[row * JT_W + J.cl] = r4;
-}
-
-// Two prefetched jobs at once, branch-free until the stores: a job without
+// Two jobs at once, branch-free until the stores: a job without
 // a constraint reads tile slot 0 and pair 0 (both staged) and stores zeros.
 template <bool FAST>
-__device__ __forceinline__ void shared_compute2(const SharedJob& J0, const SharedJob& J1, const float4* tl,
+__device__ __forceinline__ void shared_compute(const SharedJob& J0, const SharedJob& J1, const float4* tl,
                                                 const float4* lpairs, float4* res) {
     float4 pa[2], pb[2], pr[2];
     bool none[2];
@@ -735,7 +693,7 @@ __device__ __forceinline__ void shared_compute2(const SharedJob& J0, const Share
     }
     float4 r[2];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) r[u] = constraint_result_nb<FAST>(pa[u], pb[u], pr[u], none[u]);
+    for (int u = 0; u < 2; ++u) r[u] = constraint_result<FAST>(pa[u], pb[u], pr[u], none[u]);
 #pragma unroll
     for (int u = 0; u < 2; ++u)
         if (J[u]->item != 0xFFFFFFFFu) res[(J[u]->item >> 20) * JT_W + J[u]->cl] = r[u];
@@ -747,9 +705,13 @@ __device__ __forceinline__ void shared_phase1(const JacobiArgs& a, const float4*
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t njobs = a.sh.njobs_main + a.sh.njobs_edge;
 #pragma unroll
-    for (uint32_t k = 0; k < JS_JOBS_PF; k += 2) shared_compute2<FAST>(pf[k], pf[k + 1], tl, lpairs, res);   // unused: item none
+    for (uint32_t k = 0; k < JS_JOBS_PF; k += 2) shared_compute<FAST>(pf[k], pf[k + 1], tl, lpairs, res);   // unused: item none
+    SharedJob none;
+    none.item = 0xFFFFFFFFu;
+    none.cl = 0u;
+    none.pid = 0xFFFFu;
     for (uint32_t j = wave + JS_JOBS_PF * JT_ROWS; j < njobs; j += JT_ROWS)
-        shared_compute<FAST>(shared_job(a, j, r0, c0), tl, lpairs, res);
+        shared_compute<FAST>(shared_job(a, j, r0, c0), none, tl, lpairs, res);
 }
 
 // pre: the lane's first JS_WORDS_PF entry words (those below ceil(width / 2)).
