@@ -282,8 +282,9 @@ def config2_rates(X, frames: int = 40, warmup: int = 5):
     centre pinned, seeded rotation, self-collision every substep, contact
     distance 0.8 x spacing), 4 substeps x 8 iterations, both solvers (Jacobi
     omega 0.32, coloured Gauss-Seidel omega 1.0; GS in its autotuned launch
-    form and in each form: one persistent launch per substep, one launch per
-    colour), frames replayed from the hipGraph: substeps/s, the GS substep's
+    form and in each form: one persistent launch per substep with grid
+    barriers, one launch per colour, one persistent launch ordered by per-tile
+    flags), frames replayed from the hipGraph: substeps/s, the GS substep's
     time and its colours."""
     import numpy as np
     h = 1.0 / 255.0
@@ -292,7 +293,8 @@ def config2_rates(X, frames: int = 40, warmup: int = 5):
                        "4 substeps x 8 iterations (step = 1 frame)", "angle_rad": round(angle, 6)}
     for name, solver, form in (("jacobi", X.SOLVER_JACOBI, 0), ("gauss_seidel", X.SOLVER_GAUSS_SEIDEL, 0),
                                ("gauss_seidel_persistent", X.SOLVER_GAUSS_SEIDEL, 1),
-                               ("gauss_seidel_per_colour", X.SOLVER_GAUSS_SEIDEL, 2)):
+                               ("gauss_seidel_per_colour", X.SOLVER_GAUSS_SEIDEL, 2),
+                               ("gauss_seidel_tile_flags", X.SOLVER_GAUSS_SEIDEL, 3)):
         mesh = X.ClothMeshData.sheet(cols=256, rows=256, spacing=h, layout=X.LAYOUT_HORIZONTAL_XZ, height=1.0,
                                      angle=angle, pins=X.PIN_CENTER)
         cfg = X.ClothSimConfig.default(solve_substeps=4, solve_inner_iterations=8, solver=solver, use_graph=1,
@@ -309,6 +311,8 @@ def config2_rates(X, frames: int = 40, warmup: int = 5):
         if solver == X.SOLVER_GAUSS_SEIDEL:
             ms, cnt = sim.profile_frames(5, X.KERNEL_GS_SUBSTEP)
             rec.update({"gs_substep_launch_us": round(ms * 1e3, 2), "colours": len(mesh.field("batch_offsets")) - 1})
+            if form in (0, 3):
+                rec["tile_plan"] = sim.gs_tiles()
         sim.close()
         out[name] = rec
         log(f"bench: 256x256 sheet, {name}: {rate:.1f} substeps/s")
@@ -318,10 +322,11 @@ def config2_rates(X, frames: int = 40, warmup: int = 5):
 def gs_forms_1024(X, frames: int = 5, warmup: int = 2):
     """Coloured Gauss-Seidel on the 1024x1024 bench sheet (8 substeps x 10
     iterations, omega 1.0, collision every substep), frames replayed from the
-    hipGraph, in its autotuned launch form and in each form: substeps/s."""
+    hipGraph, in its autotuned launch form and in each form: substeps/s
+    (and the tile plan: particles per tile, tiles, blocks, most dependencies)."""
     out = {"workload": "1024x1024 bench sheet, coloured Gauss-Seidel 8 substeps x 10 iterations, "
                        "self-collision every substep"}
-    for name, form in (("autotuned", 0), ("persistent", 1), ("per_colour", 2)):
+    for name, form in (("autotuned", 0), ("persistent", 1), ("per_colour", 2), ("tile_flags", 3)):
         mesh, thickness, _, _ = build_workload(X, "sheet1024")
         cfg = X.ClothSimConfig.default(solve_substeps=8, solve_inner_iterations=10, solver=X.SOLVER_GAUSS_SEIDEL,
                                        use_graph=1, gs_launch=form)
@@ -334,8 +339,10 @@ def gs_forms_1024(X, frames: int = 5, warmup: int = 2):
             sim.step_frames(frames)
             sim.sync()
             rate = 8 * frames / (time.perf_counter() - t0)
-        sim.close()
         out[name] = {"substeps_per_s": round(rate, 1)}
+        if form in (0, 3):
+            out[name]["tile_plan"] = sim.gs_tiles()
+        sim.close()
         log(f"bench: 1024x1024 sheet, Gauss-Seidel {name}: {rate:.1f} substeps/s")
     return out
 

@@ -187,6 +187,7 @@ def lib():
             "xpbd_profile_frames": (i32, [vp, u32, i32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
             "xpbd_debug_collision_keys": (i32, [vp, f32, vp, vp, vp]),
             "xpbd_debug_raise_device_error": (i32, [vp]),
+            "xpbd_debug_gs_tiles": (i32, [vp, u32, u32, vp]),
             "xpbd_get_layout": (i32, [vp, vp]),
             "xpbd_selftest_division": (i32, [i32, i32, u64, u64, vp]),
             "xpbd_selftest_sqrt": (i32, [i32, i32, vp]),
@@ -492,6 +493,14 @@ class ClothComputePlugin:
     def raise_device_error(self):
         """Test hook: set the device error word, as an expired barrier wait does."""
         _check(lib().xpbd_debug_raise_device_error(self._h), "xpbd_debug_raise_device_error", self._h)
+
+    def gs_tiles(self, tile: int = 0, blocks: int = 0):
+        """Test/tuning hook: the per-tile-flag GS plan (gs_launch = 3), re-planned
+        with `tile` particles per tile on `blocks` blocks when tile > 0.  Returns
+        (particles per tile, tiles, blocks, most dependencies of one tile)."""
+        out = (C.c_uint32 * 4)()
+        _check(lib().xpbd_debug_gs_tiles(self._h, int(tile), int(blocks), out), "xpbd_debug_gs_tiles", self._h)
+        return tuple(int(v) for v in out)
 
     def residual(self) -> float:
         r = C.c_double()

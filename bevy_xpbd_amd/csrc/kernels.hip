@@ -2044,17 +2044,49 @@ struct GridSync {
     }
 };
 
+// The GS work iterate through ordinary loads and stores (ordered between
+// blocks by launch boundaries or the grid barrier's L2 writeback).
+struct PlainW {
+    float4* W;
+    __device__ __forceinline__ float4 ld(uint32_t i) const { return W[i]; }
+    __device__ __forceinline__ void st(uint32_t i, float4 v) const { W[i] = v; }
+};
+
+// The GS work iterate at device scope, for blocks that hand particles to each
+// other through flags without any L2 writeback or invalidate: buffer accesses
+// with the SC1 cache policy (the policy of agent-scope atomics on this chip)
+// miss the CU's L1 and the XCD's L2 copies on load and write through to the
+// device's coherence point on store.  A store has reached that point once the
+// wave's vmcnt has drained, so a flag stored after s_waitcnt vmcnt(0) and a
+// barrier is seen after the data it announces.  The buffer range (n x 16
+// bytes) bounds every access.
+constexpr int CPOL_SC1 = 16;
+typedef float v4f32 __attribute__((ext_vector_type(4)));
+struct DeviceW {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ DeviceW(float4* W, uint32_t n)
+        : r(__builtin_amdgcn_make_buffer_rsrc(W, 0, int(n * 16u), 0x00020000)) {}
+    __device__ __forceinline__ float4 ld(uint32_t i) const {
+        const v4f32 v = __builtin_amdgcn_raw_buffer_load_b128(r, int(i * 16u), 0, CPOL_SC1);
+        return make_float4(v.x, v.y, v.z, v.w);
+    }
+    __device__ __forceinline__ void st(uint32_t i, float4 x) const {
+        const v4f32 v = {x.x, x.y, x.z, x.w};
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, int(i * 16u), 0, CPOL_SC1);
+    }
+};
+
 // One constraint of coloured Gauss-Seidel (xpbd_cpu.rs:312-391): project
 // constraint e of the colour-sorted arrays on the work iterate W in place.
 // Constraints of one colour share no vertex, so any order within a colour,
-// and any split of a colour over launches or blocks, gives the same bits.
-__device__ __forceinline__ void gs_project(const GsArgs& a, uint32_t e) {
+// and any split of a colour over launches, blocks or tiles, gives the same bits.
+// gs_apply: the projection of constraint (ij, rc) from its endpoints' loaded
+// iterates pi, pj; gs_project: the same with its own loads.
+template <class Mem>
+__device__ __forceinline__ void gs_apply(const GsArgs& a, uint2 ij, float2 rc, float4 pi, float4 pj, const Mem& m) {
     const DevParams& p = a.p;
-    const uint2 ij = a.ij[e];
-    const float4 pi = a.W[ij.x], pj = a.W[ij.y];
     const float wi = pi.w, wj = pj.w;
     if (wj <= 0.f && wi <= 0.f) return;
-    const float2 rc = a.rc[e];
     float gx = pi.x - pj.x, gy = pi.y - pj.y, gz = pi.z - pj.z;
     const float len = len3(gx, gy, gz);
     if (len < 1e-8f) return;
@@ -2083,8 +2115,15 @@ __device__ __forceinline__ void gs_project(const GsArgs& a, uint32_t e) {
         jy = jy * s;
         jz = jz * s;
     }
-    if (wi > 0.f) a.W[ij.x] = make_float4(pi.x + ix, pi.y + iy, pi.z + iz, wi);
-    if (wj > 0.f) a.W[ij.y] = make_float4(pj.x + jx, pj.y + jy, pj.z + jz, wj);
+    if (wi > 0.f) m.st(ij.x, make_float4(pi.x + ix, pi.y + iy, pi.z + iz, wi));
+    if (wj > 0.f) m.st(ij.y, make_float4(pj.x + jx, pj.y + jy, pj.z + jz, wj));
+}
+
+template <class Mem>
+__device__ __forceinline__ void gs_project(const GsArgs& a, uint32_t e, const Mem& m) {
+    const uint2 ij = a.ij[e];
+    const float4 pi = m.ld(ij.x), pj = m.ld(ij.y);
+    gs_apply(a, ij, a.rc[e], pi, pj, m);
 }
 
 // One colour of one GS iteration as its own launch: constraints [e0, e1) of
@@ -2092,7 +2131,7 @@ __device__ __forceinline__ void gs_project(const GsArgs& a, uint32_t e) {
 // colours, in place of a grid barrier.
 __global__ __launch_bounds__(256) void k_gs_colour(GsArgs a, uint32_t e0, uint32_t e1) {
     const uint32_t e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < e1) gs_project(a, e);
+    if (e < e1) gs_project(a, e, PlainW{a.W});
 }
 
 __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
@@ -2110,7 +2149,7 @@ __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
     for (uint32_t it = 0; it < a.iterations; ++it) {
         for (uint32_t b = 0; b < a.num_batches; ++b) {
             const uint32_t e1 = a.batch_offsets[b + 1];
-            for (uint32_t e = a.batch_offsets[b] + gt; e < e1; e += nthreads) gs_project(a, e);
+            for (uint32_t e = a.batch_offsets[b] + gt; e < e1; e += nthreads) gs_project(a, e, PlainW{a.W});
             gsync.sync();
         }
     }
@@ -2128,6 +2167,113 @@ __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
         }
         block_aabb_store(lo, hi, a.aabb, blockIdx.x);
     }
+}
+
+// Wait until every tile in tile T's dependency list has finished step s - 1
+// (its flag >= s).  One wavefront polls, a lane per dependency; the wait is
+// bounded like gs_wait (on expiry the error word is set and this and every
+// later wait of the launch return at once).
+__device__ __forceinline__ void gs_tile_wait(const GsTiles& t, uint32_t T, uint32_t s, uint32_t* err) {
+    if (threadIdx.x < 64) {
+        const uint32_t d0 = t.doff[T], nd = t.doff[T + 1] - d0;
+        uint32_t* f = threadIdx.x < nd ? t.flag + t.dep[d0 + threadIdx.x] : nullptr;
+        if (nd && !__builtin_amdgcn_readfirstlane(gs_load(err))) {
+            for (uint32_t spins = 1;; ++spins) {
+                if (__all(f == nullptr || gs_load(f) >= s)) break;
+                __builtin_amdgcn_s_sleep(1);
+                if ((spins & 1023u) == 0 && __builtin_amdgcn_readfirstlane(gs_load(err))) break;
+                if (spins > (1u << 22)) {   // seconds at most: report instead of hanging
+                    if (threadIdx.x == 0) atomicExch(err, 1u);
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// One GS substep (xpbd_cpu.rs:312-391 inside the substep of :240-420) with
+// the grid barriers replaced by per-tile flags.  Block b owns tiles b,
+// b + gridDim.x, ...; for each step in order it runs each of its tiles'
+// share: the predict of the tile's particles, then each colour's constraints
+// of the tile, then the final write (or the bounds) of its particles.  A step
+// waits only for the tiles whose particles the tile's constraints can share,
+// so neighbouring tiles run several colours apart instead of every colour
+// waiting for the slowest block.  Deadlock-free for a co-resident grid: the
+// lowest unfinished step only waits for finished ones.  The iterate goes
+// through DeviceW; a tile's flag is stored after its stores have drained.
+// Constraints per thread whose loads go out together in a tile step.  2 on
+// the 1024^2 sheet (1024-particle tiles): 771 substeps/s against 734 for 1
+// and 654 for 4, whose registers leave too few blocks resident for one tile
+// each.
+constexpr uint32_t GS_TILE_K = 2;
+
+__global__ __launch_bounds__(256) void k_gs_tiles(GsArgs a, GsTiles t) {
+    if (a.probe) {   // uniform: the co-residency check of this kernel's grid
+        GridSync gsync;
+        gsync.init(a.barrier, a.error);
+        return;
+    }
+    const DevParams& p = a.p;
+    const DeviceW w(a.W, p.n);
+    const uint32_t nb = a.num_batches;
+    const uint32_t last = 1 + a.iterations * nb;   // the final step
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t s = 0; s <= last; ++s) {
+        const bool colour = s > 0 && s < last;
+        const uint32_t* to = colour ? t.toff + ((s - 1) % nb) * (t.ntiles + 1) : nullptr;
+        for (uint32_t T = blockIdx.x; T < t.ntiles; T += gridDim.x) {
+            // the tile's constraint rows of this step do not depend on other
+            // tiles: fetched before the wait, GS_TILE_K per thread
+            uint32_t e0 = 0, e1 = 0;
+            uint2 ij[GS_TILE_K];
+            float2 rc[GS_TILE_K];
+            if (colour) {
+                e0 = to[T] + threadIdx.x;
+                e1 = to[T + 1];
+#pragma unroll
+                for (uint32_t k = 0; k < GS_TILE_K; ++k) {
+                    const uint32_t e = e0 + k * blockDim.x;
+                    ij[k] = e < e1 ? a.ij[e] : make_uint2(0u, 0u);
+                    rc[k] = e < e1 ? a.rc[e] : make_float2(0.f, 0.f);
+                }
+            }
+            if (s > 0) gs_tile_wait(t, T, s, a.error);
+            const uint32_t i0 = T * t.tile, i1 = min(i0 + t.tile, p.n);
+            if (s == 0) {
+                for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) w.st(i, predict(a.X[i], a.V[i], i, p));
+            } else if (colour) {
+                // the constraints of one colour share no particle: all loads
+                // first, then the updates
+                float4 pi[GS_TILE_K], pj[GS_TILE_K];
+#pragma unroll
+                for (uint32_t k = 0; k < GS_TILE_K; ++k) {
+                    if (e0 + k * blockDim.x < e1) {
+                        pi[k] = w.ld(ij[k].x);
+                        pj[k] = w.ld(ij[k].y);
+                    }
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < GS_TILE_K; ++k)
+                    if (e0 + k * blockDim.x < e1) gs_apply(a, ij[k], rc[k], pi[k], pj[k], w);
+                for (uint32_t e = e0 + GS_TILE_K * blockDim.x; e < e1; e += blockDim.x) gs_project(a, e, w);
+            } else {
+                for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+                    const float4 x = w.ld(i);
+                    if (a.final_write) {
+                        if (x.w > 0.f) finish_particle(x, a.X[i], i, a.ground, p, a.xout, a.vout);
+                    } else {
+                        aabb_fold(true, x.x, x.y, x.z, lo, hi);
+                    }
+                }
+                continue;   // nothing waits for the final step
+            }
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_store(t.flag + T, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (!a.final_write) block_aabb_store(lo, hi, a.aabb, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2671,6 +2817,19 @@ uint32_t gs_max_blocks(uint32_t block) {
 
 void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s) {
     hipLaunchKernelGGL(k_gs_substep, dim3(blocks), dim3(block), 0, s, a);
+}
+
+uint32_t gs_tiles_max_blocks() {
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_tiles, 256, 0) != hipSuccess) return 0;
+    return uint32_t(coresident_per_cu(per_cu, 256) * std::max(0, cus));
+}
+
+void launch_gs_tiles(const GsArgs& a, const GsTiles& t, uint32_t blocks, hipStream_t s) {
+    launch_zero_u32(t.flag, t.ntiles, s);
+    hipLaunchKernelGGL(k_gs_tiles, dim3(blocks), dim3(256), 0, s, a, t);
 }
 
 void launch_publish(const float4* X, uint32_t n, const uint32_t* tris, const uint32_t* vt_off, const uint32_t* vt_tri,

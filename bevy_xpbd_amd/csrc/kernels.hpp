@@ -204,9 +204,28 @@ struct GsArgs {
     uint32_t* barrier;
     uint32_t* error;
     uint32_t final_write;
-    uint32_t probe;      // 1: only the barrier set-up and one barrier (co-residency check)
+    uint32_t probe;      // 1: only the barrier set-up (k_gs_substep: and one barrier), the co-residency check
     DevParams p;
 };
+
+// Coloured GS ordered by per-tile flags instead of grid barriers.  The
+// particles are cut into tiles of `tile` consecutive indices; a constraint
+// belongs to the tile of its lower endpoint, so within a colour (sorted by
+// lower endpoint) each tile's constraints are one run.  The steps of a substep
+// are the predict, the I x colours constraint steps and the final write (or
+// bounds); a tile's step s may start once each tile in its dependency list
+// (every other tile whose particle range [first index, highest endpoint of its
+// constraints] overlaps its own) has finished step s - 1.
+struct GsTiles {
+    const uint32_t* toff;   // colours x (ntiles + 1): first constraint of each tile in each colour
+    const uint32_t* doff;   // ntiles + 1: dependency list offsets into dep
+    const uint32_t* dep;    // dependency lists (tile ids)
+    uint32_t* flag;         // ntiles: steps finished (zeroed by launch_gs_tiles)
+    uint32_t ntiles;
+    uint32_t tile;          // particles per tile
+};
+// A tile's dependencies are checked by the lanes of one wavefront.
+constexpr uint32_t GS_TILE_DEPS_MAX = 64;
 
 // ---- launchers (kernels.hip) ----
 uint32_t jacobi_blocks(uint32_t nslices, uint32_t block);
@@ -255,6 +274,12 @@ void launch_gs(const GsArgs& a, uint32_t blocks, uint32_t block, hipStream_t s);
 // positions; a.W, a.ij, a.rc and a.p are read.
 void launch_gs_colours(const GsArgs& a, const std::vector<uint32_t>& off, uint32_t iterations, hipStream_t s);
 uint32_t gs_max_blocks(uint32_t block);
+// One GS substep ordered by per-tile flags (zeroes t.flag first).  blocks
+// must be resident at once (block b takes tiles b, b + blocks, ...); every
+// wait is bounded and sets a.error on expiry.  a.W must hold n * 16 < 2^32
+// bytes.  With a.final_write == 0 one bounds partial per block goes to a.aabb.
+void launch_gs_tiles(const GsArgs& a, const GsTiles& t, uint32_t blocks, hipStream_t s);
+uint32_t gs_tiles_max_blocks();
 void launch_copy4(const float4* a, float4* b, uint32_t n, hipStream_t s);
 // per-slice tame bytes of P (1: every coordinate 0 or of magnitude in [2^-50, 2^20])
 void launch_tame_scan(const float4* P, uint32_t n, uint8_t* slice_tame, hipStream_t s);

@@ -52,6 +52,12 @@ constexpr float THICKNESS = 0.04f;                            // cloth_compute.r
 constexpr float DEFAULT_COLL_SCALE = 0.38f;                   // cloth_compute.rs:74
 constexpr float DEFAULT_LINEAR_AIR_DRAG_PER_SEC = 1.25f;      // cloth_compute.rs:80
 
+// Coloured GS forms: one persistent launch with grid barriers, one launch per
+// colour and iteration, one persistent launch ordered by per-tile flags.
+constexpr uint32_t GS_FORM_BARRIERS = 0, GS_FORM_COLOURS = 1, GS_FORM_TILES = 2;
+// smallest GS flag tile (particles); the device tile arrays are sized for it
+constexpr uint32_t GS_TILE_MIN = 64;
+
 uint32_t next_pow2(uint32_t v) {
     uint32_t p = 1;
     while (p < v && p < 0x80000000u) p <<= 1;
@@ -128,8 +134,15 @@ struct xpbd_cloth {
 
     uint32_t jacobi_block = 256, remap = 1, fused_predict = 0;
     uint32_t gs_block = 256, gs_blocks = 0;
-    uint32_t gs_colour_tuned = 0;   // the autotuner found one launch per colour faster
-    uint32_t gs_colour = 0;         // GS substeps run one launch per colour (else persistent)
+    uint32_t gs_form_tuned = 0;     // the GS form the autotuner measured fastest (GS_FORM_*)
+    uint32_t gs_form = 0;           // the GS form substeps run (GS_FORM_*)
+    // GS with per-tile flags: the tiles in use, their grid, the device
+    // capacity of the tile arrays (tiles of at least GS_TILE_MIN particles),
+    // and the colour-sorted endpoints to plan other tile sizes from
+    GsTiles gt{};
+    bool gt_ok = false;
+    uint32_t gt_blocks = 0, gt_cap = 0, gt_deps = 0;
+    std::vector<uint2> gs_ij_host;
     uint32_t pj_blocks = 0;        // persistent Jacobi grid (co-resident blocks)
 
     // fast correctly rounded division / sqrt in the Jacobi gather: the static
@@ -334,6 +347,21 @@ void enqueue_gs_colours(xpbd_cloth* h, const GsArgs& g, bool collision, uint32_t
     else launch_finalize(g.W, g.xout, g.vout, g.ground, g.p, s);
 }
 
+// The GS part of a substep in the handle's form (gs_form); g.final_write must
+// be !collision.  With collision, nparts = the bounds partials written.
+void enqueue_gs(xpbd_cloth* h, const GsArgs& g, bool collision, uint32_t& nparts, hipStream_t s) {
+    if (h->gs_form == GS_FORM_COLOURS) {
+        enqueue_gs_colours(h, g, collision, nparts, s);
+    } else if (h->gs_form == GS_FORM_TILES) {
+        launch_gs_tiles(g, h->gt, h->gt_blocks, s);
+        nparts = h->gt_blocks;
+    } else {
+        launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
+        launch_gs(g, h->gs_blocks, h->gs_block, s);
+        nparts = h->gs_blocks;
+    }
+}
+
 // One substep (cloth_jacobi.rs:702-883 / cloth_compute.rs:1313-1488).  The
 // state is always (X, V) at substep boundaries; A/B hold the iterates.
 void enqueue_substep(xpbd_cloth* h, bool collision) {
@@ -343,7 +371,6 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
     const float4* F = nullptr;
     uint32_t nparts = 0;
     if (h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL) {
-        if (!h->gs_colour) launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
         GsArgs g{};
         g.X = h->X;
         g.V = h->V;
@@ -363,11 +390,9 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         g.p = p;
         {
             ProbeScope ps(h, XPBD_KERNEL_GS_SUBSTEP);
-            if (h->gs_colour) enqueue_gs_colours(h, g, collision, nparts, s);
-            else launch_gs(g, h->gs_blocks, h->gs_block, s);
+            enqueue_gs(h, g, collision, nparts, s);
         }
         F = h->A;
-        if (!h->gs_colour) nparts = h->gs_blocks;
     } else if (h->warm) {
         // learned warm start: predict -> (features, MLP) x0 = xp + MLP -> warm
         // iterations.  With a finite threshold, the relative residual of the
@@ -846,6 +871,86 @@ int validate_mesh(const Mesh& m, std::string& why) {
     return XPBD_OK;
 }
 
+// Tiles of `tile` consecutive particles for the per-tile-flag GS substep
+// (GsTiles): each colour's run of constraints per tile, and each tile's
+// dependency list, the other tiles U whose span [first particle, highest
+// endpoint of U's constraints] overlaps the tile's own.  Constraints whose
+// particles two tiles share make the spans overlap, so a tile that waits for
+// these tiles' previous step reads every particle after its last write.
+// Fails (false) when a list would exceed GS_TILE_DEPS_MAX or a colour is not
+// sorted by lower endpoint.
+bool plan_gs_tiles(const std::vector<uint2>& ij, const std::vector<uint32_t>& off, uint32_t n, uint32_t tile,
+                   std::vector<uint32_t>& toff, std::vector<uint32_t>& doff, std::vector<uint32_t>& dep,
+                   uint32_t& max_deps) {
+    const uint32_t nt = (n + tile - 1) / tile;
+    const size_t nb = off.size() - 1;
+    toff.assign(nb * (size_t(nt) + 1), 0u);
+    std::vector<uint32_t> reach(nt);
+    for (uint32_t t = 0; t < nt; ++t) reach[t] = std::min(n, (t + 1) * tile) - 1;
+    for (size_t b = 0; b < nb; ++b) {
+        uint32_t* to = toff.data() + b * (size_t(nt) + 1);
+        size_t k = off[b];
+        uint32_t prev_lo = 0;
+        for (size_t q = off[b]; q < off[b + 1]; ++q) {
+            const uint32_t lo = std::min(ij[q].x, ij[q].y), hi = std::max(ij[q].x, ij[q].y);
+            if (lo < prev_lo || hi >= n) return false;
+            prev_lo = lo;
+            reach[lo / tile] = std::max(reach[lo / tile], hi);
+        }
+        for (uint32_t t = 0; t <= nt; ++t) {
+            const uint64_t first = uint64_t(t) * tile;
+            while (k < off[b + 1] && std::min(ij[k].x, ij[k].y) < first) ++k;
+            to[t] = uint32_t(k);
+        }
+    }
+    std::vector<uint32_t> cnt(nt, 0u);
+    for (uint32_t u = 0; u < nt; ++u) {
+        const uint32_t last = reach[u] / tile;   // tiles u+1..last overlap u
+        if (last - u > GS_TILE_DEPS_MAX) return false;
+        for (uint32_t t = u + 1; t <= last; ++t) {
+            ++cnt[u];
+            ++cnt[t];
+        }
+    }
+    doff.assign(size_t(nt) + 1, 0u);
+    max_deps = 0;
+    for (uint32_t t = 0; t < nt; ++t) {
+        if (cnt[t] > GS_TILE_DEPS_MAX) return false;
+        max_deps = std::max(max_deps, cnt[t]);
+        doff[t + 1] = doff[t] + cnt[t];
+    }
+    dep.assign(doff[nt], 0u);
+    std::vector<uint32_t> cur(doff.begin(), doff.end() - 1);
+    for (uint32_t u = 0; u < nt; ++u)
+        for (uint32_t t = u + 1; t <= reach[u] / tile; ++t) {
+            dep[cur[u]++] = t;
+            dep[cur[t]++] = u;
+        }
+    return true;
+}
+
+// Plan GS flag tiles of `tile` particles and upload them into the handle's
+// tile arrays; gt_ok says whether they are usable.  The grid is the tiles,
+// within what can be resident at once and the bounds partial slots.
+int build_gs_tiles(xpbd_cloth* h, uint32_t tile, uint32_t blocks = 0) {
+    h->gt_ok = false;
+    if (!h->gt_cap || tile < GS_TILE_MIN) return XPBD_OK;
+    std::vector<uint32_t> toff, doff, dep;
+    uint32_t max_deps = 0;
+    if (!plan_gs_tiles(h->gs_ij_host, h->batch_offsets, h->n, tile, toff, doff, dep, max_deps)) return XPBD_OK;
+    const uint32_t nt = uint32_t(doff.size() - 1);
+    TRY(upload(h, const_cast<uint32_t*>(h->gt.toff), toff.data(), toff.size()));
+    TRY(upload(h, const_cast<uint32_t*>(h->gt.doff), doff.data(), doff.size()));
+    TRY(upload(h, const_cast<uint32_t*>(h->gt.dep), dep.data(), dep.size()));
+    h->gt.ntiles = nt;
+    h->gt.tile = tile;
+    h->gt_deps = max_deps;
+    const uint32_t cap = std::min(std::max(1u, gs_tiles_max_blocks()), h->aabb_slots);
+    h->gt_blocks = std::max(1u, std::min({nt, cap, blocks ? blocks : nt}));
+    h->gt_ok = true;
+    return XPBD_OK;
+}
+
 int create_device(xpbd_cloth* h, const Mesh& m) {
     const size_t n = h->n, e = h->e, t = h->t;
     TRY(dalloc(h, &h->X, n));
@@ -972,6 +1077,7 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
         TRY(dalloc(h, &h->gs_rc, e));
         TRY(upload(h, h->gs_ij, gij.data(), e));
         TRY(upload(h, h->gs_rc, grc.data(), e));
+        h->gs_ij_host = std::move(gij);
     }
     {
         std::vector<uint32_t> ord(e);
@@ -1047,6 +1153,24 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
         TRY(upload(h, h->cpfirst, pf.data(), pf.size()));
     }
     TRY(dalloc(h, &h->aabb, size_t(h->aabb_slots) * 8));
+    // GS per-tile flags: device arrays for tiles of GS_TILE_MIN particles or
+    // more, and the first tile size whose dependency lists fit
+    if (n < (1u << 28)) {
+        const uint32_t cap = (n + GS_TILE_MIN - 1) / GS_TILE_MIN;
+        h->gt_cap = cap;
+        uint32_t *toff = nullptr, *doff = nullptr, *dep = nullptr;
+        TRY(dalloc(h, &toff, size_t(h->num_batches) * (cap + 1)));
+        TRY(dalloc(h, &doff, size_t(cap) + 1));
+        TRY(dalloc(h, &dep, size_t(cap) * GS_TILE_DEPS_MAX));
+        TRY(dalloc(h, &h->gt.flag, cap));
+        h->gt.toff = toff;
+        h->gt.doff = doff;
+        h->gt.dep = dep;
+        for (uint32_t tile = std::max(GS_TILE_MIN, next_pow2((n + 1023) / 1024)); !h->gt_ok; tile *= 2) {
+            TRY(build_gs_tiles(h, tile));
+            if (tile >= n) break;
+        }
+    }
     {
         std::vector<uint32_t> offs = m.cloth_offsets;
         if (offs.size() < 2 || offs.front() != 0 || offs.back() != n) offs = {0u, uint32_t(n)};
@@ -1134,7 +1258,13 @@ void apply_tuning(xpbd_cloth* h) {
     // persistent Jacobi: every block must be resident at once (grid barrier)
     const uint32_t groups = std::max(1u, (h->nslices + 3) / 4);
     h->pj_blocks = std::max(1u, std::min(persistent_max_blocks(h->nbpk != nullptr, h->cfg.fast_division != 0), groups));
-    h->gs_colour = h->cfg.gs_launch == 2 ? 1u : h->cfg.gs_launch == 1 ? 0u : h->gs_colour_tuned;
+    // coloured GS: 1 barriers, 2 one launch per colour, 3 per-tile flags,
+    // 0 the form the autotuner measured fastest (barriers without autotuning)
+    h->gs_form = h->cfg.gs_launch == 1   ? GS_FORM_BARRIERS
+                 : h->cfg.gs_launch == 2 ? GS_FORM_COLOURS
+                 : h->cfg.gs_launch == 3 ? GS_FORM_TILES
+                                         : h->gs_form_tuned;
+    if (h->gs_form == GS_FORM_TILES && !h->gt_ok) h->gs_form = GS_FORM_BARRIERS;
     // grid tiles: 1 = the form the autotuner found faster, 3 = constraint-
     // shared, 2 = per entry
     h->js_on = h->js_ok && (h->cfg.grid_tiles == 3 || (h->cfg.grid_tiles == 1 && h->js_tuned)) ? 1u : 0u;
@@ -1337,11 +1467,13 @@ int autotune(xpbd_cloth* h) {
         h->gs_blocks = gbest_blocks;
     }
     // Coloured GS: one persistent launch (grid barriers between colours)
-    // against one launch per colour (launch boundaries instead), each timed
-    // as a replayed graph of one substep's GS part, the form frames replay.
-    // Barriers cost more than graph kernel boundaries on large grids, while
-    // small colours leave launches mostly latency.
-    h->gs_colour_tuned = 0;
+    // against one launch per colour (launch boundaries instead) and one
+    // persistent launch ordered by per-tile flags (at each tile size from the
+    // create-time one to 8 times it), each timed as a replayed graph of one
+    // substep's GS part, the form frames replay.  Barriers cost more than
+    // graph kernel boundaries on large grids, while small colours leave
+    // launches mostly latency.
+    h->gs_form_tuned = GS_FORM_BARRIERS;
     if (h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL && h->e) {
         GsArgs g{};
         g.X = h->X;
@@ -1386,7 +1518,30 @@ int autotune(xpbd_cloth* h) {
             uint32_t np = 0;
             enqueue_gs_colours(h, g, false, np, h->stream);
         });
-        h->gs_colour_tuned = tc < tp ? 1u : 0u;
+        float tt = 1e30f;
+        if (h->gt_ok) {
+            const uint32_t base = h->gt.tile;
+            uint32_t best_tile = base;
+            for (uint32_t tile = base; tile <= 8 * base && tile < 2 * h->n; tile *= 2) {
+                TRY(build_gs_tiles(h, tile));
+                if (!h->gt_ok) continue;
+                const float t = time_graph([&]() { launch_gs_tiles(g, h->gt, h->gt_blocks, h->stream); });
+                uint32_t flag = 0;
+                HIPC(h, hipMemcpy(&flag, h->err, sizeof(flag), hipMemcpyDeviceToHost));
+                if (flag) {   // a wait expired: this grid is not resident at once here
+                    const uint32_t zero = 0;
+                    HIPC(h, hipMemcpy(h->err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+                    continue;
+                }
+                if (t < tt) {
+                    tt = t;
+                    best_tile = tile;
+                }
+            }
+            TRY(build_gs_tiles(h, best_tile));
+            if (tt >= 1e30f) h->gt_ok = false;
+        }
+        h->gs_form_tuned = tc < tp && tc <= tt ? GS_FORM_COLOURS : tt < tp ? GS_FORM_TILES : GS_FORM_BARRIERS;
     }
     apply_tuning(h);
     (void)hipEventDestroy(e0);
@@ -1437,6 +1592,18 @@ int verify_coresident(xpbd_cloth* h, bool gs, bool persistent) {
                 launch_gs(g, b, h->gs_block, s);
             },
             h->gs_blocks, "the coloured Gauss-Seidel kernel"));
+    }
+    if (gs && h->gs_form == GS_FORM_TILES) {
+        TRY(check(
+            [&](uint32_t b) {
+                GsArgs g{};
+                g.barrier = h->barrier;
+                g.error = h->err;
+                g.probe = 1;
+                g.p = p;
+                launch_gs_tiles(g, h->gt, b, s);
+            },
+            h->gt_blocks, "the per-tile Gauss-Seidel kernel"));
     }
     if (persistent) {
         TRY(check(
@@ -1637,7 +1804,7 @@ XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, cons
                          void* stream, xpbd_cloth** out) {
     if (!mesh || !cfg || !u || !out) return XPBD_E_INVALID;
     *out = nullptr;
-    if (cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL || cfg->gs_launch > 2) return XPBD_E_INVALID;
+    if (cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL || cfg->gs_launch > 3) return XPBD_E_INVALID;
     if (cfg->jacobi_block && (cfg->jacobi_block % 64 || cfg->jacobi_block > 512)) return XPBD_E_INVALID;
     std::string why;
     if (validate_mesh(mesh->m, why) != XPBD_OK) return XPBD_E_INVALID;
@@ -1664,6 +1831,8 @@ XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, cons
     }
     int r = create_device(h, m);
     if (r != XPBD_OK) return r;
+    if (cfg->gs_launch == 3 && !h->gt_ok)
+        return fail(h, XPBD_E_INVALID, "gs_launch 3: no per-tile flag plan for this mesh");
     r = build_warm_topology(h, m);
     if (r != XPBD_OK) return r;
     r = upload_state(h, h->init_pos4.data(), nullptr);
@@ -1711,9 +1880,11 @@ XPBD_API int xpbd_set_uniforms(xpbd_cloth* h, const xpbd_sim_uniforms* u) {
 }
 
 XPBD_API int xpbd_set_config(xpbd_cloth* h, const xpbd_sim_config* cfg) {
-    if (!h || !cfg || cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL || cfg->gs_launch > 2) return XPBD_E_INVALID;
+    if (!h || !cfg || cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL || cfg->gs_launch > 3) return XPBD_E_INVALID;
     if (cfg->jacobi_block && (cfg->jacobi_block % 64 || cfg->jacobi_block > 512)) return XPBD_E_INVALID;
     const bool retune = cfg->autotune && !h->cfg.autotune;
+    if (cfg->gs_launch == 3 && !h->gt_ok)
+        return fail(h, XPBD_E_INVALID, "gs_launch 3: no per-tile flag plan for this mesh");
     h->cfg = *cfg;
     apply_tuning(h);
     drop_graph(h);
@@ -1963,14 +2134,11 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
                 launch_radix_pass(h->keys, h->ids, h->keys2, h->ids2, h->n, 0, h->hist, s, false, h->coll_digit_bits);
                 break;
             case XPBD_KERNEL_GS_SUBSTEP:
-                if (h->gs_colour) {
-                    uint32_t np = 0;
-                    enqueue_gs_colours(h, g, false, np, s);
-                } else {
-                    launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
-                    launch_gs(g, h->gs_blocks, h->gs_block, s);
-                }
+            {
+                uint32_t np = 0;
+                enqueue_gs(h, g, false, np, s);
                 break;
+            }
             case XPBD_KERNEL_COPY: launch_copy4(h->X, h->B, h->n, s); break;
             case XPBD_KERNEL_JACOBI_PERSISTENT: {
                 launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
@@ -2398,6 +2566,26 @@ XPBD_API int xpbd_debug_raise_device_error(xpbd_cloth* h) {
     const uint32_t one = 1;
     HIPC(h, hipMemcpyAsync(h->err, &one, sizeof(one), hipMemcpyHostToDevice, h->stream));
     HIPC(h, hipStreamSynchronize(h->stream));
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_debug_gs_tiles(xpbd_cloth* h, uint32_t tile, uint32_t blocks, uint32_t* out4) {
+    if (!h || !out4) return XPBD_E_INVALID;
+    if (tile) {
+        const uint32_t old_tile = h->gt.tile, old_blocks = h->gt_blocks;
+        TRY(build_gs_tiles(h, tile, blocks));
+        if (!h->gt_ok) {
+            if (old_tile) TRY(build_gs_tiles(h, old_tile, old_blocks));
+            return fail(h, XPBD_E_INVALID, "no per-tile flag plan at this tile size");
+        }
+        drop_graph(h);
+        apply_tuning(h);
+        TRY(verify_coresident(h, h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL, uses_persistent_jacobi(h)));
+    }
+    out4[0] = h->gt_ok ? h->gt.tile : 0u;
+    out4[1] = h->gt_ok ? h->gt.ntiles : 0u;
+    out4[2] = h->gt_ok ? h->gt_blocks : 0u;
+    out4[3] = h->gt_ok ? h->gt_deps : 0u;
     return XPBD_OK;
 }
 

@@ -236,10 +236,14 @@ typedef struct xpbd_sim_config {
                                              Tiles are chosen at create; all forms
                                              give the same results */
     uint32_t gs_launch;                   /* coloured GS launch form: 0 (default) =
-                                             autotuned (else persistent), 1 = one
-                                             persistent launch per substep with grid
-                                             barriers between colours, 2 = one launch
-                                             per colour and iteration (same results) */
+                                             autotuned (else 1), 1 = one persistent
+                                             launch per substep with grid barriers
+                                             between colours, 2 = one launch per colour
+                                             and iteration, 3 = one persistent launch
+                                             per substep in which tiles of particles
+                                             wait only for their neighbour tiles'
+                                             previous colour (per-tile flags).  Same
+                                             results */
 } xpbd_sim_config;
 
 /* ClothSimUniforms (cloth_compute.rs:182-202). */
@@ -375,6 +379,15 @@ XPBD_API int xpbd_debug_collision_keys(xpbd_cloth* h, float cell_size, uint32_t*
  * persistent launch (coloured Gauss-Seidel, persistent Jacobi) then returns at
  * once, and the next xpbd_sync reports XPBD_E_STATE and clears the word. */
 XPBD_API int xpbd_debug_raise_device_error(xpbd_cloth* h);
+
+/* Test/tuning hook (no reference counterpart): the per-tile-flag Gauss-Seidel
+ * plan (gs_launch = 3).  With tile > 0, re-plans with tiles of `tile`
+ * consecutive particles (at least 64) on a grid of `blocks` blocks (0: as many
+ * as there are tiles, within what can be resident at once); all plans give
+ * the same results.  out4 = {particles per tile, tiles, blocks, most
+ * dependencies of one tile}, zeros when the mesh has no plan.
+ * XPBD_E_INVALID (plan kept) when this tile size has none. */
+XPBD_API int xpbd_debug_gs_tiles(xpbd_cloth* h, uint32_t tile, uint32_t blocks, uint32_t* out4);
 
 /* Layout the handle uses (no reference counterpart; for tests and tools):
  * out4 = {adjacency: 0 plain 12-byte entries, 1 packed 4-byte entries,

@@ -82,14 +82,14 @@ def test_batch_collision_keys_and_sort_bit_exact(O, X):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["jacobi", "jacobi_persistent", "gs", "gs_colours"])
+@pytest.mark.parametrize("mode", ["jacobi", "jacobi_persistent", "gs", "gs_colours", "gs_tiles"])
 def test_batch_substeps_with_collision_bit_exact(O, X, mode):
     note(f"case batch substeps mode={mode}")
     solver = 1 if mode.startswith("gs") else 0
     mesh = X.ClothMeshData.batch([X.SheetDesc.make(**d) for d in descs()])
     f = mesh.fields()
     sim, cfg, u = make(X, mesh, solver=solver, persistent=1 if mode == "jacobi_persistent" else 0,
-                       gs_launch=2 if mode == "gs_colours" else 1 if mode == "gs" else 0,
+                       gs_launch={"gs": 1, "gs_colours": 2, "gs_tiles": 3}.get(mode, 0),
                        thickness=0.012, coll_scale=1.0, jacobi_omega=0.32 if solver == 0 else 1.0)
     pos = overlapped_state(f)
     sim.write_state(pos)

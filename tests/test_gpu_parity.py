@@ -22,9 +22,10 @@ def note(msg):
 
 
 # (solver, gs_launch): Jacobi; coloured GS as one persistent launch per
-# substep; coloured GS as one launch per colour and iteration
-SOLVER_FORMS = [(0, 0), (1, 1), (1, 2)]
-SOLVER_FORM_IDS = ["jacobi", "gs_persistent", "gs_colours"]
+# substep with grid barriers; coloured GS as one launch per colour and
+# iteration; coloured GS as one persistent launch ordered by per-tile flags
+SOLVER_FORMS = [(0, 0), (1, 1), (1, 2), (1, 3)]
+SOLVER_FORM_IDS = ["jacobi", "gs_persistent", "gs_colours", "gs_tiles"]
 
 
 def make(X, mesh, solver=0, S=8, I=10, coll_stride=1, use_graph=0, fused_predict=0, jacobi_block=0,
@@ -333,8 +334,8 @@ def test_graph_replay_equals_eager(X, persistent):
 
 
 def test_gs_launch_form_switched_at_run_time(X):
-    """set_config between frames moves coloured GS from one persistent
-    launch per substep to one launch per colour and back (the frame graph is
+    """set_config between frames moves coloured GS between its three forms
+    (barriers, one launch per colour, per-tile flags; the frame graph is
     rebuilt each time); the trajectory equals a run that never switches, and
     the GS substep is timed in each form."""
     note("case gs launch switch")
@@ -344,7 +345,7 @@ def test_gs_launch_form_switched_at_run_time(X):
                      jacobi_omega=1.0)
     b, _, _ = make(X, X.ClothMeshData.sheet(**kw), solver=1, S=4, I=6, use_graph=1, thickness=th, gs_launch=1,
                    jacobi_omega=1.0)
-    for k, form in enumerate((2, 1, 2)):
+    for k, form in enumerate((2, 3, 1, 3, 2)):
         cfg.gs_launch = form
         a.set_config(cfg)
         a.step_frames(3)

@@ -37,7 +37,8 @@ def _squeezed(X):
 
 
 @pytest.mark.parametrize("graph", [0, 1])
-@pytest.mark.parametrize("solver,gs_launch", [(0, 0), (1, 1), (1, 2)], ids=["jacobi", "gs_persistent", "gs_colours"])
+@pytest.mark.parametrize("solver,gs_launch", [(0, 0), (1, 1), (1, 2), (1, 3)],
+                         ids=["jacobi", "gs_persistent", "gs_colours", "gs_tiles"])
 @pytest.mark.parametrize("stride", [4, 8])
 def test_collision_stride_frames_bit_exact(O, X, stride, solver, gs_launch, graph):
     note(f"case stride={stride} solver={solver} gs_launch={gs_launch} graph={graph}")
@@ -64,14 +65,15 @@ def test_collision_stride_frames_bit_exact(O, X, stride, solver, gs_launch, grap
     sim.close()
 
 
-@pytest.mark.parametrize("mode", ["gauss_seidel", "persistent_jacobi"])
+@pytest.mark.parametrize("mode", ["gauss_seidel", "gauss_seidel_tiles", "persistent_jacobi"])
 def test_error_word_ends_persistent_launches_promptly(O, X, mode):
     note(f"case error word {mode}")
-    solver = 1 if mode == "gauss_seidel" else 0
+    solver = 1 if mode.startswith("gauss_seidel") else 0
     mesh = X.ClothMeshData.sheet(cols=48, rows=40, spacing=0.02, pins=2, jitter=1e-3, seed=13)
     f = mesh.fields()
     cfg = X.ClothSimConfig.default(solve_substeps=4, solve_inner_iterations=8, solver=solver, autotune=0,
-                                   persistent_jacobi=1 if solver == 0 else 0)
+                                   persistent_jacobi=1 if solver == 0 else 0,
+                                   gs_launch=3 if mode == "gauss_seidel_tiles" else 0)
     u = X.ClothSimUniforms.default(solver, 4, thickness=0.016)
     sim = X.ClothComputePlugin(mesh, cfg, u)
     sim.raise_device_error()
