@@ -319,7 +319,7 @@ def config2_rates(X, frames: int = 40, warmup: int = 5):
     return out
 
 
-def gs_forms_1024(X, frames: int = 5, warmup: int = 2):
+def gs_forms_1024(X, frames: int = 5, warmup: int = 2, only=None):
     """Coloured Gauss-Seidel on the 1024x1024 bench sheet (8 substeps x 10
     iterations, omega 1.0, collision every substep), frames replayed from the
     hipGraph, in its autotuned launch form and in each form: substeps/s
@@ -327,6 +327,8 @@ def gs_forms_1024(X, frames: int = 5, warmup: int = 2):
     out = {"workload": "1024x1024 bench sheet, coloured Gauss-Seidel 8 substeps x 10 iterations, "
                        "self-collision every substep"}
     for name, form in (("autotuned", 0), ("persistent", 1), ("per_colour", 2), ("tile_flags", 3)):
+        if only and name not in only:
+            continue
         mesh, thickness, _, _ = build_workload(X, "sheet1024")
         cfg = X.ClothSimConfig.default(solve_substeps=8, solve_inner_iterations=10, solver=X.SOLVER_GAUSS_SEIDEL,
                                        use_graph=1, gs_launch=form)
@@ -389,12 +391,19 @@ def main():
     ap.add_argument("--trace-range", default="timed frames",
                     help="--trace-summary: the ROCTX range to keep (default: the timed Jacobi frames; "
                          "'gauss-seidel per_colour frames' etc. for the 1024x1024 Gauss-Seidel records)")
+    ap.add_argument("--gs-forms", metavar="NAMES",
+                    help="only the 1024x1024 Gauss-Seidel records of these forms (comma-separated: autotuned, "
+                         "persistent, per_colour, tile_flags; --steps frames each), printed as JSON, then exit")
     ap.add_argument("--counter-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the bytes-per-dispatch table from FETCH_SIZE and WRITE_SIZE counter passes "
                          "(-f csv) of `bench.py --kernels-only`, then the Jacobi iteration's JSON record, and exit")
     args = ap.parse_args()
     if args.trace_summary:
         print(trace_summary(args.trace_summary, range_name=args.trace_range, frames=args.steps, by_grid=args.by_grid))
+        return
+    if args.gs_forms:
+        import bevy_xpbd_amd as X
+        print(json.dumps(gs_forms_1024(X, frames=args.steps, only=args.gs_forms.split(","))), flush=True)
         return
     if args.counter_summary:
         table, rec = counter_summary(*args.counter_summary)
