@@ -188,6 +188,7 @@ def lib():
             "xpbd_debug_collision_keys": (i32, [vp, f32, vp, vp, vp]),
             "xpbd_debug_raise_device_error": (i32, [vp]),
             "xpbd_debug_gs_tiles": (i32, [vp, u32, u32, vp]),
+            "xpbd_debug_gs_grid": (i32, [vp, u32, vp]),
             "xpbd_get_layout": (i32, [vp, vp]),
             "xpbd_selftest_division": (i32, [i32, i32, u64, u64, vp]),
             "xpbd_selftest_sqrt": (i32, [i32, i32, vp]),
@@ -497,9 +498,18 @@ class ClothComputePlugin:
     def gs_tiles(self, tile: int = 0, blocks: int = 0):
         """Test/tuning hook: the per-tile-flag GS plan (gs_launch = 3), re-planned
         with `tile` particles per tile on `blocks` blocks when tile > 0.  Returns
-        (particles per tile, tiles, blocks, most dependencies of one tile)."""
+        (particles per tile, tiles, blocks, most dependencies of one tile); the
+        flag form then runs these index-range tiles."""
         out = (C.c_uint32 * 4)()
         _check(lib().xpbd_debug_gs_tiles(self._h, int(tile), int(blocks), out), "xpbd_debug_gs_tiles", self._h)
+        return tuple(int(v) for v in out)
+
+    def gs_grid(self, rows: int = 0):
+        """Test/tuning hook: the flag form's LDS grid tiles (one grid cloth,
+        64 columns x rows), re-planned at `rows` rows and switched on when
+        rows > 0.  Returns (rows per tile, tiles, threads per block, in use)."""
+        out = (C.c_uint32 * 4)()
+        _check(lib().xpbd_debug_gs_grid(self._h, int(rows), out), "xpbd_debug_gs_grid", self._h)
         return tuple(int(v) for v in out)
 
     def residual(self) -> float:

@@ -2173,23 +2173,36 @@ __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
 // (its flag >= s).  One wavefront polls, a lane per dependency; the wait is
 // bounded like gs_wait (on expiry the error word is set and this and every
 // later wait of the launch return at once).
+// Called by the lanes of wavefront 0: poll until each lane's flag (nullptr:
+// none) is >= s.
+__device__ __forceinline__ void poll_flags(uint32_t* f, uint32_t s, uint32_t* err) {
+    if (!__builtin_amdgcn_readfirstlane(gs_load(err))) {
+        for (uint32_t spins = 1;; ++spins) {
+            if (__all(f == nullptr || gs_load(f) >= s)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if ((spins & 1023u) == 0 && __builtin_amdgcn_readfirstlane(gs_load(err))) break;
+            if (spins > (1u << 22)) {   // seconds at most: report instead of hanging
+                if (threadIdx.x == 0) atomicExch(err, 1u);
+                break;
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ void gs_tile_wait(const GsTiles& t, uint32_t T, uint32_t s, uint32_t* err) {
     if (threadIdx.x < 64) {
         const uint32_t d0 = t.doff[T], nd = t.doff[T + 1] - d0;
         uint32_t* f = threadIdx.x < nd ? t.flag + t.dep[d0 + threadIdx.x] : nullptr;
-        if (nd && !__builtin_amdgcn_readfirstlane(gs_load(err))) {
-            for (uint32_t spins = 1;; ++spins) {
-                if (__all(f == nullptr || gs_load(f) >= s)) break;
-                __builtin_amdgcn_s_sleep(1);
-                if ((spins & 1023u) == 0 && __builtin_amdgcn_readfirstlane(gs_load(err))) break;
-                if (spins > (1u << 22)) {   // seconds at most: report instead of hanging
-                    if (threadIdx.x == 0) atomicExch(err, 1u);
-                    break;
-                }
-            }
-        }
+        if (nd) poll_flags(f, s, err);
     }
     __syncthreads();
+}
+
+// The end of a flagged step: every wave's stores drained, then the flag.
+__device__ __forceinline__ void flag_step(uint32_t* flag, uint32_t v) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(flag, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One GS substep (xpbd_cpu.rs:312-391 inside the substep of :240-420) with
@@ -2268,9 +2281,152 @@ __global__ __launch_bounds__(256) void k_gs_tiles(GsArgs a, GsTiles t) {
                 }
                 continue;   // nothing waits for the final step
             }
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            __syncthreads();
-            if (threadIdx.x == 0) __hip_atomic_store(t.flag + T, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            flag_step(t.flag + T, s + 1);
+        }
+    }
+    if (!a.final_write) block_aabb_store(lo, hi, a.aabb, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Coloured GS on grid tiles resident in LDS, ordered by neighbour flags.  One
+// grid cloth (row-major, cols a multiple of 64) is cut into tiles of 64
+// columns x th rows, one block each, all resident at once.  A block keeps its
+// tile and a 2-particle ring around it (the window, (th + 4) x 68 particles)
+// in LDS for the whole substep.  Its constraints (those whose lower endpoint
+// is in the tile) reach at most 2 rows down and 2 columns either way, so they
+// stay inside the window; a particle more than 2 from the tile's edge is
+// touched by no other tile's constraints and never leaves LDS between the
+// predict and the final write.  The shared particles (the ring inside and
+// outside the tile's edge) go through the iterate at device scope: each step
+// re-reads them after the 8 neighbour tiles have finished the previous step,
+// and writes back the ones its constraints change.  A neighbour may change a
+// shared particle during the same step; a colour's constraints share no
+// vertex, so this tile neither uses nor writes that particle in that step and
+// reads it again in the next one.
+constexpr uint32_t GSG_W = 64;              // tile columns
+constexpr uint32_t GSG_WW = GSG_W + 4;      // window columns
+constexpr uint32_t GSG_SHARED = 0x8000u;    // window position flag: written back to the iterate
+
+__device__ __forceinline__ bool gsg_shared(uint32_t lr, uint32_t lc, uint32_t th) {
+    return lr < 4 || lr >= th || lc < 4 || lc >= GSG_W;
+}
+
+// Window accessor for gs_apply: positions are window indices with
+// GSG_SHARED set where the particle is written back.
+struct TileWindow {
+    float4* win;
+    DeviceW w;
+    int gbase;   // global index of window position 0: (r0 - 2) * cols + c0 - 2
+    uint32_t cols;
+    __device__ __forceinline__ float4 ld(uint32_t x) const { return win[x & (GSG_SHARED - 1u)]; }
+    __device__ __forceinline__ void st(uint32_t x, float4 v) const {
+        const uint32_t k = x & (GSG_SHARED - 1u);
+        win[k] = v;
+        if (x & GSG_SHARED) w.st(uint32_t(gbase + int(k / GSG_WW) * int(cols) + int(k % GSG_WW)), v);
+    }
+};
+
+// Re-read the window's shared particles that a constraint of this tile or of
+// a neighbour can touch (window rows 2..3 and th..th+3 whole, columns 0..3 and
+// 64..67 of rows 4..th-1; rows 0..1 lie above every endpoint of this tile).
+__device__ __forceinline__ void gsg_ring_load(const TileWindow& m, int r0, int c0, uint32_t th, uint32_t rows) {
+    const uint32_t nfull = 6 * GSG_WW, nring = nfull + (th - 4) * 8;
+    for (uint32_t k = threadIdx.x; k < nring; k += blockDim.x) {
+        uint32_t lr, lc;
+        if (k < nfull) {
+            const uint32_t q = k / GSG_WW;
+            lr = q < 2 ? 2 + q : th + q - 2;
+            lc = k % GSG_WW;
+        } else {
+            const uint32_t q = k - nfull, c8 = q % 8;
+            lr = 4 + q / 8;
+            lc = c8 < 4 ? c8 : GSG_W + c8 - 4;
+        }
+        const int r = r0 - 2 + int(lr), c = c0 - 2 + int(lc);
+        if (r >= 0 && r < int(rows) && c >= 0 && c < int(m.cols))
+            m.win[lr * GSG_WW + lc] = m.w.ld(uint32_t(r) * m.cols + uint32_t(c));
+    }
+}
+
+__device__ __forceinline__ void gsg_wait(const GsGrid& t, uint32_t tx, uint32_t ty, uint32_t s, uint32_t* err) {
+    if (threadIdx.x < 64) {
+        const int l = int(threadIdx.x), nx = int(tx) + l % 3 - 1, ny = int(ty) + l / 3 - 1;
+        const bool nb = l < 9 && l != 4 && nx >= 0 && ny >= 0 && nx < int(t.tx) && ny < int(t.ty);
+        poll_flags(nb ? t.flag + ny * int(t.tx) + nx : nullptr, s, err);
+    }
+    __syncthreads();
+}
+
+// One GS substep on LDS-resident grid tiles (block = tile, grid = all tiles,
+// co-resident): the predict of the tile (step 0), I x colours constraint
+// steps, then the final write of the tile's particles to the iterate and the
+// state update (or the bounds).  Same results as k_gs_substep.
+__global__ __launch_bounds__(1024) void k_gs_grid(GsArgs a, GsGrid t) {
+    if (a.probe) {   // uniform: the co-residency check of this kernel's grid
+        GridSync gsync;
+        gsync.init(a.barrier, a.error);
+        return;
+    }
+    extern __shared__ float4 win[];
+    const DevParams& p = a.p;
+    const uint32_t T = blockIdx.x, tx = T % t.tx, ty = T / t.tx, th = t.th;
+    const int r0 = int(ty * th), c0 = int(tx * GSG_W);
+    const uint32_t ntr = min(th, t.rows - ty * th);
+    const TileWindow m{win, DeviceW(a.W, p.n), (r0 - 2) * int(t.cols) + c0 - 2, t.cols};
+    for (uint32_t k = threadIdx.x; k < ntr * GSG_W; k += blockDim.x) {
+        const uint32_t lr = k / GSG_W + 2, lc = k % GSG_W + 2;
+        const uint32_t i = uint32_t(r0 + int(lr) - 2) * t.cols + uint32_t(c0 + int(lc) - 2);
+        const float4 v = predict(a.X[i], a.V[i], i, p);
+        win[lr * GSG_WW + lc] = v;
+        if (gsg_shared(lr, lc, th)) m.w.st(i, v);
+    }
+    flag_step(t.flag + T, 1);
+    const uint32_t last = 1 + a.iterations * t.colours;
+    const uint32_t* co = t.coff + size_t(T) * t.colours * 2;
+    for (uint32_t s = 1; s < last; ++s) {
+        const uint32_t c = (s - 1) % t.colours;
+        const uint32_t e0 = co[2 * c], em = co[2 * c + 1], e1 = co[2 * c + 2];
+        // the colour's constraints inside the tile's interior touch no
+        // particle of another tile: they run before the wait
+        for (uint32_t q = e0 + threadIdx.x; q < em; q += blockDim.x) {
+            const uint32_t w2 = t.con[q];
+            const uint2 ij = make_uint2(w2 & 0xFFFFu, w2 >> 16);
+            gs_apply(a, ij, t.crc[q], m.ld(ij.x), m.ld(ij.y), m);
+        }
+        const uint32_t e = em + threadIdx.x;
+        uint32_t cw = 0;
+        float2 rc = make_float2(0.f, 0.f);
+        if (e < e1) {
+            cw = t.con[e];
+            rc = t.crc[e];
+        }
+        gsg_wait(t, tx, ty, s, a.error);
+        gsg_ring_load(m, r0, c0, th, t.rows);
+        __syncthreads();
+        if (e < e1) {
+            const uint2 ij = make_uint2(cw & 0xFFFFu, cw >> 16);
+            gs_apply(a, ij, rc, m.ld(ij.x), m.ld(ij.y), m);
+        }
+        for (uint32_t q = e + blockDim.x; q < e1; q += blockDim.x) {
+            const uint32_t w2 = t.con[q];
+            const uint2 ij = make_uint2(w2 & 0xFFFFu, w2 >> 16);
+            gs_apply(a, ij, t.crc[q], m.ld(ij.x), m.ld(ij.y), m);
+        }
+        flag_step(t.flag + T, s + 1);
+    }
+    gsg_wait(t, tx, ty, last, a.error);
+    gsg_ring_load(m, r0, c0, th, t.rows);
+    __syncthreads();
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t k = threadIdx.x; k < ntr * GSG_W; k += blockDim.x) {
+        const uint32_t lr = k / GSG_W + 2, lc = k % GSG_W + 2;
+        const uint32_t i = uint32_t(r0 + int(lr) - 2) * t.cols + uint32_t(c0 + int(lc) - 2);
+        const float4 v = win[lr * GSG_WW + lc];
+        a.W[i] = v;
+        if (a.final_write) {
+            if (v.w > 0.f) finish_particle(v, a.X[i], i, a.ground, p, a.xout, a.vout);
+        } else {
+            aabb_fold(true, v.x, v.y, v.z, lo, hi);
         }
     }
     if (!a.final_write) block_aabb_store(lo, hi, a.aabb, blockIdx.x);
@@ -2830,6 +2986,30 @@ uint32_t gs_tiles_max_blocks() {
 void launch_gs_tiles(const GsArgs& a, const GsTiles& t, uint32_t blocks, hipStream_t s) {
     launch_zero_u32(t.flag, t.ntiles, s);
     hipLaunchKernelGGL(k_gs_tiles, dim3(blocks), dim3(256), 0, s, a, t);
+}
+
+uint32_t gs_grid_lds_bytes(uint32_t th) { return (th + 4) * GSG_WW * 16u; }
+
+uint32_t gs_grid_block(uint32_t th) { return th >= 64 ? 1024u : th >= 32 ? 512u : 256u; }
+
+hipError_t gs_grid_prepare(uint32_t th) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(k_gs_grid), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(gs_grid_lds_bytes(th)));
+}
+
+uint32_t gs_grid_max_blocks(uint32_t th) {
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_grid, int(gs_grid_block(th)),
+                                                     gs_grid_lds_bytes(th)) != hipSuccess)
+        return 0;
+    return uint32_t(std::max(0, per_cu) * std::max(0, cus));
+}
+
+void launch_gs_grid(const GsArgs& a, const GsGrid& t, hipStream_t s) {
+    launch_zero_u32(t.flag, t.tx * t.ty, s);
+    hipLaunchKernelGGL(k_gs_grid, dim3(t.tx * t.ty), dim3(gs_grid_block(t.th)), gs_grid_lds_bytes(t.th), s, a, t);
 }
 
 void launch_publish(const float4* X, uint32_t n, const uint32_t* tris, const uint32_t* vt_off, const uint32_t* vt_tri,

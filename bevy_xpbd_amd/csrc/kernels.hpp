@@ -227,6 +227,24 @@ struct GsTiles {
 // A tile's dependencies are checked by the lanes of one wavefront.
 constexpr uint32_t GS_TILE_DEPS_MAX = 64;
 
+// Coloured GS on LDS-resident grid tiles (k_gs_grid): one grid cloth, tiles
+// of 64 columns x th rows (th >= 8), one block each.  A constraint belongs to
+// the tile of its lower endpoint; its endpoints are stored as positions in
+// the tile's window ((th + 4) x 68 particles: the tile and 2 around it), with
+// bit 15 set on a position that other tiles can touch.
+struct GsGrid {
+    // tiles x colours x 2 + 1 offsets: each tile's constraints of each
+    // colour, first those touching only its interior, then the others
+    const uint32_t* coff;
+    const uint32_t* con;    // window positions (i | j << 16) in that order
+    const float2* crc;      // (rest, compliance) in that order
+    uint32_t* flag;         // tiles: steps finished (zeroed by launch_gs_grid)
+    uint32_t tx, ty;        // tiles per row of tiles, rows of tiles
+    uint32_t cols, rows;    // the cloth's grid
+    uint32_t th;            // tile rows
+    uint32_t colours;
+};
+
 // ---- launchers (kernels.hip) ----
 uint32_t jacobi_blocks(uint32_t nslices, uint32_t block);
 uint32_t launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s);
@@ -280,6 +298,15 @@ uint32_t gs_max_blocks(uint32_t block);
 // bytes.  With a.final_write == 0 one bounds partial per block goes to a.aabb.
 void launch_gs_tiles(const GsArgs& a, const GsTiles& t, uint32_t blocks, hipStream_t s);
 uint32_t gs_tiles_max_blocks();
+// The grid-tile GS substep (k_gs_grid): tiles x blocks of gs_grid_block(th)
+// threads and gs_grid_lds_bytes(th) of LDS, all resident at once (zeroes
+// t.flag first).  gs_grid_prepare raises the kernel's LDS limit (once per
+// tile height, before launching or capturing).
+uint32_t gs_grid_lds_bytes(uint32_t th);
+uint32_t gs_grid_block(uint32_t th);
+hipError_t gs_grid_prepare(uint32_t th);
+uint32_t gs_grid_max_blocks(uint32_t th);
+void launch_gs_grid(const GsArgs& a, const GsGrid& t, hipStream_t s);
 void launch_copy4(const float4* a, float4* b, uint32_t n, hipStream_t s);
 // per-slice tame bytes of P (1: every coordinate 0 or of magnitude in [2^-50, 2^20])
 void launch_tame_scan(const float4* P, uint32_t n, uint8_t* slice_tame, hipStream_t s);

@@ -57,6 +57,8 @@ constexpr float DEFAULT_LINEAR_AIR_DRAG_PER_SEC = 1.25f;      // cloth_compute.r
 constexpr uint32_t GS_FORM_BARRIERS = 0, GS_FORM_COLOURS = 1, GS_FORM_TILES = 2;
 // smallest GS flag tile (particles); the device tile arrays are sized for it
 constexpr uint32_t GS_TILE_MIN = 64;
+// fewest rows of an LDS grid tile for GS; its arrays are sized for it
+constexpr uint32_t GSG_MIN_ROWS = 8;
 
 uint32_t next_pow2(uint32_t v) {
     uint32_t p = 1;
@@ -143,6 +145,13 @@ struct xpbd_cloth {
     bool gt_ok = false;
     uint32_t gt_blocks = 0, gt_cap = 0, gt_deps = 0;
     std::vector<uint2> gs_ij_host;
+    // ... and on LDS-resident grid tiles (one grid cloth): the plan, whether
+    // it is usable (gg_ok) and used by the flag form (gg_on), the cloth's
+    // grid, the capacity of coff / flag (tiles of GSG_MIN_ROWS rows)
+    GsGrid gg{};
+    bool gg_ok = false, gg_on = false;
+    uint32_t gg_cols = 0, gg_rows = 0, gg_cap = 0;
+    std::vector<float2> gs_rc_host;
     uint32_t pj_blocks = 0;        // persistent Jacobi grid (co-resident blocks)
 
     // fast correctly rounded division / sqrt in the Jacobi gather: the static
@@ -352,6 +361,9 @@ void enqueue_gs_colours(xpbd_cloth* h, const GsArgs& g, bool collision, uint32_t
 void enqueue_gs(xpbd_cloth* h, const GsArgs& g, bool collision, uint32_t& nparts, hipStream_t s) {
     if (h->gs_form == GS_FORM_COLOURS) {
         enqueue_gs_colours(h, g, collision, nparts, s);
+    } else if (h->gs_form == GS_FORM_TILES && h->gg_on) {
+        launch_gs_grid(g, h->gg, s);
+        nparts = h->gg.tx * h->gg.ty;
     } else if (h->gs_form == GS_FORM_TILES) {
         launch_gs_tiles(g, h->gt, h->gt_blocks, s);
         nparts = h->gt_blocks;
@@ -951,6 +963,91 @@ int build_gs_tiles(xpbd_cloth* h, uint32_t tile, uint32_t blocks = 0) {
     return XPBD_OK;
 }
 
+// LDS grid tiles for k_gs_grid (GsGrid): tiles of 64 columns x th rows of one
+// grid cloth; each constraint goes to the tile of its lower endpoint, grouped
+// by (tile, colour, touches a shared position), each group in the
+// colour-sorted order, with its endpoints as window positions ((th + 4) x 68,
+// the tile and 2 particles around it; bit 15: the position is within 2 of the
+// tile's edge or outside it, where other tiles' constraints reach).  False
+// when an endpoint falls outside its tile's window.
+// Group of a grid-tile constraint: (tile, colour, 1 if an endpoint is shared).
+size_t grid_group(uint32_t T, uint32_t c, uint32_t nc, uint32_t pos) {
+    return (size_t(T) * nc + c) * 2 + ((pos & 0x80008000u) ? 1u : 0u);
+}
+
+bool plan_gs_grid(const std::vector<uint2>& ij, const std::vector<float2>& rc, const std::vector<uint32_t>& off,
+                  uint32_t cols, uint32_t rows, uint32_t th, std::vector<uint32_t>& coff, std::vector<uint32_t>& con,
+                  std::vector<float2>& crc) {
+    if (cols < 64 || cols % 64 || rows == 0 || th < GSG_MIN_ROWS || off.size() < 2) return false;
+    const uint32_t tx = cols / 64, ty = (rows + th - 1) / th, nt = tx * ty, nc = uint32_t(off.size() - 1);
+    const size_t e = ij.size();
+    std::vector<uint32_t> tile(e), pos(e);
+    coff.assign(size_t(nt) * nc * 2 + 1, 0u);
+    for (uint32_t c = 0; c < nc; ++c)
+        for (size_t k = off[c]; k < off[c + 1]; ++k) {
+            const uint32_t lo = std::min(ij[k].x, ij[k].y);
+            if (lo >= cols * rows || std::max(ij[k].x, ij[k].y) >= cols * rows) return false;
+            const uint32_t T = (lo / cols / th) * tx + (lo % cols) / 64;
+            const int r0 = int((T / tx) * th), c0 = int((T % tx) * 64);
+            uint32_t w[2];
+            const uint32_t q[2] = {ij[k].x, ij[k].y};
+            for (int s = 0; s < 2; ++s) {
+                const int lr = int(q[s] / cols) - r0 + 2, lc = int(q[s] % cols) - c0 + 2;
+                if (lr < 0 || lr >= int(th) + 4 || lc < 0 || lc >= 68) return false;
+                const bool shared = lr < 4 || lr >= int(th) || lc < 4 || lc >= 64;
+                w[s] = uint32_t(lr * 68 + lc) | (shared ? 0x8000u : 0u);
+            }
+            tile[k] = T;
+            pos[k] = w[0] | (w[1] << 16);
+            ++coff[grid_group(T, c, nc, pos[k])];
+        }
+    uint32_t run = 0;   // counts -> starts; the last entry: the end
+    for (size_t g = 0; g < coff.size(); ++g) {
+        const uint32_t cnt = coff[g];
+        coff[g] = run;
+        run += cnt;
+    }
+    con.assign(e, 0u);
+    crc.assign(e, make_float2(0.f, 0.f));
+    std::vector<uint32_t> cur(coff);
+    for (uint32_t c = 0; c < nc; ++c)
+        for (size_t k = off[c]; k < off[c + 1]; ++k) {
+            const uint32_t d = cur[grid_group(tile[k], c, nc, pos[k])]++;
+            con[d] = pos[k];
+            crc[d] = rc[k];
+        }
+    return true;
+}
+
+// Plan LDS grid tiles of th rows and upload them; gg_ok says whether they
+// are usable (a plan, the LDS, and every tile resident at once).
+int build_gs_grid(xpbd_cloth* h, uint32_t th) {
+    h->gg_ok = false;
+    if (!h->gg_cap || th < GSG_MIN_ROWS) return XPBD_OK;
+    int dev = 0, lds = 0;
+    HIPC(h, hipGetDevice(&dev));
+    HIPC(h, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (gs_grid_lds_bytes(th) > uint32_t(lds)) return XPBD_OK;
+    std::vector<uint32_t> coff, con;
+    std::vector<float2> crc;
+    if (!plan_gs_grid(h->gs_ij_host, h->gs_rc_host, h->batch_offsets, h->gg_cols, h->gg_rows, th, coff, con, crc))
+        return XPBD_OK;
+    HIPC(h, gs_grid_prepare(th));
+    const uint32_t nt = (h->gg_cols / 64) * ((h->gg_rows + th - 1) / th);
+    if (nt > gs_grid_max_blocks(th) || nt > h->aabb_slots) return XPBD_OK;
+    TRY(upload(h, const_cast<uint32_t*>(h->gg.coff), coff.data(), coff.size()));
+    TRY(upload(h, const_cast<uint32_t*>(h->gg.con), con.data(), con.size()));
+    TRY(upload(h, const_cast<float2*>(h->gg.crc), crc.data(), crc.size()));
+    h->gg.tx = h->gg_cols / 64;
+    h->gg.ty = (h->gg_rows + th - 1) / th;
+    h->gg.cols = h->gg_cols;
+    h->gg.rows = h->gg_rows;
+    h->gg.th = th;
+    h->gg.colours = h->num_batches;
+    h->gg_ok = true;
+    return XPBD_OK;
+}
+
 int create_device(xpbd_cloth* h, const Mesh& m) {
     const size_t n = h->n, e = h->e, t = h->t;
     TRY(dalloc(h, &h->X, n));
@@ -1078,6 +1175,7 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
         TRY(upload(h, h->gs_ij, gij.data(), e));
         TRY(upload(h, h->gs_rc, grc.data(), e));
         h->gs_ij_host = std::move(gij);
+        h->gs_rc_host = std::move(grc);
     }
     {
         std::vector<uint32_t> ord(e);
@@ -1170,6 +1268,26 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
             TRY(build_gs_tiles(h, tile));
             if (tile >= n) break;
         }
+    }
+    // GS on LDS grid tiles: one grid cloth of 64k columns; 64-row tiles
+    // where the LDS allows, else 32
+    if (n < (1u << 28) && h->num_batches && m.cloth_offsets.size() == 2 && m.cloth_cols.size() == 1 &&
+        m.cloth_cols[0] >= 64 && m.cloth_cols[0] % 64 == 0 && n % m.cloth_cols[0] == 0) {
+        h->gg_cols = m.cloth_cols[0];
+        h->gg_rows = n / h->gg_cols;
+        h->gg_cap = (h->gg_cols / 64) * ((h->gg_rows + GSG_MIN_ROWS - 1) / GSG_MIN_ROWS);
+        uint32_t *coff = nullptr, *con = nullptr;
+        float2* crc = nullptr;
+        TRY(dalloc(h, &coff, size_t(h->gg_cap) * h->num_batches * 2 + 1));
+        TRY(dalloc(h, &con, e));
+        TRY(dalloc(h, &crc, e));
+        TRY(dalloc(h, &h->gg.flag, h->gg_cap));
+        h->gg.coff = coff;
+        h->gg.con = con;
+        h->gg.crc = crc;
+        TRY(build_gs_grid(h, 64));
+        if (!h->gg_ok) TRY(build_gs_grid(h, 32));
+        h->gg_on = h->gg_ok;
     }
     {
         std::vector<uint32_t> offs = m.cloth_offsets;
@@ -1264,7 +1382,7 @@ void apply_tuning(xpbd_cloth* h) {
                  : h->cfg.gs_launch == 2 ? GS_FORM_COLOURS
                  : h->cfg.gs_launch == 3 ? GS_FORM_TILES
                                          : h->gs_form_tuned;
-    if (h->gs_form == GS_FORM_TILES && !h->gt_ok) h->gs_form = GS_FORM_BARRIERS;
+    if (h->gs_form == GS_FORM_TILES && !h->gt_ok && !h->gg_on) h->gs_form = GS_FORM_BARRIERS;
     // grid tiles: 1 = the form the autotuner found faster, 3 = constraint-
     // shared, 2 = per entry
     h->js_on = h->js_ok && (h->cfg.grid_tiles == 3 || (h->cfg.grid_tiles == 1 && h->js_tuned)) ? 1u : 0u;
@@ -1541,6 +1659,31 @@ int autotune(xpbd_cloth* h) {
             TRY(build_gs_tiles(h, best_tile));
             if (tt >= 1e30f) h->gt_ok = false;
         }
+        // the flag form on LDS grid tiles, at 64, 32 and 16 rows per tile
+        float tg = 1e30f;
+        if (h->gg_cap) {
+            uint32_t best_th = 0;
+            for (uint32_t th : {64u, 32u, 16u}) {
+                TRY(build_gs_grid(h, th));
+                if (!h->gg_ok) continue;
+                const float t = time_graph([&]() { launch_gs_grid(g, h->gg, h->stream); });
+                uint32_t flag = 0;
+                HIPC(h, hipMemcpy(&flag, h->err, sizeof(flag), hipMemcpyDeviceToHost));
+                if (flag) {   // a wait expired: these tiles are not resident at once here
+                    const uint32_t zero = 0;
+                    HIPC(h, hipMemcpy(h->err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+                    continue;
+                }
+                if (t < tg) {
+                    tg = t;
+                    best_th = th;
+                }
+            }
+            if (best_th) TRY(build_gs_grid(h, best_th));
+            else h->gg_ok = false;
+            h->gg_on = h->gg_ok && (tg < tt || !h->gt_ok);
+            tt = std::min(tt, tg);
+        }
         h->gs_form_tuned = tc < tp && tc <= tt ? GS_FORM_COLOURS : tt < tp ? GS_FORM_TILES : GS_FORM_BARRIERS;
     }
     apply_tuning(h);
@@ -1593,7 +1736,29 @@ int verify_coresident(xpbd_cloth* h, bool gs, bool persistent) {
             },
             h->gs_blocks, "the coloured Gauss-Seidel kernel"));
     }
-    if (gs && h->gs_form == GS_FORM_TILES) {
+    if (gs && h->gs_form == GS_FORM_TILES && h->gg_on) {
+        // every grid tile needs its own resident block: probed once; where
+        // the probe's wait expires the index-range tiles run instead
+        launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
+        GsArgs g{};
+        g.barrier = h->barrier;
+        g.error = h->err;
+        g.probe = 1;
+        g.p = p;
+        launch_gs_grid(g, h->gg, s);
+        HIPC(h, hipGetLastError());
+        uint32_t flag = 0;
+        HIPC(h, hipMemcpyAsync(&flag, h->err, sizeof(flag), hipMemcpyDeviceToHost, s));
+        HIPC(h, hipStreamSynchronize(s));
+        if (flag) {
+            const uint32_t zero = 0;
+            HIPC(h, hipMemcpy(h->err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+            h->gg_on = false;
+            h->gg_ok = false;
+            drop_graph(h);
+        }
+    }
+    if (gs && h->gs_form == GS_FORM_TILES && !h->gg_on) {
         TRY(check(
             [&](uint32_t b) {
                 GsArgs g{};
@@ -1831,7 +1996,7 @@ XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, cons
     }
     int r = create_device(h, m);
     if (r != XPBD_OK) return r;
-    if (cfg->gs_launch == 3 && !h->gt_ok)
+    if (cfg->gs_launch == 3 && !h->gt_ok && !h->gg_on)
         return fail(h, XPBD_E_INVALID, "gs_launch 3: no per-tile flag plan for this mesh");
     r = build_warm_topology(h, m);
     if (r != XPBD_OK) return r;
@@ -1883,7 +2048,7 @@ XPBD_API int xpbd_set_config(xpbd_cloth* h, const xpbd_sim_config* cfg) {
     if (!h || !cfg || cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL || cfg->gs_launch > 3) return XPBD_E_INVALID;
     if (cfg->jacobi_block && (cfg->jacobi_block % 64 || cfg->jacobi_block > 512)) return XPBD_E_INVALID;
     const bool retune = cfg->autotune && !h->cfg.autotune;
-    if (cfg->gs_launch == 3 && !h->gt_ok)
+    if (cfg->gs_launch == 3 && !h->gt_ok && !h->gg_on)
         return fail(h, XPBD_E_INVALID, "gs_launch 3: no per-tile flag plan for this mesh");
     h->cfg = *cfg;
     apply_tuning(h);
@@ -2578,6 +2743,7 @@ XPBD_API int xpbd_debug_gs_tiles(xpbd_cloth* h, uint32_t tile, uint32_t blocks, 
             if (old_tile) TRY(build_gs_tiles(h, old_tile, old_blocks));
             return fail(h, XPBD_E_INVALID, "no per-tile flag plan at this tile size");
         }
+        h->gg_on = false;   // the flag form runs these index-range tiles
         drop_graph(h);
         apply_tuning(h);
         TRY(verify_coresident(h, h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL, uses_persistent_jacobi(h)));
@@ -2586,6 +2752,27 @@ XPBD_API int xpbd_debug_gs_tiles(xpbd_cloth* h, uint32_t tile, uint32_t blocks, 
     out4[1] = h->gt_ok ? h->gt.ntiles : 0u;
     out4[2] = h->gt_ok ? h->gt_blocks : 0u;
     out4[3] = h->gt_ok ? h->gt_deps : 0u;
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_debug_gs_grid(xpbd_cloth* h, uint32_t rows, uint32_t* out4) {
+    if (!h || !out4) return XPBD_E_INVALID;
+    if (rows) {
+        const uint32_t old = h->gg_ok ? h->gg.th : 0u;
+        TRY(build_gs_grid(h, rows));
+        if (!h->gg_ok) {
+            if (old) TRY(build_gs_grid(h, old));
+            return fail(h, XPBD_E_INVALID, "no LDS grid-tile plan at this tile height");
+        }
+        h->gg_on = true;
+        drop_graph(h);
+        apply_tuning(h);
+        TRY(verify_coresident(h, h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL, uses_persistent_jacobi(h)));
+    }
+    out4[0] = h->gg_ok ? h->gg.th : 0u;
+    out4[1] = h->gg_ok ? h->gg.tx * h->gg.ty : 0u;
+    out4[2] = h->gg_ok ? gs_grid_block(h->gg.th) : 0u;
+    out4[3] = h->gg_on ? 1u : 0u;
     return XPBD_OK;
 }
 

@@ -386,8 +386,19 @@ XPBD_API int xpbd_debug_raise_device_error(xpbd_cloth* h);
  * as there are tiles, within what can be resident at once); all plans give
  * the same results.  out4 = {particles per tile, tiles, blocks, most
  * dependencies of one tile}, zeros when the mesh has no plan.
- * XPBD_E_INVALID (plan kept) when this tile size has none. */
+ * XPBD_E_INVALID (plan kept) when this tile size has none.  Makes the flag
+ * form run index-range tiles. */
 XPBD_API int xpbd_debug_gs_tiles(xpbd_cloth* h, uint32_t tile, uint32_t blocks, uint32_t* out4);
+
+/* Test/tuning hook (no reference counterpart): the flag form on LDS grid
+ * tiles (gs_launch = 3 on one grid cloth whose column count is a multiple of
+ * 64: tiles of 64 columns x `rows` rows, one block each, kept in LDS for the
+ * whole substep).  With rows > 0, re-plans at that height (at least 8) and
+ * makes the flag form use these tiles (xpbd_debug_gs_tiles switches it back
+ * to index-range tiles).  out4 = {rows per tile, tiles, threads per block,
+ * 1 if the flag form uses them}, zeros when there is no plan.
+ * XPBD_E_INVALID (plan kept) when this height has none. */
+XPBD_API int xpbd_debug_gs_grid(xpbd_cloth* h, uint32_t rows, uint32_t* out4);
 
 /* Layout the handle uses (no reference counterpart; for tests and tools):
  * out4 = {adjacency: 0 plain 12-byte entries, 1 packed 4-byte entries,

@@ -7,6 +7,11 @@ for the tiles whose particles its constraints can share.  Every particle still
 sees the colours in order, so the bar is that of the other GS forms: bit-exact
 f32 positions and velocities, substep by substep, whatever the tile size and
 however many tiles one block runs in turn.
+
+On one grid cloth whose column count is a multiple of 64 the flag form runs
+LDS-resident grid tiles (64 columns x 8..64 rows, one block each): a tile's
+interior stays in LDS for the whole substep and only a 2-particle ring goes
+through memory.  Same bar.
 """
 import numpy as np
 import pytest
@@ -95,21 +100,76 @@ def test_gs_tiles_frames_and_graph(O, X, graph):
 
 
 def test_gs_tiles_equal_colours_on_1024_sheet(X):
-    """The bench sheet (1024x1024): per-tile flags and one launch per colour
+    """The bench sheet (1024x1024): per-tile flags (LDS grid tiles of 64 and
+    32 rows, index-range tiles of 256 particles) and one launch per colour
     (itself pinned to the oracle at smaller sizes) give the same bits after a
-    frame of 8 substeps with collision, at two tile sizes."""
+    frame of 8 substeps with collision."""
     note("case gs tiles 1024")
     h = 1 / 1023
     kw = dict(cols=1024, rows=1024, spacing=h, pins=2, jitter=1e-3, seed=3)
     out = []
-    for form, tile in ((2, 0), (3, 0), (3, 256)):
+    for form, tile, rows in ((2, 0, 0), (3, 0, 0), (3, 256, 0), (3, 0, 32)):
         sim, _, _ = make(X, X.ClothMeshData.sheet(**kw), solver=1, S=8, I=10, gs_launch=form, use_graph=1,
                          thickness=2 * (0.4 * h), jacobi_omega=1.0)
         if tile:
             sim.gs_tiles(tile)
+        if rows:
+            sim.gs_grid(rows)
+        if form == 3:
+            assert sim.gs_grid()[3] == (0 if tile else 1)
         sim.step_frames(1)
         out.append(sim.read_state())
         sim.close()
-    for k in (1, 2):
+    for k in (1, 2, 3):
         assert bits_equal(out[k][0], out[0][0]), f"form {k}: max |d| {max_abs(out[k][0], out[0][0]):.3e}"
         assert bits_equal(out[k][1], out[0][1])
+
+
+def test_gs_grid_plan(X):
+    """Grid tiles cover the cloth (partial last row of tiles included), the
+    height is re-planned, refused below 8 rows (plan kept), and index-range
+    tiles switch them off; a cloth whose columns are not a multiple of 64 has
+    none."""
+    note("case gs grid plan")
+    sim, _, _ = make(X, _sheet(X, cols=128, rows=72), solver=1, S=4, I=6, gs_launch=3, thickness=0.016)
+    th, nt, threads, on = sim.gs_grid()
+    assert on == 1 and nt == 2 * -(-72 // th) and threads >= 256, (th, nt, threads, on)
+    assert sim.gs_grid(16) == (16, 2 * 5, 256, 1)
+    assert sim.gs_grid(8) == (8, 2 * 9, 256, 1)
+    with pytest.raises(X.XpbdError):
+        sim.gs_grid(4)
+    assert sim.gs_grid() == (8, 18, 256, 1)
+    sim.gs_tiles(128)
+    assert sim.gs_grid()[3] == 0
+    sim.close()
+    sim, _, _ = make(X, _sheet(X), solver=1, S=4, I=6, gs_launch=3, thickness=0.016)
+    assert sim.gs_grid() == (0, 0, 0, 0)
+    sim.close()
+
+
+@pytest.mark.parametrize("cols,rows,th", [(128, 72, 8), (128, 72, 16), (128, 72, 32), (128, 72, 64), (64, 100, 16),
+                                          (192, 40, 0)])
+def test_gs_grid_bit_exact(O, X, cols, rows, th):
+    """LDS grid tiles of each height, a partial last row of tiles, a single
+    column of tiles, and the create-time height, with collision."""
+    note(f"case gs grid {cols}x{rows} th={th}")
+    mesh = _sheet(X, cols=cols, rows=rows, seed=cols + rows)
+    f = mesh.fields()
+    sim, cfg, u = make(X, mesh, solver=1, S=8, I=6, gs_launch=3, thickness=0.016, jacobi_omega=1.0)
+    plan = sim.gs_grid(th) if th else sim.gs_grid()
+    assert plan[3] == 1 and (th == 0 or plan[0] == th), plan
+    _check_substeps(O, X, sim, cfg, u, f, substeps=4)
+    sim.close()
+
+
+@pytest.mark.parametrize("iters", [1, 2])
+def test_gs_grid_iteration_counts(O, X, iters):
+    """One and two iterations: the first colour step follows the predict, the
+    final write follows the last colour step directly."""
+    note(f"case gs grid I={iters}")
+    mesh = _sheet(X, cols=128, rows=48, seed=7)
+    f = mesh.fields()
+    sim, cfg, u = make(X, mesh, solver=1, S=4, I=iters, gs_launch=3, thickness=0.016, jacobi_omega=1.0)
+    assert sim.gs_grid(16)[3] == 1
+    _check_substeps(O, X, sim, cfg, u, f, substeps=3)
+    sim.close()
