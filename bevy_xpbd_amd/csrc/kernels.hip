@@ -2329,9 +2329,11 @@ struct TileWindow {
 // Re-read the window's shared particles that a constraint of this tile or of
 // a neighbour can touch (window rows 2..3 and th..th+3 whole, columns 0..3 and
 // 64..67 of rows 4..th-1; rows 0..1 lie above every endpoint of this tile).
-__device__ __forceinline__ void gsg_ring_load(const TileWindow& m, int r0, int c0, uint32_t th, uint32_t rows) {
+// Threads [0, nthreads) of the block take part, thread `tid` from k = tid.
+__device__ __forceinline__ void gsg_ring_load(const TileWindow& m, int r0, int c0, uint32_t th, uint32_t rows,
+                                              uint32_t tid, uint32_t nthreads) {
     const uint32_t nfull = 6 * GSG_WW, nring = nfull + (th - 4) * 8;
-    for (uint32_t k = threadIdx.x; k < nring; k += blockDim.x) {
+    for (uint32_t k = tid; k < nring; k += nthreads) {
         uint32_t lr, lc;
         if (k < nfull) {
             const uint32_t q = k / GSG_WW;
@@ -2348,6 +2350,34 @@ __device__ __forceinline__ void gsg_ring_load(const TileWindow& m, int r0, int c
     }
 }
 
+// One grid-tile step's constraint rows held in registers: this thread's first
+// GSG_PRE rounds of the interior group [e0, em) and its first round of the
+// shared group [em, e1) (emt: its row).
+constexpr uint32_t GSG_PRE = 2;
+struct GsgRows {
+    uint32_t e0, em, e1, emt;
+    uint32_t c[GSG_PRE + 1];
+    float2 r[GSG_PRE + 1];
+};
+
+__device__ __forceinline__ GsgRows gsg_rows(const GsGrid& t, const uint32_t* co, uint32_t colour) {
+    GsgRows g;
+    g.e0 = co[2 * colour];
+    g.em = co[2 * colour + 1];
+    g.e1 = co[2 * colour + 2];
+    g.emt = g.em + threadIdx.x;
+#pragma unroll
+    for (uint32_t k = 0; k <= GSG_PRE; ++k) {
+        const uint32_t q = k < GSG_PRE ? g.e0 + threadIdx.x + k * blockDim.x : g.emt;
+        const bool ok = k < GSG_PRE ? q < g.em : q < g.e1;
+        g.c[k] = ok ? t.con[q] : 0u;
+        g.r[k] = ok ? t.crc[q] : make_float2(0.f, 0.f);
+    }
+    return g;
+}
+
+// Wavefront 0 polls the 8 neighbour tiles' flags (lanes 0..8), then the block
+// synchronises.
 __device__ __forceinline__ void gsg_wait(const GsGrid& t, uint32_t tx, uint32_t ty, uint32_t s, uint32_t* err) {
     if (threadIdx.x < 64) {
         const int l = int(threadIdx.x), nx = int(tx) + l % 3 - 1, ny = int(ty) + l / 3 - 1;
@@ -2383,39 +2413,44 @@ __global__ __launch_bounds__(1024) void k_gs_grid(GsArgs a, GsGrid t) {
     flag_step(t.flag + T, 1);
     const uint32_t last = 1 + a.iterations * t.colours;
     const uint32_t* co = t.coff + size_t(T) * t.colours * 2;
+    // A step's constraint rows do not depend on any other tile: each step's
+    // first rounds (GSG_PRE of the interior, one of the rest) are fetched a
+    // step ahead, so their latency overlaps the previous step.
+    GsgRows cur = gsg_rows(t, co, 0u), nxt = cur;
     for (uint32_t s = 1; s < last; ++s) {
-        const uint32_t c = (s - 1) % t.colours;
-        const uint32_t e0 = co[2 * c], em = co[2 * c + 1], e1 = co[2 * c + 2];
+        if (s + 1 < last) nxt = gsg_rows(t, co, s % t.colours);
         // the colour's constraints inside the tile's interior touch no
         // particle of another tile: they run before the wait
-        for (uint32_t q = e0 + threadIdx.x; q < em; q += blockDim.x) {
+#pragma unroll
+        for (uint32_t k = 0; k < GSG_PRE; ++k) {
+            if (cur.e0 + threadIdx.x + k * blockDim.x < cur.em) {
+                const uint2 ij = make_uint2(cur.c[k] & 0xFFFFu, cur.c[k] >> 16);
+                gs_apply(a, ij, cur.r[k], m.ld(ij.x), m.ld(ij.y), m);
+            }
+        }
+        for (uint32_t q = cur.e0 + threadIdx.x + GSG_PRE * blockDim.x; q < cur.em; q += blockDim.x) {
             const uint32_t w2 = t.con[q];
             const uint2 ij = make_uint2(w2 & 0xFFFFu, w2 >> 16);
             gs_apply(a, ij, t.crc[q], m.ld(ij.x), m.ld(ij.y), m);
         }
-        const uint32_t e = em + threadIdx.x;
-        uint32_t cw = 0;
-        float2 rc = make_float2(0.f, 0.f);
-        if (e < e1) {
-            cw = t.con[e];
-            rc = t.crc[e];
-        }
         gsg_wait(t, tx, ty, s, a.error);
-        gsg_ring_load(m, r0, c0, th, t.rows);
+        gsg_ring_load(m, r0, c0, th, t.rows, threadIdx.x, blockDim.x);
         __syncthreads();
-        if (e < e1) {
-            const uint2 ij = make_uint2(cw & 0xFFFFu, cw >> 16);
-            gs_apply(a, ij, rc, m.ld(ij.x), m.ld(ij.y), m);
+        const uint32_t e = cur.emt;
+        if (e < cur.e1) {
+            const uint2 ij = make_uint2(cur.c[GSG_PRE] & 0xFFFFu, cur.c[GSG_PRE] >> 16);
+            gs_apply(a, ij, cur.r[GSG_PRE], m.ld(ij.x), m.ld(ij.y), m);
         }
-        for (uint32_t q = e + blockDim.x; q < e1; q += blockDim.x) {
+        for (uint32_t q = e + blockDim.x; q < cur.e1; q += blockDim.x) {
             const uint32_t w2 = t.con[q];
             const uint2 ij = make_uint2(w2 & 0xFFFFu, w2 >> 16);
             gs_apply(a, ij, t.crc[q], m.ld(ij.x), m.ld(ij.y), m);
         }
         flag_step(t.flag + T, s + 1);
+        cur = nxt;
     }
     gsg_wait(t, tx, ty, last, a.error);
-    gsg_ring_load(m, r0, c0, th, t.rows);
+    gsg_ring_load(m, r0, c0, th, t.rows, threadIdx.x, blockDim.x);
     __syncthreads();
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t k = threadIdx.x; k < ntr * GSG_W; k += blockDim.x) {
