@@ -189,6 +189,7 @@ def lib():
             "xpbd_debug_raise_device_error": (i32, [vp]),
             "xpbd_debug_gs_tiles": (i32, [vp, u32, u32, vp]),
             "xpbd_debug_gs_grid": (i32, [vp, u32, vp]),
+            "xpbd_debug_jacobi_resident": (i32, [vp, u32, vp]),
             "xpbd_get_layout": (i32, [vp, vp]),
             "xpbd_selftest_division": (i32, [i32, i32, u64, u64, vp]),
             "xpbd_selftest_sqrt": (i32, [i32, i32, vp]),
@@ -510,6 +511,14 @@ class ClothComputePlugin:
         rows > 0.  Returns (rows per tile, tiles, threads per block, in use)."""
         out = (C.c_uint32 * 4)()
         _check(lib().xpbd_debug_gs_grid(self._h, int(rows), out), "xpbd_debug_gs_grid", self._h)
+        return tuple(int(v) for v in out)
+
+    def jacobi_resident(self, rows: int = 0):
+        """Test/tuning hook: the LDS-resident Jacobi tiles (persistent_jacobi = 2),
+        re-planned at `rows` rows per tile when rows > 0.  Returns (rows per
+        tile, tiles, pairs staged in LDS, in use)."""
+        out = (C.c_uint32 * 4)()
+        _check(lib().xpbd_debug_jacobi_resident(self._h, int(rows), out), "xpbd_debug_jacobi_resident", self._h)
         return tuple(int(v) for v in out)
 
     def residual(self) -> float:

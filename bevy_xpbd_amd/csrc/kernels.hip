@@ -290,7 +290,6 @@ constexpr uint32_t JU = 6;
 constexpr uint32_t JT_JU = 2;
 constexpr uint32_t LDS_PAIRS_MAX = 2048;   // (rest, compliance) table staged in LDS up to this size (static)
 constexpr uint32_t LDS_PAIRS_DYN_MAX = 4096;   // k_jacobi: table sized to the mesh's pairs (dynamic LDS, <= 32 KB)
-constexpr uint32_t JT_PAIRS_MAX = 1024;    // the tiled kernel's staged table (float4)
 
 // The staged pair of packed entry word w.  Two LDS table forms: (rest,
 // compliance) as given (fewer registers: the per-slice kernels at the 64-VGPR
@@ -529,7 +528,9 @@ __device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, con
         }
         out = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, wi);
     }
-    if (MODE & JM_FINAL) {
+    if (MODE & JM_KEEP) {
+        // the caller keeps the result
+    } else if (MODE & JM_FINAL) {
         if (pi.w > 0.f) finish_particle(out, a.dst[i], i, a.ground, p, a.dst, a.vout);
     } else {
         a.dst[i] = out;
@@ -2549,6 +2550,153 @@ __global__ __launch_bounds__(256) void k_jacobi_persistent(JacobiArgs a, Persist
 }
 
 // ---------------------------------------------------------------------------
+// Jacobi substep on LDS-resident grid tiles: predict and all I iterations in
+// one launch, without grid barriers and without restaging the tile every
+// iteration.  One grid cloth whose column count is a multiple of 64 is cut
+// into tiles of 64 columns x th rows, one block each (all resident at once).
+// A block keeps its tile's iterate and a 2-particle ring around it (the
+// window, (th + 4) x 68, the layout of the per-iteration grid tiles, so the
+// same tile-offset entry words apply) in LDS, twice: the iterate read and the
+// one written.  Between iterations only the ring crosses tiles: a tile
+// publishes its own particles within 2 of its edge (at device scope, DeviceW)
+// into the iterate buffer of that iteration (A for the predict, then B, A,
+// ...), raises its flag, and before the next iteration reads the 2-particle
+// ring outside its edge that its 8 neighbours published.  Double buffering:
+// iteration k reads ring k from buffer k % 2 and publishes ring k + 1 into the
+// other, which no neighbour still reads (each waited for this tile's ring k
+// before it published its own ring k + 1).  Per particle the gather is that
+// of k_jacobi_tiled (same entries, same order, same arithmetic), so the
+// results are those of the per-iteration launches.  The fast division is
+// used in an iteration where the whole window is tame (a block-wide AND).
+// The final iterate goes to the state (final_write) or, with its bounds (one
+// partial per tile), to buffer I % 2 for the collision pass.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool jr_ring(uint32_t lr, uint32_t lc, uint32_t ntr) {
+    return lr < 4 || lr >= ntr || lc < 4 || lc >= 64;   // own positions: window rows/cols 2..ntr+1 / 2..65
+}
+
+// Load the 2-particle ring outside the tile (window rows 0, 1, ntr + 2,
+// ntr + 3 whole; columns 0, 1, 66, 67 of rows 2..ntr+1) from the published
+// iterate; tame: AND over the loaded particles.
+__device__ __forceinline__ bool jr_halo(float4* win, const DeviceW& src, int r0, int c0, uint32_t ntr, uint32_t rows,
+                                        uint32_t cols) {
+    bool tame = true;
+    const uint32_t nfull = 4 * JT_W, n = nfull + ntr * 4;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        uint32_t lr, lc;
+        if (k < nfull) {
+            const uint32_t q = k / JT_W;
+            lr = q < 2 ? q : ntr + q;
+            lc = k % JT_W;
+        } else {
+            const uint32_t q = k - nfull, c4 = q % 4;
+            lr = 2 + q / 4;
+            lc = c4 < 2 ? c4 : 64 + c4;
+        }
+        const int r = r0 - 2 + int(lr), c = c0 - 2 + int(lc);
+        if (r >= 0 && r < int(rows) && c >= 0 && c < int(cols)) {
+            const float4 v = src.ld(uint32_t(r) * cols + uint32_t(c));
+            win[lr * JT_W + lc] = v;
+            tame = tame && tame3(v);
+        }
+    }
+    return tame;
+}
+
+template <bool FASTDIV>
+__global__ __launch_bounds__(1024) void k_jacobi_resident(JacobiArgs a, PersistArgs q, JResident t) {
+    if (q.probe) {   // uniform: the co-residency check of this kernel's grid
+        GridSync gsync;
+        gsync.init(q.barrier, q.error);
+        return;
+    }
+    extern __shared__ float4 jr_smem[];
+    const DevParams& p = a.p;
+    const uint32_t cols = a.sell.tcols, rows = a.sell.trows, ntx = cols / 64, th = t.th;
+    const uint32_t T = blockIdx.x, tx = T % ntx, ty = T / ntx;
+    const int r0 = int(ty * th), c0 = int(tx * 64);
+    const uint32_t ntr = min(th, rows - ty * th);
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    float4* lpairs = jr_smem;
+    float4* cur = jr_smem + t.npairs;
+    float4* nxt = cur + (th + 4) * JT_W;
+    if (t.npairs)
+        for (uint32_t k = threadIdx.x; k < a.num_pairs; k += blockDim.x) lpairs[k] = stage_pair(a.sell.pairs[k], p);
+    // step 0: predict the tile, publish its ring into A
+    bool tame = true;
+    {
+        const DeviceW pub(q.A, p.n);
+        for (uint32_t k = threadIdx.x; k < ntr * 64; k += blockDim.x) {
+            const uint32_t lr = k / 64 + 2, lc = k % 64 + 2;
+            const uint32_t i = uint32_t(r0 + int(lr) - 2) * cols + uint32_t(c0 + int(lc) - 2);
+            const float4 v = predict(q.X[i], q.V[i], i, p);
+            cur[lr * JT_W + lc] = v;
+            tame = tame && tame3(v);
+            if (jr_ring(lr, lc, ntr)) pub.st(i, v);
+        }
+    }
+    flag_step(t.flag + T, 1);
+    for (uint32_t k = 0; k < q.iterations; ++k) {
+        const bool last = k + 1 == q.iterations;
+        const DeviceW src((k & 1) ? q.B : q.A, p.n), pub((k & 1) ? q.A : q.B, p.n);
+        if (threadIdx.x < 64) {
+            const int l = int(threadIdx.x), nx = int(tx) + l % 3 - 1, ny = int(ty) + l / 3 - 1;
+            const bool nb = l < 9 && l != 4 && nx >= 0 && ny >= 0 && nx < int(ntx) && ny < int(t.ty);
+            poll_flags(nb ? t.flag + ny * int(ntx) + nx : nullptr, k + 1, q.error);
+        }
+        __syncthreads();
+        tame = jr_halo(cur, src, r0, c0, ntr, rows, cols) && tame;
+        const int all_tame = __syncthreads_and(tame ? 1 : 0);
+        const bool fast = FASTDIV && all_tame != 0;
+        tame = true;
+        for (uint32_t row = wave; row < ntr; row += nw) {
+            const uint32_t slice = (uint32_t(r0) + row) * ntx + tx;
+            const uint32_t base = a.sell.soff[slice] + lane, width = a.sell.swid[slice];
+            uint32_t pre[JT_JU];
+#pragma unroll
+            for (uint32_t u = 0; u < JT_JU; ++u)
+                pre[u] = u < width ? __builtin_nontemporal_load(a.sell.tpacked + base + u * 64) : 0u;
+            const uint32_t lidx = (row + 2) * JT_W + lane + 2;
+            bool valid;
+            float4 out;
+            if (t.npairs)
+                out = fast ? jacobi_particle_tiled<JM_KEEP, true, true>(a, lpairs, cur, slice, lane, lidx, true, valid,
+                                                                         pre, base, width)
+                           : jacobi_particle_tiled<JM_KEEP, false, true>(a, lpairs, cur, slice, lane, lidx, true, valid,
+                                                                          pre, base, width);
+            else
+                out = fast ? jacobi_particle_tiled<JM_KEEP, true, false>(a, lpairs, cur, slice, lane, lidx, true, valid,
+                                                                          pre, base, width)
+                           : jacobi_particle_tiled<JM_KEEP, false, false>(a, lpairs, cur, slice, lane, lidx, true,
+                                                                           valid, pre, base, width);
+            nxt[lidx] = out;
+            tame = tame && tame3(out);
+            if (!last && jr_ring(row + 2, lane + 2, ntr)) pub.st(slice * 64 + lane, out);
+        }
+        if (!last) flag_step(t.flag + T, k + 2);
+        else __syncthreads();
+        float4* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+    // the final iterate of the tile
+    float4* F = (q.iterations & 1) ? q.B : q.A;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t k = threadIdx.x; k < ntr * 64; k += blockDim.x) {
+        const uint32_t lr = k / 64 + 2, lc = k % 64 + 2;
+        const uint32_t i = uint32_t(r0 + int(lr) - 2) * cols + uint32_t(c0 + int(lc) - 2);
+        const float4 v = cur[lr * JT_W + lc];
+        if (q.final_write) {
+            if (v.w > 0.f) finish_particle(v, q.X[i], i, a.ground, p, q.X, q.V);
+        } else {
+            F[i] = v;
+            aabb_fold(true, v.x, v.y, v.z, lo, hi);
+        }
+    }
+    if (!q.final_write) block_aabb_store(lo, hi, t.aabb, T);
+}
+
+// ---------------------------------------------------------------------------
 // Render publish: finalize_normals (cloth_sim_jacobi.wgsl:527-547) with the
 // face normals gathered per vertex over its incident triangles in ascending
 // order (no fixed-point atomics), fallback +Y; positions copied with w = 0.
@@ -3040,6 +3188,38 @@ uint32_t gs_grid_max_blocks(uint32_t th) {
                                                      gs_grid_lds_bytes(th)) != hipSuccess)
         return 0;
     return uint32_t(std::max(0, per_cu) * std::max(0, cus));
+}
+
+uint32_t jacobi_resident_lds_bytes(uint32_t th, uint32_t npairs) { return (npairs + 2 * (th + 4) * JT_W) * 16u; }
+
+hipError_t jacobi_resident_prepare(uint32_t th, uint32_t npairs) {
+    const int bytes = int(jacobi_resident_lds_bytes(th, npairs));
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_jacobi_resident<true>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(k_jacobi_resident<false>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+uint32_t jacobi_resident_max_blocks(uint32_t th, uint32_t npairs) {
+    int dev = 0, per_cu = 0, per_cu2 = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    const size_t bytes = jacobi_resident_lds_bytes(th, npairs);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jacobi_resident<true>, int(th * 16), bytes) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, k_jacobi_resident<false>, int(th * 16), bytes) !=
+            hipSuccess)
+        return 0;
+    return uint32_t(std::max(0, std::min(per_cu, per_cu2)) * std::max(0, cus));
+}
+
+void launch_jacobi_resident(const JacobiArgs& a, const PersistArgs& q, const JResident& t, hipStream_t s) {
+    const uint32_t tiles = (a.sell.tcols / 64) * t.ty;
+    launch_zero_u32(t.flag, tiles, s);
+    const size_t bytes = jacobi_resident_lds_bytes(t.th, t.npairs);
+    if (a.fastdiv) hipLaunchKernelGGL(k_jacobi_resident<true>, dim3(tiles), dim3(t.th * 16), bytes, s, a, q, t);
+    else hipLaunchKernelGGL(k_jacobi_resident<false>, dim3(tiles), dim3(t.th * 16), bytes, s, a, q, t);
 }
 
 void launch_gs_grid(const GsArgs& a, const GsGrid& t, hipStream_t s) {

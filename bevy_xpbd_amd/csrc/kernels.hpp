@@ -73,6 +73,7 @@ struct SharedTiles {
 };
 constexpr uint32_t JS_DIRS = 8;
 constexpr uint32_t JS_PAIRS_MAX = 1024;
+constexpr uint32_t JT_PAIRS_MAX = 1024;   // the tiled kernels' staged pair table (float4)
 // LDS of one k_jacobi_shared block (position tile, result slots, pairs).
 size_t jacobi_shared_lds_bytes(uint32_t nslots, uint32_t num_pairs);
 
@@ -97,6 +98,7 @@ enum JacobiMode : uint32_t {
     JM_PREDICT = 1,   // inputs are the substep-start state; predict on the fly
     JM_FINAL = 2,     // write the new state in place + velocity (no collision)
     JM_AABB = 4,      // accumulate the bounds of the output (collision follows)
+    JM_KEEP = 8,      // the caller stores the result (LDS-resident tiles)
 };
 
 struct JacobiArgs {
@@ -245,6 +247,20 @@ struct GsGrid {
     uint32_t colours;
 };
 
+// Jacobi substep on LDS-resident grid tiles (k_jacobi_resident): one grid
+// cloth (the grid-tile layout: sell.tcols a multiple of 64), tiles of 64
+// columns x th rows (th in {16, 32, 64}; blocks of th x 16 threads), all
+// resident at once.  With PersistArgs: predict X,V, I iterations, the state
+// (final_write) or the last iterate in A (even I) / B (odd I) with one bounds
+// partial per tile in aabb.  A and B carry the rings between tiles.
+struct JResident {
+    uint32_t* flag;      // tiles (zeroed by launch_jacobi_resident)
+    float* aabb;         // bounds partials (tiles), collision follows
+    uint32_t th;         // tile rows
+    uint32_t ty;         // rows of tiles
+    uint32_t npairs;     // pairs staged in LDS (0: the pair table is read from memory)
+};
+
 // ---- launchers (kernels.hip) ----
 uint32_t jacobi_blocks(uint32_t nslices, uint32_t block);
 uint32_t launch_jacobi(uint32_t mode, const JacobiArgs& a, uint32_t block, hipStream_t s);
@@ -307,6 +323,10 @@ uint32_t gs_grid_block(uint32_t th);
 hipError_t gs_grid_prepare(uint32_t th);
 uint32_t gs_grid_max_blocks(uint32_t th);
 void launch_gs_grid(const GsArgs& a, const GsGrid& t, hipStream_t s);
+uint32_t jacobi_resident_lds_bytes(uint32_t th, uint32_t npairs);
+hipError_t jacobi_resident_prepare(uint32_t th, uint32_t npairs);
+uint32_t jacobi_resident_max_blocks(uint32_t th, uint32_t npairs);
+void launch_jacobi_resident(const JacobiArgs& a, const PersistArgs& q, const JResident& t, hipStream_t s);
 void launch_copy4(const float4* a, float4* b, uint32_t n, hipStream_t s);
 // per-slice tame bytes of P (1: every coordinate 0 or of magnitude in [2^-50, 2^20])
 void launch_tame_scan(const float4* P, uint32_t n, uint8_t* slice_tame, hipStream_t s);

@@ -150,6 +150,11 @@ struct xpbd_cloth {
     // grid, the capacity of coff / flag (tiles of GSG_MIN_ROWS rows)
     GsGrid gg{};
     bool gg_ok = false, gg_on = false;
+    // Jacobi on LDS-resident grid tiles (k_jacobi_resident): the plan, whether
+    // it can run (jr_ok), the autotuner's finding, whether substeps use it
+    JResident jr{};
+    bool jr_ok = false, jr_tuned = false;
+    uint32_t jr_on = 0, jr_cap = 0;
     uint32_t gg_cols = 0, gg_rows = 0, gg_cap = 0;
     std::vector<float2> gs_rc_host;
     uint32_t pj_blocks = 0;        // persistent Jacobi grid (co-resident blocks)
@@ -476,7 +481,29 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         } else {
             launch_finalize(h->W0, h->X, h->V, h->has_ground ? h->ground : nullptr, p, s);
         }
-    } else if (h->cfg.persistent_jacobi) {
+    } else if (h->jr_on) {
+        // predict and all iterations in one launch on LDS-resident tiles; the
+        // result lands in X,V (no collision) or, with one bounds partial per
+        // tile, in A (even I) / B (odd I) for the collision pass.
+        JacobiArgs a = jacobi_args(h, p);
+        PersistArgs q{};
+        q.X = h->X;
+        q.V = h->V;
+        q.A = h->A;
+        q.B = h->B;
+        q.iterations = iters;
+        q.final_write = collision ? 0u : 1u;
+        q.barrier = h->barrier;
+        q.error = h->err;
+        {
+            ProbeScope ps(h, XPBD_KERNEL_JACOBI_PERSISTENT);
+            launch_jacobi_resident(a, q, h->jr, s);
+        }
+        if (collision) {
+            F = (iters % 2 == 0) ? h->A : h->B;
+            nparts = (h->tile_cols / 64) * h->jr.ty;
+        }
+    } else if (h->cfg.persistent_jacobi == 1) {
         // predict and all iterations in one launch; the result lands in X,V
         // (no collision) or in A (even I) / B (odd I) for the collision pass.
         launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
@@ -1048,6 +1075,31 @@ int build_gs_grid(xpbd_cloth* h, uint32_t th) {
     return XPBD_OK;
 }
 
+// Plan the LDS-resident Jacobi tiles of th rows (grid-tile layout only):
+// jr_ok when the window pair and the staged pairs fit the LDS and every tile
+// can be resident at once.
+int build_jacobi_resident(xpbd_cloth* h, uint32_t th) {
+    h->jr_ok = false;
+    if (!h->nbtile || !h->jr.flag || !h->tile_cols || th < 16 || th > 64) return XPBD_OK;
+    const uint32_t ty = (h->tile_rows + th - 1) / th, nt = (h->tile_cols / 64) * ty;
+    if (nt > h->jr_cap || nt > h->aabb_slots) return XPBD_OK;
+    int dev = 0, lds = 0;
+    HIPC(h, hipGetDevice(&dev));
+    HIPC(h, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const uint32_t static_lds = 1024;   // the kernel's own (grid set-up, block AND)
+    uint32_t npairs = h->num_pairs <= JT_PAIRS_MAX ? h->num_pairs : 0u;
+    if (jacobi_resident_lds_bytes(th, npairs) + static_lds > uint32_t(lds)) npairs = 0;
+    if (jacobi_resident_lds_bytes(th, npairs) + static_lds > uint32_t(lds)) return XPBD_OK;
+    HIPC(h, jacobi_resident_prepare(th, npairs));
+    if (nt > jacobi_resident_max_blocks(th, npairs)) return XPBD_OK;
+    h->jr.th = th;
+    h->jr.ty = ty;
+    h->jr.npairs = npairs;
+    h->jr.aabb = h->aabb;
+    h->jr_ok = true;
+    return XPBD_OK;
+}
+
 int create_device(xpbd_cloth* h, const Mesh& m) {
     const size_t n = h->n, e = h->e, t = h->t;
     TRY(dalloc(h, &h->X, n));
@@ -1289,6 +1341,14 @@ int create_device(xpbd_cloth* h, const Mesh& m) {
         if (!h->gg_ok) TRY(build_gs_grid(h, 32));
         h->gg_on = h->gg_ok;
     }
+    // Jacobi on LDS-resident grid tiles: 64 rows where the LDS and residency
+    // allow, else 32, else 16
+    if (h->nbtile && h->tile_cols >= 64) {
+        h->jr_cap = (h->tile_cols / 64) * ((h->tile_rows + 15) / 16);
+        TRY(dalloc(h, &h->jr.flag, h->jr_cap));
+        for (uint32_t th : {64u, 32u, 16u})
+            if (!h->jr_ok) TRY(build_jacobi_resident(h, th));
+    }
     {
         std::vector<uint32_t> offs = m.cloth_offsets;
         if (offs.size() < 2 || offs.front() != 0 || offs.back() != n) offs = {0u, uint32_t(n)};
@@ -1386,6 +1446,12 @@ void apply_tuning(xpbd_cloth* h) {
     // grid tiles: 1 = the form the autotuner found faster, 3 = constraint-
     // shared, 2 = per entry
     h->js_on = h->js_ok && (h->cfg.grid_tiles == 3 || (h->cfg.grid_tiles == 1 && h->js_tuned)) ? 1u : 0u;
+    // Jacobi substeps on LDS-resident tiles: 2 = on, 0 = where the autotuner
+    // measured it faster (per entry tiles only)
+    h->jr_on = h->jr_ok && !h->js_on &&
+                       (h->cfg.persistent_jacobi == 2 || (h->cfg.persistent_jacobi == 0 && h->jr_tuned))
+                   ? 1u
+                   : 0u;
 }
 
 // Grid neighbours and feature constants of the warm-start network.  Needs a
@@ -1540,6 +1606,76 @@ int autotune(xpbd_cloth* h) {
             (void)hipEventElapsedTime(&t_form[form], e0, e1);
         }
         h->js_tuned = t_form[1] < t_form[0];
+    }
+    // Jacobi substeps: per-iteration launches (predict fused into the first,
+    // bounds into the last) against one launch on LDS-resident tiles, each
+    // timed as a replayed graph of a collision substep's Jacobi part (the
+    // state is only read: the iterates go to A/B).
+    h->jr_tuned = false;
+    const bool js_would = h->js_ok && (h->cfg.grid_tiles == 3 || (h->cfg.grid_tiles == 1 && h->js_tuned));
+    if (h->jr_ok && h->cfg.persistent_jacobi == 0 && h->cfg.solver == XPBD_SOLVER_JACOBI && !js_would) {
+        const uint32_t iters = std::max(h->cfg.solve_inner_iterations, 1u);
+        auto time_graph = [&](auto body) -> float {
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            float ms = 1e30f;
+            if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return ms;
+            body();
+            if (hipStreamEndCapture(h->stream, &graph) != hipSuccess || !graph) return ms;
+            if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+                (void)hipGraphLaunch(exec, h->stream);   // warm-up
+                (void)hipEventRecord(e0, h->stream);
+                for (uint32_t r = 0; r < 5; ++r) (void)hipGraphLaunch(exec, h->stream);
+                (void)hipEventRecord(e1, h->stream);
+                (void)hipEventSynchronize(e1);
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                (void)hipGraphExecDestroy(exec);
+            }
+            (void)hipGraphDestroy(graph);
+            return ms;
+        };
+        const float t_iter = time_graph([&]() {
+            JacobiArgs a = jacobi_args(h, p);
+            a.sh = SharedTiles{};
+            a.src = h->X;
+            a.vel = h->V;
+            a.dst = h->A;
+            if (iters == 1) {
+                launch_jacobi(JM_PREDICT | JM_AABB, a, h->jacobi_block, h->stream);
+                return;
+            }
+            launch_jacobi(JM_PREDICT, a, h->jacobi_block, h->stream);
+            float4* cur = h->A;
+            float4* nxt = h->B;
+            for (uint32_t k = 1; k < iters; ++k) {
+                a.src = cur;
+                a.dst = nxt;
+                launch_jacobi(k + 1 < iters ? 0u : JM_AABB, a, h->jacobi_block, h->stream);
+                std::swap(cur, nxt);
+            }
+        });
+        const float t_res = time_graph([&]() {
+            JacobiArgs a = jacobi_args(h, p);
+            PersistArgs q{};
+            q.X = h->X;
+            q.V = h->V;
+            q.A = h->A;
+            q.B = h->B;
+            q.iterations = iters;
+            q.final_write = 0;
+            q.barrier = h->barrier;
+            q.error = h->err;
+            launch_jacobi_resident(a, q, h->jr, h->stream);
+        });
+        uint32_t flag = 0;
+        HIPC(h, hipMemcpy(&flag, h->err, sizeof(flag), hipMemcpyDeviceToHost));
+        if (flag) {   // a wait expired: these tiles are not resident at once here
+            const uint32_t zero = 0;
+            HIPC(h, hipMemcpy(h->err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+            h->jr_ok = false;
+        } else {
+            h->jr_tuned = t_res < t_iter;
+        }
     }
     // Coloured GS: the grid size of the persistent substep.  Every block takes
     // part in each of the I x colours grid barriers, whose cost grows with the
@@ -1770,6 +1906,28 @@ int verify_coresident(xpbd_cloth* h, bool gs, bool persistent) {
             },
             h->gt_blocks, "the per-tile Gauss-Seidel kernel"));
     }
+    if (h->cfg.solver == XPBD_SOLVER_JACOBI && h->jr_on) {
+        // every resident tile needs its own resident block: probed once; where
+        // the probe's wait expires the per-iteration launches run instead
+        launch_zero_u32(h->barrier, GRID_SYNC_WORDS, s);
+        JacobiArgs a = jacobi_args(h, p);
+        PersistArgs q{};
+        q.barrier = h->barrier;
+        q.error = h->err;
+        q.probe = 1;
+        launch_jacobi_resident(a, q, h->jr, s);
+        HIPC(h, hipGetLastError());
+        uint32_t flag = 0;
+        HIPC(h, hipMemcpyAsync(&flag, h->err, sizeof(flag), hipMemcpyDeviceToHost, s));
+        HIPC(h, hipStreamSynchronize(s));
+        if (flag) {
+            const uint32_t zero = 0;
+            HIPC(h, hipMemcpy(h->err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+            h->jr_on = 0;
+            h->jr_ok = false;
+            drop_graph(h);
+        }
+    }
     if (persistent) {
         TRY(check(
             [&](uint32_t b) {
@@ -1787,7 +1945,7 @@ int verify_coresident(xpbd_cloth* h, bool gs, bool persistent) {
 
 bool uses_persistent_jacobi(const xpbd_cloth* h) {
     return h->cfg.solver == XPBD_SOLVER_JACOBI &&
-           (h->cfg.persistent_jacobi || (h->warm && std::isfinite(h->warm_threshold)));
+           (h->cfg.persistent_jacobi == 1 || (h->warm && std::isfinite(h->warm_threshold)));
 }
 
 }  // namespace
@@ -1969,7 +2127,8 @@ XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, cons
                          void* stream, xpbd_cloth** out) {
     if (!mesh || !cfg || !u || !out) return XPBD_E_INVALID;
     *out = nullptr;
-    if (cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL || cfg->gs_launch > 3) return XPBD_E_INVALID;
+    if (cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL || cfg->gs_launch > 3 || cfg->persistent_jacobi > 2)
+        return XPBD_E_INVALID;
     if (cfg->jacobi_block && (cfg->jacobi_block % 64 || cfg->jacobi_block > 512)) return XPBD_E_INVALID;
     std::string why;
     if (validate_mesh(mesh->m, why) != XPBD_OK) return XPBD_E_INVALID;
@@ -1998,6 +2157,8 @@ XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, cons
     if (r != XPBD_OK) return r;
     if (cfg->gs_launch == 3 && !h->gt_ok && !h->gg_on)
         return fail(h, XPBD_E_INVALID, "gs_launch 3: no per-tile flag plan for this mesh");
+    if (cfg->persistent_jacobi == 2 && !h->jr_ok)
+        return fail(h, XPBD_E_INVALID, "persistent_jacobi 2: no LDS-resident tiles for this mesh");
     r = build_warm_topology(h, m);
     if (r != XPBD_OK) return r;
     r = upload_state(h, h->init_pos4.data(), nullptr);
@@ -2045,11 +2206,14 @@ XPBD_API int xpbd_set_uniforms(xpbd_cloth* h, const xpbd_sim_uniforms* u) {
 }
 
 XPBD_API int xpbd_set_config(xpbd_cloth* h, const xpbd_sim_config* cfg) {
-    if (!h || !cfg || cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL || cfg->gs_launch > 3) return XPBD_E_INVALID;
+    if (!h || !cfg || cfg->solver > XPBD_SOLVER_GAUSS_SEIDEL || cfg->gs_launch > 3 || cfg->persistent_jacobi > 2)
+        return XPBD_E_INVALID;
     if (cfg->jacobi_block && (cfg->jacobi_block % 64 || cfg->jacobi_block > 512)) return XPBD_E_INVALID;
     const bool retune = cfg->autotune && !h->cfg.autotune;
     if (cfg->gs_launch == 3 && !h->gt_ok && !h->gg_on)
         return fail(h, XPBD_E_INVALID, "gs_launch 3: no per-tile flag plan for this mesh");
+    if (cfg->persistent_jacobi == 2 && !h->jr_ok)
+        return fail(h, XPBD_E_INVALID, "persistent_jacobi 2: no LDS-resident tiles for this mesh");
     h->cfg = *cfg;
     apply_tuning(h);
     drop_graph(h);
@@ -2773,6 +2937,26 @@ XPBD_API int xpbd_debug_gs_grid(xpbd_cloth* h, uint32_t rows, uint32_t* out4) {
     out4[1] = h->gg_ok ? h->gg.tx * h->gg.ty : 0u;
     out4[2] = h->gg_ok ? gs_grid_block(h->gg.th) : 0u;
     out4[3] = h->gg_on ? 1u : 0u;
+    return XPBD_OK;
+}
+
+XPBD_API int xpbd_debug_jacobi_resident(xpbd_cloth* h, uint32_t rows, uint32_t* out4) {
+    if (!h || !out4) return XPBD_E_INVALID;
+    if (rows) {
+        const uint32_t old = h->jr_ok ? h->jr.th : 0u;
+        TRY(build_jacobi_resident(h, rows));
+        if (!h->jr_ok) {
+            if (old) TRY(build_jacobi_resident(h, old));
+            return fail(h, XPBD_E_INVALID, "no LDS-resident Jacobi tiles at this tile height");
+        }
+        drop_graph(h);
+        apply_tuning(h);
+        TRY(verify_coresident(h, h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL, uses_persistent_jacobi(h)));
+    }
+    out4[0] = h->jr_ok ? h->jr.th : 0u;
+    out4[1] = h->jr_ok ? (h->tile_cols / 64) * h->jr.ty : 0u;
+    out4[2] = h->jr_ok ? h->jr.npairs : 0u;
+    out4[3] = h->jr_on;
     return XPBD_OK;
 }
 

@@ -224,7 +224,13 @@ typedef struct xpbd_sim_config {
                                              1 = hardware order, 2 = XCD-contiguous tiles */
     uint32_t persistent_jacobi;           /* 1: predict + all I Jacobi iterations of a
                                              substep in one co-resident launch with grid
-                                             barriers (same results) */
+                                             barriers; 2: the same in one launch on
+                                             LDS-resident grid tiles (one grid cloth of
+                                             64k columns: each tile stays in LDS and only
+                                             a 2-particle ring goes between tiles); 0
+                                             (default): one launch per iteration, or 2
+                                             where the autotuner measures it faster.
+                                             Same results */
     uint32_t grid_tiles;                  /* 1 (default): a single grid cloth whose
                                              column count is a multiple of 64 runs the
                                              Jacobi gather on 64x8 tiles staged in LDS
@@ -399,6 +405,14 @@ XPBD_API int xpbd_debug_gs_tiles(xpbd_cloth* h, uint32_t tile, uint32_t blocks, 
  * 1 if the flag form uses them}, zeros when there is no plan.
  * XPBD_E_INVALID (plan kept) when this height has none. */
 XPBD_API int xpbd_debug_gs_grid(xpbd_cloth* h, uint32_t rows, uint32_t* out4);
+
+/* Test/tuning hook (no reference counterpart): the LDS-resident Jacobi tiles
+ * (persistent_jacobi = 2: tiles of 64 columns x `rows` rows of one grid cloth,
+ * one block each).  With rows in {16, 32, 64}, re-plans at that height.
+ * out4 = {rows per tile, tiles, pairs staged in LDS, 1 if substeps use them},
+ * zeros when there is no plan.  XPBD_E_INVALID (plan kept) when this height
+ * has none. */
+XPBD_API int xpbd_debug_jacobi_resident(xpbd_cloth* h, uint32_t rows, uint32_t* out4);
 
 /* Layout the handle uses (no reference counterpart; for tests and tools):
  * out4 = {adjacency: 0 plain 12-byte entries, 1 packed 4-byte entries,

@@ -42,7 +42,7 @@ def test_invalid_arguments_are_rejected(X):
     # config ranges are checked before any device call
     mesh = X.ClothMeshData.grid_cloth_hanging(4, 4, 0.1)
     u = X.ClothSimUniforms.default(X.SOLVER_JACOBI, 8)
-    for bad in (dict(solver=2), dict(gs_launch=4), dict(jacobi_block=96)):
+    for bad in (dict(solver=2), dict(gs_launch=4), dict(persistent_jacobi=3), dict(jacobi_block=96)):
         cfg = X.ClothSimConfig.default(**bad)
         sim = C.c_void_p()
         assert L.xpbd_create(mesh._h, C.byref(cfg), C.byref(u), None, C.byref(sim)) == X.XPBD_E_INVALID, bad
