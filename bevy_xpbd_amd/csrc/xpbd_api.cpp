@@ -140,7 +140,8 @@ struct xpbd_cloth {
     uint32_t gs_form = 0;           // the GS form substeps run (GS_FORM_*)
     // GS with per-tile flags: the tiles in use, their grid, the device
     // capacity of the tile arrays (tiles of at least GS_TILE_MIN particles),
-    // and the colour-sorted endpoints to plan other tile sizes from
+    // and the colour-sorted rows to plan tiles from (held while the handle is
+    // created, read back from gs_ij / gs_rc for a later re-plan)
     GsTiles gt{};
     bool gt_ok = false;
     uint32_t gt_blocks = 0, gt_cap = 0, gt_deps = 0;
@@ -971,9 +972,28 @@ bool plan_gs_tiles(const std::vector<uint2>& ij, const std::vector<uint32_t>& of
 // Plan GS flag tiles of `tile` particles and upload them into the handle's
 // tile arrays; gt_ok says whether they are usable.  The grid is the tiles,
 // within what can be resident at once and the bounds partial slots.
+// The colour-sorted constraint rows on the host (read back from the device
+// after create released them).
+int gs_host_rows(xpbd_cloth* h) {
+    if (h->gs_ij_host.size() == h->e && h->gs_rc_host.size() == h->e) return XPBD_OK;
+    h->gs_ij_host.resize(h->e);
+    h->gs_rc_host.resize(h->e);
+    if (h->e) {
+        HIPC(h, hipMemcpy(h->gs_ij_host.data(), h->gs_ij, size_t(h->e) * sizeof(uint2), hipMemcpyDeviceToHost));
+        HIPC(h, hipMemcpy(h->gs_rc_host.data(), h->gs_rc, size_t(h->e) * sizeof(float2), hipMemcpyDeviceToHost));
+    }
+    return XPBD_OK;
+}
+
+void release_gs_host_rows(xpbd_cloth* h) {
+    std::vector<uint2>().swap(h->gs_ij_host);
+    std::vector<float2>().swap(h->gs_rc_host);
+}
+
 int build_gs_tiles(xpbd_cloth* h, uint32_t tile, uint32_t blocks = 0) {
     h->gt_ok = false;
     if (!h->gt_cap || tile < GS_TILE_MIN) return XPBD_OK;
+    TRY(gs_host_rows(h));
     std::vector<uint32_t> toff, doff, dep;
     uint32_t max_deps = 0;
     if (!plan_gs_tiles(h->gs_ij_host, h->batch_offsets, h->n, tile, toff, doff, dep, max_deps)) return XPBD_OK;
@@ -1055,6 +1075,7 @@ int build_gs_grid(xpbd_cloth* h, uint32_t th) {
     HIPC(h, hipGetDevice(&dev));
     HIPC(h, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     if (gs_grid_lds_bytes(th) > uint32_t(lds)) return XPBD_OK;
+    TRY(gs_host_rows(h));
     std::vector<uint32_t> coff, con;
     std::vector<float2> crc;
     if (!plan_gs_grid(h->gs_ij_host, h->gs_rc_host, h->batch_offsets, h->gg_cols, h->gg_rows, th, coff, con, crc))
@@ -2180,6 +2201,7 @@ XPBD_API int xpbd_create(const xpbd_mesh* mesh, const xpbd_sim_config* cfg, cons
     r = verify_coresident(h, h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL, uses_persistent_jacobi(h));
     if (r != XPBD_OK) return r;
     if (hipStreamSynchronize(h->stream) != hipSuccess) return XPBD_E_HIP;
+    release_gs_host_rows(h);
     *out = guard.release();
     return XPBD_OK;
 }
@@ -2218,6 +2240,7 @@ XPBD_API int xpbd_set_config(xpbd_cloth* h, const xpbd_sim_config* cfg) {
     apply_tuning(h);
     drop_graph(h);
     if (retune) TRY(autotune(h));
+    release_gs_host_rows(h);
     return verify_coresident(h, h->cfg.solver == XPBD_SOLVER_GAUSS_SEIDEL, uses_persistent_jacobi(h));
 }
 
@@ -2905,8 +2928,10 @@ XPBD_API int xpbd_debug_gs_tiles(xpbd_cloth* h, uint32_t tile, uint32_t blocks, 
         TRY(build_gs_tiles(h, tile, blocks));
         if (!h->gt_ok) {
             if (old_tile) TRY(build_gs_tiles(h, old_tile, old_blocks));
+            release_gs_host_rows(h);
             return fail(h, XPBD_E_INVALID, "no per-tile flag plan at this tile size");
         }
+        release_gs_host_rows(h);
         h->gg_on = false;   // the flag form runs these index-range tiles
         drop_graph(h);
         apply_tuning(h);
@@ -2926,8 +2951,10 @@ XPBD_API int xpbd_debug_gs_grid(xpbd_cloth* h, uint32_t rows, uint32_t* out4) {
         TRY(build_gs_grid(h, rows));
         if (!h->gg_ok) {
             if (old) TRY(build_gs_grid(h, old));
+            release_gs_host_rows(h);
             return fail(h, XPBD_E_INVALID, "no LDS grid-tile plan at this tile height");
         }
+        release_gs_host_rows(h);
         h->gg_on = true;
         drop_graph(h);
         apply_tuning(h);
