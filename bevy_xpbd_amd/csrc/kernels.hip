@@ -24,7 +24,27 @@ constexpr float GRAB_MAX_PULL = 0.065f;
 constexpr float COLLISION_APPLY_CLAMP = 0.35f;
 constexpr uint32_t EMPTY_KEY = 0xFFFFFFFFu;
 
-__device__ __forceinline__ float len3(float x, float y, float z) { return sqrtf((x * x + y * y) + z * z); }
+__device__ __forceinline__ float sq3(float x, float y, float z) { return (x * x + y * y) + z * z; }
+__device__ __forceinline__ float len3(float x, float y, float z) { return sqrtf(sq3(x, y, z)); }
+
+// The correction cap's test without the square root: the correctly rounded
+// square root is monotone, so sqrtf(s) > JACOBI_CORRECTION_CAP (which implies
+// > 0) exactly when s > CAP_SQ_LIMIT, the largest float whose correctly
+// rounded root is at most the cap (0x1.41205ep-4; NaN fails both tests).
+// The root is then taken only for a capped correction.
+constexpr float CAP_SQ_LIMIT = 0x1.41205ep-4f;
+
+// (x, y, z) scaled to the cap where its length exceeds it (xpbd_cpu.rs's
+// `if len > cap { v *= cap / len }`, same operations).
+__device__ __forceinline__ void cap_correction(float& x, float& y, float& z) {
+    const float s = sq3(x, y, z);
+    if (s > CAP_SQ_LIMIT) {
+        const float sc = JACOBI_CORRECTION_CAP / sqrtf(s);
+        x = x * sc;
+        y = y * sc;
+        z = z * sc;
+    }
+}
 
 // predict_particle (xpbd_cpu.rs:38-65); the inverse mass rides in .w.
 __device__ __forceinline__ float4 predict(float4 x, float4 v, uint32_t i, const DevParams& p) {
@@ -394,13 +414,7 @@ __device__ __forceinline__ float4 jacobi_particle(const JacobiArgs& a, const flo
             jacobi_entry<FASTDIV>(pi, pj, pr, ax, ay, az);
         }
         float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
-        const float ml = len3(dx, dy, dz);
-        if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
-            const float sc = JACOBI_CORRECTION_CAP / ml;
-            dx = dx * sc;
-            dy = dy * sc;
-            dz = dz * sc;
-        }
+        cap_correction(dx, dy, dz);
         out = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, wi);
     }
     if (MODE & JM_FINAL) {
@@ -519,13 +533,7 @@ __device__ __forceinline__ float4 jacobi_particle_tiled(const JacobiArgs& a, con
             jacobi_entry<FAST>(pi, tile_neighbour(tl + lidx, w), tile_pair<LDSP>(a, lpairs, w), ax, ay, az);
         }
         float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
-        const float ml = len3(dx, dy, dz);
-        if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
-            const float sc = JACOBI_CORRECTION_CAP / ml;
-            dx = dx * sc;
-            dy = dy * sc;
-            dz = dz * sc;
-        }
+        cap_correction(dx, dy, dz);
         out = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, wi);
     }
     if (MODE & JM_KEEP) {
@@ -756,13 +764,7 @@ __device__ __forceinline__ float4 shared_particle(const JacobiArgs& a, const flo
             if (k + 1 < width) add(w >> 16);
         }
         float dx = p.omega * ax, dy = p.omega * ay, dz = p.omega * az;
-        const float ml = len3(dx, dy, dz);
-        if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
-            const float sc = JACOBI_CORRECTION_CAP / ml;
-            dx = dx * sc;
-            dy = dy * sc;
-            dz = dz * sc;
-        }
+        cap_correction(dx, dy, dz);
         out = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, wi);
     }
     if (MODE & JM_FINAL) {
@@ -2102,20 +2104,8 @@ __device__ __forceinline__ void gs_apply(const GsArgs& a, uint2 ij, float2 rc, f
     const float ox = p.omega * gx, oy = p.omega * gy, oz = p.omega * gz;
     float ix = (ox * wi) * dlam, iy = (oy * wi) * dlam, iz = (oz * wi) * dlam;
     float jx = -((ox * wj) * dlam), jy = -((oy * wj) * dlam), jz = -((oz * wj) * dlam);
-    float ml = len3(ix, iy, iz);
-    if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
-        const float s = JACOBI_CORRECTION_CAP / ml;
-        ix = ix * s;
-        iy = iy * s;
-        iz = iz * s;
-    }
-    ml = len3(jx, jy, jz);
-    if (ml > JACOBI_CORRECTION_CAP && ml > 0.f) {
-        const float s = JACOBI_CORRECTION_CAP / ml;
-        jx = jx * s;
-        jy = jy * s;
-        jz = jz * s;
-    }
+    cap_correction(ix, iy, iz);
+    cap_correction(jx, jy, jz);
     if (wi > 0.f) m.st(ij.x, make_float4(pi.x + ix, pi.y + iy, pi.z + iz, wi));
     if (wj > 0.f) m.st(ij.y, make_float4(pj.x + jx, pj.y + jy, pj.z + jz, wj));
 }
