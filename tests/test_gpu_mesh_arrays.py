@@ -42,9 +42,10 @@ def sim_config_arrays(f):
                 neighbor_offsets=f["neighbor_offsets"], neighbor_packed=packed, ground=f["ground"])
 
 
-def run_arrays(O, X, f, arrays, solver, S=8, I=6, substeps=3, thickness=None):
+def run_arrays(O, X, f, arrays, solver, S=8, I=6, substeps=3, thickness=None, gs_launch=0):
     mesh = X.ClothMeshData.from_arrays(**arrays)
-    cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=I, solver=solver, autotune=0)
+    cfg = X.ClothSimConfig.default(solve_substeps=S, solve_inner_iterations=I, solver=solver, autotune=0,
+                                   gs_launch=gs_launch)
     uk = {} if thickness is None else dict(thickness=thickness)
     u = X.ClothSimUniforms.default(solver, S, **uk)
     sim = X.ClothComputePlugin(mesh, cfg, u)
@@ -80,15 +81,17 @@ def test_custom_inverse_mass_sheet(O, X, solver):
     assert contacts > 0
 
 
-@pytest.mark.parametrize("solver", [0, 1])
-def test_cloth_obj_through_arrays(O, X, solver):
-    note(f"case arrays: cloth.obj solver={solver}")
+@pytest.mark.parametrize("solver,gs_launch", [(0, 0), (1, 0), (1, 3)], ids=["jacobi", "gs", "gs_tiles"])
+def test_cloth_obj_through_arrays(O, X, solver, gs_launch):
+    """The welded OBJ (its own vertex order: the per-tile-flag GS plans tiles of
+    index ranges whose dependency lists follow that order)."""
+    note(f"case arrays: cloth.obj solver={solver} gs_launch={gs_launch}")
     with open(os.path.join(GOLDEN, "cloth.obj")) as fh:
         f = O.parse_welded_obj(fh.read())
     assert f["num_particles"] == 651
     c = f["positions"].mean(axis=0)
     f["positions"] = ((f["positions"] - c) * np.float32(0.6) + c).astype(np.float32)
-    contacts = run_arrays(O, X, f, sim_config_arrays(f), solver, S=36, I=22, thickness=0.04)
+    contacts = run_arrays(O, X, f, sim_config_arrays(f), solver, S=36, I=22, thickness=0.04, gs_launch=gs_launch)
     assert contacts > 0
 
 
