@@ -2090,36 +2090,17 @@ __device__ __forceinline__ void gs_apply(const GsArgs& a, uint2 ij, float2 rc, f
     const DevParams& p = a.p;
     const float wi = pi.w, wj = pj.w;
     if (wj <= 0.f && wi <= 0.f) return;
-    // The shorter correctly rounded operations where both endpoints are tame
-    // (the conditions of jacobi_entry<true>; the mesh-wide ones, checked by
-    // the host, in a.fastdiv).  The quotients and the root are the same
-    // values; -C - alpha~ * 0 is -C (alpha~ finite); and where -C is -0 the
-    // fast dlam is +0 against -0, which only changes the sign of the zero
-    // corrections, and a tame coordinate (never -0) plus either zero is
-    // itself.  So both paths store the same bits.
-    const bool fast = (a.fastdiv != 0u) & tame6_nb(pi, pj);
     float gx = pi.x - pj.x, gy = pi.y - pj.y, gz = pi.z - pj.z;
-    const float dot = sq3(gx, gy, gz);
-    const float len = fast ? sqrt_rn(dot) : sqrtf(dot);
+    const float len = len3(gx, gy, gz);
     if (len < 1e-8f) return;
+    gx = gx / len;
+    gy = gy / len;
+    gz = gz / len;
     const float c = len - rc.x;
     const float alpha_t = rc.y * p.inv_dt_sq;
     const float wsum = wi + wj + alpha_t;
-    float dlam;
-    if (fast) {
-        const float yl = rcp_rn(len);
-        gx = div_rn_nz(gx, len, yl);
-        gy = div_rn_nz(gy, len, yl);
-        gz = div_rn_nz(gz, len, yl);
-        if (wsum < 1e-8f) return;
-        dlam = div_rn_nz(-c, wsum, rcp_rn(wsum));
-    } else {
-        gx = gx / len;
-        gy = gy / len;
-        gz = gz / len;
-        if (wsum < 1e-8f) return;
-        dlam = (-c - alpha_t * 0.0f) / wsum;
-    }
+    if (wsum < 1e-8f) return;
+    const float dlam = (-c - alpha_t * 0.0f) / wsum;
     const float ox = p.omega * gx, oy = p.omega * gy, oz = p.omega * gz;
     float ix = (ox * wi) * dlam, iy = (oy * wi) * dlam, iz = (oz * wi) * dlam;
     float jx = -((ox * wj) * dlam), jy = -((oy * wj) * dlam), jz = -((oz * wj) * dlam);

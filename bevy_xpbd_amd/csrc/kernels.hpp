@@ -93,15 +93,6 @@ __device__ __forceinline__ bool tame_coord(float c) {
     return __float_as_uint(c) == 0u || (a >= TAME_LO && a <= TAME_HI);
 }
 __device__ __forceinline__ bool tame3(float4 v) { return tame_coord(v.x) && tame_coord(v.y) && tame_coord(v.z); }
-// The same test without short-circuit branches (all six compares, then masks).
-__device__ __forceinline__ bool tame_coord_nb(float c) {
-    const float a = fabsf(c);
-    return (__float_as_uint(c) == 0u) | ((a >= TAME_LO) & (a <= TAME_HI));
-}
-__device__ __forceinline__ bool tame6_nb(float4 u, float4 v) {
-    return tame_coord_nb(u.x) & tame_coord_nb(u.y) & tame_coord_nb(u.z) & tame_coord_nb(v.x) & tame_coord_nb(v.y) &
-           tame_coord_nb(v.z);
-}
 
 enum JacobiMode : uint32_t {
     JM_PREDICT = 1,   // inputs are the substep-start state; predict on the fly
@@ -216,7 +207,6 @@ struct GsArgs {
     uint32_t* error;
     uint32_t final_write;
     uint32_t probe;      // 1: only the barrier set-up (k_gs_substep: and one barrier), the co-residency check
-    uint32_t fastdiv;    // 1: the mesh admits the fast correctly rounded operations (per constraint: tame endpoints)
     DevParams p;
 };
 

@@ -406,7 +406,6 @@ void enqueue_substep(xpbd_cloth* h, bool collision) {
         g.error = h->err;
         g.final_write = collision ? 0u : 1u;
         g.p = p;
-        g.fastdiv = fast_division_active(h) ? 1u : 0u;
         {
             ProbeScope ps(h, XPBD_KERNEL_GS_SUBSTEP);
             enqueue_gs(h, g, collision, nparts, s);
@@ -1721,7 +1720,6 @@ int autotune(xpbd_cloth* h) {
         g.aabb = h->aabb;
         g.final_write = 1;
         g.p = p;
-        g.fastdiv = fast_division_active(h) ? 1u : 0u;
         float gbest = 1e30f;
         uint32_t gbest_blocks = h->gs_blocks;
         for (uint32_t b = h->gs_blocks; b >= 64; b /= 2) {
@@ -1768,7 +1766,6 @@ int autotune(xpbd_cloth* h) {
         g.aabb = h->aabb;
         g.final_write = 1;
         g.p = p;
-        g.fastdiv = fast_division_active(h) ? 1u : 0u;
         auto time_graph = [&](auto body) -> float {
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
@@ -1905,7 +1902,6 @@ int verify_coresident(xpbd_cloth* h, bool gs, bool persistent) {
         g.error = h->err;
         g.probe = 1;
         g.p = p;
-        g.fastdiv = fast_division_active(h) ? 1u : 0u;
         launch_gs_grid(g, h->gg, s);
         HIPC(h, hipGetLastError());
         uint32_t flag = 0;
@@ -2470,7 +2466,6 @@ XPBD_API int xpbd_time_kernel(xpbd_cloth* h, int which, uint32_t reps, float* av
         g.error = h->err;
         g.final_write = 1;
         g.p = p;
-        g.fastdiv = fast_division_active(h) ? 1u : 0u;
     }
     auto one = [&]() {
         switch (which) {

@@ -75,19 +75,18 @@ def test_gs_tiles_bit_exact(O, X, tile, blocks):
     sim.close()
 
 
-@pytest.mark.parametrize("graph,fast", [(0, 0), (1, 0), (1, 1)])
-def test_gs_tiles_frames_and_graph(O, X, graph, fast):
+@pytest.mark.parametrize("graph", [0, 1])
+def test_gs_tiles_frames_and_graph(O, X, graph):
     """Whole frames (the substep loop, replayed from a hipGraph or not) on the
-    256-wide folding sheet, with its contacts; with the fast correctly rounded
-    operations on constraints whose endpoints are tame, and without."""
-    note(f"case gs tiles 256 fold graph={graph} fast={fast}")
+    256-wide folding sheet, with its contacts."""
+    note(f"case gs tiles 256 fold graph={graph}")
     h = 1 / 255
     kw = dict(cols=256, rows=96, spacing=h, layout=1, height=1.0, angle=1.643758, pins=3)
     mesh = X.ClothMeshData.sheet(**kw)
     f = mesh.fields()
     S = 4
     sim, cfg, u = make(X, mesh, solver=1, S=S, I=8, gs_launch=3, use_graph=graph, thickness=2 * (0.4 * h),
-                       jacobi_omega=1.0, fast_division=fast)
+                       jacobi_omega=1.0)
     s = O.Solver(f)
     p = oracle_params(O, X, u, cfg, collision=True)
     pos, vel = f["positions"].copy(), np.zeros_like(f["positions"])
@@ -111,7 +110,7 @@ def test_gs_tiles_equal_colours_on_1024_sheet(X):
     out = []
     for form, tile, rows in ((2, 0, 0), (3, 0, 0), (3, 256, 0), (3, 0, 32)):
         sim, _, _ = make(X, X.ClothMeshData.sheet(**kw), solver=1, S=8, I=10, gs_launch=form, use_graph=1,
-                         thickness=2 * (0.4 * h), jacobi_omega=1.0, fast_division=1)
+                         thickness=2 * (0.4 * h), jacobi_omega=1.0)
         if tile:
             sim.gs_tiles(tile)
         if rows:
@@ -148,17 +147,15 @@ def test_gs_grid_plan(X):
     sim.close()
 
 
-@pytest.mark.parametrize("cols,rows,th,fast", [(128, 72, 8, 0), (128, 72, 16, 1), (128, 72, 32, 0), (128, 72, 64, 1),
-                                               (64, 100, 16, 0), (192, 40, 0, 1)])
-def test_gs_grid_bit_exact(O, X, cols, rows, th, fast):
+@pytest.mark.parametrize("cols,rows,th", [(128, 72, 8), (128, 72, 16), (128, 72, 32), (128, 72, 64), (64, 100, 16),
+                                          (192, 40, 0)])
+def test_gs_grid_bit_exact(O, X, cols, rows, th):
     """LDS grid tiles of each height, a partial last row of tiles, a single
-    column of tiles, and the create-time height, with collision, with and
-    without the fast operations."""
-    note(f"case gs grid {cols}x{rows} th={th} fast={fast}")
+    column of tiles, and the create-time height, with collision."""
+    note(f"case gs grid {cols}x{rows} th={th}")
     mesh = _sheet(X, cols=cols, rows=rows, seed=cols + rows)
     f = mesh.fields()
-    sim, cfg, u = make(X, mesh, solver=1, S=8, I=6, gs_launch=3, thickness=0.016, jacobi_omega=1.0,
-                       fast_division=fast)
+    sim, cfg, u = make(X, mesh, solver=1, S=8, I=6, gs_launch=3, thickness=0.016, jacobi_omega=1.0)
     plan = sim.gs_grid(th) if th else sim.gs_grid()
     assert plan[3] == 1 and (th == 0 or plan[0] == th), plan
     _check_substeps(O, X, sim, cfg, u, f, substeps=4)

@@ -423,19 +423,6 @@ def test_config2_256_collision_substeps_bit_exact(O, X, solver, gs_launch):
              jacobi_omega=0.32 if solver == 0 else 1.0, gs_launch=gs_launch)
 
 
-@pytest.mark.parametrize("gs_launch", [1, 2, 3], ids=["gs_persistent", "gs_colours", "gs_tiles"])
-def test_gs_fast_operations_bit_exact(O, X, gs_launch):
-    """Coloured GS with the fast correctly rounded division and square root on
-    each constraint whose endpoints are tame (fast_division = 1), in each
-    launch form, on the config-2 sheet with its contacts."""
-    note(f"case 256 gs fast gs_launch={gs_launch}")
-    h = 1 / 255
-    rng = np.random.default_rng(2)
-    kw = dict(cols=256, rows=256, spacing=h, layout=1, height=1.0, angle=float(rng.uniform(0, 2 * np.pi)), pins=3)
-    run_pair(O, X, kw, solver=1, S=4, I=8, substeps=4, collision=True, thickness=2 * (0.4 * h), jacobi_omega=1.0,
-             gs_launch=gs_launch, fast_division=1)
-
-
 @pytest.mark.parametrize("solver", [0, 1])
 def test_autotuned_launch_shapes_bit_exact(O, X, solver):
     """Autotuned launch shapes (Jacobi block and block order; the coloured
