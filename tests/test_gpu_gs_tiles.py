@@ -173,3 +173,25 @@ def test_gs_grid_iteration_counts(O, X, iters):
     assert sim.gs_grid(16)[3] == 1
     _check_substeps(O, X, sim, cfg, u, f, substeps=3)
     sim.close()
+
+
+def test_gs_grid_long_run_equals_per_colour(X):
+    """Many colour steps of the flag protocol: 8 frames (64 substeps, 7 680
+    colour steps per tile) of the 1024x1024 sheet on LDS grid tiles stay the
+    bits of one launch per colour, frame by frame."""
+    note("case gs grid 8 frames")
+    h = 1 / 1023
+    kw = dict(cols=1024, rows=1024, spacing=h, pins=2, jitter=1e-3, seed=3)
+    sims = []
+    for form in (2, 3):
+        sim, _, _ = make(X, X.ClothMeshData.sheet(**kw), solver=1, S=8, I=10, gs_launch=form, use_graph=1,
+                         thickness=2 * (0.4 * h), jacobi_omega=1.0)
+        sims.append(sim)
+    assert sims[1].gs_grid()[3] == 1
+    for frame in range(8):
+        for sim in sims:
+            sim.step_frames(1)
+        a, b = sims[0].read_state(), sims[1].read_state()
+        assert bits_equal(a[0], b[0]) and bits_equal(a[1], b[1]), f"frame {frame}"
+    for sim in sims:
+        sim.close()

@@ -157,3 +157,25 @@ def test_resident_error_word_ends_launch_promptly(O, X):
     sim.reset()
     _check_substeps(O, X, sim, cfg, u, f, substeps=2, collision=True)
     sim.close()
+
+
+def test_resident_long_run_equals_per_iteration(X):
+    """Many substeps of the flag protocol: 24 frames (192 substeps, 1 920
+    ring exchanges per tile) of the bench sheet on resident tiles stay the
+    bits of the per-iteration launches, frame by frame."""
+    note("case jacobi resident 24 frames")
+    h = 1 / 1023
+    kw = dict(cols=1024, rows=1024, spacing=h, pins=2, jitter=1e-3, seed=3)
+    sims = []
+    for form in (0, 2):
+        sim, _, _ = make(X, X.ClothMeshData.sheet(**kw), S=8, I=10, persistent_jacobi=form, fast_division=1,
+                         use_graph=1, thickness=2 * (0.4 * h))
+        assert sim.jacobi_resident()[3] == (1 if form == 2 else 0)
+        sims.append(sim)
+    for frame in range(24):
+        for sim in sims:
+            sim.step_frames(1)
+        a, b = sims[0].read_state(), sims[1].read_state()
+        assert bits_equal(a[0], b[0]) and bits_equal(a[1], b[1]), f"frame {frame}"
+    for sim in sims:
+        sim.close()
