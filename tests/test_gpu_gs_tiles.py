@@ -195,3 +195,22 @@ def test_gs_grid_long_run_equals_per_colour(X):
         assert bits_equal(a[0], b[0]) and bits_equal(a[1], b[1]), f"frame {frame}"
     for sim in sims:
         sim.close()
+
+
+@pytest.mark.parametrize("cols,rows", [(64, 17), (128, 200), (320, 96), (640, 333)])
+def test_gs_grid_shapes_equal_per_colour(X, cols, rows):
+    """Cloth shapes with partial last rows of grid tiles: 3 frames with
+    collision on LDS grid tiles equal one launch per colour bit for bit."""
+    note(f"case gs grid shape {cols}x{rows}")
+    h = 1 / (cols - 1)
+    kw = dict(cols=cols, rows=rows, spacing=h, layout=1, height=0.6, angle=0.4, pins=3)
+    out = []
+    for form in (2, 3):
+        sim, _, _ = make(X, X.ClothMeshData.sheet(**kw), solver=1, S=4, I=6, gs_launch=form, use_graph=1,
+                         thickness=2 * (0.4 * h), jacobi_omega=1.0)
+        if form == 3:
+            assert sim.gs_grid()[3] == 1
+        sim.step_frames(3)
+        out.append(sim.read_state())
+        sim.close()
+    assert bits_equal(out[0][0], out[1][0]) and bits_equal(out[0][1], out[1][1])

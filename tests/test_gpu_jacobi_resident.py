@@ -179,3 +179,22 @@ def test_resident_long_run_equals_per_iteration(X):
         assert bits_equal(a[0], b[0]) and bits_equal(a[1], b[1]), f"frame {frame}"
     for sim in sims:
         sim.close()
+
+
+@pytest.mark.parametrize("cols,rows", [(64, 17), (128, 200), (320, 96), (640, 333)])
+def test_resident_shapes_equal_per_iteration(X, cols, rows):
+    """Cloth shapes with partial last rows of tiles at each height the plan
+    picks: 4 frames with collision on resident tiles equal the per-iteration
+    launches bit for bit."""
+    note(f"case jacobi resident shape {cols}x{rows}")
+    h = 1 / (cols - 1)
+    kw = dict(cols=cols, rows=rows, spacing=h, layout=1, height=0.6, angle=0.4, pins=3)
+    out = []
+    for form in (0, 2):
+        sim, _, _ = make(X, X.ClothMeshData.sheet(**kw), S=4, I=8, persistent_jacobi=form, fast_division=1,
+                         use_graph=1, thickness=2 * (0.4 * h))
+        assert sim.jacobi_resident()[3] == (form == 2)
+        sim.step_frames(4)
+        out.append(sim.read_state())
+        sim.close()
+    assert bits_equal(out[0][0], out[1][0]) and bits_equal(out[0][1], out[1][1])
