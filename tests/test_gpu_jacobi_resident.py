@@ -198,3 +198,27 @@ def test_resident_shapes_equal_per_iteration(X, cols, rows):
         out.append(sim.read_state())
         sim.close()
     assert bits_equal(out[0][0], out[1][0]) and bits_equal(out[0][1], out[1][1])
+
+
+def test_resident_form_switched_at_run_time(X):
+    """set_config between frames moves Jacobi between per-iteration launches,
+    resident tiles and the grid-barrier launch (the frame graph rebuilt each
+    time); the trajectory equals a run that never switches."""
+    note("case jacobi resident switch")
+    kw = dict(cols=128, rows=96, spacing=1 / 127, layout=1, height=1.0, angle=0.7, pins=3)
+    th = 2 * 0.4 / 127
+    a, cfg, _ = make(X, X.ClothMeshData.sheet(**kw), S=4, I=6, use_graph=1, fast_division=1, thickness=th)
+    b, _, _ = make(X, X.ClothMeshData.sheet(**kw), S=4, I=6, use_graph=1, fast_division=1, thickness=th)
+    for k, form in enumerate((2, 0, 1, 2, 0)):
+        cfg.persistent_jacobi = form
+        a.set_config(cfg)
+        assert a.jacobi_resident()[3] == (form == 2)
+        a.step_frames(2)
+        a.sync()
+        b.step_frames(2)
+        b.sync()
+        pa, va = a.read_state()
+        pb, vb = b.read_state()
+        assert bits_equal(pa, pb) and bits_equal(va, vb), f"after switch {k} to persistent_jacobi={form}"
+    a.close()
+    b.close()
