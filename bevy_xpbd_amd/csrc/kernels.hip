@@ -2165,14 +2165,15 @@ __global__ __launch_bounds__(256) void k_gs_substep(GsArgs a) {
 // bounded like gs_wait (on expiry the error word is set and this and every
 // later wait of the launch return at once).
 // Called by the lanes of wavefront 0: poll until each lane's flag (nullptr:
-// none) is >= s.
+// none) is >= s, sleeping SLEEP x 64 clocks between polls.
+template <int SLEEP = 1>
 __device__ __forceinline__ void poll_flags(uint32_t* f, uint32_t s, uint32_t* err) {
     if (!__builtin_amdgcn_readfirstlane(gs_load(err))) {
         for (uint32_t spins = 1;; ++spins) {
             if (__all(f == nullptr || gs_load(f) >= s)) break;
-            __builtin_amdgcn_s_sleep(1);
+            __builtin_amdgcn_s_sleep(SLEEP);
             if ((spins & 1023u) == 0 && __builtin_amdgcn_readfirstlane(gs_load(err))) break;
-            if (spins > (1u << 22)) {   // seconds at most: report instead of hanging
+            if (spins > (1u << 22) / (1u + SLEEP / 32)) {   // seconds at most: report instead of hanging
                 if (threadIdx.x == 0) atomicExch(err, 1u);
                 break;
             }
@@ -2368,12 +2369,15 @@ __device__ __forceinline__ GsgRows gsg_rows(const GsGrid& t, const uint32_t* co,
 }
 
 // Wavefront 0 polls the 8 neighbour tiles' flags (lanes 0..8), then the block
-// synchronises.
+// synchronises.  Polls 64 x 64 clocks apart: the polls' own loads at the
+// coherence point slowed the other tiles' ring traffic (sleeps of 1, 4, 16,
+// 32, 64: 1 172, 1 187-1 207, 1 234-1 242, 1 259-1 261, 1 275-1 277
+// substeps/s on the 1024^2 sheet).
 __device__ __forceinline__ void gsg_wait(const GsGrid& t, uint32_t tx, uint32_t ty, uint32_t s, uint32_t* err) {
     if (threadIdx.x < 64) {
         const int l = int(threadIdx.x), nx = int(tx) + l % 3 - 1, ny = int(ty) + l / 3 - 1;
         const bool nb = l < 9 && l != 4 && nx >= 0 && ny >= 0 && nx < int(t.tx) && ny < int(t.ty);
-        poll_flags(nb ? t.flag + ny * int(t.tx) + nx : nullptr, s, err);
+        poll_flags<64>(nb ? t.flag + ny * int(t.tx) + nx : nullptr, s, err);
     }
     __syncthreads();
 }
