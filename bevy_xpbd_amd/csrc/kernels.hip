@@ -2185,7 +2185,7 @@ __device__ __forceinline__ void gs_tile_wait(const GsTiles& t, uint32_t T, uint3
     if (threadIdx.x < 64) {
         const uint32_t d0 = t.doff[T], nd = t.doff[T + 1] - d0;
         uint32_t* f = threadIdx.x < nd ? t.flag + t.dep[d0 + threadIdx.x] : nullptr;
-        if (nd) poll_flags(f, s, err);
+        if (nd) poll_flags<16>(f, s, err);   // 16 x 64 clocks apart: 748 -> 766 at 1024^2, +4% at 256^2
     }
     __syncthreads();
 }
@@ -2371,8 +2371,8 @@ __device__ __forceinline__ GsgRows gsg_rows(const GsGrid& t, const uint32_t* co,
 // Wavefront 0 polls the 8 neighbour tiles' flags (lanes 0..8), then the block
 // synchronises.  Polls 64 x 64 clocks apart: the polls' own loads at the
 // coherence point slowed the other tiles' ring traffic (sleeps of 1, 4, 16,
-// 32, 64: 1 172, 1 187-1 207, 1 234-1 242, 1 259-1 261, 1 275-1 277
-// substeps/s on the 1024^2 sheet).
+// 32, 64, 127: 1 172, 1 187-1 207, 1 234-1 242, 1 259-1 261, 1 275-1 280,
+// 1 143 substeps/s on the 1024^2 sheet).
 __device__ __forceinline__ void gsg_wait(const GsGrid& t, uint32_t tx, uint32_t ty, uint32_t s, uint32_t* err) {
     if (threadIdx.x < 64) {
         const int l = int(threadIdx.x), nx = int(tx) + l % 3 - 1, ny = int(ty) + l / 3 - 1;
